@@ -1,0 +1,845 @@
+// HIP kernels of the analytic-element DFN solve for CDNA4 (gfx950).
+//
+// One iteration of AnDFN's fixed point (andfn/hpc/hpc_solve.py:152-239) is, on the device:
+//   sweep_kernel<RHS>   series-only potential at the K discharge points of every discharge
+//                       element side                           (build_head_matrix, hpc_solve.py:778-829)
+//   rhs_reduce_kernel   mean over the K points -> right-hand side b
+//   [rocSOLVER getrs]   y = A^-1 b with the factors of the constant discharge matrix
+//   scatter_kernel      q, fracture constants, relative discharge change (hpc_solve.py:169-174, 720-761)
+//   sweep_kernel<COLLOC> potential at every collocation point of every element, all other
+//                       elements of the fracture superposed   (calc_omega inside the Cauchy fits,
+//                                                              hpc_fracture.py:51-92)
+//   fit_kernel          Cauchy/DFT fit, element update, errors, damping (hpc_math_functions.py:243-297,
+//                       537-670; element solves in hpc_intersection.py, hpc_bounding_circle.py,
+//                       hpc_const_head_line.py, hpc_imp_object.py; hpc_solve.py:383-388)
+//   grow_kernel         coefficient-growth test                 (hpc_solve.py:465-559)
+//   blend_kernel        second damping step                     (hpc_solve.py:561-575)
+//   reduce_kernel       max element error / coefficient change  (hpc_solve.py:196-197, 260-279)
+//
+// Layout in HBM (structure of arrays):
+//   per element: type, frac0, frac1, ncoef, nint (int32); endpoints ep[4] (complex: ep0[0..1], ep1[0..1]);
+//     radius, q, phi (f64); center (complex); coef and work coef [E x M] complex, M = coefficient
+//     capacity; errors (f64); dpsi_corr [E x 2M] f64.
+//   per fracture: t, constant (f64); member slots [mem_ptr[f], mem_ptr[f+1]) of Slot records that
+//     carry the source's map (alpha, beta) in that fracture and its sign (-1 for an intersection
+//     seen from its second fracture).
+//   target sets: per fracture, the collocation (or discharge) points of its element sides are
+//     packed contiguously and cut into 64-point tiles; one wavefront per tile walks the
+//     fracture's sources, so every source parameter and coefficient is a wave-uniform scalar load.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aem_kernels.hpp"
+#include "aem_math.hpp"
+
+namespace aem {
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// --------------------------------------------------------------------------------------
+// Target points
+// --------------------------------------------------------------------------------------
+struct PointInfo {
+  double2 z;
+  int elem;     // element the point belongs to (excluded in the collocation sweep)
+  int imag;     // 1 when the fit uses Im Omega (circles, impermeable line)
+};
+
+__device__ __forceinline__ int find_side(const int* __restrict__ pt_off, int lo, int hi, int p) {
+  // largest s in [lo, hi) with pt_off[s] <= p
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (pt_off[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// collocation point i of n on element e, side `which` (calc_thetas + map_chi_to_z_*,
+// hpc_math_functions.py:673-699, hpc_geometry_functions.py:34-55, 80-99)
+__device__ __forceinline__ double2 colloc_point(const NetView& net, int e, int which, int i, int n) {
+  int ty = net.etype[e];
+  if (ty == 1) {  // full circle: theta_i = 2 pi i / n
+    double s, c;
+    sincospi(2.0 * (double)i / (double)n, &s, &c);
+    double r = net.radius[e];
+    double2 ce = net.center[e];
+    return c2(fma(r, c, ce.x), fma(r, s, ce.y));
+  }
+  double s, c;
+  sincospi((double)(2 * i + 1) / (double)(2 * n), &s, &c);  // theta_i = (i + 1/2) pi / n
+  if (ty == 4) {  // impermeable circle: half-circle angles on the full circle map
+    double r = net.radius[e];
+    double2 ce = net.center[e];
+    return c2(fma(r, c, ce.x), fma(r, s, ce.y));
+  }
+  const double2* ep = net.ep + 4 * e + 2 * which;
+  double2 a = ep[0], b = ep[1];
+  // z = ((chi + 1/chi)/2 (b - a) + a + b)/2 with chi on the unit circle: (chi + 1/chi)/2 = cos theta
+  return c2(0.5 * fma(c, b.x - a.x, a.x + b.x), 0.5 * fma(c, b.y - a.y, a.y + b.y));
+}
+
+// discharge point i of K on element e, side `which` (hpc_intersection.py:13-19,
+// hpc_const_head_line.py:167-184, hpc_well.py:111-130)
+__device__ __forceinline__ double2 discharge_point(const NetView& net, int e, int which, int i, int K) {
+  int ty = net.etype[e];
+  if (ty == 2) {
+    double th = (K > 1) ? (double)i * ((2.0 * kPi - 2.0 * kPi / K) / (double)(K - 1)) : 0.0;
+    double s, c;
+    sincos(th, &s, &c);
+    double r = net.radius[e];
+    double2 ce = net.center[e];
+    return c2(r * c + ce.x, r * s + ce.y);
+  }
+  const double2* ep = net.ep + 4 * e + 2 * which;
+  double2 a = ep[0], b = ep[1];
+  double2 step = c2((b.x - a.x) / (double)(K + 1), (b.y - a.y) / (double)(K + 1));
+  return c2(a.x + (double)(i + 1) * step.x, a.y + (double)(i + 1) * step.y);
+}
+
+// --------------------------------------------------------------------------------------
+// Sweep: Omega_f(z) = C_f + sum over the fracture's sources (hpc_fracture.py:51-92)
+// --------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
+  const int t = blockIdx.x;
+  const Tile tile = tg.tiles[t];
+  const int lane = threadIdx.x;
+  const int p = tile.p0 + lane;
+  const bool valid = p < tile.pend;
+  const int f = tile.frac;
+
+  constexpr bool colloc_pts = (MODE == MODE_COLLOC || MODE == MODE_COLLOC_HEADS);
+  constexpr bool use_q = (MODE != MODE_RHS);
+  constexpr bool heads_out = (MODE == MODE_POINTS || MODE == MODE_COLLOC_HEADS || MODE == MODE_DISCH_HEADS);
+  double2 z = c2(0.0, 0.0);
+  int excl = -1;
+  int want_imag = 0;
+  if (valid) {
+    if (MODE == MODE_POINTS) {
+      z = tg.pt_z[p];
+    } else {
+      int s = find_side(tg.pt_off, tile.sbeg, tile.send, p);
+      int e = tg.side_elem[s];
+      int which = tg.side_which[s];
+      int i = p - tg.pt_off[s];
+      if (colloc_pts) {
+        int n = net.nint[e];
+        z = colloc_point(net, e, which, i, n);
+        if (MODE == MODE_COLLOC) {
+          excl = e;
+          int ty = net.etype[e];
+          want_imag = (ty == 1 || ty == 4 || ty == 5);
+        }
+      } else {
+        z = discharge_point(net, e, which, i, K);
+      }
+    }
+  }
+  const bool any_imag = __any(want_imag);
+
+  double ore = 0.0, oim = 0.0;
+  const int kbeg = net.mem_ptr[f];
+  const int kend = net.mem_ptr[f + 1];
+  for (int k = kbeg; k < kend; ++k) {
+    const Slot sl = net.slots[k];
+    const int src = sl.elem;
+    const int m = net.ncoef[src];
+    const double2* __restrict__ cf = net.coef + (size_t)src * net.M;
+    double tre = 0.0, tim = 0.0;
+    if (sl.kind == KIND_LINE || sl.kind == KIND_IMPL) {
+      double2 Z = c2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
+                     fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
+      double2 chi = joukowsky_exterior(Z);
+      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+      double inv = 1.0 / r2;
+      double2 w = c2(chi.x * inv, -chi.y * inv);
+      double2 h = horner(cf, m, w);
+      tre = sl.sign * h.x;
+      tim = sl.sign * h.y;
+      if (sl.kind == KIND_LINE && use_q) {
+        double qq = sl.sign * net.q[src] * kInv2Pi;
+        if (qq != 0.0) {
+          tre = fma(qq, 0.5 * log(r2), tre);
+          if (any_imag) tim = fma(qq, atan2(chi.y, chi.x), tim);
+        }
+      }
+    } else if (sl.kind == KIND_BOUND) {
+      double2 chi = c2(sl.alpha.x * z.x, sl.alpha.x * z.y);  // z / R
+      double2 h = horner(cf, m, chi);
+      tre = h.x;
+      tim = h.y;
+    } else if (sl.kind == KIND_WELL) {
+      double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
+      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+      double qq = use_q ? net.q[src] * kInv2Pi : 0.0;
+      if (qq != 0.0) {
+        tre = qq * (0.5 * log(r2));
+        if (any_imag) tim = qq * atan2(chi.y, chi.x);
+      }
+      if (sqrt(r2) < 1.0 - 1e-12) {
+        tre = __builtin_nan("");
+        tim = __builtin_nan("");
+      }
+    } else {  // KIND_IMPC
+      double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
+      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+      double inv = 1.0 / r2;
+      double2 w = c2(chi.x * inv, -chi.y * inv);
+      double2 h = horner(cf, m, w);
+      tre = h.x;
+      tim = h.y;
+      if (sqrt(r2) < 1.0 - 1e-10) {
+        tre = __builtin_nan("");
+        tim = __builtin_nan("");
+      }
+    }
+    if (src != excl) {
+      ore += tre;
+      oim += tim;
+    }
+  }
+  if (valid) {
+    if (MODE == MODE_COLLOC)
+      val[p] = want_imag ? oim : ore + net.constant[f];
+    else if (heads_out)
+      val[p] = (ore + net.constant[f]) / net.t[f];
+    else
+      val[p] = ore;
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// Fit: one wavefront per element (hpc_math_functions.py:243-297, 537-670 and the element solves)
+// --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
+                                                 const double* __restrict__ val, const int* __restrict__ pt_off,
+                                                 double damping, int blend) {
+  __shared__ double sv[2][kMaxPts];
+  __shared__ double sdpsi[kMaxPts];
+  const FitItem it = items[blockIdx.x];
+  const int e = it.elem;
+  const int lane = threadIdx.x;
+  const int ty = net.etype[e];
+  const int m = net.ncoef[e];
+  const int n = net.nint[e];
+  const int nsides = (ty == 0) ? 2 : 1;
+  for (int s = 0; s < nsides; ++s) {
+    const int base = pt_off[s == 0 ? it.s0 : it.s1];
+    for (int i = lane; i < n; i += 64) sv[s][i] = val[base + i];
+  }
+  const bool circle = (ty == 1 || ty == 4 || ty == 5);
+  __syncthreads();
+  if (circle) {
+    // branch-cut correction kappa_i = sum q * sign over the cached crossings, then unwrap
+    // (get_dpsi_corr hpc_math_functions.py:406-447; cauchy_integral_domega 583-595)
+    for (int i = lane; i < n; i += 64) sdpsi[i] = 0.0;
+    __syncthreads();
+    if (lane == 0) {
+      const int cb = fv.cross_ptr[e];
+      const int cn = fv.cross_cnt[e];
+      for (int c = 0; c < cn; ++c) {
+        const int2 rec = fv.cross[cb + c];  // (position, element); sign in cross_sign
+        sdpsi[rec.x] += net.q[rec.y] * (double)fv.cross_sign[cb + c];
+      }
+      double* gd = fv.dpsi_corr + (size_t)e * 2 * net.M;
+      for (int i = 0; i < n - 1; ++i) gd[i] = sdpsi[i];
+      double psi0 = sv[0][0];
+      double prev = sv[0][0];
+      for (int i = 1; i < n; ++i) {
+        double cur = sv[0][i];
+        double d = (cur - prev) - sdpsi[i - 1];
+        prev = cur;
+        psi0 = psi0 + d;
+        sv[0][i] = psi0;
+      }
+    }
+    __syncthreads();
+  }
+
+  // per-lane coefficients j = lane, lane + 64, ...
+  double2* wc = net.wcoef + (size_t)e * net.M;
+  double2* cf = net.coef + (size_t)e * net.M;
+  double s_new_re = 0.0, s_new_im = 0.0, s_tail_re = 0.0, s_tail_im = 0.0, s_old5 = 0.0, s_diff = 0.0;
+  const int jtail = (int)(m * 0.9);
+  const double t0 = (ty == 0) ? net.t[net.efrac0[e]] : 0.0;
+  const double t1 = (ty == 0) ? net.t[net.efrac1[e]] : 0.0;
+  for (int j = lane; j < m; j += 64) {
+    double x0 = 0.0, y0 = 0.0, x1 = 0.0;
+    if (ty == 1) {
+      for (int i = 0; i < n; ++i) {
+        double s, c;
+        sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &s, &c);
+        x0 = fma(sv[0][i], c, x0);
+        y0 = fma(sv[0][i], s, y0);
+      }
+    } else {
+      const int64_t period = 4 * (int64_t)n;
+      for (int i = 0; i < n; ++i) {
+        double s, c;
+        sincospi((double)(((int64_t)j * (2 * i + 1)) % period) / (double)(2 * n), &s, &c);
+        x0 = fma(sv[0][i], c, x0);
+        y0 = fma(sv[0][i], s, y0);
+        if (ty == 0) x1 = fma(sv[1][i], c, x1);
+      }
+    }
+    const double two_n = 2.0 / (double)n;
+    double2 cnew;
+    if (ty == 0) {
+      // c_j = Re[(t0 a1_j - t1 a0_j)/(t0 + t1)], a_j = (2/n) X_j (hpc_intersection.py:95-101)
+      double a0 = two_n * x0, a1 = two_n * x1;
+      cnew = c2(j == 0 ? 0.0 : (t0 * a1 - t1 * a0) / (t0 + t1), 0.0);
+    } else if (ty == 3) {
+      cnew = c2(j == 0 ? 0.0 : -(two_n * x0), 0.0);  // hpc_const_head_line.py:71-75
+    } else {
+      // b_j = (2i/n) sum psi e^{-ij theta} = (2/n)(Y_j + i X_j); b_0 halved
+      double bre = two_n * y0, bim = two_n * x0;
+      if (j == 0) {
+        bre *= 0.5;
+        bim *= 0.5;
+      }
+      if (ty == 1) cnew = c2(-bre, -bim);                           // hpc_bounding_circle.py:151
+      else if (ty == 4) cnew = c2(j == 0 ? 0.0 : bre, j == 0 ? 0.0 : -bim);  // conj, c_0 = 0
+      else cnew = c2(0.0, j == 0 ? 0.0 : -bim);                     // -i Im b, c_0 = 0
+    }
+    const double2 old = cf[j];
+    wc[j] = cnew;
+    s_new_re += cnew.x;
+    s_new_im += cnew.y;
+    if (j >= jtail) {
+      s_tail_re += cnew.x;
+      s_tail_im += cnew.y;
+    }
+    if (j < 5) s_old5 += hypot(old.x, old.y);
+    s_diff += hypot(cnew.x - old.x, cnew.y - old.y);
+    if (blend) cf[j] = c2(damping * cnew.x + (1.0 - damping) * old.x, damping * cnew.y + (1.0 - damping) * old.y);
+  }
+  s_new_re = wave_sum(s_new_re);
+  s_new_im = wave_sum(s_new_im);
+  s_tail_re = wave_sum(s_tail_re);
+  s_tail_im = wave_sum(s_tail_im);
+  s_old5 = wave_sum(s_old5);
+  s_diff = wave_sum(s_diff);
+  if (lane == 0) {
+    double err = 0.0, errc = 0.0;
+    if (!(s_old5 < 1e-15)) {
+      double a = hypot(s_new_re - s_tail_re, s_new_im - s_tail_im);
+      double b = hypot(s_new_re + 1e-16, s_new_im);
+      err = fabs(a / b - 1.0);
+      errc = s_diff / (double)m;
+    }
+    fv.error_old2[e] = fv.error_old[e];
+    fv.error_old[e] = fv.error[e];
+    fv.error[e] = err;
+    fv.error_coef[e] = errc;
+  }
+}
+
+// second damping step for every element (hpc_solve.py:561-575); also the only update of wells
+__global__ void blend_kernel(NetView net, int ne, double damping) {
+  const int e = blockIdx.x;
+  if (e >= ne) return;
+  const int m = net.ncoef[e];
+  double2* cf = net.coef + (size_t)e * net.M;
+  const double2* wc = net.wcoef + (size_t)e * net.M;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) {
+    double2 a = wc[j], b = cf[j];
+    cf[j] = c2(damping * a.x + (1.0 - damping) * b.x, damping * a.y + (1.0 - damping) * b.y);
+  }
+}
+
+// wells: error = 0 and the first damping step (their work coefficients stay zero)
+__global__ void well_kernel(NetView net, FitView fv, const int* __restrict__ wells, int nw, double damping) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nw) return;
+  const int e = wells[k];
+  fv.error[e] = 0.0;
+  const int m = net.ncoef[e];
+  double2* cf = net.coef + (size_t)e * net.M;
+  const double2* wc = net.wcoef + (size_t)e * net.M;
+  for (int j = 0; j < m; ++j) {
+    double2 a = wc[j], b = cf[j];
+    cf[j] = c2(damping * a.x + (1.0 - damping) * b.x, damping * a.y + (1.0 - damping) * b.y);
+  }
+}
+
+// coefficient-growth test on the damped coefficients (hpc_solve.py:465-559); grows in place
+__global__ void grow_kernel(NetView net, FitView fv, int ne, GrowParams gp, const int* __restrict__ sched,
+                            int* __restrict__ grow) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  grow[e] = 0;
+  if (net.etype[e] == 2) return;
+  const int m = net.ncoef[e];
+  if (sched) {  // replay of a recorded trajectory: grow where the recorded count rises
+    if (sched[e] > m && m < gp.max_coef) {
+      net.ncoef_w[e] = m + gp.increase;
+      net.nint_w[e] = 2 * (m + gp.increase);
+      grow[e] = 1;
+    }
+    return;
+  }
+  const double2* c = net.coef + (size_t)e * net.M;
+  const double err = fv.error[e];
+  double ratio = 0.0;  // the reference's first ratio is multiplied by zero (hpc_solve.py:474-478)
+  if (err > fv.error_old[e] && err > fv.error_old2[e] && err > gp.max_error && gp.nit > 5) ratio = gp.max_ratio + 1.0;
+  double tre = 0.0, tim = 0.0, hre = 0.0, him = 0.0;
+  const int k3 = m > 3 ? m - 3 : 0;
+  for (int k = k3; k < m; ++k) {
+    tre += c[k].x;
+    tim += c[k].y;
+  }
+  for (int k = 0; k < k3; ++k) {
+    hre += c[k].x;
+    him += c[k].y;
+  }
+  hre += 1e-30;
+  // |tail / head| = |tail| / |head|
+  if (hypot(tre, tim) / hypot(hre, him) > gp.max_ratio && err > gp.max_error) ratio = gp.max_ratio + 1.0;
+  if (ratio > gp.max_ratio && m < gp.max_coef && gp.nit > 1) {
+    const int nm = m + gp.increase;
+    net.ncoef_w[e] = nm;
+    net.nint_w[e] = 2 * nm;
+    grow[e] = 1;
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// Branch-cut crossings of circle-type elements (find_branch_cuts, hpc_math_functions.py:300-392)
+// one wavefront per element; lanes over the collocation edges (i, i+1), i < n - 1
+// --------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 line_chi(const double2* ep, double2 z) {
+  double2 a = ep[0], b = ep[1];
+  double2 num = c2(2.0 * z.x - a.x - b.x, 2.0 * z.y - a.y - b.y);
+  double2 den = c2(b.x - a.x, b.y - a.y);
+  double dd = den.x * den.x + den.y * den.y;
+  double2 Z = c2((num.x * den.x + num.y * den.y) / dd, (num.y * den.x - num.x * den.y) / dd);
+  return joukowsky_exterior(Z);
+}
+
+__device__ __forceinline__ double npsign(double x) { return (double)((x > 0) - (x < 0)); }
+
+__global__ __launch_bounds__(64) void cross_kernel(NetView net, FitView fv, const int* __restrict__ circles, int nc) {
+  const int k = blockIdx.x;
+  if (k >= nc) return;
+  const int e = circles[k];
+  const int lane = threadIdx.x;
+  const int ty = net.etype[e];
+  const int n = net.nint[e];
+  const int f = net.efrac0[e];
+  const int cb = fv.cross_ptr[e];
+  const int cap = fv.cross_cap[e];
+  int cnt = 0;
+  const int kbeg = net.mem_ptr[f], kend = net.mem_ptr[f + 1];
+  for (int kk = kbeg; kk < kend; ++kk) {
+    const int src = net.slots[kk].elem;
+    const int sty = net.etype[src];
+    if (!(sty == 0 || sty == 2 || sty == 3)) continue;
+    for (int i0 = 0; i0 < n - 1; i0 += 64) {
+      const int i = i0 + lane;
+      int hit = 0, sgn = 0;
+      if (i < n - 1) {
+        double2 za = colloc_point(net, e, 0, i, n);
+        double2 zb = colloc_point(net, e, 0, i + 1, n);
+        double2 chi0, chi1;
+        int sdef = -1;
+        if (sty == 0) {
+          const bool own0 = net.efrac0[src] == f;
+          const double2* ep = net.ep + 4 * src + (own0 ? 0 : 2);
+          chi0 = line_chi(ep, za);
+          chi1 = line_chi(ep, zb);
+          double ln0 = atan2(chi0.y, chi0.x), ln1 = atan2(chi1.y, chi1.x);
+          hit = (npsign(ln0) != npsign(ln1)) && (fabs(ln0) + fabs(ln1) > kPi);
+          sdef = own0 ? -1 : 1;
+        } else {
+          if (sty == 2) {
+            double r = net.radius[src];
+            double2 ce = net.center[src];
+            chi0 = c2((za.x - ce.x) / r, (za.y - ce.y) / r);
+            chi1 = c2((zb.x - ce.x) / r, (zb.y - ce.y) / r);
+          } else {
+            chi0 = line_chi(net.ep + 4 * src, za);
+            chi1 = line_chi(net.ep + 4 * src, zb);
+          }
+          hit = (npsign(chi0.y) != npsign(chi1.y)) && (chi0.x < 0.0);
+        }
+        if (hit) sgn = (ty != 1 && chi0.y < chi1.y) ? -sdef : sdef;
+      }
+      const unsigned long long bal = __ballot(hit);
+      if (bal) {
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        const int slot = cnt + before;
+        if (hit && slot < cap) {
+          fv.cross[cb + slot] = make_int2(i, src);
+          fv.cross_sign[cb + slot] = sgn;
+        }
+        cnt += __popcll(bal);
+      }
+    }
+  }
+  if (lane == 0) {
+    fv.cross_cnt[e] = cnt < cap ? cnt : cap;
+    if (cnt > cap) atomicOr(fv.status, 1);
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// Right-hand side (build_head_matrix, hpc_solve.py:812-829)
+// --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
+                                                        const double* __restrict__ val, const int* __restrict__ pt_off,
+                                                        int K, double* __restrict__ b) {
+  const int j = blockIdx.x;
+  if (j >= nde) return;
+  const RhsItem it = items[j];
+  const int lane = threadIdx.x;
+  double s0 = 0.0, s1 = 0.0;
+  const int b0 = pt_off[it.s0];
+  for (int i = lane; i < K; i += 64) s0 += val[b0 + i];
+  if (it.s1 >= 0) {
+    const int b1 = pt_off[it.s1];
+    for (int i = lane; i < K; i += 64) s1 += val[b1 + i];
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  if (lane == 0) {
+    const int e = it.elem;
+    const double om0 = s0 / (double)K;
+    if (net.etype[e] == 0) {
+      const double om1 = s1 / (double)K;
+      b[j] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
+    } else {
+      b[j] = net.phi[e] - om0;
+    }
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// Discharge matrix (fill_discharge_matrix, hpc_solve.py:1117-1201; discharge terms
+// hpc_intersection.py:22-46, hpc_well.py:14-35, hpc_const_head_line.py:14-35)
+// one wavefront per (discharge row, side); dense column-major A (lda = n)
+// --------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg, const int* __restrict__ de_pos,
+                                                      int K, int nde, int64_t n, double* __restrict__ A) {
+  const int s = blockIdx.x;
+  const int e = tg.side_elem[s];
+  const int which = tg.side_which[s];
+  const int row = de_pos[e];
+  const int f = which ? net.efrac1[e] : net.efrac0[e];
+  const int lane = threadIdx.x;
+  const bool inter = net.etype[e] == 0;
+  const double rs = inter ? (which ? -1.0 : 1.0) / net.t[f] : 1.0;
+  for (int k = net.mem_ptr[f]; k < net.mem_ptr[f + 1]; ++k) {
+    const Slot sl = net.slots[k];
+    const int src = sl.elem;
+    if (src == e || !(sl.kind == KIND_LINE || sl.kind == KIND_WELL)) continue;
+    double acc = 0.0;
+    for (int i = lane; i < K; i += 64) {
+      double2 z = discharge_point(net, e, which, i, K);
+      double r2;
+      if (sl.kind == KIND_LINE) {
+        double2 Z = c2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
+                       fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
+        double2 chi = joukowsky_exterior(Z);
+        r2 = fma(chi.x, chi.x, chi.y * chi.y);
+      } else {
+        double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
+        r2 = fma(chi.x, chi.x, chi.y * chi.y);
+      }
+      acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log(r2);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) atomicAdd(&A[(int64_t)de_pos[src] * n + row], rs * acc / (double)K);
+  }
+  if (lane == 0) {
+    if (inter) {
+      if (which == 0) atomicAdd(&A[(int64_t)(nde + net.efrac0[e]) * n + row], 1.0 / net.t[net.efrac0[e]]);
+      else atomicAdd(&A[(int64_t)(nde + net.efrac1[e]) * n + row], -1.0 / net.t[net.efrac1[e]]);
+    } else {
+      atomicAdd(&A[(int64_t)(nde + net.efrac0[e]) * n + row], 1.0);
+    }
+  }
+}
+
+// fracture continuity rows: sum_e s_e q_e = 0 (hpc_solve.py:1187-1201)
+__global__ void continuity_kernel(NetView net, const int* __restrict__ de_pos, int nf, int nde, int64_t n,
+                                  double* __restrict__ A) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int64_t row = nde + f;
+  for (int k = net.mem_ptr[f]; k < net.mem_ptr[f + 1]; ++k) {
+    const int e = net.slots[k].elem;
+    const int ty = net.etype[e];
+    if (!(ty == 0 || ty == 2 || ty == 3)) continue;
+    const double v = (ty != 0 || net.efrac0[e] == f) ? 1.0 : -1.0;
+    A[(int64_t)de_pos[e] * n + row] += v;
+  }
+}
+
+// --------------------------------------------------------------------------------------
+// discharge scatter and relative change (hpc_solve.py:169-174, post_matrix_solve 720-761)
+// --------------------------------------------------------------------------------------
+__global__ void scatter_kernel(NetView net, const int* __restrict__ de, int nde, int nf, const double* __restrict__ y) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nde) net.q_w[de[k]] = y[k];
+  else if (k < nde + nf) net.constant_w[k - nde] = y[k];
+}
+
+__device__ __forceinline__ void nanmax_merge(double& a, int& ia, double b, int ib) {
+  // np.max / np.argmax semantics: NaN wins, first index on ties
+  const bool an = a != a, bn = b != b;
+  if (an) {
+    if (bn && ib < ia) ia = ib;
+    return;
+  }
+  if (bn || b > a || (b == a && ib < ia)) {
+    a = b;
+    ia = ib;
+  }
+}
+
+// one block of 1024 threads: max error (with argmax), max error_coef, max |y - y_old|, max |y_old|
+__global__ __launch_bounds__(1024) void reduce_kernel(FitView fv, int ne, const double* __restrict__ y,
+                                                      const double* __restrict__ yold, int n,
+                                                      IterScalars* __restrict__ out) {
+  __shared__ double se[1024], sc[1024], sd[1024], so[1024];
+  __shared__ int si[1024];
+  const int tid = threadIdx.x;
+  double em = -__builtin_inf(), cm = -__builtin_inf(), dm = 0.0, om = 0.0;
+  int ie = 0x7fffffff, dummy = 0;
+  for (int e = tid; e < ne; e += 1024) {
+    nanmax_merge(em, ie, fv.error[e], e);
+    nanmax_merge(cm, dummy, fv.error_coef[e], e);
+  }
+  for (int k = tid; k < n; k += 1024) {
+    double d = fabs(y[k] - yold[k]);
+    dm = (d != d || d > dm) ? d : dm;
+    double o = fabs(yold[k]);
+    om = (o != o || o > om) ? o : om;
+  }
+  se[tid] = em;
+  si[tid] = ie;
+  sc[tid] = cm;
+  sd[tid] = dm;
+  so[tid] = om;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (tid < s) {
+      double a = se[tid];
+      int ia = si[tid];
+      nanmax_merge(a, ia, se[tid + s], si[tid + s]);
+      se[tid] = a;
+      si[tid] = ia;
+      int d0 = 0;
+      double c = sc[tid];
+      nanmax_merge(c, d0, sc[tid + s], 0);
+      sc[tid] = c;
+      double d = sd[tid], d2 = sd[tid + s];
+      sd[tid] = (d != d) ? d : ((d2 != d2 || d2 > d) ? d2 : d);
+      double o = so[tid], o2 = so[tid + s];
+      so[tid] = (o != o) ? o : ((o2 != o2 || o2 > o) ? o2 : o);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out->error = se[0];
+    out->argmax = si[0];
+    out->error_coef = sc[0];
+    out->dq_max = sd[0];
+    out->yold_max = so[0];
+  }
+}
+
+__global__ void init_errors_kernel(FitView fv, int ne) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  fv.error_old[e] = 1e30;
+  fv.error[e] = 1e30;
+}
+
+// --------------------------------------------------------------------------------------
+// launchers
+// --------------------------------------------------------------------------------------
+void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st) {
+  if (ntiles <= 0) return;
+  if (mode == MODE_COLLOC)
+    hipLaunchKernelGGL(sweep_kernel<MODE_COLLOC>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else if (mode == MODE_RHS)
+    hipLaunchKernelGGL(sweep_kernel<MODE_RHS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else if (mode == MODE_COLLOC_HEADS)
+    hipLaunchKernelGGL(sweep_kernel<MODE_COLLOC_HEADS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else if (mode == MODE_DISCH_HEADS)
+    hipLaunchKernelGGL(sweep_kernel<MODE_DISCH_HEADS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else
+    hipLaunchKernelGGL(sweep_kernel<MODE_POINTS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+}
+
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
+                const int* pt_off, double damping, int blend, hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(fit_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
+}
+
+void launch_blend(const NetView& net, int ne, double damping, hipStream_t st) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(blend_kernel, dim3(ne), dim3(64), 0, st, net, ne, damping);
+}
+
+void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st) {
+  if (nw <= 0) return;
+  hipLaunchKernelGGL(well_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, net, fv, wells, nw, damping);
+}
+
+void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(grow_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, net, fv, ne, gp, sched, grow);
+}
+
+__global__ void permute_kernel(const double* __restrict__ b, const int* __restrict__ perm, int n, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = b[perm[i]];
+}
+
+// diagonal of nb stacked nbk x nbk blocks set to one (unit-triangular inverses)
+__global__ void unit_diag_kernel(double* __restrict__ blocks, int nb, int nbk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nb * nbk) {
+    const int b = k / nbk, i = k % nbk;
+    blocks[(size_t)b * nbk * nbk + (size_t)i * nbk + i] = 1.0;
+  }
+}
+
+void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st) {
+  hipLaunchKernelGGL(unit_diag_kernel, dim3((nb * nbk + 255) / 256), dim3(256), 0, st, blocks, nb, nbk);
+}
+
+void launch_permute(const double* b, const int* perm, int n, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b, perm, n, x);
+}
+
+void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(cross_kernel, dim3(nc), dim3(64), 0, st, net, fv, circles, nc);
+}
+
+void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
+                       double* b, hipStream_t st) {
+  if (nde <= 0) return;
+  hipLaunchKernelGGL(rhs_reduce_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, val, pt_off, K, b);
+}
+
+void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const int* de_pos, int K, int nde, int nf,
+                     int64_t n, double* A, hipStream_t st) {
+  if (nsides > 0)
+    hipLaunchKernelGGL(assemble_kernel, dim3(nsides), dim3(64), 0, st, net, tg, de_pos, K, nde, n, A);
+  if (nf > 0) hipLaunchKernelGGL(continuity_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, net, de_pos, nf, nde, n, A);
+}
+
+void launch_scatter(const NetView& net, const int* de, int nde, int nf, const double* y, hipStream_t st) {
+  int tot = nde + nf;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(scatter_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, net, de, nde, nf, y);
+}
+
+void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
+                   hipStream_t st) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, fv, ne, y, yold, n, out);
+}
+
+void launch_init_errors(const FitView& fv, int ne, hipStream_t st) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(init_errors_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, fv, ne);
+}
+
+// max over the head equations of |(A y)_j - b_j| in metres (constant-head/well rows are in
+// potential units and are divided by t of their fracture via row_scale)
+__global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict__ ay, const double* __restrict__ b,
+                                                        const double* __restrict__ row_scale, int nde,
+                                                        IterScalars* __restrict__ out) {
+  __shared__ double sr[1024];
+  __shared__ int si[1024];
+  const int tid = threadIdx.x;
+  double r = 0.0;
+  int ir = 0x7fffffff;
+  for (int j = tid; j < nde; j += 1024) {
+    double v = fabs(ay[j] - b[j]) * row_scale[j];
+    if (v != v || v > r) {
+      r = v;
+      ir = j;
+    }
+  }
+  sr[tid] = r;
+  si[tid] = ir;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (tid < s) {
+      double a = sr[tid], c = sr[tid + s];
+      if (a == a && (c != c || c > a || (c == a && si[tid + s] < si[tid]))) {
+        sr[tid] = c;
+        si[tid] = si[tid + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out->residual = sr[0];
+    out->arg_residual = si[0];
+  }
+}
+
+void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, IterScalars* out,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(1024), 0, st, ay, b, row_scale, nde, out);
+}
+
+// pointwise boundary-condition residual in metres: head continuity at every collocation point of
+// every intersection, prescribed head at every collocation point of every constant-head line and
+// at the discharge points of every well (andfn/intersection.py:197-225, const_head.py:187-206)
+__global__ __launch_bounds__(64) void bc_residual_kernel(NetView net, const FitItem* __restrict__ items, int nitems,
+                                                         const double* __restrict__ val, const int* __restrict__ pt_off,
+                                                         const RhsItem* __restrict__ witems, int nw,
+                                                         const double* __restrict__ wval,
+                                                         const int* __restrict__ wpt_off, int K,
+                                                         double* __restrict__ out_max) {
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
+  double r = 0.0;
+  if (k < nitems) {
+    const FitItem it = items[k];
+    const int e = it.elem;
+    const int ty = net.etype[e];
+    if (ty == 0 || ty == 3) {
+      const int n = net.nint[e];
+      const int b0 = pt_off[it.s0];
+      for (int i = lane; i < n; i += 64) {
+        double v = (ty == 0) ? fabs(val[b0 + i] - val[pt_off[it.s1] + i]) : fabs(val[b0 + i] - net.phi[e] / net.t[net.efrac0[e]]);
+        r = (v != v || v > r) ? v : r;
+      }
+    }
+  } else if (k - nitems < nw) {
+    const RhsItem it = witems[k - nitems];
+    const int e = it.elem;
+    const int b0 = wpt_off[it.s0];
+    for (int i = lane; i < K; i += 64) {
+      double v = fabs(wval[b0 + i] - net.phi[e] / net.t[net.efrac0[e]]);
+      r = (v != v || v > r) ? v : r;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    double v = __shfl_xor(r, o, 64);
+    r = (v != v || v > r) ? v : r;
+  }
+  if (lane == 0) out_max[k] = r;
+}
+
+void launch_bc_residual(const NetView& net, const FitItem* items, int nitems, const double* val, const int* pt_off,
+                        const RhsItem* witems, int nw, const double* wval, const int* wpt_off, int K, double* out_max,
+                        hipStream_t st) {
+  if (nitems + nw <= 0) return;
+  hipLaunchKernelGGL(bc_residual_kernel, dim3(nitems + nw), dim3(64), 0, st, net, items, nitems, val, pt_off, witems,
+                     nw, wval, wpt_off, K, out_max);
+}
+
+}  // namespace aem

@@ -1,0 +1,134 @@
+// Device-side views and kernel launchers of the analytic-element solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace aem {
+
+constexpr int kMaxPts = 512;  // >= 2 * coefficient capacity handled by the fit kernel
+
+// sweep modes: which points, whether the target's own element is excluded, whether the log
+// (discharge) terms and fracture constants are included, and what is written
+enum {
+  MODE_COLLOC = 0,        // collocation points, own element excluded, full potential; Re (+C) or Im for the fit
+  MODE_RHS = 1,           // discharge points, series only (q = C = 0), Re
+  MODE_POINTS = 2,        // explicit points, full potential, head = Re Omega / t
+  MODE_COLLOC_HEADS = 3,  // collocation points, full potential, head
+  MODE_DISCH_HEADS = 4    // discharge points, full potential, head
+};
+enum { KIND_LINE = 0, KIND_BOUND = 1, KIND_WELL = 2, KIND_IMPC = 3, KIND_IMPL = 4 };
+
+// one (fracture, member element) pair: the element as a source of potential in that fracture
+struct Slot {
+  int elem;
+  int kind;
+  double sign;     // -1 for an intersection seen from its second fracture, else +1
+  double2 alpha;   // line: 2/(b - a); circle: (1/r, 0)
+  double2 beta;    // line: -(a + b)/(b - a); circle: -c/r
+};
+
+struct Tile {
+  int frac;
+  int sbeg, send;  // side range of the fracture in the target set
+  int p0, pend;    // first point of the tile, end of the fracture's points
+};
+
+struct NetView {
+  int M;  // coefficient capacity (stride of coef / wcoef)
+  const int* etype;
+  const int* efrac0;
+  const int* efrac1;
+  const int* ncoef;
+  const int* nint;
+  int* ncoef_w;
+  int* nint_w;
+  const double2* ep;      // [E][4] = ep0[0], ep0[1], ep1[0], ep1[1]
+  const double* radius;
+  const double2* center;
+  const double* phi;
+  const double* q;
+  double* q_w;
+  double2* coef;          // [E][M]
+  double2* wcoef;         // [E][M]
+  const double* t;
+  const double* constant;
+  double* constant_w;
+  const int* mem_ptr;     // [F + 1]
+  const Slot* slots;      // [S]
+};
+
+struct FitView {
+  double* error;
+  double* error_old;
+  double* error_old2;
+  double* error_coef;
+  double* dpsi_corr;       // [E][2M]
+  const int* cross_ptr;    // [E] start of an element's crossing list
+  const int* cross_cap;    // [E] capacity of the list
+  int* cross_cnt;          // [E]
+  int2* cross;             // (edge position, crossing element)
+  int* cross_sign;
+  int* status;             // bit 0: a crossing list overflowed
+};
+
+struct TargetView {
+  const Tile* tiles;
+  const int* side_elem;
+  const int* side_which;
+  const int* pt_off;       // [nsides + 1]
+  const double2* pt_z;     // explicit points (MODE_POINTS) or null
+};
+
+struct FitItem {
+  int elem;
+  int s0, s1;  // side indices in the target set (s1 = -1 unless intersection)
+};
+
+struct RhsItem {
+  int elem;
+  int s0, s1;
+};
+
+struct GrowParams {
+  double max_error;
+  double max_ratio;
+  int nit;
+  int max_coef;
+  int increase;
+};
+
+struct IterScalars {
+  double error;
+  double error_coef;
+  double dq_max;
+  double yold_max;
+  double residual;  // max |A y - b| over the head equations, in metres
+  int argmax;
+  int arg_residual;
+};
+
+void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st);
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
+                const int* pt_off, double damping, int blend, hipStream_t st);
+void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
+void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
+void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
+                 hipStream_t st);
+void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st);
+void launch_permute(const double* b, const int* perm, int n, double* x, hipStream_t st);
+void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st);
+void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
+                       double* b, hipStream_t st);
+void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const int* de_pos, int K, int nde, int nf,
+                     int64_t n, double* A, hipStream_t st);
+void launch_scatter(const NetView& net, const int* de, int nde, int nf, const double* y, hipStream_t st);
+void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
+                   hipStream_t st);
+void launch_init_errors(const FitView& fv, int ne, hipStream_t st);
+void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, IterScalars* out,
+                     hipStream_t st);
+void launch_bc_residual(const NetView& net, const FitItem* items, int nitems, const double* val, const int* pt_off,
+                        const RhsItem* witems, int nw, const double* wval, const int* wpt_off, int K, double* out_max,
+                        hipStream_t st);
+
+}  // namespace aem

@@ -1,0 +1,71 @@
+// Device math of the analytic-element evaluation (FP64 and FP32 forms).
+//
+// The element potentials are those of AnDFN's HPC path:
+//   line source (intersection, constant-head line, impermeable line), andfn/hpc/hpc_intersection.py:115-145,
+//     hpc_const_head_line.py:88-108, hpc_imp_object.py:186-205:
+//       Z = (2z - a - b)/(b - a),  chi = Z + sqrt(Z - 1) sqrt(Z + 1)   (hpc_geometry_functions.py:11-31)
+//       Omega = sum_j c_j chi^-j + q/(2 pi) log chi                    (hpc_math_functions.py:16-40, 193-213)
+//   bounding circle: chi = z/R, Omega = sum_j c_j chi^j                (hpc_bounding_circle.py:52-71)
+//   well: chi = (z - c)/r, Omega = q/(2 pi) log chi, NaN inside        (hpc_well.py:38-60)
+//   impermeable circle: chi = (z - c)/r, Omega = sum_j c_j chi^-j, NaN inside (hpc_imp_object.py:67-88)
+//
+// Formulation (same values, fewer operations):
+//   * Z = alpha z + beta with alpha = 2/(b - a), beta = -(a + b)/(b - a) precomputed per source;
+//   * sqrt(Z - 1) sqrt(Z + 1) = +-sqrt((Z - 1)(Z + 1)), the sign chosen so that Re(conj(Z) s) >= 0,
+//     which is the condition |chi| >= 1 of the exterior branch (|Z + s| >= |Z - s|);
+//   * one reciprocal w = 1/chi, then Horner in multiply-adds: r <- r w + c_j;
+//   * Re log chi = log|chi|^2 / 2, Im log chi = atan2 only where the imaginary part is used.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace aem {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kInv2Pi = 0.15915494309189535;  // 1/(2 pi)
+
+// ---------------------------------------------------------------------------------------
+// FP64
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 c2(double re, double im) { return make_double2(re, im); }
+
+// principal square root (branch cut on the negative real axis)
+__device__ __forceinline__ double2 csqrt_principal(double x, double y) {
+  double r = sqrt(fma(x, x, y * y));
+  if (r == 0.0) return c2(0.0, y);
+  double t = sqrt(0.5 * (r + fabs(x)));
+  if (x >= 0.0) return c2(t, y / (2.0 * t));
+  return c2(fabs(y) / (2.0 * t), copysign(t, y));
+}
+
+// exterior Joukowsky inverse of Z (|chi| >= 1)
+__device__ __forceinline__ double2 joukowsky_exterior(double2 Z) {
+  double ux = fma(Z.x - 1.0, Z.x + 1.0, -Z.y * Z.y);
+  double uy = 2.0 * Z.x * Z.y;
+  double2 s = csqrt_principal(ux, uy);
+  if (fma(Z.x, s.x, Z.y * s.y) < 0.0) {
+    s.x = -s.x;
+    s.y = -s.y;
+  }
+  return c2(Z.x + s.x, Z.y + s.y);
+}
+
+// r <- r * w + c
+__device__ __forceinline__ double2 cmadd(double2 r, double2 w, double2 c) {
+  return c2(fma(r.x, w.x, fma(-r.y, w.y, c.x)), fma(r.x, w.y, fma(r.y, w.x, c.y)));
+}
+
+// sum_{j<m} c_j w^j by Horner; c is wave-uniform (scalar loads)
+__device__ __forceinline__ double2 horner(const double2* __restrict__ c, int m, double2 w) {
+  double2 r = c[m - 1];
+  for (int j = m - 2; j >= 0; --j) r = cmadd(r, w, c[j]);
+  return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// FP32 (mixed-precision far field)
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmaddf(float2 r, float2 w, float2 c) {
+  return make_float2(fmaf(r.x, w.x, fmaf(-r.y, w.y, c.x)), fmaf(r.x, w.y, fmaf(r.y, w.x, c.y)));
+}
+
+}  // namespace aem

@@ -1,0 +1,137 @@
+"""Drop-in replacement of AnDFN's HPC solve entry point.
+
+``solve(fracture_struc_array, element_struc_array, discharge_int, constants, ntype_elements)``
+has the signature and in-place semantics of ``andfn.hpc.hpc_solve.solve``
+(``andfn/hpc/hpc_solve.py:55-257``): the element records (coef, ncoef, nint, thetas, q,
+dpsi_corr and the error fields) and the fracture constants are updated in place and the
+element array is returned.  In an environment that has AnDFN, assigning
+``andfn.dfn.hpc_solve = andfn_amd.hpc_solve.solve`` routes ``DFN.solve`` here.
+
+The solve runs on the GPU through ``libandfn_mi.so``; there is no CPU path.  Like the
+reference, a solve that stops at MAX_ITERATIONS returns normally (a warning is logged); a
+singular discharge matrix raises ``RuntimeError`` (SuperLU's behaviour in the reference).
+"""
+
+import ctypes
+import logging
+
+import numpy as np
+
+from . import _lib
+
+logger = logging.getLogger("andfn")
+_last_info = None
+
+
+def last_solve_info():
+    """Dict of the last solve's iteration count, errors, timings and kernel counters."""
+    return dict(_last_info) if _last_info is not None else None
+
+
+def solve(fracture_struc_array, element_struc_array, discharge_int, constants, ntype_elements=None, precision="fp64",
+          device=0, verbose=False, lu_mode=0, growth_schedule=None, record_growth=False, head_tol=0.0):
+    """Solve in place; see the module docstring.
+
+    ``growth_schedule`` (int32 [iterations, n_elements], optional) replays another run's
+    coefficient counts; ``record_growth`` stores this run's counts per iteration in
+    ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 adds a stop as soon as the head
+    residual (max |A y - b(c)| over the head equations, metres) is at most ``head_tol``.
+    """
+    global _last_info
+    _lib.require_records(fracture_struc_array, element_struc_array)
+    lib = _lib.load()
+    me, m = _lib.capacities(fracture_struc_array, element_struc_array)
+    consts = np.ascontiguousarray(constants)
+    if consts.dtype.names is None or "MAX_ERROR" not in consts.dtype.names:
+        raise TypeError("constants must be a dtype_constants record")
+    nte = np.zeros(6, dtype=np.int64) if ntype_elements is None else np.ascontiguousarray(ntype_elements, dtype=np.int64)
+    opt = _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
+                       int(lu_mode), 0.0, None, 0, None, 0, float(head_tol))
+    sched = None
+    if growth_schedule is not None:
+        sched = np.ascontiguousarray(growth_schedule, dtype=np.int32)
+        if sched.ndim != 2 or sched.shape[1] != len(element_struc_array):
+            raise ValueError("growth_schedule must have shape (iterations, n_elements)")
+        opt.growth_schedule = sched.ctypes.data
+        opt.schedule_iterations = sched.shape[0]
+    rec = None
+    if record_growth:
+        rec = np.zeros((int(consts["MAX_ITERATIONS"]), len(element_struc_array)), dtype=np.int32)
+        opt.growth_record = rec.ctypes.data
+        opt.record_iterations = rec.shape[0]
+    info = _lib.Info()
+    rc = lib.andfn_mi_solve(_lib.ptr(fracture_struc_array), len(fracture_struc_array), me,
+                            _lib.ptr(element_struc_array), len(element_struc_array), m, int(discharge_int),
+                            _lib.ptr(consts), _lib.ptr(nte), ctypes.byref(opt), ctypes.byref(info))
+    if rc == _lib.ESINGULAR:
+        raise RuntimeError("Factor is exactly singular: " + lib.andfn_mi_last_error().decode())
+    _lib.check(rc)
+    _last_info = info.as_dict()
+    _last_info["converged"] = rc == _lib.OK
+    if rec is not None:
+        _last_info["ncoef_history"] = rec[: info.nit].copy()
+    if rc == _lib.NOT_CONVERGED:
+        logger.warning(f"solve stopped at MAX_ITERATIONS ({info.nit}) without meeting the tolerance")
+    logger.info(f"Iterations: {info.nit}, Error E: {info.error:.3e}, Q: {info.error_q:.3e}, ")
+    return element_struc_array
+
+
+def get_heads(fracture_struc_array, element_struc_array, frac, z, device=0):
+    """Heads h = Re Omega_f(z)/t_f at points z (local complex coordinates) on fractures ``frac``
+    (``andfn/hpc/hpc_fracture.py:204-250``)."""
+    _lib.require_records(fracture_struc_array, element_struc_array)
+    lib = _lib.load()
+    me, m = _lib.capacities(fracture_struc_array, element_struc_array)
+    frac = np.ascontiguousarray(frac, dtype=np.int64)
+    zz = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros(len(zz), dtype=np.float64)
+    _lib.check(lib.andfn_mi_heads(_lib.ptr(fracture_struc_array), len(fracture_struc_array), me,
+                                  _lib.ptr(element_struc_array), len(element_struc_array), m, len(zz), _lib.ptr(frac),
+                                  _lib.ptr(zz), _lib.ptr(out), int(device)))
+    return out
+
+
+def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0):
+    return _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
+                        int(lu_mode), 0.0, None, 0, None, 0, float(head_tol))
+
+
+class DeviceNetwork:
+    """A network uploaded once for repeated solves (C ABI create/run/download/destroy).
+
+    Every ``run`` restarts from the uploaded initial state with the inputs already resident in
+    device memory; ``download`` writes the last run's results into the record arrays.
+    """
+
+    def __init__(self, fracture_struc_array, element_struc_array, discharge_int, constants, device=0):
+        _lib.require_records(fracture_struc_array, element_struc_array)
+        self.lib = _lib.load()
+        self.fa = fracture_struc_array
+        self.ea = element_struc_array
+        self.consts = np.ascontiguousarray(constants)
+        me, m = _lib.capacities(fracture_struc_array, element_struc_array)
+        self.h = ctypes.c_void_p()
+        _lib.check(self.lib.andfn_mi_create(_lib.ptr(self.fa), len(self.fa), me, _lib.ptr(self.ea), len(self.ea), m,
+                                            int(discharge_int), _lib.ptr(self.consts), int(device),
+                                            ctypes.byref(self.h)))
+        self.device = device
+
+    def run(self, precision="fp64", lu_mode=0, head_tol=0.0, verbose=False):
+        opt = _options(precision, self.device, verbose, lu_mode, head_tol)
+        info = _lib.Info()
+        rc = self.lib.andfn_mi_run(self.h, ctypes.byref(opt), ctypes.byref(info))
+        _lib.check(rc)
+        d = info.as_dict()
+        d["converged"] = rc == _lib.OK
+        return d
+
+    def download(self):
+        _lib.check(self.lib.andfn_mi_download(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.andfn_mi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,151 @@
+/*
+ * andfn_mi.h -- C ABI of the MI355X analytic-element DFN solve.
+ *
+ * The boundary replaces AnDFN's HPC entry point
+ *
+ *     andfn.hpc.hpc_solve.solve(fracture_struc_array, element_struc_array, discharge_int,
+ *                               constants, ntype_elements)          (andfn/hpc/hpc_solve.py:55-257)
+ *
+ * whose only caller is DFN.solve (andfn/dfn.py:1840-1846).  The record arrays are passed as
+ * they are in memory: packed numpy structured arrays with the layouts
+ *
+ *     element_dtype_hpc   andfn/element.py:45-69    (12992 bytes per record at MAX_NCOEF = 200)
+ *     fracture_dtype_hpc  andfn/element.py:146-159  (4136 bytes per record at MAX_ELEMENTS = 500)
+ *     dtype_constants     andfn/constants.py:13-29  (0-d record, packed, 76 bytes)
+ *
+ * and are updated in place exactly as the reference updates them: element coef, ncoef, nint,
+ * thetas, q, dpsi_corr, error, error_old, error_old2, error_coef; fracture constant.
+ *
+ * All entry points return 0 on success, a positive value for a warning (not converged) and a
+ * negative ANDFN_MI_E* code on failure; andfn_mi_last_error() gives the message (per thread).
+ * Calls are blocking: results are in the caller's arrays when the call returns.
+ */
+#ifndef ANDFN_MI_H
+#define ANDFN_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANDFN_MI_OK 0
+#define ANDFN_MI_NOT_CONVERGED 1          /* reached MAX_ITERATIONS (the reference returns silently) */
+#define ANDFN_MI_EINVAL (-1)              /* invalid argument or inconsistent records */
+#define ANDFN_MI_ESINGULAR (-2)           /* discharge matrix is singular (reference: SuperLU RuntimeError) */
+#define ANDFN_MI_ECAPACITY (-3)           /* capacity exceeded (reference: IndexError) */
+#define ANDFN_MI_EHIP (-4)                /* HIP / rocSOLVER failure */
+#define ANDFN_MI_ENODEV (-5)              /* no GPU device */
+
+/* Arithmetic of the solve. */
+#define ANDFN_MI_FP64 0   /* every kernel in FP64 (the reference's arithmetic) */
+#define ANDFN_MI_MIXED 1  /* FP32 far-field series, FP64 near field, LU, residual and refinement */
+
+/* Per-iteration discharge solve with the LU factors of the constant discharge matrix. */
+#define ANDFN_MI_LU_TRSV 0     /* row permutation + two triangular solves (rocBLAS trsv) */
+#define ANDFN_MI_LU_GETRS 1    /* rocSOLVER getrs */
+#define ANDFN_MI_LU_INVERSE 2  /* explicit inverse formed once (getri), then one gemv per iteration */
+
+typedef struct andfn_mi_options {
+  int32_t precision;     /* ANDFN_MI_FP64 or ANDFN_MI_MIXED */
+  int32_t device;        /* HIP device ordinal */
+  int32_t verbose;       /* >0: per-iteration line on stderr, as hpc_solve.py:200-215 logs */
+  int32_t lu_mode;       /* ANDFN_MI_LU_* */
+  double far_field;      /* mixed form: |chi| above which a line source is evaluated in FP32 */
+  /* Optional coefficient-count schedule: row k (k = 0, 1, ...) holds every element's ncoef after
+   * iteration k + 1.  When given, the growth step of iteration k + 1 grows exactly the elements
+   * whose count rises in row k instead of applying the growth test (rows beyond
+   * schedule_iterations fall back to the test).  Used to replay another run's trajectory. */
+  const int32_t* growth_schedule;
+  int64_t schedule_iterations;
+  /* Optional output: every element's ncoef after each iteration, record_iterations rows. */
+  int32_t* growth_record;
+  int64_t record_iterations;
+  /* >0: also stop as soon as the head residual (below) of the current iterate is <= head_tol
+   * metres; it is evaluated at the start of every iteration from the new right-hand side. */
+  double head_tol;
+} andfn_mi_options;
+
+typedef struct andfn_mi_info {
+  int64_t nit;            /* outer iterations taken (hpc_solve.py:152) */
+  int64_t n_unknowns;     /* discharge elements + fractures */
+  int64_t n_grown;        /* coefficient-growth events over the solve */
+  int64_t max_error_element;
+  double error;           /* max element error of the last iteration */
+  double error_coef;      /* max element coefficient change of the last iteration */
+  double error_q;         /* relative discharge change of the last iteration */
+  double t_setup;         /* seconds: upload, tables, discharge-matrix assembly */
+  double t_factor;        /* seconds: LU factorisation */
+  double t_iterate;       /* seconds: the fixed-point iterations */
+  double t_total;         /* seconds: whole call, records in to records out */
+  double k_sweep_ms;      /* summed device time of the collocation sweep kernel */
+  double k_rhs_ms;        /* summed device time of the head right-hand-side sweep kernel */
+  double k_fit_ms;        /* summed device time of the fit kernel */
+  double k_lu_ms;         /* summed device time of the per-iteration discharge solve */
+  int64_t n_sweep;        /* launches of the collocation sweep kernel */
+  int64_t n_rhs;          /* launches of the right-hand-side sweep kernel */
+  int64_t pairs_sweep;    /* target-point x source pairs evaluated by all collocation sweeps */
+  int64_t pairs_rhs;      /* target-point x source pairs evaluated by all right-hand-side sweeps */
+  double flops_sweep;     /* algorithmic flops of all collocation sweeps (line 8m+61, circle 8m-6, well 24) */
+  double flops_rhs;       /* algorithmic flops of all right-hand-side sweeps */
+  /* head residual of the returned state, metres: max over the discharge (head) equations of
+   * |A y - b(c)|, i.e. how far the returned discharges and constants are from satisfying the
+   * averaged head conditions of the returned coefficients (intersection rows are head
+   * differences; constant-head and well rows are divided by the fracture's t) */
+  double head_residual;
+  int64_t head_residual_row;
+  /* pointwise boundary-condition residual, metres: max over intersection collocation points of
+   * |h_f0 - h_f1|, over constant-head-line collocation points and well discharge points of
+   * |h - h_prescribed|; bounded below by the series truncation, not by the iteration */
+  double bc_residual;
+} andfn_mi_info;
+
+/*
+ * Drop-in for andfn.hpc.hpc_solve.solve (andfn/hpc/hpc_solve.py:55-79).
+ *   fractures, n_fractures, max_elements : fracture_struc_array, len(), elements[] capacity
+ *   elements, n_elements, max_ncoef      : element_struc_array, len(), coef[] capacity
+ *   discharge_int                        : number of discharge-integration points K
+ *   constants                            : the 0-d dtype_constants record
+ *   ntype_elements                       : int64[6] element counts per type (unused, as in the reference)
+ *   opt                                  : NULL for defaults (FP64, device 0)
+ *   info                                 : NULL or receives iteration count, errors and timings
+ */
+int andfn_mi_solve(void* fractures, int64_t n_fractures, int64_t max_elements, void* elements,
+                   int64_t n_elements, int64_t max_ncoef, int64_t discharge_int, const void* constants,
+                   const int64_t* ntype_elements, const andfn_mi_options* opt, andfn_mi_info* info);
+
+/*
+ * The same solve in three steps, for callers that solve one network repeatedly (benchmarks,
+ * parameter sweeps): create parses the records and uploads them to the device (the records must
+ * stay alive until destroy); run solves from the uploaded initial state on the device, and may be
+ * called again (every run restarts from that state); download writes the last run's results into
+ * the records given to create.  andfn_mi_solve is create + run + download + destroy.
+ */
+typedef struct andfn_mi_handle andfn_mi_handle;
+int andfn_mi_create(void* fractures, int64_t n_fractures, int64_t max_elements, void* elements,
+                    int64_t n_elements, int64_t max_ncoef, int64_t discharge_int, const void* constants,
+                    int32_t device, andfn_mi_handle** handle);
+int andfn_mi_run(andfn_mi_handle* handle, const andfn_mi_options* opt, andfn_mi_info* info);
+int andfn_mi_download(andfn_mi_handle* handle);
+void andfn_mi_destroy(andfn_mi_handle* handle);
+
+/*
+ * Hydraulic heads at points of the solved network: h = Re Omega_f(z) / t_f
+ * (andfn/hpc/hpc_fracture.py:204-250, get_heads 302-343).  z is interleaved complex
+ * (re, im) in each fracture's local plane; frac gives the fracture index of every point.
+ */
+int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
+                   int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
+                   double* heads, int32_t device);
+
+/* Number of visible HIP devices (0 when none). */
+int andfn_mi_device_count(void);
+
+/* Message of the last failure on the calling thread. */
+const char* andfn_mi_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANDFN_MI_H */

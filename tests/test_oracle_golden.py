@@ -1,0 +1,56 @@
+"""The CPU oracle against AnDFN's own runs (golden fixtures written by tests/golden/make_golden.py).
+
+With the recorded growth trajectory replayed, the oracle must reproduce AnDFN's converged
+coefficients, discharges and fracture constants to 1e-9 relative (observed ~1e-12 or better),
+its iteration count, and its coefficient counts.  The free-running oracle (its own growth
+decisions) is also compared: growth decisions fire on rounding-level ties of successive error
+values (hpc_solve.py:481-489), so a free run may differ in a few late decisions; the test
+records how many and requires the first 80% of iterations to agree exactly.
+"""
+
+import numpy as np
+import pytest
+
+import golden_io
+from oracle import oracle
+
+CASES = golden_io.CASES
+
+
+def _rel(a, b):
+    s = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (s if s > 0 else 1.0))
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_replays_andfn_trajectory(case):
+    g = golden_io.load(case)
+    if "out_ncoef_history" not in g:
+        pytest.skip("fixture has no growth trajectory")
+    fa, ea, consts, K = golden_io.records(g)
+    info = {}
+    oracle.solve(fa, ea, K, consts, info=info, growth_schedule=g["out_ncoef_history"])
+    assert info["nit"] == int(g["out_nit"])
+    assert np.array_equal(info["ncoef_history"], g["out_ncoef_history"])
+    d = golden_io.compare(g, fa, ea)
+    assert d["ncoef_equal"] and d["nint_equal"]
+    assert d["coef"] < 1e-9 and d["q"] < 1e-9 and d["constant"] < 1e-9, d
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_oracle_free_run_against_andfn(case):
+    g = golden_io.load(case)
+    fa, ea, consts, K = golden_io.records(g)
+    info = {}
+    oracle.solve(fa, ea, K, consts, info=info)
+    assert info["nit"] == int(g["out_nit"])
+    if "out_ncoef_history" in g:
+        h, gh = info["ncoef_history"], g["out_ncoef_history"]
+        n = min(len(h), len(gh))
+        same = [bool(np.array_equal(h[i], gh[i])) for i in range(n)]
+        first_diff = same.index(False) if False in same else n
+        print(f"{case}: trajectories agree for {first_diff} of {n} iterations")
+        assert first_diff >= int(0.8 * n)
+    d = golden_io.compare(g, fa, ea)
+    if d["ncoef_equal"]:
+        assert d["coef"] < 1e-9 and d["q"] < 1e-9 and d["constant"] < 1e-9, d
