@@ -154,7 +154,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
                      fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
       double2 chi = joukowsky_exterior(Z);
       double r2 = fma(chi.x, chi.x, chi.y * chi.y);
-      double inv = 1.0 / r2;
+      double inv = rcp_pos(r2);
       double2 w = c2(chi.x * inv, -chi.y * inv);
       double2 h = horner(cf, m, w);
       tre = sl.sign * h.x;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
     } else {  // KIND_IMPC
       double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
       double r2 = fma(chi.x, chi.x, chi.y * chi.y);
-      double inv = 1.0 / r2;
+      double inv = rcp_pos(r2);
       double2 w = c2(chi.x * inv, -chi.y * inv);
       double2 h = horner(cf, m, w);
       tre = h.x;
@@ -697,9 +697,133 @@ void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, c
   hipLaunchKernelGGL(grow_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, net, fv, ne, gp, sched, grow);
 }
 
-__global__ void permute_kernel(const double* __restrict__ b, const int* __restrict__ perm, int n, double* __restrict__ x) {
+// x[i] = rs[perm[i]] * b[perm[i]]: row equilibration and the LU row permutation in one gather
+__global__ void permute_kernel(const double* __restrict__ b, const int* __restrict__ perm,
+                               const double* __restrict__ rs, int n, double* __restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) x[i] = b[perm[i]];
+  if (i < n) {
+    const int p = perm[i];
+    x[i] = b[p] * rs[p];
+  }
+}
+
+// row equilibration of a dense column-major matrix: rs[j] = 1 / max_k |A[j, k]|, A[j, :] *= rs[j].
+// Pass 1: each thread takes one row over a chunk of columns and merges its maximum with an
+// integer atomic max on the bit pattern (ordered like the value for non-negative doubles).
+constexpr int kColChunk = 256;
+__global__ void row_absmax_kernel(const double* __restrict__ A, int64_t n, unsigned long long* __restrict__ mx) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int64_t k0 = (int64_t)blockIdx.y * kColChunk;
+  const int64_t k1 = k0 + kColChunk < n ? k0 + kColChunk : n;
+  double m = 0.0;
+  for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(A[k * n + j]));
+  atomicMax(&mx[j], (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void row_apply_kernel(double* __restrict__ A, int64_t n, const unsigned long long* __restrict__ mx,
+                                 double* __restrict__ rs) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double m = __longlong_as_double((long long)mx[j]);
+  const double s = m > 0.0 ? 1.0 / m : 1.0;
+  if (blockIdx.y == 0) rs[j] = s;
+  const int64_t k0 = (int64_t)blockIdx.y * kColChunk;
+  const int64_t k1 = k0 + kColChunk < n ? k0 + kColChunk : n;
+  for (int64_t k = k0; k < k1; ++k) A[k * n + j] *= s;
+}
+
+void launch_row_scale(double* A, int64_t n, double* rs, unsigned long long* work, hipStream_t st) {
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + kColChunk - 1) / kColChunk));
+  (void)hipMemsetAsync(work, 0, sizeof(unsigned long long) * n, st);
+  hipLaunchKernelGGL(row_absmax_kernel, grid, dim3(256), 0, st, A, n, work);
+  hipLaunchKernelGGL(row_apply_kernel, grid, dim3(256), 0, st, A, n, work, rs);
+}
+
+__global__ void scale_kernel(const double* __restrict__ b, const double* __restrict__ rs, int n, double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = b[i] * rs[i];
+}
+
+void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(scale_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b, rs, n, x);
+}
+
+// CSR copy of the assembled (unscaled, unfactored) dense matrix: val[k] = A[row, col[k]]
+__global__ void csr_gather_kernel(const double* __restrict__ A, int64_t n, const int* __restrict__ row_ptr,
+                                  const int* __restrict__ col, double* __restrict__ val) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) val[k] = A[(int64_t)col[k] * n + r];
+}
+
+void launch_csr_gather(const double* A, int64_t n, const int* row_ptr, const int* col, double* val, hipStream_t st) {
+  hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n, row_ptr, col, val);
+}
+
+// y = A x for a CSR matrix (one thread per row)
+__global__ void spmv_kernel(int n, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                            const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double s = 0.0;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s = fma(val[k], x[col[k]], s);
+  y[r] = s;
+}
+
+void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, const double* x, double* y,
+                 hipStream_t st) {
+  hipLaunchKernelGGL(spmv_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, row_ptr, col, val, x, y);
+}
+
+// y[0:m] = beta * y + alpha * A[0:m, 0:ncols] x for a column-major block (lda) with ncols <= 512:
+// 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 4 wavefronts
+// and their partial sums reduced in LDS.  Used for the blocked triangular solves.
+constexpr int kGemvCols = 512;
+__global__ __launch_bounds__(256) void gemv_cm_kernel(const double* __restrict__ A, int64_t lda, int m, int ncols,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      double alpha, double beta) {
+  __shared__ double sx[kGemvCols];
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c = threadIdx.x; c < ncols; c += 256) sx[c] = x[c];
+  __syncthreads();
+  const int row = blockIdx.x * 64 + lane;
+  double s = 0.0;
+  if (row < m) {
+    const int per = (ncols + 3) / 4;
+    const int c0 = wv * per, c1 = min(ncols, c0 + per);
+    const double* a = A + (int64_t)c0 * lda + row;
+    // eight independent loads in flight per lane
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8, a += 8 * lda) {
+      const double a0 = a[0], a1 = a[lda], a2 = a[2 * lda], a3 = a[3 * lda];
+      const double a4 = a[4 * lda], a5 = a[5 * lda], a6 = a[6 * lda], a7 = a[7 * lda];
+      s0 = fma(a0, sx[c], s0);
+      s1 = fma(a1, sx[c + 1], s1);
+      s2 = fma(a2, sx[c + 2], s2);
+      s3 = fma(a3, sx[c + 3], s3);
+      s0 = fma(a4, sx[c + 4], s0);
+      s1 = fma(a5, sx[c + 5], s1);
+      s2 = fma(a6, sx[c + 6], s2);
+      s3 = fma(a7, sx[c + 7], s3);
+    }
+    for (; c < c1; ++c, a += lda) s0 = fma(*a, sx[c], s0);
+    s = (s0 + s1) + (s2 + s3);
+  }
+  part[wv][lane] = s;
+  __syncthreads();
+  if (wv == 0 && row < m) {
+    const double t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    y[row] = (beta == 0.0 ? 0.0 : beta * y[row]) + alpha * t;
+  }
+}
+
+void launch_gemv_cm(const double* A, int64_t lda, int m, int ncols, const double* x, double* y, double alpha,
+                    double beta, hipStream_t st) {
+  if (m <= 0 || ncols <= 0) return;
+  hipLaunchKernelGGL(gemv_cm_kernel, dim3((m + 63) / 64), dim3(256), 0, st, A, lda, m, ncols, x, y, alpha, beta);
 }
 
 // diagonal of nb stacked nbk x nbk blocks set to one (unit-triangular inverses)
@@ -715,8 +839,8 @@ void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st) {
   hipLaunchKernelGGL(unit_diag_kernel, dim3((nb * nbk + 255) / 256), dim3(256), 0, st, blocks, nb, nbk);
 }
 
-void launch_permute(const double* b, const int* perm, int n, double* x, hipStream_t st) {
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b, perm, n, x);
+void launch_permute(const double* b, const int* perm, const double* rs, int n, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b, perm, rs, n, x);
 }
 
 void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st) {

@@ -114,8 +114,15 @@ void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
 void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
                  hipStream_t st);
+void launch_csr_gather(const double* A, int64_t n, const int* row_ptr, const int* col, double* val, hipStream_t st);
+void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, const double* x, double* y,
+                 hipStream_t st);
+void launch_gemv_cm(const double* A, int64_t lda, int m, int ncols, const double* x, double* y, double alpha,
+                    double beta, hipStream_t st);
 void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st);
-void launch_permute(const double* b, const int* perm, int n, double* x, hipStream_t st);
+void launch_permute(const double* b, const int* perm, const double* rs, int n, double* x, hipStream_t st);
+void launch_row_scale(double* A, int64_t n, double* rs, unsigned long long* work, hipStream_t st);
+void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st);
 void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st);
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
                        double* b, hipStream_t st);

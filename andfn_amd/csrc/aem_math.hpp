@@ -54,11 +54,34 @@ __device__ __forceinline__ double2 cmadd(double2 r, double2 w, double2 c) {
   return c2(fma(r.x, w.x, fma(-r.y, w.y, c.x)), fma(r.x, w.y, fma(r.y, w.x, c.y)));
 }
 
-// sum_{j<m} c_j w^j by Horner; c is wave-uniform (scalar loads)
+// sum_{j<m} c_j w^j by Horner; c is wave-uniform (scalar loads).  Coefficients are fetched
+// eight at a time ahead of their multiply-adds so the scalar-load latency is paid once per chunk.
 __device__ __forceinline__ double2 horner(const double2* __restrict__ c, int m, double2 w) {
   double2 r = c[m - 1];
-  for (int j = m - 2; j >= 0; --j) r = cmadd(r, w, c[j]);
+  int j = m - 2;
+  for (; j >= 7; j -= 8) {
+    const double2 c0 = c[j], c1 = c[j - 1], c2_ = c[j - 2], c3 = c[j - 3];
+    const double2 c4 = c[j - 4], c5 = c[j - 5], c6 = c[j - 6], c7 = c[j - 7];
+    r = cmadd(r, w, c0);
+    r = cmadd(r, w, c1);
+    r = cmadd(r, w, c2_);
+    r = cmadd(r, w, c3);
+    r = cmadd(r, w, c4);
+    r = cmadd(r, w, c5);
+    r = cmadd(r, w, c6);
+    r = cmadd(r, w, c7);
+  }
+  for (; j >= 0; --j) r = cmadd(r, w, c[j]);
   return r;
+}
+
+// 1/x for x >= 1 of moderate size: hardware reciprocal refined by two Newton steps
+__device__ __forceinline__ double rcp_pos(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
 }
 
 // ---------------------------------------------------------------------------------------

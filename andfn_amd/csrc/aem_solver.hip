@@ -193,6 +193,7 @@ class Solver {
   }
 
   ~Solver() {
+    drop_graph();
     if (rb_) rocblas_destroy_handle(rb_);
     for (auto& e : ev_) (void)hipEventDestroy(e);
     if (st_) (void)hipStreamDestroy(st_);
@@ -231,10 +232,12 @@ class Solver {
   DBuf<double> d_radius_, d_phi_, d_q_, d_t_, d_constant_, d_error_, d_error_old_, d_error_old2_, d_error_coef_;
   DBuf<double> d_dpsi_, d_val_, d_valg_, d_valr_, d_A_, d_b_, d_y_, d_yold_, d_x_;
   DBuf<int> d_perm_, d_sched_;
-  DBuf<double> d_A0_, d_rowscale_, d_ay_, d_bcmax_;
+  DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
+  DBuf<int> d_csr_ptr_, d_csr_col_;
+  int64_t nnz_A_ = 0;
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
   static constexpr int kNB = 512;
-  DBuf<double> d_linv_, d_uinv_, d_w_, d_v_;
+  DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   // initial state of the mutable arrays, restored at the start of every run
   DBuf<double2> s_coef_;
   DBuf<double> s_q_, s_constant_, s_error_, s_error_old_, s_error_old2_, s_error_coef_, s_dpsi_;
@@ -304,6 +307,17 @@ class Solver {
   double set_flops(const TargetSet& ts, bool with_q, int64_t* pairs) const;
   void assemble_and_factor();
   double* discharge_solve(double* b, double* x);
+  void solve_rhs();
+  hipGraph_t lu_graph_ = nullptr;
+  hipGraphExec_t lu_exec_ = nullptr;
+  bool lu_graph_tried_ = false;
+  void drop_graph() {
+    if (lu_exec_) (void)hipGraphExecDestroy(lu_exec_);
+    if (lu_graph_) (void)hipGraphDestroy(lu_graph_);
+    lu_exec_ = nullptr;
+    lu_graph_ = nullptr;
+    lu_graph_tried_ = false;
+  }
   double head_residual(const double* b, int* row);
   double bc_residual();
 };
@@ -591,15 +605,50 @@ double Solver::set_flops(const TargetSet& ts, bool with_q, int64_t* pairs) const
 }
 
 void Solver::assemble_and_factor() {
+  drop_graph();
   const int nde = (int)de_.size();
   const int64_t n = nde + F_;
   d_A_.zero((size_t)(n * n), st_);
   aem::TargetView tv = rhs_.view();
   aem::launch_assemble(net(), tv, (int)rhs_.side_elem.size(), d_de_pos_.p, K_, nde, F_, n, d_A_.p, st_);
   HIP_CHECK(hipGetLastError());
-  // the unfactored matrix is kept for the head residual A y - b
-  d_A0_.alloc((size_t)(n * n));
-  HIP_CHECK(hipMemcpyAsync(d_A0_.p, d_A_.p, sizeof(double) * (size_t)(n * n), hipMemcpyDeviceToDevice, st_));
+  // a CSR copy of the unfactored matrix is kept for the head residual A y - b
+  {
+    std::vector<int> rp(1, 0), cols;
+    std::vector<int> tmp;
+    for (int j = 0; j < nde; ++j) {
+      const int e = de_[j];
+      tmp.clear();
+      const int sides = etype_[e] == 0 ? 2 : 1;
+      for (int sd = 0; sd < sides; ++sd) {
+        const int f = sd ? efrac1_[e] : efrac0_[e];
+        for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
+          const int src = slots_[k].elem;
+          if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+        }
+        tmp.push_back(nde + f);
+      }
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      cols.insert(cols.end(), tmp.begin(), tmp.end());
+      rp.push_back((int)cols.size());
+    }
+    for (int f = 0; f < F_; ++f) {
+      tmp.clear();
+      for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+        if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
+      std::sort(tmp.begin(), tmp.end());
+      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+      cols.insert(cols.end(), tmp.begin(), tmp.end());
+      rp.push_back((int)cols.size());
+    }
+    d_csr_ptr_.upload(rp, st_);
+    d_csr_col_.upload(cols, st_);
+    d_csr_val_.alloc(cols.size());
+    aem::launch_csr_gather(d_A_.p, n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+    HIP_CHECK(hipStreamSynchronize(st_));
+    nnz_A_ = (int64_t)cols.size();
+  }
   {
     std::vector<double> rs(nde, 1.0);
     for (int j = 0; j < nde; ++j)
@@ -608,6 +657,15 @@ void Solver::assemble_and_factor() {
     HIP_CHECK(hipStreamSynchronize(st_));
   }
   d_ay_.alloc(n);
+  // row equilibration (rows of intersections scale with 1/t), as SuperLU equilibrates by default
+  d_req_.alloc(n);
+  {
+    DBuf<unsigned long long> mx;
+    mx.alloc(n);
+    aem::launch_row_scale(d_A_.p, n, d_req_.p, mx.p, st_);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(st_));
+  }
   d_ipiv_.alloc(n);
   d_info_.zero(1, st_);
   RB_CHECK(rocsolver_dgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
@@ -616,6 +674,7 @@ void Solver::assemble_and_factor() {
   HIP_CHECK(hipStreamSynchronize(st_));
   if (info > 0) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot at %d)", (int)info);
   d_x_.alloc(n);
+  d_w_.alloc(n);
   if (lu_mode_ == ANDFN_MI_LU_TRSV) {
     // LAPACK row interchanges -> one gather permutation: (P b)[i] = b[perm[i]]
     std::vector<rocblas_int> ipiv(n);
@@ -656,40 +715,70 @@ void Solver::assemble_and_factor() {
   }
 }
 
+// d_x_ = A^-1 d_b_.  The triangular-solve sequence (one permutation gather and two gemv per
+// diagonal block) is launch-bound, so it is captured once into a HIP graph and replayed.
+void Solver::solve_rhs() {
+  if (lu_mode_ != ANDFN_MI_LU_TRSV) {
+    (void)discharge_solve(d_b_.p, d_x_.p);
+    return;
+  }
+  if (!lu_graph_tried_) {
+    lu_graph_tried_ = true;
+    (void)discharge_solve(d_b_.p, d_x_.p);  // uncaptured first call: any lazy library set-up happens here
+    HIP_CHECK(hipStreamSynchronize(st_));
+    if (hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      (void)discharge_solve(d_b_.p, d_x_.p);
+      hipGraph_t g = nullptr;
+      if (hipStreamEndCapture(st_, &g) == hipSuccess && g) {
+        if (hipGraphInstantiate(&lu_exec_, g, nullptr, nullptr, 0) == hipSuccess) {
+          lu_graph_ = g;
+        } else {
+          (void)hipGraphDestroy(g);
+          lu_exec_ = nullptr;
+        }
+      }
+      (void)hipGetLastError();
+    }
+    return;  // the first call's result is already in d_x_
+  }
+  if (lu_exec_) {
+    HIP_CHECK(hipGraphLaunch(lu_exec_, st_));
+  } else {
+    (void)discharge_solve(d_b_.p, d_x_.p);
+  }
+}
+
 // x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
 double* Solver::discharge_solve(double* b, double* x) {
   const rocblas_int n = (rocblas_int)(de_.size() + F_);
   if (lu_mode_ == ANDFN_MI_LU_GETRS) {
-    RB_CHECK(rocsolver_dgetrs(rb_, rocblas_operation_none, n, 1, d_A_.p, n, d_ipiv_.p, b, n));
-    return b;
+    aem::launch_scale(b, d_req_.p, n, x, st_);
+    RB_CHECK(rocsolver_dgetrs(rb_, rocblas_operation_none, n, 1, d_A_.p, n, d_ipiv_.p, x, n));
+    return x;
   }
   if (lu_mode_ == ANDFN_MI_LU_INVERSE) {
     const double one = 1.0, zero = 0.0;
-    RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A_.p, n, b, 1, &zero, x, 1));
+    aem::launch_scale(b, d_req_.p, n, d_w_.p, st_);
+    RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A_.p, n, d_w_.p, 1, &zero, x, 1));
     return x;
   }
-  // P b, then L z = P b and U x = z block by block: each block is one multiply by the inverse
+  // R P b, then L z = P b and U x = z block by block: each block is one multiply by the inverse
   // diagonal block and one gemv update of the remaining right-hand side (the factors are read once)
-  aem::launch_permute(b, d_perm_.p, n, x, st_);
+  aem::launch_permute(b, d_perm_.p, d_req_.p, n, x, st_);
   const double one = 1.0, zero = 0.0, mone = -1.0;
   const int nb = (n + kNB - 1) / kNB;
   double* w = d_w_.p;
   double* v = d_v_.p;
   for (int j = 0; j < nb; ++j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, bs, bs, &one, d_linv_.p + (size_t)j * kNB * kNB, kNB, x + r0, 1,
-                           &zero, w + r0, 1));
+    aem::launch_gemv_cm(d_linv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x + r0, w + r0, one, zero, st_);
     const int rest = n - r0 - bs;
-    if (rest > 0)
-      RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, rest, bs, &mone, d_A_.p + (size_t)r0 * n + r0 + bs, n, w + r0, 1,
-                             &one, x + r0 + bs, 1));
+    if (rest > 0) aem::launch_gemv_cm(d_A_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w + r0, x + r0 + bs, mone, one, st_);
   }
   for (int j = nb - 1; j >= 0; --j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, bs, bs, &one, d_uinv_.p + (size_t)j * kNB * kNB, kNB, w + r0, 1,
-                           &zero, v + r0, 1));
-    if (r0 > 0)
-      RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, r0, bs, &mone, d_A_.p + (size_t)r0 * n, n, v + r0, 1, &one, w, 1));
+    aem::launch_gemv_cm(d_uinv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w + r0, v + r0, one, zero, st_);
+    if (r0 > 0) aem::launch_gemv_cm(d_A_.p + (size_t)r0 * n, n, r0, bs, v + r0, w, mone, one, st_);
   }
   HIP_CHECK(hipMemcpyAsync(x, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   return x;
@@ -697,9 +786,8 @@ double* Solver::discharge_solve(double* b, double* x) {
 
 // max_j |(A0 y)_j - b_j| * row_scale_j over the head equations for the current y
 double Solver::head_residual(const double* b, int* row) {
-  const rocblas_int n = (rocblas_int)(de_.size() + F_);
-  const double one = 1.0, zero = 0.0;
-  RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A0_.p, n, d_y_.p, 1, &zero, d_ay_.p, 1));
+  const int n = (int)(de_.size() + F_);
+  aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
   aem::launch_residual(d_ay_.p, b, d_rowscale_.p, (int)de_.size(), d_scal_.p, st_);
   aem::IterScalars sc{};
   HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
@@ -862,13 +950,11 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       }
     }
     HIP_CHECK(hipEventRecord(ev_[4], st_));
-    double* xs = discharge_solve(d_b_.p, d_x_.p);
+    solve_rhs();  // d_x_ = A^-1 d_b_
     HIP_CHECK(hipEventRecord(ev_[5], st_));
-    DBuf<double>& xb = (xs == d_b_.p) ? d_b_ : d_x_;
     std::swap(d_yold_.p, d_y_.p);
     std::swap(d_yold_.n, d_y_.n);
-    std::swap(d_y_.p, xb.p);
-    std::swap(d_y_.n, xb.n);
+    HIP_CHECK(hipMemcpyAsync(d_y_.p, d_x_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
     aem::launch_scatter(nv, d_de_.p, nde, F_, d_y_.p, st_);
     fl_rhs += rhs_flops;
     pr_rhs += rhs_pairs;

@@ -232,12 +232,15 @@ class DFN:
         r = np.array([f.radius for f in fracs], dtype=np.float64)
         return c, n, x, y, r
 
-    def get_fracture_intersections(self, ncoef=5, nint=10, se_factor=None):
+    def get_fracture_intersections(self, ncoef=5, nint=10, se_factor=None, rim_clearance=0.0):
         """Add an Intersection for every intersecting pair i < k, in (i, k) order.
 
         Same pairs and creation order as ``andfn/dfn.py:1437-1455``; the O(N^2) pair loop is
         replaced by a KD-tree pre-filter on |c_i - c_k| <= r_i + r_k (+2e-8, the candidate
-        tolerance), which cannot drop an intersecting pair.
+        tolerance), which cannot drop an intersecting pair.  ``rim_clearance`` > 0 (not in
+        AnDFN; default off) skips an intersection whose shortened endpoints come closer than
+        rim_clearance * R to the rim of either fracture: the segment is clipped to the part
+        inside (1 - rim_clearance) R of both fractures, and dropped only when nothing remains.
         """
         if se_factor is None:
             se_factor = float(self.constants["SE_FACTOR"])
@@ -249,14 +252,21 @@ class DFN:
             e0, e1, ok = geo.fracture_intersection(c[i], n[i], x[i], y[i], r[i], c[k], n[k], x[k], y[k], r[k])
             e0 = geo.shorten_line(e0, se_factor)
             e1 = geo.shorten_line(e1, se_factor)
+            if rim_clearance > 0:
+                e0, e1, ok2 = geo.clip_to_disks(e0, r[i], e1, r[k], 1.0 - rim_clearance)
+                ok = ok & ok2
             for p in np.nonzero(ok)[0]:
                 a, b = fr[i[p]], fr[k[p]]
                 Intersection(f"{a.label}_{b.label}", e0[p], e1[p], a, b, ncoef, nint)
         self.get_elements()
 
     def set_constant_head_boundary(self, center, normal, radius, head, label="Constant Head Boundary",
-                                   ncoef=5, nint=10, se_factor=None, tolerance=-1):
-        """Constant-head lines where a boundary disk cuts the fractures (``andfn/geometry_functions.py:878-926``)."""
+                                   ncoef=5, nint=10, se_factor=None, tolerance=-1, rim_clearance=0.0):
+        """Constant-head lines where a boundary disk cuts the fractures (``andfn/geometry_functions.py:878-926``).
+
+        ``rim_clearance`` > 0 (not in AnDFN; default off) clips a line to the part inside
+        (1 - rim_clearance) R and skips it when nothing remains.
+        """
         if se_factor is None:
             se_factor = float(self.constants["SE_FACTOR"])
         fr = self.fractures
@@ -278,7 +288,13 @@ class DFN:
             length = abs(e1[p][0] - e1[p][1])
             if length < tolerance * f.radius:
                 continue
-            ConstantHeadLine(f"{label}_{f.label}", geo.shorten_line(e1[p], se_factor), head, f, ncoef, nint)
+            ends = geo.shorten_line(e1[p], se_factor)
+            if rim_clearance > 0:
+                cl, _, okc = geo.clip_to_disks(ends[None, :], np.array([f.radius]), rho=1.0 - rim_clearance)
+                if not okc[0]:
+                    continue
+                ends = cl[0]
+            ConstantHeadLine(f"{label}_{f.label}", ends, head, f, ncoef, nint)
 
     def fracture_graph(self):
         """Fracture adjacency through intersections and the boundary flag per fracture."""

@@ -141,6 +141,42 @@ def shorten_line(endpoints, se_factor):
     return np.stack([center + (z0 - center) * se_factor, center + (z1 - center) * se_factor], axis=-1)
 
 
+def clip_to_disks(e0, r0, e1=None, r1=None, rho=1.0):
+    """Clip segments to the part that lies within rho * radius of their disk centre(s).
+
+    e0 (P, 2) are segments in disk 0's plane (radius r0); e1/r1 optionally give the same
+    segments in a second disk's plane (an intersection seen from both fractures): the same
+    fraction of the segment is kept in both planes.  Returns (e0', e1', ok); ok is False where
+    nothing of a segment stays inside.
+    """
+    def interval(e, r):
+        a = e[:, 0]
+        d = e[:, 1] - e[:, 0]
+        A = np.abs(d) ** 2
+        B = np.real(a * np.conj(d))
+        C = np.abs(a) ** 2 - (rho * r) ** 2
+        disc = B * B - A * C
+        sq = np.sqrt(np.maximum(disc, 0.0))
+        As = np.where(A > 0, A, 1.0)
+        lo = (-B - sq) / As
+        hi = (-B + sq) / As
+        return lo, hi, (disc >= 0) & (A > 0)
+
+    lo, hi, ok = interval(e0, r0)
+    if e1 is not None:
+        lo1, hi1, ok1 = interval(e1, r1)
+        lo, hi, ok = np.maximum(lo, lo1), np.minimum(hi, hi1), ok & ok1
+    lo = np.clip(lo, 0.0, 1.0)
+    hi = np.clip(hi, 0.0, 1.0)
+    ok = ok & (hi - lo > 1e-9)
+
+    def cut(e):
+        d = e[:, 1] - e[:, 0]
+        return np.stack([e[:, 0] + lo * d, e[:, 0] + hi * d], axis=1)
+
+    return cut(e0), (cut(e1) if e1 is not None else None), ok
+
+
 def candidate_pairs(centers, radii):
     """All pairs (i < k) whose bounding spheres overlap, |c_i - c_k| <= r_i + r_k, sorted.
 

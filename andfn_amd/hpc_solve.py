@@ -104,6 +104,7 @@ class DeviceNetwork:
     """
 
     def __init__(self, fracture_struc_array, element_struc_array, discharge_int, constants, device=0):
+        self.h = None
         _lib.require_records(fracture_struc_array, element_struc_array)
         self.lib = _lib.load()
         self.fa = fracture_struc_array

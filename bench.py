@@ -33,10 +33,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="dfn5000")
+    ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--max-iterations", type=int, default=300)
-    ap.add_argument("--max-error", type=float, default=1e-6)
     ap.add_argument("--lu-mode", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle iterations in the CPU sample")
@@ -108,7 +106,7 @@ def main():
     if _lib.load().andfn_mi_device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    spec = workload(args.workload, constants=dict(MAX_ERROR=args.max_error, MAX_ITERATIONS=args.max_iterations))
+    spec = workload(args.workload)
     dfn = build_network(spec)
     dfn.consolidate_dfn()
     fa, ea = dfn.fractures_struc_array_hpc, dfn.elements_struc_array_hpc
@@ -154,8 +152,10 @@ def main():
             "elements": int(len(ea)),
             "unknowns": int(last["n_unknowns"]),
             "precision": args.precision,
-            "stop": f"AnDFN rule MAX_ERROR {args.max_error:g} or head residual <= {HEAD_TOL:g} m, "
-                    f"MAX_ITERATIONS {args.max_iterations}",
+            "stop": f"AnDFN rule (MAX_ERROR {float(dfn.constants['MAX_ERROR']):g}) or head residual <= "
+                    f"{HEAD_TOL:g} m, MAX_ITERATIONS {int(dfn.constants['MAX_ITERATIONS'])}",
+            "filters": f"rim clearance {spec.rim_clearance:g} R, near-coincident pairs < {spec.min_separation:g} L "
+                       f"at < {spec.min_angle_deg:g} deg",
         },
         "nit": int(last["nit"]),
         "head_residual": last["head_residual"],
