@@ -776,10 +776,10 @@ void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, c
   hipLaunchKernelGGL(spmv_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, row_ptr, col, val, x, y);
 }
 
-// y[0:m] = beta * y + alpha * A[0:m, 0:ncols] x for a column-major block (lda) with ncols <= 512:
+// y[0:m] = beta * y + alpha * A[0:m, 0:ncols] x for a column-major block (lda) with ncols <= 1024:
 // 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 4 wavefronts
 // and their partial sums reduced in LDS.  Used for the blocked triangular solves.
-constexpr int kGemvCols = 512;
+constexpr int kGemvCols = 1024;
 __global__ __launch_bounds__(256) void gemv_cm_kernel(const double* __restrict__ A, int64_t lda, int m, int ncols,
                                                       const double* __restrict__ x, double* __restrict__ y,
                                                       double alpha, double beta) {

@@ -236,7 +236,7 @@ class Solver {
   DBuf<int> d_csr_ptr_, d_csr_col_;
   int64_t nnz_A_ = 0;
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
-  static constexpr int kNB = 512;
+  static constexpr int kNB = 1024;
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   // initial state of the mutable arrays, restored at the start of every run
   DBuf<double2> s_coef_;
