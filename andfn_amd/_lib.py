@@ -68,6 +68,7 @@ class Info(ctypes.Structure):
         ("head_residual", ctypes.c_double),
         ("head_residual_row", ctypes.c_int64),
         ("bc_residual", ctypes.c_double),
+        ("nit_fp32", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -102,7 +102,7 @@ __device__ __forceinline__ double2 discharge_point(const NetView& net, int e, in
 // --------------------------------------------------------------------------------------
 // Sweep: Omega_f(z) = C_f + sum over the fracture's sources (hpc_fracture.py:51-92)
 // --------------------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, bool MIX>
 __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
   const int t = blockIdx.x;
   const Tile tile = tg.tiles[t];
@@ -141,6 +141,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
   const bool any_imag = __any(want_imag);
 
   double ore = 0.0, oim = 0.0;
+  float fre = 0.0f, fim = 0.0f;  // FP32 accumulators of the far-field terms (mixed precision)
   const int kbeg = net.mem_ptr[f];
   const int kend = net.mem_ptr[f + 1];
   for (int k = kbeg; k < kend; ++k) {
@@ -152,6 +153,29 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
     if (sl.kind == KIND_LINE || sl.kind == KIND_IMPL) {
       double2 Z = c2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
                      fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
+      if (MIX && __all(!valid || fma(Z.x, Z.x, Z.y * Z.y) >= net.far2)) {
+        // far field of this source for the whole wavefront: FP32 map, series and log
+        const float2 Zf = make_float2((float)Z.x, (float)Z.y);
+        const float2 chif = joukowsky_exterior_f(Zf);
+        const float r2f = fmaf(chif.x, chif.x, chif.y * chif.y);
+        const float invf = 1.0f / r2f;
+        const float2 wf = make_float2(chif.x * invf, -chif.y * invf);
+        const float2 hf = horner_f(net.coef32 + (size_t)src * net.M, m, wf);
+        const float sg = (float)sl.sign;
+        float gre = sg * hf.x, gim = sg * hf.y;
+        if (sl.kind == KIND_LINE && use_q) {
+          const float qq = (float)(sl.sign * net.q[src] * kInv2Pi);
+          if (qq != 0.0f) {
+            gre = fmaf(qq, 0.5f * logf(r2f), gre);
+            if (any_imag) gim = fmaf(qq, atan2f(chif.y, chif.x), gim);
+          }
+        }
+        if (src != excl) {
+          fre += gre;
+          fim += gim;
+        }
+        continue;
+      }
       double2 chi = joukowsky_exterior(Z);
       double r2 = fma(chi.x, chi.x, chi.y * chi.y);
       double inv = rcp_pos(r2);
@@ -201,6 +225,10 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
       oim += tim;
     }
   }
+  if (MIX) {
+    ore += (double)fre;
+    oim += (double)fim;
+  }
   if (valid) {
     if (MODE == MODE_COLLOC)
       val[p] = want_imag ? oim : ore + net.constant[f];
@@ -214,6 +242,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
 // --------------------------------------------------------------------------------------
 // Fit: one wavefront per element (hpc_math_functions.py:243-297, 537-670 and the element solves)
 // --------------------------------------------------------------------------------------
+template <bool F32>
 __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
                                                  const double* __restrict__ val, const int* __restrict__ pt_off,
                                                  double damping, int blend) {
@@ -268,7 +297,32 @@ __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const 
   const double t1 = (ty == 0) ? net.t[net.efrac1[e]] : 0.0;
   for (int j = lane; j < m; j += 64) {
     double x0 = 0.0, y0 = 0.0, x1 = 0.0;
-    if (ty == 1) {
+    if (F32) {
+      // mixed precision: FP32 DFT with FP32 accumulation
+      float fx0 = 0.0f, fy0 = 0.0f, fx1 = 0.0f;
+      if (ty == 1) {
+        for (int i = 0; i < n; ++i) {
+          float sn, cs;
+          sincospif(2.0f * (float)((int64_t)i * j % n) / (float)n, &sn, &cs);
+          const float v = (float)sv[0][i];
+          fx0 = fmaf(v, cs, fx0);
+          fy0 = fmaf(v, sn, fy0);
+        }
+      } else {
+        const int64_t period = 4 * (int64_t)n;
+        for (int i = 0; i < n; ++i) {
+          float sn, cs;
+          sincospif((float)(((int64_t)j * (2 * i + 1)) % period) / (float)(2 * n), &sn, &cs);
+          const float v = (float)sv[0][i];
+          fx0 = fmaf(v, cs, fx0);
+          fy0 = fmaf(v, sn, fy0);
+          if (ty == 0) fx1 = fmaf((float)sv[1][i], cs, fx1);
+        }
+      }
+      x0 = fx0;
+      y0 = fy0;
+      x1 = fx1;
+    } else if (ty == 1) {
       for (int i = 0; i < n; ++i) {
         double s, c;
         sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &s, &c);
@@ -662,24 +716,47 @@ __global__ void init_errors_kernel(FitView fv, int ne) {
 // --------------------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------------------
-void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st) {
+void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
+                  bool mixed) {
   if (ntiles <= 0) return;
-  if (mode == MODE_COLLOC)
-    hipLaunchKernelGGL(sweep_kernel<MODE_COLLOC>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  if (mode == MODE_COLLOC && mixed)
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, true>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else if (mode == MODE_COLLOC)
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+  else if (mode == MODE_RHS && mixed)
+    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, true>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
   else if (mode == MODE_RHS)
-    hipLaunchKernelGGL(sweep_kernel<MODE_RHS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
   else if (mode == MODE_COLLOC_HEADS)
-    hipLaunchKernelGGL(sweep_kernel<MODE_COLLOC_HEADS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC_HEADS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
   else if (mode == MODE_DISCH_HEADS)
-    hipLaunchKernelGGL(sweep_kernel<MODE_DISCH_HEADS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_DISCH_HEADS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
   else
-    hipLaunchKernelGGL(sweep_kernel<MODE_POINTS>, dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_POINTS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+}
+
+// single-precision copy of every element's coefficients (mixed-precision far field)
+__global__ void coef_to_f32_kernel(NetView net, int ne, float2* __restrict__ out) {
+  const int e = blockIdx.x;
+  if (e >= ne) return;
+  const int m = net.ncoef[e];
+  const double2* c = net.coef + (size_t)e * net.M;
+  float2* o = out + (size_t)e * net.M;
+  for (int j = threadIdx.x; j < m; j += blockDim.x) o[j] = make_float2((float)c[j].x, (float)c[j].y);
+}
+
+void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st) {
+  if (ne <= 0) return;
+  hipLaunchKernelGGL(coef_to_f32_kernel, dim3(ne), dim3(64), 0, st, net, ne, out);
 }
 
 void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
-                const int* pt_off, double damping, int blend, hipStream_t st) {
+                const int* pt_off, double damping, int blend, hipStream_t st, bool f32) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(fit_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
+  if (f32)
+    hipLaunchKernelGGL(fit_kernel<true>, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
+  else
+    hipLaunchKernelGGL(fit_kernel<false>, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
 }
 
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st) {
@@ -780,26 +857,26 @@ void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, c
 // 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 4 wavefronts
 // and their partial sums reduced in LDS.  Used for the blocked triangular solves.
 constexpr int kGemvCols = 1024;
-__global__ __launch_bounds__(256) void gemv_cm_kernel(const double* __restrict__ A, int64_t lda, int m, int ncols,
-                                                      const double* __restrict__ x, double* __restrict__ y,
-                                                      double alpha, double beta) {
-  __shared__ double sx[kGemvCols];
-  __shared__ double part[4][64];
+template <class T>
+__global__ __launch_bounds__(256) void gemv_cm_kernel(const T* __restrict__ A, int64_t lda, int m, int ncols,
+                                                      const T* __restrict__ x, T* __restrict__ y, T alpha, T beta) {
+  __shared__ T sx[kGemvCols];
+  __shared__ T part[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int c = threadIdx.x; c < ncols; c += 256) sx[c] = x[c];
   __syncthreads();
   const int row = blockIdx.x * 64 + lane;
-  double s = 0.0;
+  T s = 0;
   if (row < m) {
     const int per = (ncols + 3) / 4;
     const int c0 = wv * per, c1 = min(ncols, c0 + per);
-    const double* a = A + (int64_t)c0 * lda + row;
+    const T* a = A + (int64_t)c0 * lda + row;
     // eight independent loads in flight per lane
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int c = c0;
     for (; c + 8 <= c1; c += 8, a += 8 * lda) {
-      const double a0 = a[0], a1 = a[lda], a2 = a[2 * lda], a3 = a[3 * lda];
-      const double a4 = a[4 * lda], a5 = a[5 * lda], a6 = a[6 * lda], a7 = a[7 * lda];
+      const T a0 = a[0], a1 = a[lda], a2 = a[2 * lda], a3 = a[3 * lda];
+      const T a4 = a[4 * lda], a5 = a[5 * lda], a6 = a[6 * lda], a7 = a[7 * lda];
       s0 = fma(a0, sx[c], s0);
       s1 = fma(a1, sx[c + 1], s1);
       s2 = fma(a2, sx[c + 2], s2);
@@ -815,28 +892,73 @@ __global__ __launch_bounds__(256) void gemv_cm_kernel(const double* __restrict__
   part[wv][lane] = s;
   __syncthreads();
   if (wv == 0 && row < m) {
-    const double t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
-    y[row] = (beta == 0.0 ? 0.0 : beta * y[row]) + alpha * t;
+    const T t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    y[row] = (beta == T(0) ? T(0) : beta * y[row]) + alpha * t;
   }
 }
 
-void launch_gemv_cm(const double* A, int64_t lda, int m, int ncols, const double* x, double* y, double alpha,
-                    double beta, hipStream_t st) {
+template <class T>
+void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st) {
   if (m <= 0 || ncols <= 0) return;
-  hipLaunchKernelGGL(gemv_cm_kernel, dim3((m + 63) / 64), dim3(256), 0, st, A, lda, m, ncols, x, y, alpha, beta);
+  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64), dim3(256), 0, st, A, lda, m, ncols, x, y, alpha, beta);
 }
+template void launch_gemv_cm<double>(const double*, int64_t, int, int, const double*, double*, double, double,
+                                     hipStream_t);
+template void launch_gemv_cm<float>(const float*, int64_t, int, int, const float*, float*, float, float, hipStream_t);
 
 // diagonal of nb stacked nbk x nbk blocks set to one (unit-triangular inverses)
-__global__ void unit_diag_kernel(double* __restrict__ blocks, int nb, int nbk) {
+template <class T>
+__global__ void unit_diag_kernel(T* __restrict__ blocks, int nb, int nbk) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < nb * nbk) {
     const int b = k / nbk, i = k % nbk;
-    blocks[(size_t)b * nbk * nbk + (size_t)i * nbk + i] = 1.0;
+    blocks[(size_t)b * nbk * nbk + (size_t)i * nbk + i] = T(1);
   }
 }
 
-void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st) {
-  hipLaunchKernelGGL(unit_diag_kernel, dim3((nb * nbk + 255) / 256), dim3(256), 0, st, blocks, nb, nbk);
+template <class T>
+void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st) {
+  hipLaunchKernelGGL(unit_diag_kernel<T>, dim3((nb * nbk + 255) / 256), dim3(256), 0, st, blocks, nb, nbk);
+}
+template void launch_unit_diag<double>(double*, int, int, hipStream_t);
+template void launch_unit_diag<float>(float*, int, int, hipStream_t);
+
+// single-precision copy of a matrix
+__global__ void to_f32_kernel(const double* __restrict__ a, int64_t n, float* __restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st) {
+  hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, b);
+}
+
+// refinement step of the mixed-precision discharge solve:
+// r = b - A y (FP64, A y given), x32[i] = rs[p] * r[p] with p = perm[i]
+__global__ void refine_rhs_kernel(const double* __restrict__ b, const double* __restrict__ ay,
+                                  const int* __restrict__ perm, const double* __restrict__ rs, int n,
+                                  float* __restrict__ x32) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int p = perm[i];
+    x32[i] = (float)((b[p] - ay[p]) * rs[p]);
+  }
+}
+
+// y_new = y + d (d from the single-precision triangular solves)
+__global__ void refine_update_kernel(const double* __restrict__ y, const float* __restrict__ d, int n,
+                                     double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = y[i] + (double)d[i];
+}
+
+void launch_refine_rhs(const double* b, const double* ay, const int* perm, const double* rs, int n, float* x32,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(refine_rhs_kernel, dim3((n + 255) / 256), dim3(256), 0, st, b, ay, perm, rs, n, x32);
+}
+
+void launch_refine_update(const double* y, const float* d, int n, double* x, hipStream_t st) {
+  hipLaunchKernelGGL(refine_update_kernel, dim3((n + 255) / 256), dim3(256), 0, st, y, d, n, x);
 }
 
 void launch_permute(const double* b, const int* perm, const double* rs, int n, double* x, hipStream_t st) {

@@ -50,6 +50,8 @@ struct NetView {
   double* q_w;
   double2* coef;          // [E][M]
   double2* wcoef;         // [E][M]
+  const float2* coef32;   // [E][M] single-precision copy (mixed-precision far field)
+  double far2;            // mixed precision: a line source is evaluated in FP32 where |Z|^2 >= far2 on every lane
   const double* t;
   const double* constant;
   double* constant_w;
@@ -107,9 +109,11 @@ struct IterScalars {
   int arg_residual;
 };
 
-void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st);
+void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
+                  bool mixed = false);
+void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
-                const int* pt_off, double damping, int blend, hipStream_t st);
+                const int* pt_off, double damping, int blend, hipStream_t st, bool f32 = false);
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
 void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
@@ -117,9 +121,14 @@ void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, c
 void launch_csr_gather(const double* A, int64_t n, const int* row_ptr, const int* col, double* val, hipStream_t st);
 void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t st);
-void launch_gemv_cm(const double* A, int64_t lda, int m, int ncols, const double* x, double* y, double alpha,
-                    double beta, hipStream_t st);
-void launch_unit_diag(double* blocks, int nb, int nbk, hipStream_t st);
+template <class T>
+void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st);
+template <class T>
+void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st);
+void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);
+void launch_refine_rhs(const double* b, const double* ay, const int* perm, const double* rs, int n, float* x32,
+                       hipStream_t st);
+void launch_refine_update(const double* y, const float* d, int n, double* x, hipStream_t st);
 void launch_permute(const double* b, const int* perm, const double* rs, int n, double* x, hipStream_t st);
 void launch_row_scale(double* A, int64_t n, double* rs, unsigned long long* work, hipStream_t st);
 void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st);

@@ -91,4 +91,43 @@ __device__ __forceinline__ float2 cmaddf(float2 r, float2 w, float2 c) {
   return make_float2(fmaf(r.x, w.x, fmaf(-r.y, w.y, c.x)), fmaf(r.x, w.y, fmaf(r.y, w.x, c.y)));
 }
 
+__device__ __forceinline__ float2 csqrt_principal_f(float x, float y) {
+  float r = sqrtf(fmaf(x, x, y * y));
+  if (r == 0.0f) return make_float2(0.0f, y);
+  float t = sqrtf(0.5f * (r + fabsf(x)));
+  if (x >= 0.0f) return make_float2(t, y / (2.0f * t));
+  return make_float2(fabsf(y) / (2.0f * t), copysignf(t, y));
+}
+
+__device__ __forceinline__ float2 joukowsky_exterior_f(float2 Z) {
+  float ux = fmaf(Z.x - 1.0f, Z.x + 1.0f, -Z.y * Z.y);
+  float uy = 2.0f * Z.x * Z.y;
+  float2 s = csqrt_principal_f(ux, uy);
+  if (fmaf(Z.x, s.x, Z.y * s.y) < 0.0f) {
+    s.x = -s.x;
+    s.y = -s.y;
+  }
+  return make_float2(Z.x + s.x, Z.y + s.y);
+}
+
+// FP32 Horner, coefficients fetched eight at a time (wave-uniform scalar loads)
+__device__ __forceinline__ float2 horner_f(const float2* __restrict__ c, int m, float2 w) {
+  float2 r = c[m - 1];
+  int j = m - 2;
+  for (; j >= 7; j -= 8) {
+    const float2 c0 = c[j], c1 = c[j - 1], c2_ = c[j - 2], c3 = c[j - 3];
+    const float2 c4 = c[j - 4], c5 = c[j - 5], c6 = c[j - 6], c7 = c[j - 7];
+    r = cmaddf(r, w, c0);
+    r = cmaddf(r, w, c1);
+    r = cmaddf(r, w, c2_);
+    r = cmaddf(r, w, c3);
+    r = cmaddf(r, w, c4);
+    r = cmaddf(r, w, c5);
+    r = cmaddf(r, w, c6);
+    r = cmaddf(r, w, c7);
+  }
+  for (; j >= 0; --j) r = cmaddf(r, w, c[j]);
+  return r;
+}
+
 }  // namespace aem

@@ -238,6 +238,12 @@ class Solver {
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
   static constexpr int kNB = 1024;
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
+  // mixed precision: single-precision factors and blocked-solve buffers
+  DBuf<float> d_A32_, d_linv32_, d_uinv32_, d_x32_, d_w32_, d_v32_;
+  DBuf<float2> d_coef32_;
+  bool mixed_phase_ = false;
+  double far2_ = 16.0;
+  int precision_ = ANDFN_MI_FP64;
   // initial state of the mutable arrays, restored at the start of every run
   DBuf<double2> s_coef_;
   DBuf<double> s_q_, s_constant_, s_error_, s_error_old_, s_error_old2_, s_error_coef_, s_dpsi_;
@@ -279,6 +285,8 @@ class Solver {
     v.constant_w = d_constant_.p;
     v.mem_ptr = d_mem_ptr_.p;
     v.slots = d_slots_.p;
+    v.coef32 = d_coef32_.p;
+    v.far2 = far2_;
     return v;
   }
   aem::FitView fit() const {
@@ -668,7 +676,15 @@ void Solver::assemble_and_factor() {
   }
   d_ipiv_.alloc(n);
   d_info_.zero(1, st_);
-  RB_CHECK(rocsolver_dgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
+  if (precision_ == ANDFN_MI_MIXED) {
+    // single-precision factors of the equilibrated matrix; the FP64 residual of the CSR copy
+    // drives the refinement, so the iteration's fixed point is the FP64 one
+    d_A32_.alloc((size_t)(n * n));
+    aem::launch_to_f32(d_A_.p, n * n, d_A32_.p, st_);
+    RB_CHECK(rocsolver_sgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A32_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
+  } else {
+    RB_CHECK(rocsolver_dgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
+  }
   rocblas_int info = 0;
   HIP_CHECK(hipMemcpyAsync(&info, d_info_.p, sizeof(info), hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
@@ -688,7 +704,32 @@ void Solver::assemble_and_factor() {
     const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
     d_linv_.zero((size_t)nb * kNB * kNB, st_);
     d_uinv_.zero((size_t)nb * kNB * kNB, st_);
-    if (nfull > 0) {
+    if (precision_ == ANDFN_MI_MIXED) {
+      d_linv32_.zero((size_t)nb * kNB * kNB, st_);
+      d_uinv32_.zero((size_t)nb * kNB * kNB, st_);
+      if (nfull > 0) {
+        RB_CHECK(rocblas_strtri_strided_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, d_A32_.p,
+                                                (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_linv32_.p, kNB,
+                                                (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
+        RB_CHECK(rocblas_strtri_strided_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, kNB, d_A32_.p,
+                                                (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_uinv32_.p, kNB,
+                                                (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
+      }
+      if (nb > nfull) {
+        const int64_t r0 = nfull * kNB;
+        const rocblas_int rem = (rocblas_int)(n - r0);
+        float* blk = d_A32_.p + r0 * (n + 1);
+        RB_CHECK(rocblas_strtri(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rem, blk, (rocblas_int)n,
+                                d_linv32_.p + nfull * kNB * kNB, kNB));
+        RB_CHECK(rocblas_strtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, (rocblas_int)n,
+                                d_uinv32_.p + nfull * kNB * kNB, kNB));
+      }
+      aem::launch_unit_diag<float>(d_linv32_.p, (int)nb, kNB, st_);
+      d_x32_.alloc(n);
+      d_w32_.alloc(n);
+      d_v32_.alloc(n);
+      HIP_CHECK(hipStreamSynchronize(st_));
+    } else if (nfull > 0) {
       RB_CHECK(rocblas_dtrtri_strided_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, d_A_.p,
                                               (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_linv_.p, kNB,
                                               (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
@@ -696,7 +737,7 @@ void Solver::assemble_and_factor() {
                                               (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_uinv_.p, kNB,
                                               (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
     }
-    if (nb > nfull) {
+    if (precision_ != ANDFN_MI_MIXED && nb > nfull) {
       const int64_t r0 = nfull * kNB;
       const rocblas_int rem = (rocblas_int)(n - r0);
       double* blk = d_A_.p + r0 * (n + 1);
@@ -705,7 +746,7 @@ void Solver::assemble_and_factor() {
       RB_CHECK(rocblas_dtrtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, (rocblas_int)n,
                               d_uinv_.p + nfull * kNB * kNB, kNB));
     }
-    aem::launch_unit_diag(d_linv_.p, (int)nb, kNB, st_);
+    if (precision_ != ANDFN_MI_MIXED) aem::launch_unit_diag<double>(d_linv_.p, (int)nb, kNB, st_);
     d_w_.alloc(n);
     d_v_.alloc(n);
     HIP_CHECK(hipStreamSynchronize(st_));
@@ -762,6 +803,30 @@ double* Solver::discharge_solve(double* b, double* x) {
     RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A_.p, n, d_w_.p, 1, &zero, x, 1));
     return x;
   }
+  if (precision_ == ANDFN_MI_MIXED) {
+    // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
+    aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
+    aem::launch_refine_rhs(b, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
+    const int nb = (n + kNB - 1) / kNB;
+    float* x32 = d_x32_.p;
+    float* w32 = d_w32_.p;
+    float* v32 = d_v32_.p;
+    for (int j = 0; j < nb; ++j) {
+      const int r0 = j * kNB, bs = std::min(kNB, n - r0);
+      aem::launch_gemv_cm<float>(d_linv32_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x32 + r0, w32 + r0, 1.f, 0.f, st_);
+      const int rest = n - r0 - bs;
+      if (rest > 0)
+        aem::launch_gemv_cm<float>(d_A32_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w32 + r0, x32 + r0 + bs, -1.f, 1.f,
+                                   st_);
+    }
+    for (int j = nb - 1; j >= 0; --j) {
+      const int r0 = j * kNB, bs = std::min(kNB, n - r0);
+      aem::launch_gemv_cm<float>(d_uinv32_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w32 + r0, v32 + r0, 1.f, 0.f, st_);
+      if (r0 > 0) aem::launch_gemv_cm<float>(d_A32_.p + (size_t)r0 * n, n, r0, bs, v32 + r0, w32, -1.f, 1.f, st_);
+    }
+    aem::launch_refine_update(d_y_.p, v32, n, x, st_);
+    return x;
+  }
   // R P b, then L z = P b and U x = z block by block: each block is one multiply by the inverse
   // diagonal block and one gemv update of the remaining right-hand side (the factors are read once)
   aem::launch_permute(b, d_perm_.p, d_req_.p, n, x, st_);
@@ -771,14 +836,14 @@ double* Solver::discharge_solve(double* b, double* x) {
   double* v = d_v_.p;
   for (int j = 0; j < nb; ++j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm(d_linv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x + r0, w + r0, one, zero, st_);
+    aem::launch_gemv_cm<double>(d_linv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x + r0, w + r0, one, zero, st_);
     const int rest = n - r0 - bs;
-    if (rest > 0) aem::launch_gemv_cm(d_A_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w + r0, x + r0 + bs, mone, one, st_);
+    if (rest > 0) aem::launch_gemv_cm<double>(d_A_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w + r0, x + r0 + bs, mone, one, st_);
   }
   for (int j = nb - 1; j >= 0; --j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm(d_uinv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w + r0, v + r0, one, zero, st_);
-    if (r0 > 0) aem::launch_gemv_cm(d_A_.p + (size_t)r0 * n, n, r0, bs, v + r0, w, mone, one, st_);
+    aem::launch_gemv_cm<double>(d_uinv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w + r0, v + r0, one, zero, st_);
+    if (r0 > 0) aem::launch_gemv_cm<double>(d_A_.p + (size_t)r0 * n, n, r0, bs, v + r0, w, mone, one, st_);
   }
   HIP_CHECK(hipMemcpyAsync(x, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   return x;
@@ -839,6 +904,18 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const double damping = C_.DAMPING;
   const int verbose = opt ? opt->verbose : 0;
   lu_mode_ = opt ? opt->lu_mode : ANDFN_MI_LU_TRSV;
+  precision_ = opt ? opt->precision : ANDFN_MI_FP64;
+  if (precision_ != ANDFN_MI_FP64 && precision_ != ANDFN_MI_MIXED) fail(ANDFN_MI_EINVAL, "precision %d unknown", precision_);
+  if (precision_ == ANDFN_MI_MIXED && lu_mode_ != ANDFN_MI_LU_TRSV)
+    fail(ANDFN_MI_EINVAL, "the mixed-precision form uses the blocked triangular solves (lu_mode 0)");
+  mixed_phase_ = precision_ == ANDFN_MI_MIXED;
+  if (mixed_phase_ && !(opt && opt->head_tol > 0.0))
+    fail(ANDFN_MI_EINVAL, "the mixed-precision form stops on the FP64 head residual: set head_tol > 0");
+  {
+    const double ff = (opt && opt->far_field > 0.0) ? opt->far_field : 4.0;
+    far2_ = ff * ff;
+  }
+  if (mixed_phase_) d_coef32_.zero(coef_.size(), st_);
   if (lu_mode_ < 0 || lu_mode_ > 2) fail(ANDFN_MI_EINVAL, "lu_mode %d unknown", lu_mode_);
   const int32_t* sched = opt ? opt->growth_schedule : nullptr;
   const int64_t sched_n = opt ? opt->schedule_iterations : 0;
@@ -847,6 +924,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const double head_tol = opt ? opt->head_tol : 0.0;
   double resid = -1.0;
   int resid_row = -1;
+  std::vector<double> rhist;
+  int64_t nit_fp32 = 0;
   if (K_ < 1 || K_ > 2 * L_.M) fail(ANDFN_MI_EINVAL, "discharge_int %d outside [1, %lld]", K_, (long long)(2 * L_.M));
   if (max_coef + coef_increase - 1 > L_.M)
     fail(ANDFN_MI_ECAPACITY, "MAX_NCOEF %d + COEF_INCREASE %d exceeds the coefficient capacity %lld", max_coef,
@@ -903,9 +982,10 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
 
   auto sweep_full = [&](bool blend_now) {
     HIP_CHECK(hipEventRecord(ev_[0], st_));
-    aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(), (int)full_.tiles.size(), K_, d_val_.p, st_);
+    aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(), (int)full_.tiles.size(), K_, d_val_.p, st_, mixed_phase_);
     HIP_CHECK(hipEventRecord(ev_[1], st_));
-    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, damping, blend_now ? 1 : 0, st_);
+    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, damping, blend_now ? 1 : 0, st_,
+                    mixed_phase_);
     aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_);
     HIP_CHECK(hipEventRecord(ev_[2], st_));
     int64_t pr = 0;
@@ -935,14 +1015,30 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   while (cnt_error < 2 && nit < max_iterations) {
     ++nit;
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
+    if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
     HIP_CHECK(hipEventRecord(ev_[3], st_));
-    aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_);
+    aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_, mixed_phase_);
     aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_b_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_b_.p + nde, 0, sizeof(double) * F_, st_));
     if (head_tol > 0.0 && nit >= 2) {
       // head residual of the previous iterate against the right-hand side of its coefficients
       resid = head_residual(d_b_.p, &resid_row);
-      if (resid <= head_tol) {
+      if (mixed_phase_) {
+        // The FP32 phase ends near the tolerance, or when the best residual of the last 10
+        // iterations is no better than 0.7 of the best of the 10 before (the FP32 floor).
+        rhist.push_back(resid);
+        const size_t h = rhist.size();
+        bool stalled = false;
+        if (h >= 20) {
+          const double last = *std::min_element(rhist.end() - 10, rhist.end());
+          const double prev = *std::min_element(rhist.end() - 20, rhist.end() - 10);
+          stalled = last > 0.7 * prev;
+        }
+        if (resid <= 30.0 * head_tol || stalled) {
+          mixed_phase_ = false;
+          nit_fp32 = nit - 1;
+        }
+      } else if (resid <= head_tol) {
         --nit;
         HIP_CHECK(hipEventRecord(ev_[4], st_));
         HIP_CHECK(hipEventRecord(ev_[5], st_));
@@ -952,8 +1048,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipEventRecord(ev_[4], st_));
     solve_rhs();  // d_x_ = A^-1 d_b_
     HIP_CHECK(hipEventRecord(ev_[5], st_));
-    std::swap(d_yold_.p, d_y_.p);
-    std::swap(d_yold_.n, d_y_.n);
+    HIP_CHECK(hipMemcpyAsync(d_yold_.p, d_y_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
     HIP_CHECK(hipMemcpyAsync(d_y_.p, d_x_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
     aem::launch_scatter(nv, d_de_.p, nde, F_, d_y_.p, st_);
     fl_rhs += rhs_flops;
@@ -997,11 +1092,13 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
           aem::launch_cross(nv, fv, d_grown_circles_.p, (int)gcirc.size(), st_);
         }
         HIP_CHECK(hipEventRecord(ev_[6], st_));
-        aem::launch_sweep(aem::MODE_COLLOC, nv, grown_.view(), (int)grown_.tiles.size(), K_, d_valg_.p, st_);
+        if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
+        aem::launch_sweep(aem::MODE_COLLOC, nv, grown_.view(), (int)grown_.tiles.size(), K_, d_valg_.p, st_, mixed_phase_);
         HIP_CHECK(hipEventRecord(ev_[7], st_));
         grown_timed = true;
         ++n_sweep;
-        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_valg_.p, grown_.d_pt_off.p, damping, 0, st_);
+        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_valg_.p, grown_.d_pt_off.p, damping, 0, st_,
+                        mixed_phase_);
         int64_t pr = 0;
         fl_sweep += set_flops(grown_, true, &pr);
         pr_sweep += pr;
@@ -1078,7 +1175,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->head_residual = resid;
     info->head_residual_row = resid_row;
     info->bc_residual = bcres;
+    info->nit_fp32 = precision_ == ANDFN_MI_MIXED ? (mixed_phase_ ? nit : nit_fp32) : 0;
   }
+  mixed_phase_ = false;
   return (cnt_error < 2 && !stopped_on_head) ? ANDFN_MI_NOT_CONVERGED : ANDFN_MI_OK;
 }
 

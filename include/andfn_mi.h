@@ -51,7 +51,7 @@ typedef struct andfn_mi_options {
   int32_t device;        /* HIP device ordinal */
   int32_t verbose;       /* >0: per-iteration line on stderr, as hpc_solve.py:200-215 logs */
   int32_t lu_mode;       /* ANDFN_MI_LU_* */
-  double far_field;      /* mixed form: |chi| above which a line source is evaluated in FP32 */
+  double far_field;      /* mixed form: |Z| above which a line source is evaluated in FP32 (0: default 4) */
   /* Optional coefficient-count schedule: row k (k = 0, 1, ...) holds every element's ncoef after
    * iteration k + 1.  When given, the growth step of iteration k + 1 grows exactly the elements
    * whose count rises in row k instead of applying the growth test (rows beyond
@@ -98,6 +98,7 @@ typedef struct andfn_mi_info {
    * |h_f0 - h_f1|, over constant-head-line collocation points and well discharge points of
    * |h - h_prescribed|; bounded below by the series truncation, not by the iteration */
   double bc_residual;
+  int64_t nit_fp32;       /* mixed form: iterations run with FP32 far field and fit before the FP64 phase */
 } andfn_mi_info;
 
 /*
