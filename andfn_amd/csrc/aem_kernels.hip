@@ -854,22 +854,29 @@ void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, c
 }
 
 // y[0:m] = beta * y + alpha * A[0:m, 0:ncols] x for a column-major block (lda) with ncols <= 1024:
-// 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 4 wavefronts
-// and their partial sums reduced in LDS.  Used for the blocked triangular solves.
+// 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 16 wavefronts
+// and their partial sums reduced in LDS in a fixed order.  Sixteen wavefronts with eight loads in
+// flight each keep enough bytes in flight per CU to stream a panel at HBM rate.  ``tri`` skips the
+// zero half of a triangular block (1: lower, column c <= row; 2: upper, column c >= row).
+// Used for the blocked triangular solves.
 constexpr int kGemvCols = 1024;
+constexpr int kGemvWaves = 16;
 template <class T>
-__global__ __launch_bounds__(256) void gemv_cm_kernel(const T* __restrict__ A, int64_t lda, int m, int ncols,
-                                                      const T* __restrict__ x, T* __restrict__ y, T alpha, T beta) {
+__device__ __forceinline__ void gemv_cm_block(const T* __restrict__ A, int64_t lda, int m, int ncols,
+                                              const T* __restrict__ x, T* __restrict__ y, T alpha, T beta, int tri,
+                                              int rblk) {
   __shared__ T sx[kGemvCols];
-  __shared__ T part[4][64];
+  __shared__ T part[kGemvWaves][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int c = threadIdx.x; c < ncols; c += 256) sx[c] = x[c];
+  for (int c = threadIdx.x; c < ncols; c += 64 * kGemvWaves) sx[c] = x[c];
   __syncthreads();
-  const int row = blockIdx.x * 64 + lane;
+  const int row = rblk * 64 + lane;
+  const int per = (ncols + kGemvWaves - 1) / kGemvWaves;
+  int c0 = wv * per, c1 = min(ncols, c0 + per);
+  if (tri == 1) c1 = min(c1, rblk * 64 + 64);  // lower: columns beyond the block's last row are zero
+  if (tri == 2) c0 = max(c0, rblk * 64);       // upper: columns before the block's first row are zero
   T s = 0;
-  if (row < m) {
-    const int per = (ncols + 3) / 4;
-    const int c0 = wv * per, c1 = min(ncols, c0 + per);
+  if (row < m && c0 < c1) {
     const T* a = A + (int64_t)c0 * lda + row;
     // eight independent loads in flight per lane
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
@@ -892,19 +899,51 @@ __global__ __launch_bounds__(256) void gemv_cm_kernel(const T* __restrict__ A, i
   part[wv][lane] = s;
   __syncthreads();
   if (wv == 0 && row < m) {
-    const T t = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    T t = 0;
+#pragma unroll
+    for (int k = 0; k < kGemvWaves; ++k) t += part[k][lane];
     y[row] = (beta == T(0) ? T(0) : beta * y[row]) + alpha * t;
   }
 }
 
 template <class T>
+__global__ __launch_bounds__(64 * kGemvWaves) void gemv_cm_kernel(const T* __restrict__ A, int64_t lda, int m,
+                                                                  int ncols, const T* __restrict__ x,
+                                                                  T* __restrict__ y, T alpha, T beta) {
+  gemv_cm_block<T>(A, lda, m, ncols, x, y, alpha, beta, 0, blockIdx.x);
+}
+
+// y_j = B_j x_j for the nb stacked nbk x nbk triangular blocks B_j (block j covers rows
+// j*nbk .. min(n, (j+1)*nbk)); blockIdx.y selects the block
+template <class T>
+__global__ __launch_bounds__(64 * kGemvWaves) void gemv_diag_kernel(const T* __restrict__ B, int nbk, int n, int tri,
+                                                                    const T* __restrict__ x, T* __restrict__ y) {
+  const int j = blockIdx.y;
+  const int r0 = j * nbk;
+  const int bs = min(nbk, n - r0);
+  if ((int)blockIdx.x * 64 >= bs) return;  // uniform per workgroup, before any barrier
+  gemv_cm_block<T>(B + (size_t)j * nbk * nbk, nbk, bs, bs, x + r0, y + r0, T(1), T(0), tri, blockIdx.x);
+}
+
+template <class T>
 void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st) {
   if (m <= 0 || ncols <= 0) return;
-  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64), dim3(256), 0, st, A, lda, m, ncols, x, y, alpha, beta);
+  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64), dim3(64 * kGemvWaves), 0, st, A, lda, m, ncols, x, y,
+                     alpha, beta);
 }
 template void launch_gemv_cm<double>(const double*, int64_t, int, int, const double*, double*, double, double,
                                      hipStream_t);
 template void launch_gemv_cm<float>(const float*, int64_t, int, int, const float*, float*, float, float, hipStream_t);
+
+template <class T>
+void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st) {
+  if (n <= 0) return;
+  const int nb = (n + nbk - 1) / nbk;
+  hipLaunchKernelGGL(gemv_diag_kernel<T>, dim3((nbk + 63) / 64, nb), dim3(64 * kGemvWaves), 0, st, blocks, nbk, n, tri,
+                     x, y);
+}
+template void launch_gemv_diag<double>(const double*, int, int, int, const double*, double*, hipStream_t);
+template void launch_gemv_diag<float>(const float*, int, int, int, const float*, float*, hipStream_t);
 
 // diagonal of nb stacked nbk x nbk blocks set to one (unit-triangular inverses)
 template <class T>

@@ -123,6 +123,9 @@ void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, c
                  hipStream_t st);
 template <class T>
 void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st);
+// y_j = B_j x_j for the stacked nbk x nbk diagonal blocks (tri: 0 full, 1 lower, 2 upper)
+template <class T>
+void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st);
 template <class T>
 void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st);
 void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);

@@ -238,6 +238,10 @@ class Solver {
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
   static constexpr int kNB = 1024;
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
+  template <class T>
+  void prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv);
+  template <class T>
+  void blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v);
   // mixed precision: single-precision factors and blocked-solve buffers
   DBuf<float> d_A32_, d_linv32_, d_uinv32_, d_x32_, d_w32_, d_v32_;
   DBuf<float2> d_coef32_;
@@ -612,6 +616,75 @@ double Solver::set_flops(const TargetSet& ts, bool with_q, int64_t* pairs) const
   return fl;
 }
 
+// rocBLAS triangular inverse / multiply by element type
+static rocblas_status rb_trtri_batched(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int nbk, const double* A,
+                                       int64_t lda, int64_t sa, double* inv, int ldi, int64_t si, int cnt) {
+  return rocblas_dtrtri_strided_batched(h, f, d, nbk, A, (rocblas_int)lda, sa, inv, ldi, si, cnt);
+}
+static rocblas_status rb_trtri_batched(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int nbk, const float* A,
+                                       int64_t lda, int64_t sa, float* inv, int ldi, int64_t si, int cnt) {
+  return rocblas_strtri_strided_batched(h, f, d, nbk, A, (rocblas_int)lda, sa, inv, ldi, si, cnt);
+}
+static rocblas_status rb_trtri(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, const double* A,
+                               int64_t lda, double* inv, int ldi) {
+  return rocblas_dtrtri(h, f, d, m, A, (rocblas_int)lda, inv, ldi);
+}
+static rocblas_status rb_trtri(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, const float* A,
+                               int64_t lda, float* inv, int ldi) {
+  return rocblas_strtri(h, f, d, m, A, (rocblas_int)lda, inv, ldi);
+}
+static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
+                                    const double* T, int ldt, double* B, int64_t ldb) {
+  const double one = 1.0;
+  return rocblas_dtrmm(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, (rocblas_int)ldb,
+                       B, (rocblas_int)ldb);
+}
+static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
+                                    const float* T, int ldt, float* B, int64_t ldb) {
+  const float one = 1.0f;
+  return rocblas_strmm(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, (rocblas_int)ldb,
+                       B, (rocblas_int)ldb);
+}
+
+// Blocked triangular-solve operators from the LU factors in A (n x n, column-major, kNB blocks):
+// Linv_j / Uinv_j = inverses of the diagonal blocks of L (unit) and U, and, in place in A, the
+// panels G_j = L[below j, j] Linv_j and H_j = U[above j, j] Uinv_j.  With them, L z = c becomes
+// "c[below j] -= G_j c_j for j ascending, then z_j = Linv_j c_j for all j at once" (and U likewise,
+// descending): the sequential chain is one gemv per block and the diagonal blocks are applied by
+// a single batched launch.
+template <class T>
+void Solver::prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv) {
+  const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
+  linv.zero((size_t)nb * kNB * kNB, st_);
+  uinv.zero((size_t)nb * kNB * kNB, st_);
+  if (nfull > 0) {
+    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, A, n, kNB * (n + 1), linv.p, kNB,
+                              (int64_t)kNB * kNB, (int)nfull));
+    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, kNB, A, n, kNB * (n + 1), uinv.p,
+                              kNB, (int64_t)kNB * kNB, (int)nfull));
+  }
+  if (nb > nfull) {
+    const int64_t r0 = nfull * kNB;
+    const int rem = (int)(n - r0);
+    const T* blk = A + r0 * (n + 1);
+    RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rem, blk, n, linv.p + nfull * kNB * kNB, kNB));
+    RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, n, uinv.p + nfull * kNB * kNB,
+                      kNB));
+  }
+  aem::launch_unit_diag<T>(linv.p, (int)nb, kNB, st_);
+  for (int64_t j = 0; j < nb; ++j) {
+    const int64_t r0 = j * kNB;
+    const int bs = (int)std::min<int64_t>(kNB, n - r0);
+    const int rest = (int)(n - r0 - bs);
+    if (rest > 0)
+      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, linv.p + j * kNB * kNB, kNB,
+                             A + r0 * n + r0 + bs, n));
+    if (r0 > 0)
+      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, uinv.p + j * kNB * kNB,
+                             kNB, A + r0 * n, n));
+  }
+}
+
 void Solver::assemble_and_factor() {
   drop_graph();
   const int nde = (int)de_.size();
@@ -700,53 +773,14 @@ void Solver::assemble_and_factor() {
     for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
     for (int64_t i = 0; i < n; ++i) std::swap(perm[i], perm[ipiv[i] - 1]);
     d_perm_.upload(perm, st_);
-    // diagonal-block inverses: full blocks batched, then the trailing partial block
-    const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
-    d_linv_.zero((size_t)nb * kNB * kNB, st_);
-    d_uinv_.zero((size_t)nb * kNB * kNB, st_);
     if (precision_ == ANDFN_MI_MIXED) {
-      d_linv32_.zero((size_t)nb * kNB * kNB, st_);
-      d_uinv32_.zero((size_t)nb * kNB * kNB, st_);
-      if (nfull > 0) {
-        RB_CHECK(rocblas_strtri_strided_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, d_A32_.p,
-                                                (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_linv32_.p, kNB,
-                                                (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
-        RB_CHECK(rocblas_strtri_strided_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, kNB, d_A32_.p,
-                                                (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_uinv32_.p, kNB,
-                                                (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
-      }
-      if (nb > nfull) {
-        const int64_t r0 = nfull * kNB;
-        const rocblas_int rem = (rocblas_int)(n - r0);
-        float* blk = d_A32_.p + r0 * (n + 1);
-        RB_CHECK(rocblas_strtri(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rem, blk, (rocblas_int)n,
-                                d_linv32_.p + nfull * kNB * kNB, kNB));
-        RB_CHECK(rocblas_strtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, (rocblas_int)n,
-                                d_uinv32_.p + nfull * kNB * kNB, kNB));
-      }
-      aem::launch_unit_diag<float>(d_linv32_.p, (int)nb, kNB, st_);
+      prepare_blocked<float>(d_A32_.p, n, d_linv32_, d_uinv32_);
       d_x32_.alloc(n);
       d_w32_.alloc(n);
       d_v32_.alloc(n);
-      HIP_CHECK(hipStreamSynchronize(st_));
-    } else if (nfull > 0) {
-      RB_CHECK(rocblas_dtrtri_strided_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, d_A_.p,
-                                              (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_linv_.p, kNB,
-                                              (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
-      RB_CHECK(rocblas_dtrtri_strided_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, kNB, d_A_.p,
-                                              (rocblas_int)n, (rocblas_stride)(kNB * (n + 1)), d_uinv_.p, kNB,
-                                              (rocblas_stride)kNB * kNB, (rocblas_int)nfull));
+    } else {
+      prepare_blocked<double>(d_A_.p, n, d_linv_, d_uinv_);
     }
-    if (precision_ != ANDFN_MI_MIXED && nb > nfull) {
-      const int64_t r0 = nfull * kNB;
-      const rocblas_int rem = (rocblas_int)(n - r0);
-      double* blk = d_A_.p + r0 * (n + 1);
-      RB_CHECK(rocblas_dtrtri(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rem, blk, (rocblas_int)n,
-                              d_linv_.p + nfull * kNB * kNB, kNB));
-      RB_CHECK(rocblas_dtrtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, (rocblas_int)n,
-                              d_uinv_.p + nfull * kNB * kNB, kNB));
-    }
-    if (precision_ != ANDFN_MI_MIXED) aem::launch_unit_diag<double>(d_linv_.p, (int)nb, kNB, st_);
     d_w_.alloc(n);
     d_v_.alloc(n);
     HIP_CHECK(hipStreamSynchronize(st_));
@@ -790,6 +824,22 @@ void Solver::solve_rhs() {
 }
 
 // x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
+// v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten)
+template <class T>
+void Solver::blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v) {
+  const int nb = (n + kNB - 1) / kNB;
+  for (int j = 0; j < nb; ++j) {
+    const int r0 = j * kNB, bs = std::min(kNB, n - r0), rest = n - r0 - bs;
+    if (rest > 0) aem::launch_gemv_cm<T>(A + (size_t)r0 * n + r0 + bs, n, rest, bs, x + r0, x + r0 + bs, T(-1), T(1), st_);
+  }
+  aem::launch_gemv_diag<T>(linv, kNB, n, 1, x, w, st_);
+  for (int j = nb - 1; j > 0; --j) {
+    const int r0 = j * kNB, bs = std::min(kNB, n - r0);
+    aem::launch_gemv_cm<T>(A + (size_t)r0 * n, n, r0, bs, w + r0, w, T(-1), T(1), st_);
+  }
+  aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_);
+}
+
 double* Solver::discharge_solve(double* b, double* x) {
   const rocblas_int n = (rocblas_int)(de_.size() + F_);
   if (lu_mode_ == ANDFN_MI_LU_GETRS) {
@@ -807,44 +857,15 @@ double* Solver::discharge_solve(double* b, double* x) {
     // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
     aem::launch_refine_rhs(b, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    const int nb = (n + kNB - 1) / kNB;
-    float* x32 = d_x32_.p;
-    float* w32 = d_w32_.p;
+    blocked_solve<float>(d_A32_.p, n, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
     float* v32 = d_v32_.p;
-    for (int j = 0; j < nb; ++j) {
-      const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-      aem::launch_gemv_cm<float>(d_linv32_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x32 + r0, w32 + r0, 1.f, 0.f, st_);
-      const int rest = n - r0 - bs;
-      if (rest > 0)
-        aem::launch_gemv_cm<float>(d_A32_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w32 + r0, x32 + r0 + bs, -1.f, 1.f,
-                                   st_);
-    }
-    for (int j = nb - 1; j >= 0; --j) {
-      const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-      aem::launch_gemv_cm<float>(d_uinv32_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w32 + r0, v32 + r0, 1.f, 0.f, st_);
-      if (r0 > 0) aem::launch_gemv_cm<float>(d_A32_.p + (size_t)r0 * n, n, r0, bs, v32 + r0, w32, -1.f, 1.f, st_);
-    }
     aem::launch_refine_update(d_y_.p, v32, n, x, st_);
     return x;
   }
-  // R P b, then L z = P b and U x = z block by block: each block is one multiply by the inverse
-  // diagonal block and one gemv update of the remaining right-hand side (the factors are read once)
+  // R P b, then the blocked triangular solves
   aem::launch_permute(b, d_perm_.p, d_req_.p, n, x, st_);
-  const double one = 1.0, zero = 0.0, mone = -1.0;
-  const int nb = (n + kNB - 1) / kNB;
-  double* w = d_w_.p;
   double* v = d_v_.p;
-  for (int j = 0; j < nb; ++j) {
-    const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm<double>(d_linv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, x + r0, w + r0, one, zero, st_);
-    const int rest = n - r0 - bs;
-    if (rest > 0) aem::launch_gemv_cm<double>(d_A_.p + (size_t)r0 * n + r0 + bs, n, rest, bs, w + r0, x + r0 + bs, mone, one, st_);
-  }
-  for (int j = nb - 1; j >= 0; --j) {
-    const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm<double>(d_uinv_.p + (size_t)j * kNB * kNB, kNB, bs, bs, w + r0, v + r0, one, zero, st_);
-    if (r0 > 0) aem::launch_gemv_cm<double>(d_A_.p + (size_t)r0 * n, n, r0, bs, v + r0, w, mone, one, st_);
-  }
+  blocked_solve<double>(d_A_.p, n, d_linv_.p, d_uinv_.p, x, d_w_.p, v);
   HIP_CHECK(hipMemcpyAsync(x, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   return x;
 }
@@ -1130,7 +1151,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       ms_sweep += a;
       ms_fit += b;
     }
-    if (error_coef < max_error && error_q < max_error) {
+    // AnDFN's stop rule (hpc_solve.py:237-239); with a head tolerance the head-residual test
+    // at the top of the loop is the stop rule instead
+    if (head_tol <= 0.0 && error_coef < max_error && error_q < max_error) {
       ++cnt_error;
       error_q = 1e30;
     }

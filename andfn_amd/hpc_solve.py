@@ -34,8 +34,9 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
 
     ``growth_schedule`` (int32 [iterations, n_elements], optional) replays another run's
     coefficient counts; ``record_growth`` stores this run's counts per iteration in
-    ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 adds a stop as soon as the head
-    residual (max |A y - b(c)| over the head equations, metres) is at most ``head_tol``.
+    ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 replaces AnDFN's stop rule: the
+    solve stops as soon as the head residual (max |A y - b(c)| over the head equations, metres)
+    is at most ``head_tol`` (or at MAX_ITERATIONS).
     """
     global _last_info
     _lib.require_records(fracture_struc_array, element_struc_array)

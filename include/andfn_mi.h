@@ -61,8 +61,9 @@ typedef struct andfn_mi_options {
   /* Optional output: every element's ncoef after each iteration, record_iterations rows. */
   int32_t* growth_record;
   int64_t record_iterations;
-  /* >0: also stop as soon as the head residual (below) of the current iterate is <= head_tol
-   * metres; it is evaluated at the start of every iteration from the new right-hand side. */
+  /* >0: stop as soon as the head residual (below) of the current iterate is <= head_tol metres,
+   * instead of AnDFN's coefficient/discharge-change rule; it is evaluated at the start of every
+   * iteration from the new right-hand side.  MAX_ITERATIONS still bounds the solve. */
   double head_tol;
 } andfn_mi_options;
 
