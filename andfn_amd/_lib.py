@@ -22,6 +22,9 @@ ENODEV = -5
 FP64 = 0
 MIXED = 1
 
+ADJ_GMRES = 0
+ADJ_FIXED_POINT = 1
+
 LU_TRSV = 0
 LU_GETRS = 1
 LU_INVERSE = 2
@@ -75,6 +78,41 @@ class Info(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Objective(ctypes.Structure):
+    _fields_ = [
+        ("dq", ctypes.c_void_p),
+        ("dconstant", ctypes.c_void_p),
+        ("n_obs", ctypes.c_int64),
+        ("obs_frac", ctypes.c_void_p),
+        ("obs_z", ctypes.c_void_p),
+        ("dhead", ctypes.c_void_p),
+    ]
+
+
+class AdjointOptions(ctypes.Structure):
+    _fields_ = [
+        ("tol", ctypes.c_double),
+        ("max_iterations", ctypes.c_int64),
+        ("damping", ctypes.c_double),
+        ("verbose", ctypes.c_int32),
+        ("method", ctypes.c_int32),
+    ]
+
+
+class AdjointInfo(ctypes.Structure):
+    _fields_ = [
+        ("nit", ctypes.c_int64),
+        ("delta", ctypes.c_double),
+        ("objective", ctypes.c_double),
+        ("t_setup", ctypes.c_double),
+        ("t_iterate", ctypes.c_double),
+        ("t_total", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # every exported symbol and its signature
 SIGNATURES = {
     "andfn_mi_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
@@ -89,6 +127,12 @@ SIGNATURES = {
     "andfn_mi_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Options), ctypes.POINTER(Info)]),
     "andfn_mi_download": (ctypes.c_int, [ctypes.c_void_p]),
     "andfn_mi_destroy": (None, [ctypes.c_void_p]),
+    "andfn_mi_adjoint": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Objective), ctypes.POINTER(AdjointOptions),
+                                        ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(AdjointInfo)]),
+    "andfn_mi_set_parameters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "andfn_mi_state_heads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
+    "andfn_mi_state_discharges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "andfn_mi_device_count": (ctypes.c_int, []),
     "andfn_mi_last_error": (ctypes.c_char_p, []),
 }

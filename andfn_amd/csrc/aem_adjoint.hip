@@ -1,0 +1,508 @@
+// Transposed operators of the analytic-element fixed point, for the implicit adjoint.
+//
+// At a converged state the solve satisfies (coefficients c, discharges y = [q; C]):
+//   c = FIT(EVAL(c, y), t)          collocation sweep + Cauchy/DFT fit (one Jacobi step)
+//   A(t) y = RHS(c, t, h)           head equations with the series part on the right
+// For an objective J(c, y, t) the Lagrangian L = J + lam.(FIT(EVAL(c, y)) - c) + mu.(RHS(c) - A y)
+// is stationary in (c, y) when
+//   mu  = A^-T (J_y + EVAL_y^T FIT^T lam)
+//   lam = J_c + EVAL_c^T FIT^T lam + RHS_c^T mu,
+// which is the transpose of the forward iteration and converges at the same rate when damped the
+// same way.  dJ/dp = J_p + lam.FIT_p + mu.(RHS_p - A_p y) then gives every parameter gradient
+// from one adjoint solve.
+//
+// Kernels here:
+//   adj_fitT_kernel    FIT^T: dual weights at an element's collocation points from its
+//                      coefficient adjoint (the DFT transposed; circle unwrapping transposed into
+//                      suffix sums that feed the branch-cut discharge terms)
+//   adj_gather_kernel  EVAL^T: for each source element, the sum over every weighted target point of
+//                      its fracture(s) of the derivative of the point's value with respect to the
+//                      source's coefficients (powers of 1/chi or chi) and discharge (log chi):
+//                      one wavefront per source, lanes over targets, wave reductions at the end
+//   adj_cross_kernel   branch-cut terms of FIT^T into the discharge adjoint
+//   adj_csum_kernel    fracture-constant adjoint (sum of the Re weights of a fracture's points)
+//   adj_rhsT_kernel    RHS^T: dual weights at the discharge points from mu
+//   adj_update_kernel  damped update of lam with the change / magnitude maxima
+//   adj_tgrad_kernel   d c / d t of intersections contracted with lam (uses the forward values)
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "aem_kernels.hpp"
+#include "aem_math.hpp"
+
+namespace aem {
+
+namespace {
+
+__device__ __forceinline__ double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* p, double v) {
+  // non-negative doubles order like their bit patterns
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+}  // namespace
+
+constexpr int kAdjChunk = 20;  // coefficient adjoints accumulated per pass over the targets
+
+// FIT^T for one element per wavefront.  Writes the dual weight of every collocation point of the
+// element (Re weight for line types, Im weight for circle types) and, for circle types, the suffix
+// sums sfx[e][k] = sum_{i > k} w_i that the branch-cut correction kappa_k receives.
+__global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem* __restrict__ items, int nitems,
+                                                      const double2* __restrict__ lam, const int* __restrict__ pt_off,
+                                                      double2* __restrict__ w, double* __restrict__ sfx) {
+  __shared__ double sw[kMaxPts];
+  const int b = blockIdx.x;
+  if (b >= nitems) return;
+  const FitItem it = items[b];
+  const int e = it.elem;
+  const int lane = threadIdx.x;
+  const int ty = net.etype[e];
+  const int m = net.ncoef[e];
+  const int n = net.nint[e];
+  const double2* l = lam + (size_t)e * net.M;
+  const double two_n = 2.0 / (double)n;
+  const bool circle = (ty == 1 || ty == 4 || ty == 5);
+  double f0 = 1.0, f1 = 0.0;
+  if (ty == 0) {
+    // c_j = (t0 a1_j - t1 a0_j)/(t0 + t1): side 0 weighs -t1/(t0+t1), side 1 t0/(t0+t1)
+    const double t0 = net.t[net.efrac0[e]], t1 = net.t[net.efrac1[e]];
+    f0 = -t1 / (t0 + t1);
+    f1 = t0 / (t0 + t1);
+  } else if (ty == 3) {
+    f0 = -1.0;  // c_j = -a_j
+  }
+  const int p0 = pt_off[it.s0];
+  const int p1 = (ty == 0) ? pt_off[it.s1] : 0;
+  for (int i = lane; i < n; i += 64) {
+    double acc = 0.0;
+    if (ty == 1) {
+      // c_j = -(2/n) h_j (Y_j + i X_j), h_0 = 1/2, angles 2 pi i j / n
+      for (int j = 0; j < m; ++j) {
+        double s, c;
+        sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &s, &c);
+        const double h = (j == 0) ? 0.5 : 1.0;
+        acc = fma(h, fma(l[j].x, s, l[j].y * c), acc);
+      }
+      acc *= -two_n;
+    } else {
+      const int64_t period = 4 * (int64_t)n;
+      for (int j = 1; j < m; ++j) {
+        double s, c;
+        sincospi((double)(((int64_t)j * (2 * i + 1)) % period) / (double)(2 * n), &s, &c);
+        if (ty == 0 || ty == 3) acc = fma(l[j].x, c, acc);
+        else if (ty == 4) acc += l[j].x * s - l[j].y * c;  // c_j = ((2/n) Y_j, -(2/n) X_j)
+        else acc -= l[j].y * c;                              // c_j = (0, -(2/n) X_j)
+      }
+      acc *= two_n;
+    }
+    if (circle) {
+      sw[i] = acc;
+      w[p0 + i] = make_double2(0.0, acc);
+    } else {
+      w[p0 + i] = make_double2(f0 * acc, 0.0);
+      if (ty == 0) w[p1 + i] = make_double2(f1 * acc, 0.0);
+    }
+  }
+  if (circle) {
+    __syncthreads();
+    if (lane == 0) {
+      // psi~_i = psi_i - sum_{k < i} kappa_k  =>  dJ/dkappa_k = -sum_{i > k} w_i
+      double* o = sfx + (size_t)e * 2 * net.M;
+      double run = 0.0;
+      for (int k = n - 2; k >= 0; --k) {
+        run += sw[k + 1];
+        o[k] = run;
+      }
+    }
+  }
+}
+
+// EVAL^T for one source element per wavefront (items sorted heaviest first).
+//   eslot[eslot_ptr[e] .. eslot_ptr[e+1]) = (slot, fracture) of the element's source sides
+//   target points of fracture f: [fp_beg[f], fp_end[f]) with coordinates pz and owner pelem
+//   w: (Re, Im) dual weight of every target point
+//   excl: skip the source's own points (collocation); use_q: include the log terms
+//   accumulate: add into gc / gq instead of overwriting
+__global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* __restrict__ items, int nitems,
+                                                        const int* __restrict__ eslot_ptr,
+                                                        const int2* __restrict__ eslot, const int* __restrict__ fp_beg,
+                                                        const int* __restrict__ fp_end, const double2* __restrict__ pz,
+                                                        const int* __restrict__ pelem, const double2* __restrict__ w,
+                                                        int excl, int use_q, int accumulate, double2* __restrict__ gc,
+                                                        double* __restrict__ gq) {
+  const int b = blockIdx.x;
+  if (b >= nitems) return;
+  const int src = items[b];
+  const int lane = threadIdx.x;
+  const int ty = net.etype[src];
+  const int m = (ty == 2) ? 0 : net.ncoef[src];
+  const bool has_q = use_q && (ty == 0 || ty == 2 || ty == 3);
+  const int sb = eslot_ptr[src], se = eslot_ptr[src + 1];
+  double aq = 0.0;
+  int k0 = 0;
+  do {
+    const int kc = min(kAdjChunk, m - k0);
+    const bool do_q = has_q && k0 == 0;
+    double ar[kAdjChunk], ai[kAdjChunk];
+#pragma unroll
+    for (int k = 0; k < kAdjChunk; ++k) {
+      ar[k] = 0.0;
+      ai[k] = 0.0;
+    }
+    for (int s = sb; s < se; ++s) {
+      const int2 sf = eslot[s];
+      const Slot sl = net.slots[sf.x];
+      const int pb = fp_beg[sf.y], pe = fp_end[sf.y];
+      const bool line = (sl.kind == KIND_LINE || sl.kind == KIND_IMPL);
+      const double sg = line ? sl.sign : 1.0;
+      for (int p = pb + lane; p < pe; p += 64) {
+        if (excl && pelem[p] == src) continue;
+        const double2 wd = w[p];
+        if (wd.x == 0.0 && wd.y == 0.0) continue;
+        const double2 z = pz[p];
+        double2 base = make_double2(0.0, 0.0);
+        if (line) {
+          const double2 Z = make_double2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
+                                         fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
+          const double2 chi = joukowsky_exterior(Z);
+          const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+          const double inv = rcp_pos(r2);
+          base = make_double2(chi.x * inv, -chi.y * inv);  // 1/chi
+          if (do_q && sl.kind == KIND_LINE)
+            aq = fma(sl.sign * kInv2Pi, fma(wd.x, 0.5 * log(r2), wd.y * atan2(chi.y, chi.x)), aq);
+        } else if (sl.kind == KIND_BOUND) {
+          base = make_double2(sl.alpha.x * z.x, sl.alpha.x * z.y);  // z / R
+        } else {
+          const double2 chi = make_double2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
+          const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+          if (sl.kind == KIND_WELL) {
+            if (do_q) aq = fma(kInv2Pi, fma(wd.x, 0.5 * log(r2), wd.y * atan2(chi.y, chi.x)), aq);
+          } else {
+            const double inv = rcp_pos(r2);
+            base = make_double2(chi.x * inv, -chi.y * inv);
+          }
+        }
+        if (kc > 0) {
+          const double wr = sg * wd.x, wi = sg * wd.y;
+          double2 P = make_double2(1.0, 0.0);
+          for (int k = 0; k < k0; ++k) P = cmul(P, base);
+#pragma unroll
+          for (int k = 0; k < kAdjChunk; ++k) {
+            if (k < kc) {
+              // value = wr Re(c_k P_k) + wi Im(c_k P_k)
+              ar[k] = fma(wr, P.x, fma(wi, P.y, ar[k]));
+              ai[k] = fma(-wr, P.y, fma(wi, P.x, ai[k]));
+              P = cmul(P, base);
+            }
+          }
+        }
+      }
+    }
+    double2* o = gc + (size_t)src * net.M + k0;
+#pragma unroll
+    for (int k = 0; k < kAdjChunk; ++k) {
+      if (k < kc) {
+        const double a = wsum(ar[k]);
+        const double c = wsum(ai[k]);
+        if (lane == 0) {
+          if (accumulate) o[k] = make_double2(o[k].x + a, o[k].y + c);
+          else o[k] = make_double2(a, c);
+        }
+      }
+    }
+    k0 += kAdjChunk;
+  } while (k0 < m);
+  if (has_q) {
+    aq = wsum(aq);
+    if (lane == 0) gq[src] = accumulate ? gq[src] + aq : aq;
+  }
+}
+
+// branch-cut terms: gq[src] -= sign * sfx[e][pos] over the crossings (e, pos, sign) of src
+__global__ void adj_cross_kernel(int nsrc, const int* __restrict__ srcs, const int* __restrict__ cptr,
+                                 const int4* __restrict__ cent, const double* __restrict__ sfx, int M,
+                                 double* __restrict__ gq) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsrc) return;
+  const int src = srcs[k];
+  double acc = 0.0;
+  for (int c = cptr[k]; c < cptr[k + 1]; ++c) {
+    const int4 r = cent[c];  // (circle element, position, sign, -)
+    acc -= (double)r.z * sfx[(size_t)r.x * 2 * M + r.y];
+  }
+  gq[src] += acc;
+}
+
+// gC[f] (+)= sum of the Re weights of fracture f's points
+__global__ __launch_bounds__(64) void adj_csum_kernel(int nf, const int* __restrict__ fp_beg,
+                                                      const int* __restrict__ fp_end, const double2* __restrict__ w,
+                                                      int accumulate, double* __restrict__ gC) {
+  const int f = blockIdx.x;
+  if (f >= nf) return;
+  double s = 0.0;
+  for (int p = fp_beg[f] + (int)threadIdx.x; p < fp_end[f]; p += 64) s += w[p].x;
+  s = wsum(s);
+  if (threadIdx.x == 0) gC[f] = accumulate ? gC[f] + s : s;
+}
+
+// z = J_y + [gq of the discharge elements; gC]
+__global__ void adj_build_z_kernel(int nde, int nf, const int* __restrict__ de, const double* __restrict__ jy,
+                                   const double* __restrict__ gq, const double* __restrict__ gC, double* __restrict__ z) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < nde) z[j] = jy[j] + gq[de[j]];
+  else if (j < nde + nf) z[j] = jy[j] + gC[j - nde];
+}
+
+// RHS^T: b_j = S1/t1 - S0/t0 (intersection) or phi - S0, S = mean of the series over K points
+__global__ __launch_bounds__(64) void adj_rhsT_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
+                                                      const double* __restrict__ mu, const int* __restrict__ pt_off,
+                                                      int K, double2* __restrict__ w) {
+  const int j = blockIdx.x;
+  if (j >= nde) return;
+  const RhsItem it = items[j];
+  const int e = it.elem;
+  const double m = mu[j] / (double)K;
+  double v0, v1 = 0.0;
+  if (net.etype[e] == 0) {
+    v0 = -m / net.t[net.efrac0[e]];
+    v1 = m / net.t[net.efrac1[e]];
+  } else {
+    v0 = -m;
+  }
+  const int b0 = pt_off[it.s0];
+  for (int i = threadIdx.x; i < K; i += 64) w[b0 + i] = make_double2(v0, 0.0);
+  if (it.s1 >= 0) {
+    const int b1 = pt_off[it.s1];
+    for (int i = threadIdx.x; i < K; i += 64) w[b1 + i] = make_double2(v1, 0.0);
+  }
+}
+
+// lam += damping * (J_c + gc - lam); stats[0] = max |J_c + gc - lam|, stats[1] = max |J_c + gc|
+__global__ void adj_update_kernel(int64_t cnt, const double2* __restrict__ jc, const double2* __restrict__ gc,
+                                  double2* __restrict__ lam, double damping, unsigned long long* __restrict__ stats) {
+  double dmax = 0.0, vmax = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 t = make_double2(jc[i].x + gc[i].x, jc[i].y + gc[i].y);
+    const double2 l = lam[i];
+    const double2 d = make_double2(t.x - l.x, t.y - l.y);
+    lam[i] = make_double2(fma(damping, d.x, l.x), fma(damping, d.y, l.y));
+    dmax = fmax(dmax, fmax(fabs(d.x), fabs(d.y)));
+    vmax = fmax(vmax, fmax(fabs(t.x), fabs(t.y)));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&stats[0], dmax);
+    atomic_max_pos(&stats[1], vmax);
+  }
+}
+
+// s_e = sum_i (v0_i + v1_i) omega_i, omega_i = (2/n) sum_{j >= 1} lam_j cos(j theta_i), for the
+// intersections: lam . dc/dt0 = s_e t1/(t0+t1)^2 and lam . dc/dt1 = -s_e t0/(t0+t1)^2
+__global__ __launch_bounds__(64) void adj_tgrad_kernel(NetView net, const FitItem* __restrict__ items, int nitems,
+                                                       const double2* __restrict__ lam, const double* __restrict__ val,
+                                                       const int* __restrict__ pt_off, double* __restrict__ s_out) {
+  const int b = blockIdx.x;
+  if (b >= nitems) return;
+  const FitItem it = items[b];
+  const int e = it.elem;
+  const int m = net.ncoef[e];
+  const int n = net.nint[e];
+  const double2* l = lam + (size_t)e * net.M;
+  const int64_t period = 4 * (int64_t)n;
+  const int p0 = pt_off[it.s0], p1 = pt_off[it.s1];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 64) {
+    double om = 0.0;
+    for (int j = 1; j < m; ++j) {
+      double s, c;
+      sincospi((double)(((int64_t)j * (2 * i + 1)) % period) / (double)(2 * n), &s, &c);
+      om = fma(l[j].x, c, om);
+    }
+    acc = fma(val[p0 + i] + val[p1 + i], om, acc);
+  }
+  acc = wsum(acc) * (2.0 / (double)n);
+  if (threadIdx.x == 0) s_out[b] = acc;
+}
+
+// B = A^T for an n x n column-major matrix (32 x 32 tiles through LDS)
+template <class T>
+__global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ A, int64_t n, T* __restrict__ B) {
+  __shared__ T tile[32][33];
+  const int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = bx + tx, c = by + k;
+    if (r < n && c < n) tile[k][tx] = A[c * n + r];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int64_t r = by + tx, c = bx + k;  // B(r, c) = A(c, r)
+    if (r < n && c < n) B[c * n + r] = tile[tx][k];
+  }
+}
+
+// x = (T)(z - A^T mu)  (the refinement right-hand side of the transposed solve)
+template <class T>
+__global__ void adj_refine_in_kernel(const double* __restrict__ z, const double* __restrict__ aty, int n,
+                                     T* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (T)(z[i] - aty[i]);
+}
+
+// mu[perm[i]] += rs[perm[i]] * v[i]  (A^-T = R P^T L^-T U^-T); stats[2] = max |correction|,
+// stats[3] = max |mu|
+template <class T>
+__global__ void adj_refine_out_kernel(const T* __restrict__ v, const int* __restrict__ perm,
+                                      const double* __restrict__ rs, int n, double* __restrict__ mu,
+                                      unsigned long long* __restrict__ stats) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double d = 0.0, a = 0.0;
+  if (i < n) {
+    const int p = perm[i];
+    d = rs[p] * (double)v[i];
+    const double u = mu[p] + d;
+    mu[p] = u;
+    d = fabs(d);
+    a = fabs(u);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    d = fmax(d, __shfl_xor(d, o, 64));
+    a = fmax(a, __shfl_xor(a, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_pos(&stats[2], d);
+    atomic_max_pos(&stats[3], a);
+  }
+}
+
+// packed coefficient vectors (only k < ncoef of elements with coefficients): pidx maps a packed
+// entry to its [E][M] position
+__global__ void adj_pack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ full,
+                                double2* __restrict__ packed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) packed[i] = full[pidx[i]];
+}
+__global__ void adj_unpack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ packed,
+                                  double2* __restrict__ full) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) full[pidx[i]] = packed[i];
+}
+// out = v - packed(gc)
+__global__ void adj_residual_op_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ v,
+                                       const double2* __restrict__ gc, double2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) {
+    const double2 a = v[i], b = gc[pidx[i]];
+    out[i] = make_double2(a.x - b.x, a.y - b.y);
+  }
+}
+
+// ---- launchers
+void launch_adj_fitT(const NetView& net, const FitItem* items, int nitems, const double2* lam, const int* pt_off,
+                     double2* w, double* sfx, hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(adj_fitT_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, lam, pt_off, w, sfx);
+}
+
+void launch_adj_gather(const NetView& net, const int* items, int nitems, const int* eslot_ptr, const int2* eslot,
+                       const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem, const double2* w,
+                       int excl, int use_q, int accumulate, double2* gc, double* gq, hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(adj_gather_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, eslot_ptr, eslot, fp_beg,
+                     fp_end, pz, pelem, w, excl, use_q, accumulate, gc, gq);
+}
+
+void launch_adj_cross(int nsrc, const int* srcs, const int* cptr, const int4* cent, const double* sfx, int M,
+                      double* gq, hipStream_t st) {
+  if (nsrc <= 0) return;
+  hipLaunchKernelGGL(adj_cross_kernel, dim3((nsrc + 255) / 256), dim3(256), 0, st, nsrc, srcs, cptr, cent, sfx, M, gq);
+}
+
+void launch_adj_csum(int nf, const int* fp_beg, const int* fp_end, const double2* w, int accumulate, double* gC,
+                     hipStream_t st) {
+  if (nf <= 0) return;
+  hipLaunchKernelGGL(adj_csum_kernel, dim3(nf), dim3(64), 0, st, nf, fp_beg, fp_end, w, accumulate, gC);
+}
+
+void launch_adj_build_z(int nde, int nf, const int* de, const double* jy, const double* gq, const double* gC,
+                        double* z, hipStream_t st) {
+  const int n = nde + nf;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(adj_build_z_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nde, nf, de, jy, gq, gC, z);
+}
+
+void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const double* mu, const int* pt_off, int K,
+                     double2* w, hipStream_t st) {
+  if (nde <= 0) return;
+  hipLaunchKernelGGL(adj_rhsT_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, mu, pt_off, K, w);
+}
+
+void launch_adj_update(int64_t cnt, const double2* jc, const double2* gc, double2* lam, double damping,
+                       unsigned long long* stats, hipStream_t st) {
+  if (cnt <= 0) return;
+  int64_t nb = (cnt + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(adj_update_kernel, dim3((unsigned)nb), dim3(256), 0, st, cnt, jc, gc, lam, damping, stats);
+}
+
+void launch_adj_tgrad(const NetView& net, const FitItem* items, int nitems, const double2* lam, const double* val,
+                      const int* pt_off, double* s_out, hipStream_t st) {
+  if (nitems <= 0) return;
+  hipLaunchKernelGGL(adj_tgrad_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, lam, val, pt_off, s_out);
+}
+
+void launch_adj_pack(int64_t np, const int64_t* pidx, const double2* full, double2* packed, hipStream_t st) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(adj_pack_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, full, packed);
+}
+void launch_adj_unpack(int64_t np, const int64_t* pidx, const double2* packed, double2* full, hipStream_t st) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(adj_unpack_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, packed, full);
+}
+void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, const double2* gc, double2* out,
+                            hipStream_t st) {
+  if (np <= 0) return;
+  hipLaunchKernelGGL(adj_residual_op_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, v, gc,
+                     out);
+}
+
+template <class T>
+void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st) {
+  if (n <= 0) return;
+  const unsigned g = (unsigned)((n + 31) / 32);
+  hipLaunchKernelGGL(transpose_kernel<T>, dim3(g, g), dim3(256), 0, st, A, n, B);
+}
+template void launch_transpose<double>(const double*, int64_t, double*, hipStream_t);
+template void launch_transpose<float>(const float*, int64_t, float*, hipStream_t);
+
+template <class T>
+void launch_adj_refine_in(const double* z, const double* aty, int n, T* x, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(adj_refine_in_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, z, aty, n, x);
+}
+template void launch_adj_refine_in<double>(const double*, const double*, int, double*, hipStream_t);
+template void launch_adj_refine_in<float>(const double*, const double*, int, float*, hipStream_t);
+
+template <class T>
+void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, unsigned long long* stats,
+                           hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(adj_refine_out_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, v, perm, rs, n, mu, stats);
+}
+template void launch_adj_refine_out<double>(const double*, const int*, const double*, int, double*,
+                                            unsigned long long*, hipStream_t);
+template void launch_adj_refine_out<float>(const float*, const int*, const double*, int, double*, unsigned long long*,
+                                           hipStream_t);
+
+}  // namespace aem

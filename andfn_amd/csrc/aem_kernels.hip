@@ -239,6 +239,26 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
   }
 }
 
+// coordinates and owner element of every point of a target set (used by the adjoint gathers)
+__global__ __launch_bounds__(64) void points_kernel(NetView net, TargetView tg, int K, int colloc,
+                                                    double2* __restrict__ pz, int* __restrict__ pelem) {
+  const Tile tile = tg.tiles[blockIdx.x];
+  const int p = tile.p0 + (int)threadIdx.x;
+  if (p >= tile.pend) return;
+  const int s = find_side(tg.pt_off, tile.sbeg, tile.send, p);
+  const int e = tg.side_elem[s];
+  const int which = tg.side_which[s];
+  const int i = p - tg.pt_off[s];
+  pz[p] = colloc ? colloc_point(net, e, which, i, net.nint[e]) : discharge_point(net, e, which, i, K);
+  pelem[p] = e;
+}
+
+void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, int colloc, double2* pz, int* pelem,
+                   hipStream_t st) {
+  if (ntiles <= 0) return;
+  hipLaunchKernelGGL(points_kernel, dim3(ntiles), dim3(64), 0, st, net, tg, K, colloc, pz, pelem);
+}
+
 // --------------------------------------------------------------------------------------
 // Fit: one wavefront per element (hpc_math_functions.py:243-297, 537-670 and the element solves)
 // --------------------------------------------------------------------------------------

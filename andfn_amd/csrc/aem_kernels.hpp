@@ -150,4 +150,37 @@ void launch_bc_residual(const NetView& net, const FitItem* items, int nitems, co
                         const RhsItem* witems, int nw, const double* wval, const int* wpt_off, int K, double* out_max,
                         hipStream_t st);
 
+
+// ---- implicit adjoint (aem_adjoint.hip) and its point tables
+void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, int colloc, double2* pz, int* pelem,
+                   hipStream_t st);
+void launch_adj_fitT(const NetView& net, const FitItem* items, int nitems, const double2* lam, const int* pt_off,
+                     double2* w, double* sfx, hipStream_t st);
+void launch_adj_gather(const NetView& net, const int* items, int nitems, const int* eslot_ptr, const int2* eslot,
+                       const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem, const double2* w,
+                       int excl, int use_q, int accumulate, double2* gc, double* gq, hipStream_t st);
+void launch_adj_cross(int nsrc, const int* srcs, const int* cptr, const int4* cent, const double* sfx, int M,
+                      double* gq, hipStream_t st);
+void launch_adj_csum(int nf, const int* fp_beg, const int* fp_end, const double2* w, int accumulate, double* gC,
+                     hipStream_t st);
+void launch_adj_build_z(int nde, int nf, const int* de, const double* jy, const double* gq, const double* gC,
+                        double* z, hipStream_t st);
+void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const double* mu, const int* pt_off, int K,
+                     double2* w, hipStream_t st);
+void launch_adj_update(int64_t cnt, const double2* jc, const double2* gc, double2* lam, double damping,
+                       unsigned long long* stats, hipStream_t st);
+void launch_adj_tgrad(const NetView& net, const FitItem* items, int nitems, const double2* lam, const double* val,
+                      const int* pt_off, double* s_out, hipStream_t st);
+void launch_adj_pack(int64_t np, const int64_t* pidx, const double2* full, double2* packed, hipStream_t st);
+void launch_adj_unpack(int64_t np, const int64_t* pidx, const double2* packed, double2* full, hipStream_t st);
+void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, const double2* gc, double2* out,
+                            hipStream_t st);
+template <class T>
+void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st);
+template <class T>
+void launch_adj_refine_in(const double* z, const double* aty, int n, T* x, hipStream_t st);
+template <class T>
+void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, unsigned long long* stats,
+                           hipStream_t st);
+
 }  // namespace aem

@@ -156,6 +156,7 @@ double pair_flops(int kind, int m, bool with_q) {
 
 struct TargetSet {
   std::vector<int> side_elem, side_which, pt_off;
+  std::vector<int> fsb;  // [F + 1]: sides of fracture f are [fsb[f], fsb[f + 1])
   std::vector<Tile> tiles;
   std::vector<int> frac_nslots;  // sources per tile's fracture, for work accounting
   DBuf<int> d_side_elem, d_side_which, d_pt_off;
@@ -239,7 +240,7 @@ class Solver {
   static constexpr int kNB = 1024;
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   template <class T>
-  void prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv);
+  void prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
   template <class T>
   void blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v);
   // mixed precision: single-precision factors and blocked-solve buffers
@@ -332,6 +333,42 @@ class Solver {
   }
   double head_residual(const double* b, int* row);
   double bc_residual();
+
+  // ---- implicit adjoint
+  struct AdjTables {
+    DBuf<double2> pz;  // point coordinates
+    DBuf<int> pelem;   // element owning the point (-1: observation point)
+    DBuf<int> fpb, fpe;  // [F] point range of every fracture
+  };
+  bool have_state_ = false, adjT_valid_ = false;
+  std::vector<double> head_;  // prescribed head of constant-head elements (phi / t)
+  AdjTables tab_full_, tab_rhs_, tab_obs_;
+  DBuf<int> d_eslot_ptr_, d_gitems_, d_xsrc_, d_xptr_;
+  DBuf<int2> d_eslot_;
+  DBuf<int4> d_xent_;
+  DBuf<FitItem> d_titems_;
+  DBuf<double2> d_lam_, d_jc_, d_gc_, d_wfull_, d_wrhs_, d_wobs_;
+  DBuf<double> d_gq_, d_gC_, d_jy0_, d_jy_, d_z_, d_mu_, d_aty_, d_sfx_, d_ts_, d_hw_;
+  DBuf<unsigned long long> d_stats_;
+  DBuf<double> d_jzero_, d_hv_;
+  DBuf<int64_t> d_pidx_;
+  DBuf<double2> d_gv_, d_xv_, d_wv_;
+  DBuf<double> d_V_;
+  DBuf<double> d_AT_, d_linvT_, d_uinvT_, d_csrT_val_;
+  DBuf<float> d_AT32_, d_linvT32_, d_uinvT32_;
+  DBuf<int> d_csrT_ptr_, d_csrT_col_;
+  void adj_tables(const TargetSet& ts, bool colloc, AdjTables& t);
+  template <class T>
+  void make_transposed(const T* A, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT);
+  void make_csr_transpose();
+  template <class T>
+  void adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v);
+
+ public:
+  int adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_options* opt, double* grad_t, double* grad_head,
+              andfn_mi_adjoint_info* info);
+  void set_parameters(const double* t, const double* head);
+  void state_discharges(double* q, double* constant);
 };
 
 void Solver::parse() {
@@ -419,6 +456,9 @@ void Solver::parse() {
     }
     mem_ptr_[f + 1] = (int)slots_.size();
   }
+  head_.assign(E_, 0.0);
+  for (int e = 0; e < E_; ++e)
+    if (etype_[e] == 2 || etype_[e] == 3) head_[e] = phi_[e] / t_[efrac0_[e]];
   de_.clear();
   de_pos_.assign(E_, -1);
   wells_.clear();
@@ -540,8 +580,10 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
   ts.pt_off.assign(1, 0);
   ts.tiles.clear();
   ts.frac_nslots.clear();
+  ts.fsb.assign(F_ + 1, 0);
   for (int f = 0; f < F_; ++f) {
     const int sbeg = (int)ts.side_elem.size();
+    ts.fsb[f] = sbeg;
     const int pbeg = ts.pt_off.back();
     for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
       const int e = slots_[k].elem;
@@ -559,6 +601,7 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
       ts.frac_nslots.push_back(mem_ptr_[f + 1] - mem_ptr_[f]);
     }
   }
+  ts.fsb[F_] = (int)ts.side_elem.size();
   // heaviest fractures first (work of a tile ~ number of sources of its fracture)
   std::vector<int> order(ts.tiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
@@ -652,41 +695,45 @@ static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_di
 // "c[below j] -= G_j c_j for j ascending, then z_j = Linv_j c_j for all j at once" (and U likewise,
 // descending): the sequential chain is one gemv per block and the diagonal blocks are applied by
 // a single batched launch.
+// lower_unit: L has the unit diagonal (LU factors); false for the transposed factors U^T L^T,
+// whose lower triangle carries the diagonal and whose upper triangle is unit.
 template <class T>
-void Solver::prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv) {
+void Solver::prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
   const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
+  const rocblas_diagonal dl = lower_unit ? rocblas_diagonal_unit : rocblas_diagonal_non_unit;
+  const rocblas_diagonal du = lower_unit ? rocblas_diagonal_non_unit : rocblas_diagonal_unit;
   linv.zero((size_t)nb * kNB * kNB, st_);
   uinv.zero((size_t)nb * kNB * kNB, st_);
   if (nfull > 0) {
-    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, kNB, A, n, kNB * (n + 1), linv.p, kNB,
-                              (int64_t)kNB * kNB, (int)nfull));
-    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, kNB, A, n, kNB * (n + 1), uinv.p,
-                              kNB, (int64_t)kNB * kNB, (int)nfull));
+    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, A, n, kNB * (n + 1), linv.p, kNB, (int64_t)kNB * kNB,
+                              (int)nfull));
+    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, kNB, A, n, kNB * (n + 1), uinv.p, kNB, (int64_t)kNB * kNB,
+                              (int)nfull));
   }
   if (nb > nfull) {
     const int64_t r0 = nfull * kNB;
     const int rem = (int)(n - r0);
     const T* blk = A + r0 * (n + 1);
-    RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rem, blk, n, linv.p + nfull * kNB * kNB, kNB));
-    RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, rem, blk, n, uinv.p + nfull * kNB * kNB,
-                      kNB));
+    RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, dl, rem, blk, n, linv.p + nfull * kNB * kNB, kNB));
+    RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, du, rem, blk, n, uinv.p + nfull * kNB * kNB, kNB));
   }
-  aem::launch_unit_diag<T>(linv.p, (int)nb, kNB, st_);
+  // a unit-triangular inverse is written without its diagonal
+  aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)nb, kNB, st_);
   for (int64_t j = 0; j < nb; ++j) {
     const int64_t r0 = j * kNB;
     const int bs = (int)std::min<int64_t>(kNB, n - r0);
     const int rest = (int)(n - r0 - bs);
     if (rest > 0)
-      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, linv.p + j * kNB * kNB, kNB,
-                             A + r0 * n + r0 + bs, n));
+      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + j * kNB * kNB, kNB, A + r0 * n + r0 + bs,
+                             n));
     if (r0 > 0)
-      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, uinv.p + j * kNB * kNB,
-                             kNB, A + r0 * n, n));
+      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + j * kNB * kNB, kNB, A + r0 * n, n));
   }
 }
 
 void Solver::assemble_and_factor() {
   drop_graph();
+  adjT_valid_ = false;
   const int nde = (int)de_.size();
   const int64_t n = nde + F_;
   d_A_.zero((size_t)(n * n), st_);
@@ -955,6 +1002,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     if (etype_[e] != 2 && is_circle_like(etype_[e]) && radius_[e] <= 0)
       fail(ANDFN_MI_EINVAL, "element %d: circle radius must be positive", e);
 
+  have_state_ = false;
   restore();
   const int nde = (int)de_.size();
   const int n = nde + F_;
@@ -1201,6 +1249,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->nit_fp32 = precision_ == ANDFN_MI_MIXED ? (mixed_phase_ ? nit : nit_fp32) : 0;
   }
   mixed_phase_ = false;
+  have_state_ = true;
   return (cnt_error < 2 && !stopped_on_head) ? ANDFN_MI_NOT_CONVERGED : ANDFN_MI_OK;
 }
 
@@ -1281,6 +1330,520 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
   for (int64_t k = 0; k < npts; ++k) out[order[k]] = v[k];
 }
 
+// ---------------------------------------------------------------------------------------------
+// Implicit adjoint
+// ---------------------------------------------------------------------------------------------
+
+void Solver::adj_tables(const TargetSet& ts, bool colloc, AdjTables& t) {
+  const int64_t np = ts.npts();
+  t.pz.alloc(std::max<int64_t>(np, 1));
+  t.pelem.alloc(std::max<int64_t>(np, 1));
+  aem::launch_points(net(), ts.view(), (int)ts.tiles.size(), K_, colloc ? 1 : 0, t.pz.p, t.pelem.p, st_);
+  std::vector<int> b(F_), e(F_);
+  for (int f = 0; f < F_; ++f) {
+    b[f] = ts.pt_off[ts.fsb[f]];
+    e[f] = ts.pt_off[ts.fsb[f + 1]];
+  }
+  t.fpb.upload(b, st_);
+  t.fpe.upload(e, st_);
+}
+
+// Transposed factors for A^-T = R P^T L^-T U^-T: the LU factors with the panel products undone
+// (L[below j, j] = G_j L_jj, U[above j, j] = H_j U_jj), transposed, and prepared for the same
+// blocked solves with a non-unit lower (U^T) and a unit upper (L^T) triangle.
+template <class T>
+void Solver::make_transposed(const T* A, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT) {
+  const int64_t n = (int64_t)de_.size() + F_;
+  const int64_t nb = (n + kNB - 1) / kNB;
+  {
+    DBuf<T> tmp;
+    tmp.alloc((size_t)(n * n));
+    HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * (size_t)(n * n), hipMemcpyDeviceToDevice, st_));
+    for (int64_t j = 0; j < nb; ++j) {
+      const int64_t r0 = j * kNB;
+      const int bs = (int)std::min<int64_t>(kNB, n - r0);
+      const int rest = (int)(n - r0 - bs);
+      const T* djj = tmp.p + r0 * (n + 1);
+      if (rest > 0)
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n,
+                               tmp.p + r0 * n + r0 + bs, n));
+      if (r0 > 0)
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
+                               tmp.p + r0 * n, n));
+    }
+    AT.alloc((size_t)(n * n));
+    aem::launch_transpose<T>(tmp.p, n, AT.p, st_);
+    HIP_CHECK(hipStreamSynchronize(st_));
+  }
+  prepare_blocked<T>(AT.p, n, linvT, uinvT, false);
+  HIP_CHECK(hipStreamSynchronize(st_));
+}
+
+// CSR of A^T (for the FP64 residual z - A^T mu of the transposed refinement)
+void Solver::make_csr_transpose() {
+  const int n = (int)(de_.size() + F_);
+  std::vector<int> rp(n + 1), col(nnz_A_);
+  std::vector<double> val(nnz_A_);
+  HIP_CHECK(hipMemcpyAsync(rp.data(), d_csr_ptr_.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(col.data(), d_csr_col_.p, sizeof(int) * nnz_A_, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(val.data(), d_csr_val_.p, sizeof(double) * nnz_A_, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipStreamSynchronize(st_));
+  std::vector<int> tp(n + 1, 0), tc(nnz_A_);
+  std::vector<double> tv(nnz_A_);
+  for (int64_t k = 0; k < nnz_A_; ++k) ++tp[col[k] + 1];
+  for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+  std::vector<int> fill(tp.begin(), tp.end() - 1);
+  for (int r = 0; r < n; ++r)
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int d = fill[col[k]]++;
+      tc[d] = r;
+      tv[d] = val[k];
+    }
+  d_csrT_ptr_.upload(tp, st_);
+  d_csrT_col_.upload(tc, st_);
+  d_csrT_val_.upload(tv, st_);
+}
+
+// mu += A^-T (z - A^T mu) with the transposed factors in precision T
+template <class T>
+void Solver::adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v) {
+  const int n = (int)(de_.size() + F_);
+  aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
+  aem::launch_adj_refine_in<T>(d_z_.p, d_aty_.p, n, x, st_);
+  blocked_solve<T>(AT, n, linvT, uinvT, x, w, v);
+  aem::launch_adj_refine_out<T>(v, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
+}
+
+int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_options* opt, double* grad_t,
+                    double* grad_head, andfn_mi_adjoint_info* info) {
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  if (!have_state_) fail(ANDFN_MI_EINVAL, "no solved state on this handle: run a solve first");
+  if (lu_mode_ != ANDFN_MI_LU_TRSV) fail(ANDFN_MI_EINVAL, "the adjoint uses the blocked triangular solves (lu_mode 0)");
+  if (!grad_t || !grad_head) fail(ANDFN_MI_EINVAL, "null gradient pointer");
+  const double tol = (opt && opt->tol > 0) ? opt->tol : 1e-10;
+  const int64_t maxit = (opt && opt->max_iterations > 0) ? opt->max_iterations : 5000;
+  const double damp = (opt && opt->damping > 0) ? opt->damping : 0.75;
+  if (damp > 1.0) fail(ANDFN_MI_EINVAL, "damping %g outside (0, 1]", damp);
+  const int verbose = opt ? opt->verbose : 0;
+  const int64_t nobs = obj ? obj->n_obs : 0;
+  if (nobs < 0 || (nobs > 0 && (!obj->obs_frac || !obj->obs_z || !obj->dhead)))
+    fail(ANDFN_MI_EINVAL, "observation arrays missing");
+  for (int64_t k = 0; k < nobs; ++k)
+    if (obj->obs_frac[k] < 0 || obj->obs_frac[k] >= F_)
+      fail(ANDFN_MI_EINVAL, "observation %lld: fracture %lld out of range", (long long)k, (long long)obj->obs_frac[k]);
+  const int nde = (int)de_.size(), n = nde + F_;
+  const size_t ncf = (size_t)E_ * L_.M;
+  aem::NetView nv = net();
+
+  // point tables of the collocation and discharge point sets at the converged counts
+  adj_tables(full_, true, tab_full_);
+  adj_tables(rhs_, false, tab_rhs_);
+  // source sides of every element as (slot, fracture)
+  std::vector<int> eptr(E_ + 1, 0);
+  for (const Slot& sl : slots_) ++eptr[sl.elem + 1];
+  for (int e = 0; e < E_; ++e) eptr[e + 1] += eptr[e];
+  std::vector<int2> esl(std::max<size_t>(slots_.size(), 1));
+  {
+    std::vector<int> fill(eptr.begin(), eptr.end() - 1);
+    for (int f = 0; f < F_; ++f)
+      for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) esl[fill[slots_[k].elem]++] = make_int2(k, f);
+  }
+  d_eslot_ptr_.upload(eptr, st_);
+  d_eslot_.upload(esl, st_);
+  // gather order: heaviest sources (most target points on their fractures) first
+  std::vector<int64_t> work(E_, 0);
+  for (int e = 0; e < E_; ++e)
+    for (int k = eptr[e]; k < eptr[e + 1]; ++k) {
+      const int f = esl[k].y;
+      work[e] += full_.pt_off[full_.fsb[f + 1]] - full_.pt_off[full_.fsb[f]];
+    }
+  std::vector<int> gitems(E_);
+  for (int e = 0; e < E_; ++e) gitems[e] = e;
+  std::stable_sort(gitems.begin(), gitems.end(), [&](int a, int b) { return work[a] > work[b]; });
+  d_gitems_.upload(gitems, st_);
+  // branch-cut crossings regrouped by crossing source: (circle element, position, sign)
+  int nxsrc = 0;
+  {
+    std::vector<int> ccnt(E_);
+    std::vector<int2> cr(std::max(cross_total_, 1));
+    std::vector<int> cs(std::max(cross_total_, 1));
+    HIP_CHECK(hipMemcpyAsync(ccnt.data(), d_cross_cnt_.p, sizeof(int) * E_, hipMemcpyDeviceToHost, st_));
+    if (cross_total_ > 0) {
+      HIP_CHECK(hipMemcpyAsync(cr.data(), d_cross_.p, sizeof(int2) * cross_total_, hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipMemcpyAsync(cs.data(), d_cross_sign_.p, sizeof(int) * cross_total_, hipMemcpyDeviceToHost, st_));
+    }
+    HIP_CHECK(hipStreamSynchronize(st_));
+    std::vector<std::vector<int4>> by(E_);
+    for (int e : circles_)
+      for (int c = 0; c < ccnt[e]; ++c) {
+        const int2 r = cr[cross_ptr_[e] + c];
+        by[r.y].push_back(make_int4(e, r.x, cs[cross_ptr_[e] + c], 0));
+      }
+    std::vector<int> xsrc, xptr(1, 0);
+    std::vector<int4> xent;
+    for (int e = 0; e < E_; ++e)
+      if (!by[e].empty()) {
+        xsrc.push_back(e);
+        xent.insert(xent.end(), by[e].begin(), by[e].end());
+        xptr.push_back((int)xent.size());
+      }
+    d_xsrc_.upload(xsrc, st_);
+    d_xptr_.upload(xptr, st_);
+    d_xent_.upload(xent, st_);
+    nxsrc = (int)xsrc.size();
+  }
+  // transposed factors, once per factorisation
+  if (!adjT_valid_) {
+    if (precision_ == ANDFN_MI_MIXED) make_transposed<float>(d_A32_.p, d_AT32_, d_linvT32_, d_uinvT32_);
+    else make_transposed<double>(d_A_.p, d_AT_, d_linvT_, d_uinvT_);
+    make_csr_transpose();
+    adjT_valid_ = true;
+  }
+  if (precision_ == ANDFN_MI_MIXED) {
+    d_x32_.alloc(n);
+    d_w32_.alloc(n);
+    d_v32_.alloc(n);
+  } else {
+    d_x_.alloc(n);
+    d_w_.alloc(n);
+    d_v_.alloc(n);
+  }
+
+  // objective: J_c and J_y (discharge / constant weights plus the observation-head terms)
+  std::vector<double> jy0(n, 0.0);
+  if (obj && obj->dq)
+    for (int j = 0; j < nde; ++j) jy0[j] = obj->dq[de_[j]];
+  if (obj && obj->dconstant)
+    for (int f = 0; f < F_; ++f) jy0[nde + f] = obj->dconstant[f];
+  d_gc_.zero(ncf, st_);
+  d_gq_.zero(E_, st_);
+  d_gC_.zero(F_, st_);
+  std::vector<int64_t> oorder(nobs);
+  if (nobs > 0) {
+    for (int64_t k = 0; k < nobs; ++k) oorder[k] = k;
+    std::stable_sort(oorder.begin(), oorder.end(),
+                     [&](int64_t a, int64_t b) { return obj->obs_frac[a] < obj->obs_frac[b]; });
+    std::vector<double2> pz(nobs), wv(nobs);
+    std::vector<int> pe(nobs, -1), fb(F_, 0), fe(F_, 0);
+    for (int64_t i = 0; i < nobs; ++i) {
+      const int64_t k = oorder[i];
+      const int f = (int)obj->obs_frac[k];
+      pz[i] = make_double2(obj->obs_z[2 * k], obj->obs_z[2 * k + 1]);
+      wv[i] = make_double2(obj->dhead[k] / t_[f], 0.0);  // h = (Re Omega + C) / t
+    }
+    for (int64_t i = 0; i < nobs;) {
+      const int f = (int)obj->obs_frac[oorder[i]];
+      fb[f] = (int)i;
+      while (i < nobs && obj->obs_frac[oorder[i]] == f) ++i;
+      fe[f] = (int)i;
+    }
+    tab_obs_.pz.upload(pz, st_);
+    tab_obs_.pelem.upload(pe, st_);
+    tab_obs_.fpb.upload(fb, st_);
+    tab_obs_.fpe.upload(fe, st_);
+    d_wobs_.upload(wv, st_);
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_obs_.fpb.p, tab_obs_.fpe.p,
+                           tab_obs_.pz.p, tab_obs_.pelem.p, d_wobs_.p, 0, 1, 0, d_gc_.p, d_gq_.p, st_);
+    aem::launch_adj_csum(F_, tab_obs_.fpb.p, tab_obs_.fpe.p, d_wobs_.p, 0, d_gC_.p, st_);
+  }
+  d_jc_.alloc(ncf);
+  HIP_CHECK(hipMemcpyAsync(d_jc_.p, d_gc_.p, sizeof(double2) * ncf, hipMemcpyDeviceToDevice, st_));
+  d_jy0_.upload(jy0, st_);
+  d_jy_.alloc(n);
+  aem::launch_adj_build_z(nde, F_, d_de_.p, d_jy0_.p, d_gq_.p, d_gC_.p, d_jy_.p, st_);
+
+  // adjoint solve
+  const int method = opt ? opt->method : ANDFN_MI_ADJ_GMRES;
+  if (method != ANDFN_MI_ADJ_GMRES && method != ANDFN_MI_ADJ_FIXED_POINT) fail(ANDFN_MI_EINVAL, "adjoint method %d unknown", method);
+  d_lam_.zero(ncf, st_);
+  d_mu_.zero(n, st_);
+  d_z_.alloc(n);
+  d_aty_.alloc(n);
+  d_gc_.zero(ncf, st_);
+  d_jzero_.zero(n, st_);
+  d_wfull_.zero(std::max<int64_t>(full_.npts(), 1), st_);
+  d_wrhs_.zero(std::max<int64_t>(rhs_.npts(), 1), st_);
+  d_sfx_.zero((size_t)E_ * 2 * L_.M, st_);
+  d_stats_.zero(4, st_);
+  // A^-T with the run's factors: one FP64 step, or FP32 steps refined against the FP64 matrix
+  const int nref = precision_ == ANDFN_MI_MIXED ? 3 : 1;
+  // d_gc_ = B lam (+ RHS_c^T A^-T J_y when with_j) and d_mu_ = A^-T (S_y^T lam (+ J_y)), where B is
+  // the fixed-point map's linear part on the coefficients: FIT^T, EVAL^T at the collocation
+  // points (coefficients, discharges, constants, branch cuts), A^-T, RHS^T
+  auto apply = [&](bool with_j) {
+    HIP_CHECK(hipMemsetAsync(d_gc_.p, 0, sizeof(double2) * ncf, st_));
+    aem::launch_adj_fitT(nv, d_items_.p, (int)items_.size(), d_lam_.p, full_.d_pt_off.p, d_wfull_.p, d_sfx_.p, st_);
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_full_.fpb.p, tab_full_.fpe.p,
+                           tab_full_.pz.p, tab_full_.pelem.p, d_wfull_.p, 1, 1, 0, d_gc_.p, d_gq_.p, st_);
+    aem::launch_adj_cross(nxsrc, d_xsrc_.p, d_xptr_.p, d_xent_.p, d_sfx_.p, (int)L_.M, d_gq_.p, st_);
+    aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
+    aem::launch_adj_build_z(nde, F_, d_de_.p, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
+    HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
+    for (int r = 0; r < nref; ++r) {
+      if (precision_ == ANDFN_MI_MIXED)
+        adj_mu_step<float>(d_AT32_.p, d_linvT32_.p, d_uinvT32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+      else
+        adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
+    }
+    aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
+                           tab_rhs_.pz.p, tab_rhs_.pelem.p, d_wrhs_.p, 0, 0, 1, d_gc_.p, d_gq_.p, st_);
+  };
+  auto read_stats = [&](double* v) {
+    unsigned long long st4[4];
+    HIP_CHECK(hipMemcpyAsync(st4, d_stats_.p, sizeof(st4), hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    for (int k = 0; k < 4; ++k) std::memcpy(&v[k], &st4[k], sizeof(double));
+  };
+  HIP_CHECK(hipStreamSynchronize(st_));
+  HIP_CHECK(hipGetLastError());
+  auto t_set = clk::now();
+  int64_t nit = 0;
+  double delta = 1.0;
+  bool conv = false;
+  if (method == ANDFN_MI_ADJ_FIXED_POINT) {
+    while (nit < maxit) {
+      ++nit;
+      apply(true);
+      HIP_CHECK(hipMemsetAsync(d_stats_.p, 0, sizeof(unsigned long long) * 4, st_));
+      aem::launch_adj_update((int64_t)ncf, d_jc_.p, d_gc_.p, d_lam_.p, damp, d_stats_.p, st_);
+      double v[4];
+      read_stats(v);
+      delta = v[1] > 0 ? v[0] / v[1] : 0.0;
+      if (verbose && (nit % 50 == 0 || nit == 1))
+        fprintf(stderr, "adjoint iteration %lld: relative change %.3e\n", (long long)nit, delta);
+      if (!(delta == delta)) fail(ANDFN_MI_EHIP, "adjoint iteration produced NaN at iteration %lld", (long long)nit);
+      if (nit >= 3 && delta <= tol) {
+        conv = true;
+        break;
+      }
+    }
+  } else {
+    // restarted GMRES on (I - B) lam = g, g = J_c + RHS_c^T A^-T J_y, in the packed space of the
+    // active coefficients (k < ncoef of every element with coefficients)
+    std::vector<int64_t> pidx;
+    for (int e = 0; e < E_; ++e)
+      if (etype_[e] != 2)
+        for (int k = 0; k < ncoef_[e]; ++k) pidx.push_back((int64_t)e * L_.M + k);
+    const int64_t np = (int64_t)pidx.size();
+    const rocblas_int N = (rocblas_int)(2 * np);
+    const int restart = 40;
+    d_pidx_.upload(pidx, st_);
+    d_gv_.alloc(np);
+    d_xv_.zero(np, st_);
+    d_wv_.alloc(np);
+    d_V_.alloc((size_t)N * (restart + 1));
+    d_hv_.alloc(restart + 1);
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    // g (lam = 0)
+    apply(true);
+    HIP_CHECK(hipMemsetAsync(d_stats_.p, 0, sizeof(unsigned long long) * 4, st_));
+    aem::launch_adj_update((int64_t)ncf, d_jc_.p, d_gc_.p, d_lam_.p, 1.0, d_stats_.p, st_);  // lam = J_c + gc
+    aem::launch_adj_pack(np, d_pidx_.p, d_lam_.p, d_gv_.p, st_);
+    ++nit;
+    auto op = [&](const double2* v, double2* out) {  // out = (I - B) v
+      HIP_CHECK(hipMemsetAsync(d_lam_.p, 0, sizeof(double2) * ncf, st_));
+      aem::launch_adj_unpack(np, d_pidx_.p, v, d_lam_.p, st_);
+      apply(false);
+      aem::launch_adj_residual_op(np, d_pidx_.p, v, d_gc_.p, out, st_);
+      ++nit;
+    };
+    double gnorm = 0.0;
+    RB_CHECK(rocblas_dnrm2(rb_, N, (const double*)d_gv_.p, 1, &gnorm));
+    double* V = d_V_.p;
+    std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), sv(restart + 1), h(restart + 1),
+        h2(restart + 1);
+    double* w = (double*)d_wv_.p;
+    if (gnorm == 0.0) {
+      conv = true;
+      delta = 0.0;
+    }
+    bool first = true;
+    while (!conv && nit < maxit) {
+      // r = g - (I - B) x
+      if (first) {
+        HIP_CHECK(hipMemcpyAsync(w, d_gv_.p, sizeof(double) * N, hipMemcpyDeviceToDevice, st_));
+        first = false;
+      } else {
+        op(d_xv_.p, d_wv_.p);
+        RB_CHECK(rocblas_dscal(rb_, N, &mone, w, 1));
+        RB_CHECK(rocblas_daxpy(rb_, N, &one, (const double*)d_gv_.p, 1, w, 1));
+      }
+      double beta = 0.0;
+      RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &beta));
+      delta = beta / gnorm;
+      if (delta <= tol) {
+        conv = true;
+        break;
+      }
+      const double ib = 1.0 / beta;
+      RB_CHECK(rocblas_dcopy(rb_, N, w, 1, V, 1));
+      RB_CHECK(rocblas_dscal(rb_, N, &ib, V, 1));
+      std::fill(sv.begin(), sv.end(), 0.0);
+      sv[0] = beta;
+      int jd = 0;
+      for (int j = 0; j < restart && nit < maxit; ++j) {
+        op((const double2*)(V + (size_t)j * N), d_wv_.p);
+        // classical Gram-Schmidt, twice
+        for (int pass = 0; pass < 2; ++pass) {
+          RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_transpose, N, j + 1, &one, V, N, w, 1, &zero, d_hv_.p, 1));
+          RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, N, j + 1, &mone, V, N, d_hv_.p, 1, &one, w, 1));
+          HIP_CHECK(hipMemcpyAsync(pass ? h2.data() : h.data(), d_hv_.p, sizeof(double) * (j + 1),
+                                   hipMemcpyDeviceToHost, st_));
+        }
+        double hn = 0.0;
+        RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &hn));
+        HIP_CHECK(hipStreamSynchronize(st_));
+        for (int i = 0; i <= j; ++i) H[(size_t)i * restart + j] = h[i] + h2[i];
+        H[(size_t)(j + 1) * restart + j] = hn;
+        if (hn > 0.0) {
+          const double ih = 1.0 / hn;
+          RB_CHECK(rocblas_dcopy(rb_, N, w, 1, V + (size_t)(j + 1) * N, 1));
+          RB_CHECK(rocblas_dscal(rb_, N, &ih, V + (size_t)(j + 1) * N, 1));
+        }
+        // Givens rotations
+        for (int i = 0; i < j; ++i) {
+          const double a = H[(size_t)i * restart + j], b = H[(size_t)(i + 1) * restart + j];
+          H[(size_t)i * restart + j] = cs[i] * a + sn[i] * b;
+          H[(size_t)(i + 1) * restart + j] = -sn[i] * a + cs[i] * b;
+        }
+        const double a = H[(size_t)j * restart + j], b = H[(size_t)(j + 1) * restart + j];
+        const double r = std::hypot(a, b);
+        cs[j] = r > 0 ? a / r : 1.0;
+        sn[j] = r > 0 ? b / r : 0.0;
+        H[(size_t)j * restart + j] = r;
+        H[(size_t)(j + 1) * restart + j] = 0.0;
+        sv[j + 1] = -sn[j] * sv[j];
+        sv[j] = cs[j] * sv[j];
+        jd = j + 1;
+        delta = std::fabs(sv[j + 1]) / gnorm;
+        if (verbose && (nit % 20 == 0))
+          fprintf(stderr, "adjoint GMRES %lld: relative residual %.3e\n", (long long)nit, delta);
+        if (!(delta == delta)) fail(ANDFN_MI_EHIP, "adjoint GMRES produced NaN");
+        if (delta <= tol || hn == 0.0) break;
+      }
+      // x += V y,  H y = s (upper triangular, jd x jd)
+      std::vector<double> y(jd);
+      for (int i = jd - 1; i >= 0; --i) {
+        double acc = sv[i];
+        for (int k = i + 1; k < jd; ++k) acc -= H[(size_t)i * restart + k] * y[k];
+        y[i] = acc / H[(size_t)i * restart + i];
+      }
+      HIP_CHECK(hipMemcpyAsync(d_hv_.p, y.data(), sizeof(double) * jd, hipMemcpyHostToDevice, st_));
+      RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, N, jd, &one, V, N, d_hv_.p, 1, &one, (double*)d_xv_.p, 1));
+      HIP_CHECK(hipStreamSynchronize(st_));
+      if (delta <= tol) conv = true;
+    }
+    HIP_CHECK(hipMemsetAsync(d_lam_.p, 0, sizeof(double2) * ncf, st_));
+    aem::launch_adj_unpack(np, d_pidx_.p, d_xv_.p, d_lam_.p, st_);
+  }
+  // mu of the final lam
+  apply(true);
+  HIP_CHECK(hipGetLastError());
+  auto t_it = clk::now();
+
+  // parameter terms at the solved state
+  std::vector<FitItem> titems;
+  for (const FitItem& it : items_)
+    if (etype_[it.elem] == 0) titems.push_back(it);
+  d_titems_.upload(titems, st_);
+  d_ts_.alloc(std::max<size_t>(titems.size(), 1));
+  d_val_.alloc(std::max<int64_t>(full_.npts(), 1));
+  aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(), (int)full_.tiles.size(), K_, d_val_.p, st_);
+  aem::launch_adj_tgrad(nv, d_titems_.p, (int)titems.size(), d_lam_.p, d_val_.p, full_.d_pt_off.p, d_ts_.p, st_);
+  d_hw_.alloc(std::max<int64_t>(rhs_.npts(), 1));
+  aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_hw_.p, st_);
+  HIP_CHECK(hipGetLastError());
+  std::vector<double> ts(titems.size()), hw(rhs_.npts()), mu(n), qv(E_), cv(F_);
+  if (!titems.empty())
+    HIP_CHECK(hipMemcpyAsync(ts.data(), d_ts_.p, sizeof(double) * titems.size(), hipMemcpyDeviceToHost, st_));
+  if (!hw.empty()) HIP_CHECK(hipMemcpyAsync(hw.data(), d_hw_.p, sizeof(double) * hw.size(), hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(mu.data(), d_mu_.p, sizeof(double) * n, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(qv.data(), d_q_.p, sizeof(double) * E_, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(cv.data(), d_constant_.p, sizeof(double) * F_, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipStreamSynchronize(st_));
+  std::vector<double> hk(nobs);
+  if (nobs > 0) heads(nobs, obj->obs_frac, obj->obs_z, hk.data());
+
+  for (int f = 0; f < F_; ++f) grad_t[f] = 0.0;
+  for (int e = 0; e < E_; ++e) grad_head[e] = 0.0;
+  double J = 0.0;
+  if (obj && obj->dq)
+    for (int e = 0; e < E_; ++e) J += obj->dq[e] * qv[e];
+  if (obj && obj->dconstant)
+    for (int f = 0; f < F_; ++f) J += obj->dconstant[f] * cv[f];
+  // observation heads h = Phi / t:  dh/dt = -h / t
+  for (int64_t k = 0; k < nobs; ++k) {
+    const int f = (int)obj->obs_frac[k];
+    J += obj->dhead[k] * hk[k];
+    grad_t[f] -= obj->dhead[k] * hk[k] / t_[f];
+  }
+  // intersection fits c_j = (t0 a1_j - t1 a0_j)/(t0 + t1)
+  for (size_t b = 0; b < titems.size(); ++b) {
+    const int e = titems[b].elem;
+    const int f0 = efrac0_[e], f1 = efrac1_[e];
+    const double t0 = t_[f0], t1 = t_[f1], ss = (t0 + t1) * (t0 + t1);
+    grad_t[f0] += ts[b] * t1 / ss;
+    grad_t[f1] -= ts[b] * t0 / ss;
+  }
+  // head equations: intersection rows Phi0/t0 - Phi1/t1 = 0 give mu (h0/t0, -h1/t1); constant-head
+  // rows Phi = head t give mu head (d/dt) and mu t (d/dhead)
+  for (int j = 0; j < nde; ++j) {
+    const RhsItem& it = rhs_items_[j];
+    const int e = it.elem;
+    const int b0 = rhs_.pt_off[it.s0];
+    double h0 = 0.0;
+    for (int i = 0; i < K_; ++i) h0 += hw[b0 + i];
+    h0 /= (double)K_;
+    if (etype_[e] == 0) {
+      const int b1 = rhs_.pt_off[it.s1];
+      double h1 = 0.0;
+      for (int i = 0; i < K_; ++i) h1 += hw[b1 + i];
+      h1 /= (double)K_;
+      grad_t[efrac0_[e]] += mu[j] * h0 / t_[efrac0_[e]];
+      grad_t[efrac1_[e]] -= mu[j] * h1 / t_[efrac1_[e]];
+    } else {
+      const int f = efrac0_[e];
+      grad_t[f] += mu[j] * head_[e];
+      grad_head[e] = mu[j] * t_[f];
+    }
+  }
+  auto t_end = clk::now();
+  if (info) {
+    info->nit = nit;
+    info->delta = delta;
+    info->objective = J;
+    info->t_setup = std::chrono::duration<double>(t_set - t0).count();
+    info->t_iterate = std::chrono::duration<double>(t_it - t_set).count();
+    info->t_total = std::chrono::duration<double>(t_end - t0).count();
+  }
+  return conv ? ANDFN_MI_OK : ANDFN_MI_NOT_CONVERGED;
+}
+
+void Solver::set_parameters(const double* t, const double* head) {
+  if (t)
+    for (int f = 0; f < F_; ++f)
+      if (!(t[f] > 0.0) || !std::isfinite(t[f])) fail(ANDFN_MI_EINVAL, "t[%d] = %g must be positive and finite", f, t[f]);
+  if (t) std::copy(t, t + F_, t_.begin());
+  for (int e = 0; e < E_; ++e)
+    if (etype_[e] == 2 || etype_[e] == 3) {
+      if (head) head_[e] = head[e];
+      phi_[e] = head_[e] * t_[efrac0_[e]];
+    }
+  d_t_.upload(t_, st_);
+  d_phi_.upload(phi_, st_);
+  HIP_CHECK(hipStreamSynchronize(st_));
+  have_state_ = false;
+  adjT_valid_ = false;
+}
+
+void Solver::state_discharges(double* q, double* constant) {
+  if (q) HIP_CHECK(hipMemcpyAsync(q, d_q_.p, sizeof(double) * E_, hipMemcpyDeviceToHost, st_));
+  if (constant) HIP_CHECK(hipMemcpyAsync(constant, d_constant_.p, sizeof(double) * F_, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipStreamSynchronize(st_));
+}
+
 int guarded(const std::function<int()>& fn) {
   try {
     return fn();
@@ -1359,6 +1922,39 @@ int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_eleme
     if (!fractures || !elements || (n_points > 0 && (!frac || !z || !heads))) fail(ANDFN_MI_EINVAL, "null pointer");
     Solver s((void*)fractures, n_fractures, max_elements, (void*)elements, n_elements, max_ncoef, 1, nullptr, device);
     if (n_points > 0) s.heads(n_points, frac, z, heads);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_adjoint(andfn_mi_handle* h, const andfn_mi_objective* objective, const andfn_mi_adjoint_options* opt,
+                     double* grad_t, double* grad_head, andfn_mi_adjoint_info* info) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    return h->s->adjoint(objective, opt, grad_t, grad_head, info);
+  });
+}
+
+int andfn_mi_set_parameters(andfn_mi_handle* h, const double* t, const double* head) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    h->s->set_parameters(t, head);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_heads(andfn_mi_handle* h, int64_t n_points, const int64_t* frac, const double* z, double* heads) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (n_points > 0 && (!frac || !z || !heads)) fail(ANDFN_MI_EINVAL, "null pointer");
+    if (n_points > 0) h->s->heads(n_points, frac, z, heads);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_discharges(andfn_mi_handle* h, double* q, double* constant) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    h->s->state_discharges(q, constant);
     return ANDFN_MI_OK;
   });
 }

@@ -130,6 +130,68 @@ class DeviceNetwork:
     def download(self):
         _lib.check(self.lib.andfn_mi_download(self.h))
 
+    def set_parameters(self, t=None, head=None):
+        """Replace fracture transmissivities t[F] and/or prescribed heads head[E] of the
+        constant-head elements (phi = head * t); the next ``run`` factors the new matrix."""
+        tt = None if t is None else np.ascontiguousarray(t, dtype=np.float64)
+        hh = None if head is None else np.ascontiguousarray(head, dtype=np.float64)
+        if tt is not None and tt.shape != (len(self.fa),):
+            raise ValueError("t must have one value per fracture")
+        if hh is not None and hh.shape != (len(self.ea),):
+            raise ValueError("head must have one value per element")
+        _lib.check(self.lib.andfn_mi_set_parameters(self.h, None if tt is None else _lib.ptr(tt),
+                                                    None if hh is None else _lib.ptr(hh)))
+
+    def state_discharges(self):
+        """(q[E], constant[F]) of the current state."""
+        q = np.zeros(len(self.ea))
+        c = np.zeros(len(self.fa))
+        _lib.check(self.lib.andfn_mi_state_discharges(self.h, _lib.ptr(q), _lib.ptr(c)))
+        return q, c
+
+    def state_heads(self, frac, z):
+        """Heads at points (fracture index, local complex coordinate) of the current state."""
+        frac = np.ascontiguousarray(frac, dtype=np.int64)
+        zz = np.ascontiguousarray(z, dtype=np.complex128)
+        out = np.zeros(len(zz))
+        _lib.check(self.lib.andfn_mi_state_heads(self.h, len(zz), _lib.ptr(frac), _lib.ptr(zz), _lib.ptr(out)))
+        return out
+
+    def adjoint(self, dq=None, dconstant=None, obs_frac=None, obs_z=None, dhead=None, tol=1e-10,
+                max_iterations=5000, damping=0.75, verbose=False, method="gmres"):
+        """Gradient of J = dq.q + dconstant.C + dhead.h(obs) at the last run's state with respect
+        to every fracture transmissivity and every prescribed head (implicit adjoint of the
+        fixed point).  Returns (dJ/dt [F], dJ/dhead [E], info)."""
+        keep = []
+
+        def arr(a, dtype, n=None):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dtype)
+            if n is not None and a.shape[0] != n:
+                raise ValueError("array length mismatch")
+            keep.append(a)
+            return a.ctypes.data
+
+        nobs = 0 if obs_frac is None else len(obs_frac)
+        obj = _lib.Objective(arr(dq, np.float64, len(self.ea)), arr(dconstant, np.float64, len(self.fa)), nobs,
+                             arr(obs_frac, np.int64, nobs) if nobs else None,
+                             arr(obs_z, np.complex128, nobs) if nobs else None,
+                             arr(dhead, np.float64, nobs) if nobs else None)
+        if method not in ("gmres", "fixed_point"):
+            raise ValueError("method must be 'gmres' or 'fixed_point'")
+        opt = _lib.AdjointOptions(float(tol), int(max_iterations), float(damping), 1 if verbose else 0,
+                                  _lib.ADJ_GMRES if method == "gmres" else _lib.ADJ_FIXED_POINT)
+        gt = np.zeros(len(self.fa))
+        gh = np.zeros(len(self.ea))
+        info = _lib.AdjointInfo()
+        rc = self.lib.andfn_mi_adjoint(self.h, ctypes.byref(obj), ctypes.byref(opt), _lib.ptr(gt), _lib.ptr(gh),
+                                       ctypes.byref(info))
+        _lib.check(rc)
+        d = info.as_dict()
+        d["converged"] = rc == _lib.OK
+        return gt, gh, d
+
     def close(self):
         if self.h:
             self.lib.andfn_mi_destroy(self.h)

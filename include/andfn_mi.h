@@ -42,7 +42,7 @@ extern "C" {
 #define ANDFN_MI_MIXED 1  /* FP32 far-field series, FP64 near field, LU, residual and refinement */
 
 /* Per-iteration discharge solve with the LU factors of the constant discharge matrix. */
-#define ANDFN_MI_LU_TRSV 0     /* row permutation + two triangular solves (rocBLAS trsv) */
+#define ANDFN_MI_LU_TRSV 0     /* row permutation + blocked triangular solves (the library's own kernels) */
 #define ANDFN_MI_LU_GETRS 1    /* rocSOLVER getrs */
 #define ANDFN_MI_LU_INVERSE 2  /* explicit inverse formed once (getri), then one gemv per iteration */
 
@@ -139,6 +139,72 @@ void andfn_mi_destroy(andfn_mi_handle* handle);
 int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
                    int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
                    double* heads, int32_t device);
+
+/*
+ * Implicit adjoint of the converged fixed point.  No counterpart in the reference (AnDFN has no
+ * gradients); it is the gradient companion of andfn_mi_run.  For the objective
+ *
+ *     J = sum_e dq[e] q_e + sum_f dconstant[f] C_f + sum_k dhead[k] h(obs_z[k] in fracture obs_frac[k])
+ *
+ * at the state of the last andfn_mi_run on the handle, it returns dJ/dt_f for every fracture
+ * transmissivity and dJ/dhead_e for every constant-head element (constant-head lines and wells;
+ * 0 for other elements), with the coefficient counts frozen at their converged values.  The
+ * adjoint equations are the transpose of the fixed-point iteration (collocation sweep, fit,
+ * discharge solve): (I - B^T) lam = g with B the iteration's linear map on the coefficients.
+ * They are solved with the transposed operators (A^-T from the run's LU factors, refined against
+ * the FP64 matrix) by restarted GMRES or by the forward's damped iteration.  Returns
+ * ANDFN_MI_NOT_CONVERGED if max_iterations is reached first.
+ */
+typedef struct andfn_mi_objective {
+  const double* dq;         /* [n_elements] weights of the element discharges, or NULL */
+  const double* dconstant;  /* [n_fractures] weights of the fracture constants, or NULL */
+  int64_t n_obs;            /* observation points */
+  const int64_t* obs_frac;  /* [n_obs] fracture index of each point */
+  const double* obs_z;      /* [2 n_obs] interleaved (re, im) in the fracture's local plane */
+  const double* dhead;      /* [n_obs] weights of the heads at the points */
+} andfn_mi_objective;
+
+#define ANDFN_MI_ADJ_GMRES 0      /* restarted GMRES on (I - B^T) lam = g */
+#define ANDFN_MI_ADJ_FIXED_POINT 1 /* the forward's damped iteration, transposed */
+
+typedef struct andfn_mi_adjoint_options {
+  double tol;               /* GMRES: relative residual; fixed point: max |change| / max |adjoint| (0: 1e-10) */
+  int64_t max_iterations;   /* operator applications (0: 5000) */
+  double damping;           /* fixed point: 0 means 0.75, the forward's effective damping (hpc_solve.py:383-388, 561-575) */
+  int32_t verbose;          /* >0: progress lines on stderr */
+  int32_t method;           /* ANDFN_MI_ADJ_* */
+} andfn_mi_adjoint_options;
+
+typedef struct andfn_mi_adjoint_info {
+  int64_t nit;              /* adjoint iterations */
+  double delta;             /* relative change of the adjoint at the last iteration */
+  double objective;         /* J at the solved state */
+  double t_setup;           /* seconds: point tables, transposed factors, objective terms */
+  double t_iterate;         /* seconds: the adjoint iterations */
+  double t_total;           /* seconds: whole call */
+} andfn_mi_adjoint_info;
+
+int andfn_mi_adjoint(andfn_mi_handle* handle, const andfn_mi_objective* objective,
+                     const andfn_mi_adjoint_options* opt, double* grad_t, double* grad_head,
+                     andfn_mi_adjoint_info* info);
+
+/*
+ * Parameters of a handle's network (no counterpart in the reference, where they are fixed when
+ * the DFN is built: const_head.py:45, well.py:38).  t[n_fractures] (> 0) replaces the fracture
+ * transmissivities and head[n_elements] the prescribed heads of the constant-head elements (other
+ * entries are ignored); phi of a constant-head element becomes head * t of its fracture.  NULL
+ * leaves a parameter as it is.  The next run assembles and factors the new discharge matrix; the
+ * caller's records are not changed.
+ */
+int andfn_mi_set_parameters(andfn_mi_handle* handle, const double* t, const double* head);
+
+/* Heads at points of the handle's current state (after a run), as andfn_mi_heads. */
+int andfn_mi_state_heads(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
+                         double* heads);
+
+/* Discharges q[n_elements] and fracture constants constant[n_fractures] of the handle's current
+ * state (either pointer may be NULL), without writing the records. */
+int andfn_mi_state_discharges(andfn_mi_handle* handle, double* q, double* constant);
 
 /* Number of visible HIP devices (0 when none). */
 int andfn_mi_device_count(void);

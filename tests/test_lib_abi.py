@@ -32,10 +32,13 @@ def test_options_and_info_layout_match_header():
     def fields(struct):
         body = re.search(r"typedef struct " + struct + r" \{(.*?)\} " + struct, text, re.S).group(1)
         body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-        return re.findall(r"\b(?:int32_t|int64_t|double|const int32_t\*|int32_t\*)\s*\*?\s*([a-z_0-9]+);", body)
+        return re.findall(r"\b(?:const\s+)?(?:int32_t|int64_t|double)\s*\*?\s*([a-z_0-9]+);", body)
 
     assert fields("andfn_mi_options") == [f for f, _ in _lib.Options._fields_]
     assert fields("andfn_mi_info") == [f for f, _ in _lib.Info._fields_]
+    assert fields("andfn_mi_objective") == [f for f, _ in _lib.Objective._fields_]
+    assert fields("andfn_mi_adjoint_options") == [f for f, _ in _lib.AdjointOptions._fields_]
+    assert fields("andfn_mi_adjoint_info") == [f for f, _ in _lib.AdjointInfo._fields_]
 
 
 def test_no_device_or_bad_input_is_an_error_not_a_crash():
@@ -50,3 +53,6 @@ def test_no_device_or_bad_input_is_an_error_not_a_crash():
     assert len(lib.andfn_mi_last_error()) > 0
     rc = lib.andfn_mi_solve(None, 1, 500, None, 1, 200, 50, None, None, None, None)
     assert rc == _lib.EINVAL
+    for rc in (lib.andfn_mi_adjoint(None, None, None, None, None, None), lib.andfn_mi_set_parameters(None, None, None),
+               lib.andfn_mi_state_heads(None, 0, None, None, None), lib.andfn_mi_state_discharges(None, None, None)):
+        assert rc == _lib.EINVAL
