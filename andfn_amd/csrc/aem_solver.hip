@@ -21,6 +21,7 @@
 #include <functional>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -848,7 +849,10 @@ void Solver::solve_rhs() {
     lu_graph_tried_ = true;
     (void)discharge_solve(d_b_.p, d_x_.p);  // uncaptured first call: any lazy library set-up happens here
     HIP_CHECK(hipStreamSynchronize(st_));
-    if (hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+    // ANDFN_MI_NO_GRAPH set: launch the solve's kernels directly (profiler counter collection
+    // instruments every dispatch one by one)
+    static const bool no_graph = std::getenv("ANDFN_MI_NO_GRAPH") != nullptr;
+    if (!no_graph && hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess) {
       (void)discharge_solve(d_b_.p, d_x_.p);
       hipGraph_t g = nullptr;
       if (hipStreamEndCapture(st_, &g) == hipSuccess && g) {
@@ -870,7 +874,6 @@ void Solver::solve_rhs() {
   }
 }
 
-// x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
 // v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten)
 template <class T>
 void Solver::blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v) {
@@ -887,6 +890,7 @@ void Solver::blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x
   aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_);
 }
 
+// x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
 double* Solver::discharge_solve(double* b, double* x) {
   const rocblas_int n = (rocblas_int)(de_.size() + F_);
   if (lu_mode_ == ANDFN_MI_LU_GETRS) {

@@ -237,9 +237,14 @@ WORKLOADS = {
 
 
 def workload(name, **override):
+    """The named network spec with fields replaced by ``override``; a ``constants`` override is
+    merged into the spec's own solver constants."""
     spec = WORKLOADS[name]
     if override:
         d = dict(spec.__dict__)
+        if "constants" in override:
+            override = dict(override)
+            override["constants"] = {**spec.constants, **override["constants"]}
         d.update(override)
         spec = NetworkSpec(**d)
     return spec

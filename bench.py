@@ -4,8 +4,12 @@ A step is one complete solve of the seeded 5,000-fracture network (500 m cube, r
 [10, 150] m, 20 coefficients / 40 collocation points per element): discharge-matrix assembly,
 LU factorisation, the fixed-point iterations with coefficient growth, and the final head
 residual, with the network already resident in device memory when the timed region starts.
-The solve stops on AnDFN's rule (coefficient and discharge change below MAX_ERROR twice) or as
-soon as the head residual (max |A y - b(c)| over the head equations) is at most 1e-6 m.
+The solve stops as soon as the head residual (max |A y - b(c)| over the head equations) is at
+most 1e-6 m.  ``value`` is the FP64 form (``--precision``); ``forms`` adds the other form (the
+mixed-precision form: FP32 factors refined in FP64, FP32 far field and fits) with its error
+against the FP64 result; ``adjoint`` times the implicit adjoint (gradient of the x = 0 boundary
+discharge plus 100 observation heads with respect to every log10 transmissivity and both
+boundary heads) against one forward solve.
 
 Prints one JSON line.  ``roofline`` describes the collocation sweep kernel (FP64 vector ALU
 bound); ``cpu_baseline`` times the CPU FP64 oracle (a restatement of AnDFN's Numba path with
@@ -26,6 +30,7 @@ sys.path.insert(0, str(ROOT))
 
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), flop/s
 HEAD_TOL = 1e-6
+ADJOINT_TOL = 1e-10
 
 
 def parse():
@@ -37,6 +42,11 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--lu-mode", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="time only the --precision form")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--max-iterations", type=int, default=0, help="override MAX_ITERATIONS (short profiling runs)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle iterations in the CPU sample")
     return ap.parse_args()
 
@@ -93,6 +103,27 @@ def cpu_baseline(fa, ea, K, consts, nit_gpu, n_iter):
     }
 
 
+def _obs_points(fa, n, seed=3):
+    rng = np.random.default_rng(seed)
+    of = rng.integers(0, len(fa), n)
+    oz = fa["radius"][of] * rng.uniform(0.1, 0.7, n) * np.exp(1j * rng.uniform(0, 2 * np.pi, n))
+    return of, oz
+
+
+def _time_form(net, precision, lu_mode, warmup, steps, label):
+    for _ in range(warmup):
+        net.run(precision, lu_mode, HEAD_TOL)
+    infos, walls = [], []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        info = net.run(precision, lu_mode, HEAD_TOL)
+        walls.append(time.perf_counter() - t0)
+        infos.append(info)
+        print(f"# {label} step {k}: {walls[-1]:.3f}s nit={info['nit']} head_residual={info['head_residual']:.2e} "
+              f"factor={info['t_factor']:.3f}s iterate={info['t_iterate']:.3f}s", file=sys.stderr, flush=True)
+    return infos, walls
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -100,13 +131,25 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     from andfn_amd import _lib
+    from andfn_amd.autograd import boundary_groups
     from andfn_amd.generator import build_network, workload
     from andfn_amd.hpc_solve import DeviceNetwork
 
     if _lib.load().andfn_mi_device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
-    spec = workload(args.workload)
+    if args.calibrate_copy_mib > 0:
+        import torch
+
+        nb = args.calibrate_copy_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# calibration copy: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        del src, dst
+    spec = workload(args.workload, constants={"MAX_ITERATIONS": args.max_iterations}) if args.max_iterations \
+        else workload(args.workload)
     dfn = build_network(spec)
     dfn.consolidate_dfn()
     fa, ea = dfn.fractures_struc_array_hpc, dfn.elements_struc_array_hpc
@@ -114,20 +157,61 @@ def main():
     net = DeviceNetwork(fa, ea, dfn.discharge_int, dfn.constants)
     print(f"# network: {len(fa)} fractures, {len(ea)} elements; warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        net.run(args.precision, args.lu_mode, HEAD_TOL)
-    infos = []
-    walls = []
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        info = net.run(args.precision, args.lu_mode, HEAD_TOL)
-        walls.append(time.perf_counter() - t0)
-        infos.append(info)
-        print(f"# step {k}: {walls[-1]:.3f}s nit={info['nit']} head_residual={info['head_residual']:.2e} "
-              f"factor={info['t_factor']:.3f}s iterate={info['t_iterate']:.3f}s", file=sys.stderr, flush=True)
+    other = "mixed" if args.precision == "fp64" else "fp64"
+    forms = {}
+    states = {}
+    obs_frac, obs_z = _obs_points(fa, 100)
+    for prec in ([args.precision] + ([] if args.no_compare else [other])):
+        infos, walls = _time_form(net, prec, args.lu_mode, args.warmup, args.steps, prec)
+        q, c = net.state_discharges()
+        net.download()
+        states[prec] = (ea["coef"].copy(), q, c, net.state_heads(obs_frac, obs_z))
+        last = infos[-1]
+        forms[prec] = {"solves_per_s": args.steps / sum(walls), "ms_per_solve": 1e3 * sum(walls) / args.steps,
+                       "nit": int(last["nit"]), "nit_fp32": int(last["nit_fp32"]),
+                       "head_residual": last["head_residual"], "t_factor_s": last["t_factor"],
+                       "t_iterate_s": last["t_iterate"], "info": last, "walls": walls}
+    main_form = forms[args.precision]
+    last = main_form.pop("info")
+    walls = main_form.pop("walls")
+    if other in forms:
+        forms[other].pop("info")
+        forms[other].pop("walls")
+        a, b = states["fp64"], states["mixed"]
+
+        def rel(x, y):
+            return float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+
+        forms["mixed"]["error_vs_fp64"] = {"coef_rel": rel(b[0], a[0]), "q_rel": rel(b[1], a[1]),
+                                           "constant_rel": rel(b[2], a[2]),
+                                           "heads_abs_m": float(np.max(np.abs(b[3] - a[3]))),
+                                           "note": "both forms stopped at head residual <= 1e-6 m; free-running "
+                                                   "coefficient growth; heads at 100 observation points"}
+    adjoint = None
+    if not args.no_adjoint:
+        # gradient of Q(x=0) + sum of 100 observation heads w.r.t. every log10 T and the two boundary heads
+        net.run("fp64", args.lu_mode, HEAD_TOL)
+        t_fwd = forms["fp64"]["ms_per_solve"] / 1e3 if "fp64" in forms else None
+        groups = boundary_groups(dfn)
+        dq = np.zeros(len(ea))
+        dq[groups["x0"]] = 1.0
+        aw = []
+        for k in range(1 + args.steps):
+            t0 = time.perf_counter()
+            gt, gh, ai = net.adjoint(dq=dq, obs_frac=obs_frac, obs_z=obs_z, dhead=np.ones(len(obs_frac)),
+                                     tol=ADJOINT_TOL)
+            aw.append(time.perf_counter() - t0)
+            print(f"# adjoint {k}: {aw[-1]:.3f}s iterations={ai['nit']} residual={ai['delta']:.2e}",
+                  file=sys.stderr, flush=True)
+        t_adj = sum(aw[1:]) / args.steps
+        adjoint = {"objective": "Q through x=0 + sum of heads at 100 observation points",
+                   "gradient": f"d/d log10 T for all {len(fa)} fractures and d/d head of the 2 boundary planes",
+                   "method": "implicit fixed-point adjoint, GMRES on the transposed iteration (FP64)",
+                   "tol": ADJOINT_TOL, "iterations": int(ai["nit"]), "residual": ai["delta"],
+                   "ms": 1e3 * t_adj, "first_call_ms": 1e3 * aw[0],
+                   "adjoint_forward_ratio": (t_adj / t_fwd) if t_fwd else None}
     net.close()
     total = sum(walls)
-    last = infos[-1]
     sweep_ms = last["k_sweep_ms"] / max(last["n_sweep"], 1)
     flops_launch = last["flops_sweep"] / max(last["n_sweep"], 1)
     achieved = flops_launch / (sweep_ms * 1e-3) if sweep_ms > 0 else 0.0
@@ -152,8 +236,8 @@ def main():
             "elements": int(len(ea)),
             "unknowns": int(last["n_unknowns"]),
             "precision": args.precision,
-            "stop": f"AnDFN rule (MAX_ERROR {float(dfn.constants['MAX_ERROR']):g}) or head residual <= "
-                    f"{HEAD_TOL:g} m, MAX_ITERATIONS {int(dfn.constants['MAX_ITERATIONS'])}",
+            "stop": f"head residual <= {HEAD_TOL:g} m (max |A y - b(c)| over the head equations), "
+                    f"MAX_ITERATIONS {int(dfn.constants['MAX_ITERATIONS'])}",
             "filters": f"rim clearance {spec.rim_clearance:g} R, near-coincident pairs < {spec.min_separation:g} L "
                        f"at < {spec.min_angle_deg:g} deg",
         },
@@ -163,6 +247,8 @@ def main():
         "converged": bool(last["converged"]),
         "t_factor_s": last["t_factor"],
         "t_iterate_s": last["t_iterate"],
+        "forms": forms,
+        "adjoint": adjoint,
         "roofline": {
             "bound": "mfma",
             "kernel": "sweep_kernel<MODE_COLLOC> (FP64 vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",

@@ -73,12 +73,12 @@ def test_mixed_matches_fp64_fixed_counts(name):
     _compare(fa, ea, fm, em, i64, last_solve_info())
 
 
-@pytest.mark.parametrize("name", ["cfg1", "dfn40w"])
-def test_mixed_matches_fp64_on_fp64_growth_trajectory(name):
+@pytest.mark.parametrize("name,consts", [("cfg1", {"MAX_NCOEF": 20}), ("dfn40w", {})])
+def test_mixed_matches_fp64_on_fp64_growth_trajectory(name, consts):
     # coefficient growth reacts to the FP32 error estimates, so the mixed run replays the FP64
     # run's coefficient counts to make the two states comparable element by element
     print(f"mixed vs fp64, growth replay, {name}", flush=True)
-    fa, ea, fm, em, K, consts = _pair(name)
+    fa, ea, fm, em, K, consts = _pair(name, **consts)
     solve(fa, ea, K, consts, head_tol=HEAD_TOL, record_growth=True)
     i64 = last_solve_info()
     solve(fm, em, K, consts, precision="mixed", head_tol=HEAD_TOL, growth_schedule=i64["ncoef_history"])
