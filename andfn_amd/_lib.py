@@ -72,6 +72,8 @@ class Info(ctypes.Structure):
         ("head_residual_row", ctypes.c_int64),
         ("bc_residual", ctypes.c_double),
         ("nit_fp32", ctypes.c_int64),
+        ("solves_fp32", ctypes.c_int64),
+        ("refine_rate", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -248,6 +248,12 @@ class Solver {
   DBuf<float> d_A32_, d_linv32_, d_uinv32_, d_x32_, d_w32_, d_v32_;
   DBuf<float2> d_coef32_;
   bool mixed_phase_ = false;
+  bool solve32_ = false;      // discharge solves by refinement with the FP32-rounded factors
+  bool a64_blocked_ = false;  // d_A_ holds the FP64 blocked operators (panel products applied)
+  int64_t solves32_ = 0;      // iterations of the current run that used solve32_
+  double refine_rate_ = 0.0;  // measured contraction of the FP32-factor refinement
+  void switch_to_fp64_solves();
+  double refine_probe();
   double far2_ = 16.0;
   int precision_ = ANDFN_MI_FP64;
   // initial state of the mutable arrays, restored at the start of every run
@@ -260,7 +266,7 @@ class Solver {
   DBuf<Slot> d_slots_;
   DBuf<FitItem> d_items_, d_items_g_;
   DBuf<RhsItem> d_rhs_items_;
-  DBuf<rocblas_int> d_ipiv_, d_info_;
+  DBuf<int64_t> d_ipiv_, d_info_;  // 64-bit LAPACK interface: n^2 exceeds 2^31 from ~46k unknowns
   DBuf<aem::IterScalars> d_scal_;
 
   TargetSet full_, grown_, rhs_;
@@ -360,7 +366,7 @@ class Solver {
   DBuf<int> d_csrT_ptr_, d_csrT_col_;
   void adj_tables(const TargetSet& ts, bool colloc, AdjTables& t);
   template <class T>
-  void make_transposed(const T* A, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT);
+  void make_transposed(const T* A, bool blocked, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT);
   void make_csr_transpose();
   template <class T>
   void adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v);
@@ -680,14 +686,12 @@ static rocblas_status rb_trtri(rocblas_handle h, rocblas_fill f, rocblas_diagona
 static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
                                     const double* T, int ldt, double* B, int64_t ldb) {
   const double one = 1.0;
-  return rocblas_dtrmm(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, (rocblas_int)ldb,
-                       B, (rocblas_int)ldb);
+  return rocblas_dtrmm_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, ldb, B, ldb);
 }
 static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
                                     const float* T, int ldt, float* B, int64_t ldb) {
   const float one = 1.0f;
-  return rocblas_strmm(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, (rocblas_int)ldb,
-                       B, (rocblas_int)ldb);
+  return rocblas_strmm_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, ldb, B, ldb);
 }
 
 // Blocked triangular-solve operators from the LU factors in A (n x n, column-major, kNB blocks):
@@ -797,25 +801,28 @@ void Solver::assemble_and_factor() {
   }
   d_ipiv_.alloc(n);
   d_info_.zero(1, st_);
-  if (precision_ == ANDFN_MI_MIXED) {
-    // single-precision factors of the equilibrated matrix; the FP64 residual of the CSR copy
-    // drives the refinement, so the iteration's fixed point is the FP64 one
-    d_A32_.alloc((size_t)(n * n));
-    aem::launch_to_f32(d_A_.p, n * n, d_A32_.p, st_);
-    RB_CHECK(rocsolver_sgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A32_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
-  } else {
-    RB_CHECK(rocsolver_dgetrf(rb_, (rocblas_int)n, (rocblas_int)n, d_A_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
-  }
-  rocblas_int info = 0;
+  RB_CHECK(rocsolver_dgetrf_64(rb_, n, n, d_A_.p, n, d_ipiv_.p, d_info_.p));
+  int64_t info = 0;
   HIP_CHECK(hipMemcpyAsync(&info, d_info_.p, sizeof(info), hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
-  if (info > 0) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot at %d)", (int)info);
+  if (info > 0) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot at %lld)", (long long)info);
+  solve32_ = false;
+  a64_blocked_ = false;
+  if (precision_ == ANDFN_MI_MIXED) {
+    // the per-iteration solves use the factors rounded to FP32 (half the bytes); the FP64 residual
+    // of the CSR copy drives the refinement, so the iteration's fixed point is the FP64 one.
+    // (rocSOLVER's sgetrf_64 in this ROCm reports spurious zero pivots at n = 65,690, on random
+    // matrices too, and is no faster than dgetrf below that size, so the factorisation is FP64.)
+    d_A32_.alloc((size_t)(n * n));
+    aem::launch_to_f32(d_A_.p, n * n, d_A32_.p, st_);
+    solve32_ = true;
+  }
   d_x_.alloc(n);
   d_w_.alloc(n);
   if (lu_mode_ == ANDFN_MI_LU_TRSV) {
     // LAPACK row interchanges -> one gather permutation: (P b)[i] = b[perm[i]]
-    std::vector<rocblas_int> ipiv(n);
-    HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(rocblas_int) * n, hipMemcpyDeviceToHost, st_));
+    std::vector<int64_t> ipiv(n);
+    HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
     std::vector<int> perm(n);
     for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
@@ -826,16 +833,77 @@ void Solver::assemble_and_factor() {
       d_x32_.alloc(n);
       d_w32_.alloc(n);
       d_v32_.alloc(n);
+      // the refinement must contract well for the FP32-rounded factors to pay off; otherwise the
+      // run solves with the FP64 factors (large or ill-conditioned systems)
+      refine_rate_ = refine_probe();
+      if (!(refine_rate_ <= 0.25)) switch_to_fp64_solves();
     } else {
       prepare_blocked<double>(d_A_.p, n, d_linv_, d_uinv_);
+      a64_blocked_ = true;
     }
     d_w_.alloc(n);
     d_v_.alloc(n);
     HIP_CHECK(hipStreamSynchronize(st_));
   } else if (lu_mode_ == ANDFN_MI_LU_INVERSE) {
-    RB_CHECK(rocsolver_dgetri(rb_, (rocblas_int)n, d_A_.p, (rocblas_int)n, d_ipiv_.p, d_info_.p));
+    // getri has only the 32-bit interface here (this mode is for comparisons at moderate n)
+    std::vector<int64_t> p64(n);
+    HIP_CHECK(hipMemcpyAsync(p64.data(), d_ipiv_.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    std::vector<rocblas_int> p32(p64.begin(), p64.end());
+    DBuf<rocblas_int> ip32, inf32;
+    ip32.upload(p32, st_);
+    inf32.zero(1, st_);
+    RB_CHECK(rocsolver_dgetri(rb_, (rocblas_int)n, d_A_.p, (rocblas_int)n, ip32.p, inf32.p));
     HIP_CHECK(hipStreamSynchronize(st_));
   }
+}
+
+// Mixed form: contraction rate of the refinement y <- y + (LU)_32^-1 R P (b - A y) on the probe
+// system A y = A 1: sqrt(r3 / r1) with r_k the head residual after k steps from y = 0.
+double Solver::refine_probe() {
+  const int n = (int)(de_.size() + F_);
+  const int nde = (int)de_.size();
+  DBuf<double> ones, b0, y, x;
+  ones.upload(std::vector<double>(n, 1.0), st_);
+  b0.alloc(n);
+  y.zero(n, st_);
+  x.alloc(n);
+  d_ay_.alloc(n);
+  aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, ones.p, b0.p, st_);
+  double r[4] = {0, 0, 0, 0};
+  for (int k = 0; k < 4; ++k) {
+    aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, y.p, d_ay_.p, st_);
+    aem::launch_residual(d_ay_.p, b0.p, d_rowscale_.p, nde, d_scal_.p, st_);
+    aem::IterScalars sc{};
+    HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    r[k] = sc.residual;
+    if (k == 3) break;
+    aem::launch_refine_rhs(b0.p, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
+    blocked_solve<float>(d_A32_.p, n, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+    aem::launch_refine_update(y.p, d_v32_.p, n, x.p, st_);
+    HIP_CHECK(hipMemcpyAsync(y.p, x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+  }
+  HIP_CHECK(hipStreamSynchronize(st_));
+  if (!(r[1] > 0.0)) return 0.0;
+  return std::sqrt(r[3] / r[1]);
+}
+
+// Mixed form: continue with the FP64 factors (the refinement with the FP32-rounded factors
+// stopped contracting).  Builds the FP64 blocked operators on first use.
+void Solver::switch_to_fp64_solves() {
+  if (!solve32_) return;
+  const int64_t n = (int64_t)de_.size() + F_;
+  if (!a64_blocked_) {
+    prepare_blocked<double>(d_A_.p, n, d_linv_, d_uinv_);
+    d_w_.alloc(n);
+    d_v_.alloc(n);
+    a64_blocked_ = true;
+  }
+  solve32_ = false;
+  drop_graph();
+  adjT_valid_ = false;
+  HIP_CHECK(hipStreamSynchronize(st_));
 }
 
 // d_x_ = A^-1 d_b_.  The triangular-solve sequence (one permutation gather and two gemv per
@@ -895,7 +963,7 @@ double* Solver::discharge_solve(double* b, double* x) {
   const rocblas_int n = (rocblas_int)(de_.size() + F_);
   if (lu_mode_ == ANDFN_MI_LU_GETRS) {
     aem::launch_scale(b, d_req_.p, n, x, st_);
-    RB_CHECK(rocsolver_dgetrs(rb_, rocblas_operation_none, n, 1, d_A_.p, n, d_ipiv_.p, x, n));
+    RB_CHECK(rocsolver_dgetrs_64(rb_, rocblas_operation_none, n, 1, d_A_.p, n, d_ipiv_.p, x, n));
     return x;
   }
   if (lu_mode_ == ANDFN_MI_LU_INVERSE) {
@@ -904,7 +972,7 @@ double* Solver::discharge_solve(double* b, double* x) {
     RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A_.p, n, d_w_.p, 1, &zero, x, 1));
     return x;
   }
-  if (precision_ == ANDFN_MI_MIXED) {
+  if (solve32_) {
     // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
     aem::launch_refine_rhs(b, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
@@ -998,6 +1066,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   int resid_row = -1;
   std::vector<double> rhist;
   int64_t nit_fp32 = 0;
+  double best_resid = HUGE_VAL;
+  solves32_ = 0;
   if (K_ < 1 || K_ > 2 * L_.M) fail(ANDFN_MI_EINVAL, "discharge_int %d outside [1, %lld]", K_, (long long)(2 * L_.M));
   if (max_coef + coef_increase - 1 > L_.M)
     fail(ANDFN_MI_ECAPACITY, "MAX_NCOEF %d + COEF_INCREASE %d exceeds the coefficient capacity %lld", max_coef,
@@ -1096,6 +1166,21 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     if (head_tol > 0.0 && nit >= 2) {
       // head residual of the previous iterate against the right-hand side of its coefficients
       resid = head_residual(d_b_.p, &resid_row);
+      if (solve32_) {
+        // the FP32-factor refinement must contract: a residual 4x above the best so far (or not
+        // finite) hands the solve to the FP64 factors for the rest of the run
+        if (!(resid == resid) || !std::isfinite(resid) || (nit > 3 && resid > 4.0 * best_resid)) {
+          if (verbose)
+            fprintf(stderr, "mixed form: head residual %.3e at iteration %d (best %.3e); FP64 factors from here\n",
+                    resid, nit, best_resid);
+          switch_to_fp64_solves();
+          if (mixed_phase_) {
+            mixed_phase_ = false;
+            nit_fp32 = nit - 1;
+          }
+        }
+        if (resid == resid) best_resid = std::min(best_resid, resid);
+      }
       if (mixed_phase_) {
         // The FP32 phase ends near the tolerance, or when the best residual of the last 10
         // iterations is no better than 0.7 of the best of the 10 before (the FP32 floor).
@@ -1119,6 +1204,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       }
     }
     HIP_CHECK(hipEventRecord(ev_[4], st_));
+    if (solve32_) ++solves32_;
     solve_rhs();  // d_x_ = A^-1 d_b_
     HIP_CHECK(hipEventRecord(ev_[5], st_));
     HIP_CHECK(hipMemcpyAsync(d_yold_.p, d_y_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
@@ -1251,6 +1337,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->head_residual_row = resid_row;
     info->bc_residual = bcres;
     info->nit_fp32 = precision_ == ANDFN_MI_MIXED ? (mixed_phase_ ? nit : nit_fp32) : 0;
+    info->solves_fp32 = solves32_;
+    info->refine_rate = precision_ == ANDFN_MI_MIXED ? refine_rate_ : 0.0;
   }
   mixed_phase_ = false;
   have_state_ = true;
@@ -1356,14 +1444,14 @@ void Solver::adj_tables(const TargetSet& ts, bool colloc, AdjTables& t) {
 // (L[below j, j] = G_j L_jj, U[above j, j] = H_j U_jj), transposed, and prepared for the same
 // blocked solves with a non-unit lower (U^T) and a unit upper (L^T) triangle.
 template <class T>
-void Solver::make_transposed(const T* A, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT) {
+void Solver::make_transposed(const T* A, bool blocked, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT) {
   const int64_t n = (int64_t)de_.size() + F_;
   const int64_t nb = (n + kNB - 1) / kNB;
   {
     DBuf<T> tmp;
     tmp.alloc((size_t)(n * n));
     HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * (size_t)(n * n), hipMemcpyDeviceToDevice, st_));
-    for (int64_t j = 0; j < nb; ++j) {
+    for (int64_t j = 0; blocked && j < nb; ++j) {
       const int64_t r0 = j * kNB;
       const int bs = (int)std::min<int64_t>(kNB, n - r0);
       const int rest = (int)(n - r0 - bs);
@@ -1498,13 +1586,15 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     nxsrc = (int)xsrc.size();
   }
   // transposed factors, once per factorisation
+  // the adjoint uses the factors the run finished with: FP32-rounded (refined) or FP64
+  const bool a32 = solve32_;
   if (!adjT_valid_) {
-    if (precision_ == ANDFN_MI_MIXED) make_transposed<float>(d_A32_.p, d_AT32_, d_linvT32_, d_uinvT32_);
-    else make_transposed<double>(d_A_.p, d_AT_, d_linvT_, d_uinvT_);
+    if (a32) make_transposed<float>(d_A32_.p, true, d_AT32_, d_linvT32_, d_uinvT32_);
+    else make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
     make_csr_transpose();
     adjT_valid_ = true;
   }
-  if (precision_ == ANDFN_MI_MIXED) {
+  if (a32) {
     d_x32_.alloc(n);
     d_w32_.alloc(n);
     d_v32_.alloc(n);
@@ -1571,7 +1661,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   d_sfx_.zero((size_t)E_ * 2 * L_.M, st_);
   d_stats_.zero(4, st_);
   // A^-T with the run's factors: one FP64 step, or FP32 steps refined against the FP64 matrix
-  const int nref = precision_ == ANDFN_MI_MIXED ? 3 : 1;
+  const int nref = a32 ? 3 : 1;
   // d_gc_ = B lam (+ RHS_c^T A^-T J_y when with_j) and d_mu_ = A^-T (S_y^T lam (+ J_y)), where B is
   // the fixed-point map's linear part on the coefficients: FIT^T, EVAL^T at the collocation
   // points (coefficients, discharges, constants, branch cuts), A^-T, RHS^T
@@ -1585,7 +1675,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     aem::launch_adj_build_z(nde, F_, d_de_.p, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
     for (int r = 0; r < nref; ++r) {
-      if (precision_ == ANDFN_MI_MIXED)
+      if (a32)
         adj_mu_step<float>(d_AT32_.p, d_linvT32_.p, d_uinvT32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
       else
         adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);

@@ -100,6 +100,10 @@ typedef struct andfn_mi_info {
    * |h - h_prescribed|; bounded below by the series truncation, not by the iteration */
   double bc_residual;
   int64_t nit_fp32;       /* mixed form: iterations run with FP32 far field and fit before the FP64 phase */
+  int64_t solves_fp32;    /* mixed form: discharge solves done by refinement with the FP32-rounded factors
+                           * (the rest use the FP64 factors: after the refinement stopped contracting) */
+  double refine_rate;     /* mixed form: measured contraction per refinement step with the FP32-rounded
+                           * factors (above 0.25 the run solves with the FP64 factors throughout) */
 } andfn_mi_info;
 
 /*
