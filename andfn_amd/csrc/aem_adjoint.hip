@@ -177,14 +177,14 @@ __global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* 
           const double inv = rcp_pos(r2);
           base = make_double2(chi.x * inv, -chi.y * inv);  // 1/chi
           if (do_q && sl.kind == KIND_LINE)
-            aq = fma(sl.sign * kInv2Pi, fma(wd.x, 0.5 * log(r2), wd.y * atan2(chi.y, chi.x)), aq);
+            aq = fma(sl.sign * kInv2Pi, fma(wd.x, 0.5 * log_pos(r2), wd.y * atan2(chi.y, chi.x)), aq);
         } else if (sl.kind == KIND_BOUND) {
           base = make_double2(sl.alpha.x * z.x, sl.alpha.x * z.y);  // z / R
         } else {
           const double2 chi = make_double2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
           const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
           if (sl.kind == KIND_WELL) {
-            if (do_q) aq = fma(kInv2Pi, fma(wd.x, 0.5 * log(r2), wd.y * atan2(chi.y, chi.x)), aq);
+            if (do_q) aq = fma(kInv2Pi, fma(wd.x, 0.5 * log_pos(r2), wd.y * atan2(chi.y, chi.x)), aq);
           } else {
             const double inv = rcp_pos(r2);
             base = make_double2(chi.x * inv, -chi.y * inv);

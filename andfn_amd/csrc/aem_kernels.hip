@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
       if (sl.kind == KIND_LINE && use_q) {
         double qq = sl.sign * net.q[src] * kInv2Pi;
         if (qq != 0.0) {
-          tre = fma(qq, 0.5 * log(r2), tre);
+          tre = fma(qq, 0.5 * log_pos(r2), tre);
           if (any_imag) tim = fma(qq, atan2(chi.y, chi.x), tim);
         }
       }
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
       double r2 = fma(chi.x, chi.x, chi.y * chi.y);
       double qq = use_q ? net.q[src] * kInv2Pi : 0.0;
       if (qq != 0.0) {
-        tre = qq * (0.5 * log(r2));
+        tre = qq * (0.5 * log_pos(r2));
         if (any_imag) tim = qq * atan2(chi.y, chi.x);
       }
       if (sqrt(r2) < 1.0 - 1e-12) {
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg
         double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
         r2 = fma(chi.x, chi.x, chi.y * chi.y);
       }
-      acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log(r2);
+      acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log_pos(r2);
     }
     acc = wave_sum(acc);
     if (lane == 0) atomicAdd(&A[(int64_t)de_pos[src] * n + row], rs * acc / (double)K);

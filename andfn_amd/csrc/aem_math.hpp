@@ -28,13 +28,66 @@ constexpr double kInv2Pi = 0.15915494309189535;  // 1/(2 pi)
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double2 c2(double re, double im) { return make_double2(re, im); }
 
+// 1/x for x > 0 (normal): hardware reciprocal refined by two Newton steps
+__device__ __forceinline__ double rcp_pos(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// sqrt(x) for x > 0 (normal): hardware reciprocal square root refined by two Goldschmidt steps
+// (g -> sqrt x, h -> 1/(2 sqrt x)) and a final residual correction; within ~1 ulp
+__device__ __forceinline__ double sqrt_pos(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
+// log(x) for x > 0 (normal): x = m 2^e with m in [1/sqrt2, sqrt2), log m = 2 atanh(s),
+// s = (m - 1)/(m + 1), |s| <= 0.172, by its odd series to s^21 (truncation < 1e-18 relative)
+__device__ __forceinline__ double log_pos(double x) {
+  int e;
+  double m = frexp(x, &e);  // m in [0.5, 1)
+  if (m < 0.7071067811865476) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) * rcp_pos(m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  const double de = (double)e;
+  const double t = fma(2.0 * s * s2, p, de * ln2_lo);
+  return fma(de, ln2_hi, fma(2.0, s, t));
+}
+
 // principal square root (branch cut on the negative real axis)
 __device__ __forceinline__ double2 csqrt_principal(double x, double y) {
-  double r = sqrt(fma(x, x, y * y));
-  if (r == 0.0) return c2(0.0, y);
-  double t = sqrt(0.5 * (r + fabs(x)));
-  if (x >= 0.0) return c2(t, y / (2.0 * t));
-  return c2(fabs(y) / (2.0 * t), copysign(t, y));
+  const double r2 = fma(x, x, y * y);
+  if (r2 == 0.0) return c2(0.0, y);
+  const double r = sqrt_pos(r2);
+  const double t = sqrt_pos(0.5 * (r + fabs(x)));
+  const double h = 0.5 * rcp_pos(t);
+  if (x >= 0.0) return c2(t, y * h);
+  return c2(fabs(y) * h, copysign(t, y));
 }
 
 // exterior Joukowsky inverse of Z (|chi| >= 1)
@@ -73,15 +126,6 @@ __device__ __forceinline__ double2 horner(const double2* __restrict__ c, int m, 
   }
   for (; j >= 0; --j) r = cmadd(r, w, c[j]);
   return r;
-}
-
-// 1/x for x >= 1 of moderate size: hardware reciprocal refined by two Newton steps
-__device__ __forceinline__ double rcp_pos(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
 }
 
 // ---------------------------------------------------------------------------------------
