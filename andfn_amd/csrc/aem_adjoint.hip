@@ -253,23 +253,25 @@ __global__ __launch_bounds__(64) void adj_csum_kernel(int nf, const int* __restr
   if (threadIdx.x == 0) gC[f] = accumulate ? gC[f] + s : s;
 }
 
-// z = J_y + [gq of the discharge elements; gC]
-__global__ void adj_build_z_kernel(int nde, int nf, const int* __restrict__ de, const double* __restrict__ jy,
-                                   const double* __restrict__ gq, const double* __restrict__ gC, double* __restrict__ z) {
+// z = J_y + (gq at the discharge unknowns, gC at the fracture-constant unknowns); the launcher
+// copies J_y into z first, so padding unknowns keep J_y = 0
+__global__ void adj_build_z_kernel(int nde, int nf, const int* __restrict__ de, SysLayout sys,
+                                   const double* __restrict__ jy, const double* __restrict__ gq,
+                                   const double* __restrict__ gC, double* __restrict__ z) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < nde) z[j] = jy[j] + gq[de[j]];
-  else if (j < nde + nf) z[j] = jy[j] + gC[j - nde];
+  if (j < nde) z[sys.de_pos[de[j]]] = jy[sys.de_pos[de[j]]] + gq[de[j]];
+  else if (j < nde + nf) z[sys.fpos[j - nde]] = jy[sys.fpos[j - nde]] + gC[j - nde];
 }
 
 // RHS^T: b_j = S1/t1 - S0/t0 (intersection) or phi - S0, S = mean of the series over K points
 __global__ __launch_bounds__(64) void adj_rhsT_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
-                                                      const double* __restrict__ mu, const int* __restrict__ pt_off,
-                                                      int K, double2* __restrict__ w) {
+                                                      const int* __restrict__ de_pos, const double* __restrict__ mu,
+                                                      const int* __restrict__ pt_off, int K, double2* __restrict__ w) {
   const int j = blockIdx.x;
   if (j >= nde) return;
   const RhsItem it = items[j];
   const int e = it.elem;
-  const double m = mu[j] / (double)K;
+  const double m = mu[de_pos[e]] / (double)K;
   double v0, v1 = 0.0;
   if (net.etype[e] == 0) {
     v0 = -m / net.t[net.efrac0[e]];
@@ -339,6 +341,8 @@ __global__ __launch_bounds__(64) void adj_tgrad_kernel(NetView net, const FitIte
 template <class T>
 __global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ A, int64_t n, T* __restrict__ B) {
   __shared__ T tile[32][33];
+  A += (int64_t)blockIdx.z * n * n;
+  B += (int64_t)blockIdx.z * n * n;
   const int64_t bx = (int64_t)blockIdx.x * 32, by = (int64_t)blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
@@ -435,17 +439,20 @@ void launch_adj_csum(int nf, const int* fp_beg, const int* fp_end, const double2
   hipLaunchKernelGGL(adj_csum_kernel, dim3(nf), dim3(64), 0, st, nf, fp_beg, fp_end, w, accumulate, gC);
 }
 
-void launch_adj_build_z(int nde, int nf, const int* de, const double* jy, const double* gq, const double* gC,
-                        double* z, hipStream_t st) {
-  const int n = nde + nf;
+void launch_adj_build_z(int nde, int nf, const int* de, const SysLayout& sys, int64_t n, const double* jy,
+                        const double* gq, const double* gC, double* z, hipStream_t st) {
+  // copy first, then the element/fracture entries (two launches: the scatter may cross workgroups)
   if (n <= 0) return;
-  hipLaunchKernelGGL(adj_build_z_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nde, nf, de, jy, gq, gC, z);
+  (void)hipMemcpyAsync(z, jy, sizeof(double) * n, hipMemcpyDeviceToDevice, st);
+  const int m = nde + nf;
+  if (m <= 0) return;
+  hipLaunchKernelGGL(adj_build_z_kernel, dim3((m + 255) / 256), dim3(256), 0, st, nde, nf, de, sys, jy, gq, gC, z);
 }
 
-void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const double* mu, const int* pt_off, int K,
-                     double2* w, hipStream_t st) {
+void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const int* de_pos, const double* mu,
+                     const int* pt_off, int K, double2* w, hipStream_t st) {
   if (nde <= 0) return;
-  hipLaunchKernelGGL(adj_rhsT_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, mu, pt_off, K, w);
+  hipLaunchKernelGGL(adj_rhsT_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, de_pos, mu, pt_off, K, w);
 }
 
 void launch_adj_update(int64_t cnt, const double2* jc, const double2* gc, double2* lam, double damping,
@@ -478,13 +485,13 @@ void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, c
 }
 
 template <class T>
-void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st) {
-  if (n <= 0) return;
+void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st, int nblk) {
+  if (n <= 0 || nblk <= 0) return;
   const unsigned g = (unsigned)((n + 31) / 32);
-  hipLaunchKernelGGL(transpose_kernel<T>, dim3(g, g), dim3(256), 0, st, A, n, B);
+  hipLaunchKernelGGL(transpose_kernel<T>, dim3(g, g, nblk), dim3(256), 0, st, A, n, B);
 }
-template void launch_transpose<double>(const double*, int64_t, double*, hipStream_t);
-template void launch_transpose<float>(const float*, int64_t, float*, hipStream_t);
+template void launch_transpose<double>(const double*, int64_t, double*, hipStream_t, int);
+template void launch_transpose<float>(const float*, int64_t, float*, hipStream_t, int);
 
 template <class T>
 void launch_adj_refine_in(const double* z, const double* aty, int n, T* x, hipStream_t st) {

@@ -564,7 +564,8 @@ __global__ __launch_bounds__(64) void cross_kernel(NetView net, FitView fv, cons
 // --------------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
                                                         const double* __restrict__ val, const int* __restrict__ pt_off,
-                                                        int K, double* __restrict__ b) {
+                                                        int K, const int* __restrict__ de_pos,
+                                                        double* __restrict__ b) {
   const int j = blockIdx.x;
   if (j >= nde) return;
   const RhsItem it = items[j];
@@ -583,9 +584,9 @@ __global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsIt
     const double om0 = s0 / (double)K;
     if (net.etype[e] == 0) {
       const double om1 = s1 / (double)K;
-      b[j] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
+      b[de_pos[e]] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
     } else {
-      b[j] = net.phi[e] - om0;
+      b[de_pos[e]] = net.phi[e] - om0;
     }
   }
 }
@@ -595,12 +596,12 @@ __global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsIt
 // hpc_intersection.py:22-46, hpc_well.py:14-35, hpc_const_head_line.py:14-35)
 // one wavefront per (discharge row, side); dense column-major A (lda = n)
 // --------------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg, const int* __restrict__ de_pos,
-                                                      int K, int nde, int64_t n, double* __restrict__ A) {
+__global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg, SysLayout sys, int K,
+                                                      double* __restrict__ A) {
   const int s = blockIdx.x;
   const int e = tg.side_elem[s];
   const int which = tg.side_which[s];
-  const int row = de_pos[e];
+  const int64_t row = sys.de_pos[e];
   const int f = which ? net.efrac1[e] : net.efrac0[e];
   const int lane = threadIdx.x;
   const bool inter = net.etype[e] == 0;
@@ -625,40 +626,48 @@ __global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg
       acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log_pos(r2);
     }
     acc = wave_sum(acc);
-    if (lane == 0) atomicAdd(&A[(int64_t)de_pos[src] * n + row], rs * acc / (double)K);
+    if (lane == 0) atomicAdd(&A[mat_at(row, sys.de_pos[src], sys.npad)], rs * acc / (double)K);
   }
   if (lane == 0) {
     if (inter) {
-      if (which == 0) atomicAdd(&A[(int64_t)(nde + net.efrac0[e]) * n + row], 1.0 / net.t[net.efrac0[e]]);
-      else atomicAdd(&A[(int64_t)(nde + net.efrac1[e]) * n + row], -1.0 / net.t[net.efrac1[e]]);
+      if (which == 0) atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys.npad)], 1.0 / net.t[net.efrac0[e]]);
+      else atomicAdd(&A[mat_at(row, sys.fpos[net.efrac1[e]], sys.npad)], -1.0 / net.t[net.efrac1[e]]);
     } else {
-      atomicAdd(&A[(int64_t)(nde + net.efrac0[e]) * n + row], 1.0);
+      atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys.npad)], 1.0);
     }
   }
 }
 
 // fracture continuity rows: sum_e s_e q_e = 0 (hpc_solve.py:1187-1201)
-__global__ void continuity_kernel(NetView net, const int* __restrict__ de_pos, int nf, int nde, int64_t n,
-                                  double* __restrict__ A) {
+__global__ void continuity_kernel(NetView net, SysLayout sys, int nf, double* __restrict__ A) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  const int64_t row = nde + f;
+  const int64_t row = sys.fpos[f];
   for (int k = net.mem_ptr[f]; k < net.mem_ptr[f + 1]; ++k) {
     const int e = net.slots[k].elem;
     const int ty = net.etype[e];
     if (!(ty == 0 || ty == 2 || ty == 3)) continue;
     const double v = (ty != 0 || net.efrac0[e] == f) ? 1.0 : -1.0;
-    A[(int64_t)de_pos[e] * n + row] += v;
+    A[mat_at(row, sys.de_pos[e], sys.npad)] += v;
   }
+}
+
+// identity rows/columns of the padding unknowns of every block
+__global__ void pad_identity_kernel(double* __restrict__ A, int64_t npad, int nblk, const int* __restrict__ blk_n) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= npad * nblk) return;
+  const int64_t b = g / npad, i = g - b * npad;
+  if (i >= blk_n[b]) A[b * npad * npad + i * npad + i] = 1.0;
 }
 
 // --------------------------------------------------------------------------------------
 // discharge scatter and relative change (hpc_solve.py:169-174, post_matrix_solve 720-761)
 // --------------------------------------------------------------------------------------
-__global__ void scatter_kernel(NetView net, const int* __restrict__ de, int nde, int nf, const double* __restrict__ y) {
+__global__ void scatter_kernel(NetView net, const int* __restrict__ de, int nde, int nf, SysLayout sys,
+                               const double* __restrict__ y) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < nde) net.q_w[de[k]] = y[k];
-  else if (k < nde + nf) net.constant_w[k - nde] = y[k];
+  if (k < nde) net.q_w[de[k]] = y[sys.de_pos[de[k]]];
+  else if (k < nde + nf) net.constant_w[k - nde] = y[sys.fpos[k - nde]];
 }
 
 __device__ __forceinline__ void nanmax_merge(double& a, int& ia, double b, int ib) {
@@ -808,33 +817,39 @@ __global__ void permute_kernel(const double* __restrict__ b, const int* __restri
 // Pass 1: each thread takes one row over a chunk of columns and merges its maximum with an
 // integer atomic max on the bit pattern (ordered like the value for non-negative doubles).
 constexpr int kColChunk = 256;
-__global__ void row_absmax_kernel(const double* __restrict__ A, int64_t n, unsigned long long* __restrict__ mx) {
+__global__ void row_absmax_kernel(const double* __restrict__ A, int64_t npad, int nblk,
+                                  unsigned long long* __restrict__ mx) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= npad * nblk) return;
+  const int64_t b = j / npad, r = j - b * npad;
+  const double* a = A + b * npad * npad + r;
   const int64_t k0 = (int64_t)blockIdx.y * kColChunk;
-  const int64_t k1 = k0 + kColChunk < n ? k0 + kColChunk : n;
+  const int64_t k1 = k0 + kColChunk < npad ? k0 + kColChunk : npad;
   double m = 0.0;
-  for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(A[k * n + j]));
+  for (int64_t k = k0; k < k1; ++k) m = fmax(m, fabs(a[k * npad]));
   atomicMax(&mx[j], (unsigned long long)__double_as_longlong(m));
 }
 
-__global__ void row_apply_kernel(double* __restrict__ A, int64_t n, const unsigned long long* __restrict__ mx,
-                                 double* __restrict__ rs) {
+__global__ void row_apply_kernel(double* __restrict__ A, int64_t npad, int nblk,
+                                 const unsigned long long* __restrict__ mx, double* __restrict__ rs) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= npad * nblk) return;
   const double m = __longlong_as_double((long long)mx[j]);
   const double s = m > 0.0 ? 1.0 / m : 1.0;
   if (blockIdx.y == 0) rs[j] = s;
+  const int64_t b = j / npad, r = j - b * npad;
+  double* a = A + b * npad * npad + r;
   const int64_t k0 = (int64_t)blockIdx.y * kColChunk;
-  const int64_t k1 = k0 + kColChunk < n ? k0 + kColChunk : n;
-  for (int64_t k = k0; k < k1; ++k) A[k * n + j] *= s;
+  const int64_t k1 = k0 + kColChunk < npad ? k0 + kColChunk : npad;
+  for (int64_t k = k0; k < k1; ++k) a[k * npad] *= s;
 }
 
-void launch_row_scale(double* A, int64_t n, double* rs, unsigned long long* work, hipStream_t st) {
-  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + kColChunk - 1) / kColChunk));
-  (void)hipMemsetAsync(work, 0, sizeof(unsigned long long) * n, st);
-  hipLaunchKernelGGL(row_absmax_kernel, grid, dim3(256), 0, st, A, n, work);
-  hipLaunchKernelGGL(row_apply_kernel, grid, dim3(256), 0, st, A, n, work, rs);
+void launch_row_scale(double* A, int64_t npad, int nblk, double* rs, unsigned long long* work, hipStream_t st) {
+  const int64_t rows = npad * nblk;
+  dim3 grid((unsigned)((rows + 255) / 256), (unsigned)((npad + kColChunk - 1) / kColChunk));
+  (void)hipMemsetAsync(work, 0, sizeof(unsigned long long) * rows, st);
+  hipLaunchKernelGGL(row_absmax_kernel, grid, dim3(256), 0, st, A, npad, nblk, work);
+  hipLaunchKernelGGL(row_apply_kernel, grid, dim3(256), 0, st, A, npad, nblk, work, rs);
 }
 
 __global__ void scale_kernel(const double* __restrict__ b, const double* __restrict__ rs, int n, double* __restrict__ x) {
@@ -847,15 +862,18 @@ void launch_scale(const double* b, const double* rs, int n, double* x, hipStream
 }
 
 // CSR copy of the assembled (unscaled, unfactored) dense matrix: val[k] = A[row, col[k]]
-__global__ void csr_gather_kernel(const double* __restrict__ A, int64_t n, const int* __restrict__ row_ptr,
-                                  const int* __restrict__ col, double* __restrict__ val) {
+__global__ void csr_gather_kernel(const double* __restrict__ A, int64_t npad, int64_t nrows,
+                                  const int* __restrict__ row_ptr, const int* __restrict__ col,
+                                  double* __restrict__ val) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) val[k] = A[(int64_t)col[k] * n + r];
+  if (r >= nrows) return;
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) val[k] = A[mat_at(r, col[k], npad)];
 }
 
-void launch_csr_gather(const double* A, int64_t n, const int* row_ptr, const int* col, double* val, hipStream_t st) {
-  hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n, row_ptr, col, val);
+void launch_csr_gather(const double* A, int64_t npad, int64_t nrows, const int* row_ptr, const int* col, double* val,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, A, npad, nrows,
+                     row_ptr, col, val);
 }
 
 // y = A x for a CSR matrix (one thread per row)
@@ -929,41 +947,50 @@ __device__ __forceinline__ void gemv_cm_block(const T* __restrict__ A, int64_t l
 template <class T>
 __global__ __launch_bounds__(64 * kGemvWaves) void gemv_cm_kernel(const T* __restrict__ A, int64_t lda, int m,
                                                                   int ncols, const T* __restrict__ x,
-                                                                  T* __restrict__ y, T alpha, T beta) {
-  gemv_cm_block<T>(A, lda, m, ncols, x, y, alpha, beta, 0, blockIdx.x);
+                                                                  T* __restrict__ y, T alpha, T beta, int64_t sA,
+                                                                  int64_t sx) {
+  const int64_t z = blockIdx.z;
+  gemv_cm_block<T>(A + z * sA, lda, m, ncols, x + z * sx, y + z * sx, alpha, beta, 0, blockIdx.x);
 }
 
 // y_j = B_j x_j for the nb stacked nbk x nbk triangular blocks B_j (block j covers rows
 // j*nbk .. min(n, (j+1)*nbk)); blockIdx.y selects the block
 template <class T>
 __global__ __launch_bounds__(64 * kGemvWaves) void gemv_diag_kernel(const T* __restrict__ B, int nbk, int n, int tri,
-                                                                    const T* __restrict__ x, T* __restrict__ y) {
+                                                                    const T* __restrict__ x, T* __restrict__ y,
+                                                                    int64_t sB) {
   const int j = blockIdx.y;
   const int r0 = j * nbk;
   const int bs = min(nbk, n - r0);
   if ((int)blockIdx.x * 64 >= bs) return;  // uniform per workgroup, before any barrier
-  gemv_cm_block<T>(B + (size_t)j * nbk * nbk, nbk, bs, bs, x + r0, y + r0, T(1), T(0), tri, blockIdx.x);
+  const int64_t z = blockIdx.z;
+  gemv_cm_block<T>(B + z * sB + (size_t)j * nbk * nbk, nbk, bs, bs, x + z * n + r0, y + z * n + r0, T(1), T(0), tri,
+                   blockIdx.x);
 }
 
 template <class T>
-void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st) {
-  if (m <= 0 || ncols <= 0) return;
-  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64), dim3(64 * kGemvWaves), 0, st, A, lda, m, ncols, x, y,
-                     alpha, beta);
+void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st,
+                    int batch, int64_t sA, int64_t sx) {
+  if (m <= 0 || ncols <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64, 1, batch), dim3(64 * kGemvWaves), 0, st, A, lda, m, ncols,
+                     x, y, alpha, beta, sA, sx);
 }
 template void launch_gemv_cm<double>(const double*, int64_t, int, int, const double*, double*, double, double,
-                                     hipStream_t);
-template void launch_gemv_cm<float>(const float*, int64_t, int, int, const float*, float*, float, float, hipStream_t);
+                                     hipStream_t, int, int64_t, int64_t);
+template void launch_gemv_cm<float>(const float*, int64_t, int, int, const float*, float*, float, float, hipStream_t,
+                                    int, int64_t, int64_t);
 
 template <class T>
-void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st) {
-  if (n <= 0) return;
+void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st, int batch,
+                      int64_t sB) {
+  if (n <= 0 || batch <= 0) return;
   const int nb = (n + nbk - 1) / nbk;
-  hipLaunchKernelGGL(gemv_diag_kernel<T>, dim3((nbk + 63) / 64, nb), dim3(64 * kGemvWaves), 0, st, blocks, nbk, n, tri,
-                     x, y);
+  hipLaunchKernelGGL(gemv_diag_kernel<T>, dim3((nbk + 63) / 64, nb, batch), dim3(64 * kGemvWaves), 0, st, blocks, nbk,
+                     n, tri, x, y, sB);
 }
-template void launch_gemv_diag<double>(const double*, int, int, int, const double*, double*, hipStream_t);
-template void launch_gemv_diag<float>(const float*, int, int, int, const float*, float*, hipStream_t);
+template void launch_gemv_diag<double>(const double*, int, int, int, const double*, double*, hipStream_t, int,
+                                       int64_t);
+template void launch_gemv_diag<float>(const float*, int, int, int, const float*, float*, hipStream_t, int, int64_t);
 
 // diagonal of nb stacked nbk x nbk blocks set to one (unit-triangular inverses)
 template <class T>
@@ -1030,22 +1057,28 @@ void launch_cross(const NetView& net, const FitView& fv, const int* circles, int
 }
 
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
-                       double* b, hipStream_t st) {
+                       const int* de_pos, double* b, hipStream_t st) {
   if (nde <= 0) return;
-  hipLaunchKernelGGL(rhs_reduce_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, val, pt_off, K, b);
+  hipLaunchKernelGGL(rhs_reduce_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, val, pt_off, K, de_pos, b);
 }
 
-void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const int* de_pos, int K, int nde, int nf,
-                     int64_t n, double* A, hipStream_t st) {
-  if (nsides > 0)
-    hipLaunchKernelGGL(assemble_kernel, dim3(nsides), dim3(64), 0, st, net, tg, de_pos, K, nde, n, A);
-  if (nf > 0) hipLaunchKernelGGL(continuity_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, net, de_pos, nf, nde, n, A);
+void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
+                     double* A, hipStream_t st) {
+  if (nsides > 0) hipLaunchKernelGGL(assemble_kernel, dim3(nsides), dim3(64), 0, st, net, tg, sys, K, A);
+  if (nf > 0) hipLaunchKernelGGL(continuity_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, net, sys, nf, A);
 }
 
-void launch_scatter(const NetView& net, const int* de, int nde, int nf, const double* y, hipStream_t st) {
+void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st) {
+  const int64_t tot = npad * nblk;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(pad_identity_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, npad, nblk, blk_n);
+}
+
+void launch_scatter(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* y,
+                    hipStream_t st) {
   int tot = nde + nf;
   if (tot <= 0) return;
-  hipLaunchKernelGGL(scatter_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, net, de, nde, nf, y);
+  hipLaunchKernelGGL(scatter_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, net, de, nde, nf, sys, y);
 }
 
 void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
@@ -1059,17 +1092,18 @@ void launch_init_errors(const FitView& fv, int ne, hipStream_t st) {
 }
 
 // max over the head equations of |(A y)_j - b_j| in metres (constant-head/well rows are in
-// potential units and are divided by t of their fracture via row_scale)
+// potential units and are divided by t of their fracture via row_scale, which is 0 on the
+// continuity and padding rows)
 __global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict__ ay, const double* __restrict__ b,
-                                                        const double* __restrict__ row_scale, int nde,
+                                                        const double* __restrict__ row_scale, int nrows,
                                                         IterScalars* __restrict__ out) {
   __shared__ double sr[1024];
   __shared__ int si[1024];
   const int tid = threadIdx.x;
   double r = 0.0;
   int ir = 0x7fffffff;
-  for (int j = tid; j < nde; j += 1024) {
-    double v = fabs(ay[j] - b[j]) * row_scale[j];
+  for (int j = tid; j < nrows; j += 1024) {
+    double v = fabs(ay[j] - b[j]) * row_scale[j];  // row_scale 0 outside the head equations
     if (v != v || v > r) {
       r = v;
       ir = j;

@@ -73,6 +73,22 @@ struct FitView {
   int* status;             // bit 0: a crossing list overflowed
 };
 
+// Unknowns of the discharge system: nblk independent blocks (connected components of the
+// network) of npad unknowns each, real unknowns first and identity padding after.  Unknown
+// g = b npad + local; block b's matrix is the column-major npad x npad slab at b npad^2.
+struct SysLayout {
+  int64_t npad;
+  int nblk;
+  const int* de_pos;  // [E] unknown of each discharge element (-1 for other elements)
+  const int* fpos;    // [F] unknown of each fracture constant
+};
+
+// position of (row r, column c) of the same block in the block-slab storage
+__host__ __device__ __forceinline__ int64_t mat_at(int64_t r, int64_t c, int64_t npad) {
+  const int64_t b = r / npad;
+  return b * npad * npad + (c - b * npad) * npad + (r - b * npad);
+}
+
 struct TargetView {
   const Tile* tiles;
   const int* side_elem;
@@ -118,14 +134,19 @@ void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
 void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
                  hipStream_t st);
-void launch_csr_gather(const double* A, int64_t n, const int* row_ptr, const int* col, double* val, hipStream_t st);
+void launch_csr_gather(const double* A, int64_t npad, int64_t nrows, const int* row_ptr, const int* col, double* val,
+                       hipStream_t st);
 void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t st);
+// batch > 1: the same product on `batch` slabs, A + z sA, x + z sx, y + z sx
 template <class T>
-void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st);
-// y_j = B_j x_j for the stacked nbk x nbk diagonal blocks (tri: 0 full, 1 lower, 2 upper)
+void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st,
+                    int batch = 1, int64_t sA = 0, int64_t sx = 0);
+// y_j = B_j x_j for the stacked nbk x nbk diagonal blocks (tri: 0 full, 1 lower, 2 upper) of
+// `batch` systems of n unknowns (blocks + z sB, x and y + z n)
 template <class T>
-void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st);
+void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y, hipStream_t st, int batch = 1,
+                      int64_t sB = 0);
 template <class T>
 void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st);
 void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);
@@ -133,14 +154,17 @@ void launch_refine_rhs(const double* b, const double* ay, const int* perm, const
                        hipStream_t st);
 void launch_refine_update(const double* y, const float* d, int n, double* x, hipStream_t st);
 void launch_permute(const double* b, const int* perm, const double* rs, int n, double* x, hipStream_t st);
-void launch_row_scale(double* A, int64_t n, double* rs, unsigned long long* work, hipStream_t st);
+void launch_row_scale(double* A, int64_t npad, int nblk, double* rs, unsigned long long* work, hipStream_t st);
 void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st);
 void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st);
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
-                       double* b, hipStream_t st);
-void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const int* de_pos, int K, int nde, int nf,
-                     int64_t n, double* A, hipStream_t st);
-void launch_scatter(const NetView& net, const int* de, int nde, int nf, const double* y, hipStream_t st);
+                       const int* de_pos, double* b, hipStream_t st);
+void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
+                     double* A, hipStream_t st);
+// A[b][i][i] = 1 for the padding unknowns i >= blk_n[b] of every block
+void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st);
+void launch_scatter(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* y,
+                    hipStream_t st);
 void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
                    hipStream_t st);
 void launch_init_errors(const FitView& fv, int ne, hipStream_t st);
@@ -163,10 +187,10 @@ void launch_adj_cross(int nsrc, const int* srcs, const int* cptr, const int4* ce
                       double* gq, hipStream_t st);
 void launch_adj_csum(int nf, const int* fp_beg, const int* fp_end, const double2* w, int accumulate, double* gC,
                      hipStream_t st);
-void launch_adj_build_z(int nde, int nf, const int* de, const double* jy, const double* gq, const double* gC,
-                        double* z, hipStream_t st);
-void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const double* mu, const int* pt_off, int K,
-                     double2* w, hipStream_t st);
+void launch_adj_build_z(int nde, int nf, const int* de, const SysLayout& sys, int64_t n, const double* jy,
+                        const double* gq, const double* gC, double* z, hipStream_t st);
+void launch_adj_rhsT(const NetView& net, const RhsItem* items, int nde, const int* de_pos, const double* mu,
+                     const int* pt_off, int K, double2* w, hipStream_t st);
 void launch_adj_update(int64_t cnt, const double2* jc, const double2* gc, double2* lam, double damping,
                        unsigned long long* stats, hipStream_t st);
 void launch_adj_tgrad(const NetView& net, const FitItem* items, int nitems, const double2* lam, const double* val,
@@ -175,8 +199,9 @@ void launch_adj_pack(int64_t np, const int64_t* pidx, const double2* full, doubl
 void launch_adj_unpack(int64_t np, const int64_t* pidx, const double2* packed, double2* full, hipStream_t st);
 void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, const double2* gc, double2* out,
                             hipStream_t st);
+// B_b = A_b^T for the nblk column-major n x n slabs
 template <class T>
-void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st);
+void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st, int nblk = 1);
 template <class T>
 void launch_adj_refine_in(const double* z, const double* aty, int n, T* x, hipStream_t st);
 template <class T>

@@ -237,11 +237,25 @@ class Solver {
   DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
   DBuf<int> d_csr_ptr_, d_csr_col_;
   int64_t nnz_A_ = 0;
+  // discharge-system layout: one dense block per connected component of the network, padded to
+  // npad_ unknowns (discharge elements in element order, then fracture constants, then padding)
+  int64_t npad_ = 0, N_ = 0;
+  int nblk_ = 1;
+  std::vector<int> fpos_, blk_n_;
+  DBuf<int> d_fpos_, d_blk_n_;
+  aem::SysLayout sys() const {
+    aem::SysLayout l;
+    l.npad = npad_;
+    l.nblk = nblk_;
+    l.de_pos = d_de_pos_.p;
+    l.fpos = d_fpos_.p;
+    return l;
+  }
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
   static constexpr int kNB = 1024;
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   template <class T>
-  void prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
+  void prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
   template <class T>
   void blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v);
   // mixed precision: single-precision factors and blocked-solve buffers
@@ -478,6 +492,38 @@ void Solver::parse() {
     if (etype_[e] == 2) wells_.push_back(e);
     if (is_circle_like(etype_[e])) circles_.push_back(e);
   }
+  // connected components of the fractures through intersections: independent discharge blocks
+  {
+    std::vector<int> par(F_);
+    for (int f = 0; f < F_; ++f) par[f] = f;
+    auto root = [&](int a) {
+      while (par[a] != a) a = par[a] = par[par[a]];
+      return a;
+    };
+    for (int e = 0; e < E_; ++e)
+      if (etype_[e] == 0) {
+        const int a = root(efrac0_[e]), b = root(efrac1_[e]);
+        if (a != b) par[std::max(a, b)] = std::min(a, b);
+      }
+    std::vector<int> bid_of_root(F_, -1), bid(F_);
+    nblk_ = 0;
+    for (int f = 0; f < F_; ++f) {
+      const int r = root(f);
+      if (bid_of_root[r] < 0) bid_of_root[r] = nblk_++;
+      bid[f] = bid_of_root[r];
+    }
+    std::vector<int> cnt(nblk_, 0), local_e(E_, -1), local_f(F_);
+    for (int e : de_) local_e[e] = cnt[bid[efrac0_[e]]]++;
+    for (int f = 0; f < F_; ++f) local_f[f] = cnt[bid[f]]++;
+    npad_ = 0;
+    for (int b = 0; b < nblk_; ++b) npad_ = std::max<int64_t>(npad_, cnt[b]);
+    blk_n_ = cnt;
+    N_ = npad_ * nblk_;
+    if (N_ > INT32_MAX) fail(ANDFN_MI_ECAPACITY, "discharge system of %lld padded unknowns", (long long)N_);
+    fpos_.assign(F_, 0);
+    for (int e : de_) de_pos_[e] = (int)(bid[efrac0_[e]] * npad_ + local_e[e]);
+    for (int f = 0; f < F_; ++f) fpos_[f] = (int)(bid[f] * npad_ + local_f[f]);
+  }
   cross_ptr_.assign(E_, 0);
   cross_cap_.assign(E_, 0);
   int tot = 0;
@@ -520,6 +566,8 @@ void Solver::upload() {
   d_slots_.upload(slots_, st_);
   d_de_.upload(de_, st_);
   d_de_pos_.upload(de_pos_, st_);
+  d_fpos_.upload(fpos_, st_);
+  d_blk_n_.upload(blk_n_, st_);
   d_wells_.upload(wells_, st_);
   d_circles_.upload(circles_, st_);
   d_cross_ptr_.upload(cross_ptr_, st_);
@@ -552,10 +600,9 @@ void Solver::snapshot() {
   dcopy(s_nint_, d_nint_, E_, st_);
   ncoef0_ = ncoef_;
   nint0_ = nint_;
-  const int nde = (int)de_.size();
-  y0_.assign(nde + F_, 0.0);
-  for (int j = 0; j < nde; ++j) y0_[j] = q_[de_[j]];
-  for (int f = 0; f < F_; ++f) y0_[nde + f] = constant_[f];
+  y0_.assign(N_, 0.0);
+  for (int e : de_) y0_[de_pos_[e]] = q_[e];
+  for (int f = 0; f < F_; ++f) y0_[fpos_[f]] = constant_[f];
   HIP_CHECK(hipStreamSynchronize(st_));
 }
 
@@ -703,73 +750,89 @@ static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_di
 // lower_unit: L has the unit diagonal (LU factors); false for the transposed factors U^T L^T,
 // whose lower triangle carries the diagonal and whose upper triangle is unit.
 template <class T>
-void Solver::prepare_blocked(T* A, int64_t n, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
+void Solver::prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
   const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
   const rocblas_diagonal dl = lower_unit ? rocblas_diagonal_unit : rocblas_diagonal_non_unit;
   const rocblas_diagonal du = lower_unit ? rocblas_diagonal_non_unit : rocblas_diagonal_unit;
-  linv.zero((size_t)nb * kNB * kNB, st_);
-  uinv.zero((size_t)nb * kNB * kNB, st_);
-  if (nfull > 0) {
-    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, A, n, kNB * (n + 1), linv.p, kNB, (int64_t)kNB * kNB,
-                              (int)nfull));
-    RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, kNB, A, n, kNB * (n + 1), uinv.p, kNB, (int64_t)kNB * kNB,
-                              (int)nfull));
-  }
-  if (nb > nfull) {
-    const int64_t r0 = nfull * kNB;
-    const int rem = (int)(n - r0);
-    const T* blk = A + r0 * (n + 1);
-    RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, dl, rem, blk, n, linv.p + nfull * kNB * kNB, kNB));
-    RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, du, rem, blk, n, uinv.p + nfull * kNB * kNB, kNB));
+  const int64_t sinv = nb * kNB * kNB;  // inverse blocks of one system
+  linv.zero((size_t)(nblk * sinv), st_);
+  uinv.zero((size_t)(nblk * sinv), st_);
+  for (int b = 0; b < nblk; ++b) {
+    T* Ab = A + (int64_t)b * n * n;
+    T* li = linv.p + b * sinv;
+    T* ui = uinv.p + b * sinv;
+    if (nfull > 0) {
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, Ab, n, kNB * (n + 1), li, kNB, (int64_t)kNB * kNB,
+                                (int)nfull));
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, kNB, Ab, n, kNB * (n + 1), ui, kNB, (int64_t)kNB * kNB,
+                                (int)nfull));
+    }
+    if (nb > nfull) {
+      const int64_t r0 = nfull * kNB;
+      const int rem = (int)(n - r0);
+      const T* blk = Ab + r0 * (n + 1);
+      RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, dl, rem, blk, n, li + nfull * kNB * kNB, kNB));
+      RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, du, rem, blk, n, ui + nfull * kNB * kNB, kNB));
+    }
   }
   // a unit-triangular inverse is written without its diagonal
-  aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)nb, kNB, st_);
-  for (int64_t j = 0; j < nb; ++j) {
-    const int64_t r0 = j * kNB;
-    const int bs = (int)std::min<int64_t>(kNB, n - r0);
-    const int rest = (int)(n - r0 - bs);
-    if (rest > 0)
-      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + j * kNB * kNB, kNB, A + r0 * n + r0 + bs,
-                             n));
-    if (r0 > 0)
-      RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + j * kNB * kNB, kNB, A + r0 * n, n));
+  aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)(nb * nblk), kNB, st_);
+  for (int b = 0; b < nblk; ++b) {
+    T* Ab = A + (int64_t)b * n * n;
+    for (int64_t j = 0; j < nb; ++j) {
+      const int64_t r0 = j * kNB;
+      const int bs = (int)std::min<int64_t>(kNB, n - r0);
+      const int rest = (int)(n - r0 - bs);
+      if (rest > 0)
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + b * sinv + j * kNB * kNB, kNB,
+                               Ab + r0 * n + r0 + bs, n));
+      if (r0 > 0)
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + b * sinv + j * kNB * kNB, kNB,
+                               Ab + r0 * n, n));
+    }
   }
 }
 
 void Solver::assemble_and_factor() {
   drop_graph();
   adjT_valid_ = false;
-  const int nde = (int)de_.size();
-  const int64_t n = nde + F_;
-  d_A_.zero((size_t)(n * n), st_);
+  const int64_t npad = npad_, N = N_;
+  const int nblk = nblk_;
+  if (nblk > 1 && lu_mode_ != ANDFN_MI_LU_TRSV)
+    fail(ANDFN_MI_EINVAL, "a network of %d independent parts is solved with the blocked triangular solves (lu_mode 0)",
+         nblk);
+  d_A_.zero((size_t)(N * npad), st_);
   aem::TargetView tv = rhs_.view();
-  aem::launch_assemble(net(), tv, (int)rhs_.side_elem.size(), d_de_pos_.p, K_, nde, F_, n, d_A_.p, st_);
+  aem::launch_assemble(net(), tv, (int)rhs_.side_elem.size(), sys(), K_, F_, d_A_.p, st_);
+  aem::launch_pad_identity(d_A_.p, npad, nblk, d_blk_n_.p, st_);
   HIP_CHECK(hipGetLastError());
-  // a CSR copy of the unfactored matrix is kept for the head residual A y - b
+  // a CSR copy of the unfactored matrix (rows in unknown order) for the head residual A y - b;
+  // row_scale: 1 on intersection rows, 1/t on constant-head and well rows, 0 elsewhere
   {
-    std::vector<int> rp(1, 0), cols;
-    std::vector<int> tmp;
-    for (int j = 0; j < nde; ++j) {
-      const int e = de_[j];
+    std::vector<int> row_e(N, -1), row_f(N, -1);
+    for (int e : de_) row_e[de_pos_[e]] = e;
+    for (int f = 0; f < F_; ++f) row_f[fpos_[f]] = f;
+    std::vector<int> rp(1, 0), cols, tmp;
+    std::vector<double> rs(N, 0.0);
+    for (int64_t g = 0; g < N; ++g) {
       tmp.clear();
-      const int sides = etype_[e] == 0 ? 2 : 1;
-      for (int sd = 0; sd < sides; ++sd) {
-        const int f = sd ? efrac1_[e] : efrac0_[e];
-        for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
-          const int src = slots_[k].elem;
-          if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+      if (row_e[g] >= 0) {
+        const int e = row_e[g];
+        const int sides = etype_[e] == 0 ? 2 : 1;
+        for (int sd = 0; sd < sides; ++sd) {
+          const int f = sd ? efrac1_[e] : efrac0_[e];
+          for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
+            const int src = slots_[k].elem;
+            if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+          }
+          tmp.push_back(fpos_[f]);
         }
-        tmp.push_back(nde + f);
+        rs[g] = etype_[e] == 0 ? 1.0 : 1.0 / t_[efrac0_[e]];
+      } else if (row_f[g] >= 0) {
+        const int f = row_f[g];
+        for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+          if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
       }
-      std::sort(tmp.begin(), tmp.end());
-      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      cols.insert(cols.end(), tmp.begin(), tmp.end());
-      rp.push_back((int)cols.size());
-    }
-    for (int f = 0; f < F_; ++f) {
-      tmp.clear();
-      for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-        if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
       std::sort(tmp.begin(), tmp.end());
       tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
       cols.insert(cols.end(), tmp.begin(), tmp.end());
@@ -778,34 +841,37 @@ void Solver::assemble_and_factor() {
     d_csr_ptr_.upload(rp, st_);
     d_csr_col_.upload(cols, st_);
     d_csr_val_.alloc(cols.size());
-    aem::launch_csr_gather(d_A_.p, n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+    aem::launch_csr_gather(d_A_.p, npad, N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+    d_rowscale_.upload(rs, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
     nnz_A_ = (int64_t)cols.size();
   }
-  {
-    std::vector<double> rs(nde, 1.0);
-    for (int j = 0; j < nde; ++j)
-      if (etype_[de_[j]] != 0) rs[j] = 1.0 / t_[efrac0_[de_[j]]];
-    d_rowscale_.upload(rs, st_);
-    HIP_CHECK(hipStreamSynchronize(st_));
-  }
-  d_ay_.alloc(n);
+  d_ay_.alloc(N);
   // row equilibration (rows of intersections scale with 1/t), as SuperLU equilibrates by default
-  d_req_.alloc(n);
+  d_req_.alloc(N);
   {
     DBuf<unsigned long long> mx;
-    mx.alloc(n);
-    aem::launch_row_scale(d_A_.p, n, d_req_.p, mx.p, st_);
+    mx.alloc(N);
+    aem::launch_row_scale(d_A_.p, npad, nblk, d_req_.p, mx.p, st_);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st_));
   }
-  d_ipiv_.alloc(n);
-  d_info_.zero(1, st_);
-  RB_CHECK(rocsolver_dgetrf_64(rb_, n, n, d_A_.p, n, d_ipiv_.p, d_info_.p));
-  int64_t info = 0;
-  HIP_CHECK(hipMemcpyAsync(&info, d_info_.p, sizeof(info), hipMemcpyDeviceToHost, st_));
-  HIP_CHECK(hipStreamSynchronize(st_));
-  if (info > 0) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot at %lld)", (long long)info);
+  d_ipiv_.alloc(N);
+  d_info_.zero(nblk, st_);
+  if (nblk == 1)
+    RB_CHECK(rocsolver_dgetrf_64(rb_, npad, npad, d_A_.p, npad, d_ipiv_.p, d_info_.p));
+  else
+    RB_CHECK(rocsolver_dgetrf_strided_batched_64(rb_, npad, npad, d_A_.p, npad, (rocblas_stride)(npad * npad),
+                                                 d_ipiv_.p, (rocblas_stride)npad, d_info_.p, nblk));
+  {
+    std::vector<int64_t> info(nblk, 0);
+    HIP_CHECK(hipMemcpyAsync(info.data(), d_info_.p, sizeof(int64_t) * nblk, hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    for (int b = 0; b < nblk; ++b)
+      if (info[b] > 0)
+        fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (block %d, zero pivot at %lld)", b,
+             (long long)info[b]);
+  }
   solve32_ = false;
   a64_blocked_ = false;
   if (precision_ == ANDFN_MI_MIXED) {
@@ -813,39 +879,43 @@ void Solver::assemble_and_factor() {
     // of the CSR copy drives the refinement, so the iteration's fixed point is the FP64 one.
     // (rocSOLVER's sgetrf_64 in this ROCm reports spurious zero pivots at n = 65,690, on random
     // matrices too, and is no faster than dgetrf below that size, so the factorisation is FP64.)
-    d_A32_.alloc((size_t)(n * n));
-    aem::launch_to_f32(d_A_.p, n * n, d_A32_.p, st_);
+    d_A32_.alloc((size_t)(N * npad));
+    aem::launch_to_f32(d_A_.p, N * npad, d_A32_.p, st_);
     solve32_ = true;
   }
-  d_x_.alloc(n);
-  d_w_.alloc(n);
+  d_x_.alloc(N);
+  d_w_.alloc(N);
   if (lu_mode_ == ANDFN_MI_LU_TRSV) {
-    // LAPACK row interchanges -> one gather permutation: (P b)[i] = b[perm[i]]
-    std::vector<int64_t> ipiv(n);
-    HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
+    // LAPACK row interchanges of every block -> one gather permutation: (P b)[i] = b[perm[i]]
+    std::vector<int64_t> ipiv(N);
+    HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
-    std::vector<int> perm(n);
-    for (int64_t i = 0; i < n; ++i) perm[i] = (int)i;
-    for (int64_t i = 0; i < n; ++i) std::swap(perm[i], perm[ipiv[i] - 1]);
+    std::vector<int> perm(N);
+    for (int b = 0; b < nblk; ++b) {
+      const int64_t o = (int64_t)b * npad;
+      for (int64_t i = 0; i < npad; ++i) perm[o + i] = (int)(o + i);
+      for (int64_t i = 0; i < npad; ++i) std::swap(perm[o + i], perm[o + ipiv[o + i] - 1]);
+    }
     d_perm_.upload(perm, st_);
     if (precision_ == ANDFN_MI_MIXED) {
-      prepare_blocked<float>(d_A32_.p, n, d_linv32_, d_uinv32_);
-      d_x32_.alloc(n);
-      d_w32_.alloc(n);
-      d_v32_.alloc(n);
+      prepare_blocked<float>(d_A32_.p, npad, nblk, d_linv32_, d_uinv32_);
+      d_x32_.alloc(N);
+      d_w32_.alloc(N);
+      d_v32_.alloc(N);
       // the refinement must contract well for the FP32-rounded factors to pay off; otherwise the
       // run solves with the FP64 factors (large or ill-conditioned systems)
       refine_rate_ = refine_probe();
       if (!(refine_rate_ <= 0.25)) switch_to_fp64_solves();
     } else {
-      prepare_blocked<double>(d_A_.p, n, d_linv_, d_uinv_);
+      prepare_blocked<double>(d_A_.p, npad, nblk, d_linv_, d_uinv_);
       a64_blocked_ = true;
     }
-    d_w_.alloc(n);
-    d_v_.alloc(n);
+    d_w_.alloc(N);
+    d_v_.alloc(N);
     HIP_CHECK(hipStreamSynchronize(st_));
   } else if (lu_mode_ == ANDFN_MI_LU_INVERSE) {
     // getri has only the 32-bit interface here (this mode is for comparisons at moderate n)
+    const int64_t n = npad;
     std::vector<int64_t> p64(n);
     HIP_CHECK(hipMemcpyAsync(p64.data(), d_ipiv_.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
@@ -861,8 +931,7 @@ void Solver::assemble_and_factor() {
 // Mixed form: contraction rate of the refinement y <- y + (LU)_32^-1 R P (b - A y) on the probe
 // system A y = A 1: sqrt(r3 / r1) with r_k the head residual after k steps from y = 0.
 double Solver::refine_probe() {
-  const int n = (int)(de_.size() + F_);
-  const int nde = (int)de_.size();
+  const int n = (int)N_;
   DBuf<double> ones, b0, y, x;
   ones.upload(std::vector<double>(n, 1.0), st_);
   b0.alloc(n);
@@ -873,14 +942,14 @@ double Solver::refine_probe() {
   double r[4] = {0, 0, 0, 0};
   for (int k = 0; k < 4; ++k) {
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, y.p, d_ay_.p, st_);
-    aem::launch_residual(d_ay_.p, b0.p, d_rowscale_.p, nde, d_scal_.p, st_);
+    aem::launch_residual(d_ay_.p, b0.p, d_rowscale_.p, n, d_scal_.p, st_);
     aem::IterScalars sc{};
     HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
     r[k] = sc.residual;
     if (k == 3) break;
     aem::launch_refine_rhs(b0.p, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    blocked_solve<float>(d_A32_.p, n, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+    blocked_solve<float>(d_A32_.p, (int)npad_, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
     aem::launch_refine_update(y.p, d_v32_.p, n, x.p, st_);
     HIP_CHECK(hipMemcpyAsync(y.p, x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   }
@@ -893,11 +962,10 @@ double Solver::refine_probe() {
 // stopped contracting).  Builds the FP64 blocked operators on first use.
 void Solver::switch_to_fp64_solves() {
   if (!solve32_) return;
-  const int64_t n = (int64_t)de_.size() + F_;
   if (!a64_blocked_) {
-    prepare_blocked<double>(d_A_.p, n, d_linv_, d_uinv_);
-    d_w_.alloc(n);
-    d_v_.alloc(n);
+    prepare_blocked<double>(d_A_.p, npad_, nblk_, d_linv_, d_uinv_);
+    d_w_.alloc(N_);
+    d_v_.alloc(N_);
     a64_blocked_ = true;
   }
   solve32_ = false;
@@ -942,25 +1010,29 @@ void Solver::solve_rhs() {
   }
 }
 
-// v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten)
+// v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten), for all nblk_
+// blocks of n unknowns at once (every launch covers the same step of every block)
 template <class T>
 void Solver::blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v) {
   const int nb = (n + kNB - 1) / kNB;
+  const int64_t sA = (int64_t)n * n, sinv = (int64_t)nb * kNB * kNB;
   for (int j = 0; j < nb; ++j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0), rest = n - r0 - bs;
-    if (rest > 0) aem::launch_gemv_cm<T>(A + (size_t)r0 * n + r0 + bs, n, rest, bs, x + r0, x + r0 + bs, T(-1), T(1), st_);
+    if (rest > 0)
+      aem::launch_gemv_cm<T>(A + (size_t)r0 * n + r0 + bs, n, rest, bs, x + r0, x + r0 + bs, T(-1), T(1), st_, nblk_,
+                             sA, n);
   }
-  aem::launch_gemv_diag<T>(linv, kNB, n, 1, x, w, st_);
+  aem::launch_gemv_diag<T>(linv, kNB, n, 1, x, w, st_, nblk_, sinv);
   for (int j = nb - 1; j > 0; --j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm<T>(A + (size_t)r0 * n, n, r0, bs, w + r0, w, T(-1), T(1), st_);
+    aem::launch_gemv_cm<T>(A + (size_t)r0 * n, n, r0, bs, w + r0, w, T(-1), T(1), st_, nblk_, sA, n);
   }
-  aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_);
+  aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_, nblk_, sinv);
 }
 
 // x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
 double* Solver::discharge_solve(double* b, double* x) {
-  const rocblas_int n = (rocblas_int)(de_.size() + F_);
+  const rocblas_int n = (rocblas_int)N_;  // one block in the getrs / inverse modes
   if (lu_mode_ == ANDFN_MI_LU_GETRS) {
     aem::launch_scale(b, d_req_.p, n, x, st_);
     RB_CHECK(rocsolver_dgetrs_64(rb_, rocblas_operation_none, n, 1, d_A_.p, n, d_ipiv_.p, x, n));
@@ -976,7 +1048,7 @@ double* Solver::discharge_solve(double* b, double* x) {
     // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
     aem::launch_refine_rhs(b, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    blocked_solve<float>(d_A32_.p, n, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+    blocked_solve<float>(d_A32_.p, (int)npad_, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
     float* v32 = d_v32_.p;
     aem::launch_refine_update(d_y_.p, v32, n, x, st_);
     return x;
@@ -984,16 +1056,16 @@ double* Solver::discharge_solve(double* b, double* x) {
   // R P b, then the blocked triangular solves
   aem::launch_permute(b, d_perm_.p, d_req_.p, n, x, st_);
   double* v = d_v_.p;
-  blocked_solve<double>(d_A_.p, n, d_linv_.p, d_uinv_.p, x, d_w_.p, v);
+  blocked_solve<double>(d_A_.p, (int)npad_, d_linv_.p, d_uinv_.p, x, d_w_.p, v);
   HIP_CHECK(hipMemcpyAsync(x, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   return x;
 }
 
 // max_j |(A0 y)_j - b_j| * row_scale_j over the head equations for the current y
 double Solver::head_residual(const double* b, int* row) {
-  const int n = (int)(de_.size() + F_);
+  const int n = (int)N_;
   aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
-  aem::launch_residual(d_ay_.p, b, d_rowscale_.p, (int)de_.size(), d_scal_.p, st_);
+  aem::launch_residual(d_ay_.p, b, d_rowscale_.p, n, d_scal_.p, st_);
   aem::IterScalars sc{};
   HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
@@ -1079,7 +1151,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   have_state_ = false;
   restore();
   const int nde = (int)de_.size();
-  const int n = nde + F_;
+  const int n = (int)N_;  // unknowns of the block layout (padding included)
   aem::NetView nv = net();
   aem::FitView fv = fit();
 
@@ -1161,8 +1233,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
     HIP_CHECK(hipEventRecord(ev_[3], st_));
     aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_, mixed_phase_);
-    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_b_.p, st_);
-    HIP_CHECK(hipMemsetAsync(d_b_.p + nde, 0, sizeof(double) * F_, st_));
+    // only the head rows are written; the continuity and padding rows of b stay zero
+    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
     if (head_tol > 0.0 && nit >= 2) {
       // head residual of the previous iterate against the right-hand side of its coefficients
       resid = head_residual(d_b_.p, &resid_row);
@@ -1209,7 +1281,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipEventRecord(ev_[5], st_));
     HIP_CHECK(hipMemcpyAsync(d_yold_.p, d_y_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
     HIP_CHECK(hipMemcpyAsync(d_y_.p, d_x_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
-    aem::launch_scatter(nv, d_de_.p, nde, F_, d_y_.p, st_);
+    aem::launch_scatter(nv, d_de_.p, nde, F_, sys(), d_y_.p, st_);
     fl_rhs += rhs_flops;
     pr_rhs += rhs_pairs;
     ++n_rhs;
@@ -1302,7 +1374,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   if (!stopped_on_head) {
     // right-hand side of the returned coefficients, then the residual of the returned discharges
     aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_);
-    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_b_.p, st_);
+    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
     resid = head_residual(d_b_.p, &resid_row);
   }
   const double bcres = bc_residual();
@@ -1313,7 +1385,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   auto t_end = clk::now();
   if (info) {
     info->nit = nit;
-    info->n_unknowns = n;
+    info->n_unknowns = nde + F_;
     info->n_grown = n_grown_;
     info->max_error_element = argmax;
     info->error = error;
@@ -1445,35 +1517,39 @@ void Solver::adj_tables(const TargetSet& ts, bool colloc, AdjTables& t) {
 // blocked solves with a non-unit lower (U^T) and a unit upper (L^T) triangle.
 template <class T>
 void Solver::make_transposed(const T* A, bool blocked, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT) {
-  const int64_t n = (int64_t)de_.size() + F_;
+  const int64_t n = npad_;
   const int64_t nb = (n + kNB - 1) / kNB;
+  const size_t tot = (size_t)(n * n * nblk_);
   {
     DBuf<T> tmp;
-    tmp.alloc((size_t)(n * n));
-    HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * (size_t)(n * n), hipMemcpyDeviceToDevice, st_));
-    for (int64_t j = 0; blocked && j < nb; ++j) {
-      const int64_t r0 = j * kNB;
-      const int bs = (int)std::min<int64_t>(kNB, n - r0);
-      const int rest = (int)(n - r0 - bs);
-      const T* djj = tmp.p + r0 * (n + 1);
-      if (rest > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n,
-                               tmp.p + r0 * n + r0 + bs, n));
-      if (r0 > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
-                               tmp.p + r0 * n, n));
+    tmp.alloc(tot);
+    HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * tot, hipMemcpyDeviceToDevice, st_));
+    for (int b = 0; blocked && b < nblk_; ++b) {
+      T* Ab = tmp.p + (int64_t)b * n * n;
+      for (int64_t j = 0; j < nb; ++j) {
+        const int64_t r0 = j * kNB;
+        const int bs = (int)std::min<int64_t>(kNB, n - r0);
+        const int rest = (int)(n - r0 - bs);
+        const T* djj = Ab + r0 * (n + 1);
+        if (rest > 0)
+          RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n,
+                                 Ab + r0 * n + r0 + bs, n));
+        if (r0 > 0)
+          RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
+                                 Ab + r0 * n, n));
+      }
     }
-    AT.alloc((size_t)(n * n));
-    aem::launch_transpose<T>(tmp.p, n, AT.p, st_);
+    AT.alloc(tot);
+    aem::launch_transpose<T>(tmp.p, n, AT.p, st_, nblk_);
     HIP_CHECK(hipStreamSynchronize(st_));
   }
-  prepare_blocked<T>(AT.p, n, linvT, uinvT, false);
+  prepare_blocked<T>(AT.p, n, nblk_, linvT, uinvT, false);
   HIP_CHECK(hipStreamSynchronize(st_));
 }
 
 // CSR of A^T (for the FP64 residual z - A^T mu of the transposed refinement)
 void Solver::make_csr_transpose() {
-  const int n = (int)(de_.size() + F_);
+  const int n = (int)N_;
   std::vector<int> rp(n + 1), col(nnz_A_);
   std::vector<double> val(nnz_A_);
   HIP_CHECK(hipMemcpyAsync(rp.data(), d_csr_ptr_.p, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st_));
@@ -1499,10 +1575,10 @@ void Solver::make_csr_transpose() {
 // mu += A^-T (z - A^T mu) with the transposed factors in precision T
 template <class T>
 void Solver::adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v) {
-  const int n = (int)(de_.size() + F_);
+  const int n = (int)N_;
   aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
   aem::launch_adj_refine_in<T>(d_z_.p, d_aty_.p, n, x, st_);
-  blocked_solve<T>(AT, n, linvT, uinvT, x, w, v);
+  blocked_solve<T>(AT, (int)npad_, linvT, uinvT, x, w, v);
   aem::launch_adj_refine_out<T>(v, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
 }
 
@@ -1524,7 +1600,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   for (int64_t k = 0; k < nobs; ++k)
     if (obj->obs_frac[k] < 0 || obj->obs_frac[k] >= F_)
       fail(ANDFN_MI_EINVAL, "observation %lld: fracture %lld out of range", (long long)k, (long long)obj->obs_frac[k]);
-  const int nde = (int)de_.size(), n = nde + F_;
+  const int nde = (int)de_.size(), n = (int)N_;
   const size_t ncf = (size_t)E_ * L_.M;
   aem::NetView nv = net();
 
@@ -1607,9 +1683,9 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   // objective: J_c and J_y (discharge / constant weights plus the observation-head terms)
   std::vector<double> jy0(n, 0.0);
   if (obj && obj->dq)
-    for (int j = 0; j < nde; ++j) jy0[j] = obj->dq[de_[j]];
+    for (int e : de_) jy0[de_pos_[e]] = obj->dq[e];
   if (obj && obj->dconstant)
-    for (int f = 0; f < F_; ++f) jy0[nde + f] = obj->dconstant[f];
+    for (int f = 0; f < F_; ++f) jy0[fpos_[f]] = obj->dconstant[f];
   d_gc_.zero(ncf, st_);
   d_gq_.zero(E_, st_);
   d_gC_.zero(F_, st_);
@@ -1645,7 +1721,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   HIP_CHECK(hipMemcpyAsync(d_jc_.p, d_gc_.p, sizeof(double2) * ncf, hipMemcpyDeviceToDevice, st_));
   d_jy0_.upload(jy0, st_);
   d_jy_.alloc(n);
-  aem::launch_adj_build_z(nde, F_, d_de_.p, d_jy0_.p, d_gq_.p, d_gC_.p, d_jy_.p, st_);
+  aem::launch_adj_build_z(nde, F_, d_de_.p, sys(), n, d_jy0_.p, d_gq_.p, d_gC_.p, d_jy_.p, st_);
 
   // adjoint solve
   const int method = opt ? opt->method : ANDFN_MI_ADJ_GMRES;
@@ -1672,7 +1748,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
                            tab_full_.pz.p, tab_full_.pelem.p, d_wfull_.p, 1, 1, 0, d_gc_.p, d_gq_.p, st_);
     aem::launch_adj_cross(nxsrc, d_xsrc_.p, d_xptr_.p, d_xent_.p, d_sfx_.p, (int)L_.M, d_gq_.p, st_);
     aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
-    aem::launch_adj_build_z(nde, F_, d_de_.p, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
+    aem::launch_adj_build_z(nde, F_, d_de_.p, sys(), n, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
     for (int r = 0; r < nref; ++r) {
       if (a32)
@@ -1680,7 +1756,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       else
         adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
     }
-    aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
+    aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_de_pos_.p, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
     aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
                            tab_rhs_.pz.p, tab_rhs_.pelem.p, d_wrhs_.p, 0, 0, 1, d_gc_.p, d_gq_.p, st_);
   };
@@ -1895,12 +1971,12 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       double h1 = 0.0;
       for (int i = 0; i < K_; ++i) h1 += hw[b1 + i];
       h1 /= (double)K_;
-      grad_t[efrac0_[e]] += mu[j] * h0 / t_[efrac0_[e]];
-      grad_t[efrac1_[e]] -= mu[j] * h1 / t_[efrac1_[e]];
+      grad_t[efrac0_[e]] += mu[de_pos_[e]] * h0 / t_[efrac0_[e]];
+      grad_t[efrac1_[e]] -= mu[de_pos_[e]] * h1 / t_[efrac1_[e]];
     } else {
       const int f = efrac0_[e];
-      grad_t[f] += mu[j] * head_[e];
-      grad_head[e] = mu[j] * t_[f];
+      grad_t[f] += mu[de_pos_[e]] * head_[e];
+      grad_head[e] = mu[de_pos_[e]] * t_[f];
     }
   }
   auto t_end = clk::now();

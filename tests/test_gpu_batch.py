@@ -1,0 +1,103 @@
+"""Independent realisations solved as one network (one discharge block per realisation).
+
+The batched solve must reproduce every realisation's own solve: the element kernels see each
+realisation's fractures only, and each discharge block is factored and solved on its own (a
+strided-batched LU and batched triangular kernels instead of one LU per network).  Fixed
+iteration count (the change-based stop off) and fixed coefficient counts, so both runs take the
+same 60 iterations.  Tolerance: 1e-10 relative to the largest magnitude of each quantity (the
+batched and single factorisations round differently).  The adjoint of the summed boundary
+discharge must equal the per-realisation adjoints (1e-8 relative).
+"""
+
+import numpy as np
+import pytest
+
+from andfn_amd.autograd import boundary_groups
+from andfn_amd.batch import build_batch, realisation_specs, split
+from andfn_amd.hpc_solve import DeviceNetwork
+
+pytestmark = pytest.mark.gpu
+CONSTS = dict(MAX_ERROR=0.0, MAX_NCOEF=10, MAX_ITERATIONS=60)
+
+
+def _rel(a, b):
+    s = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (s if s > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def batch():
+    print("building 3 realisations", flush=True)
+    specs = realisation_specs(3, seed=7, nmin=150, nmax=400)
+    dfns, fa, ea, off, K, c = build_batch(specs, CONSTS)
+    return dfns, fa, ea, off, K, c
+
+
+def test_batched_solve_matches_separate_solves(batch):
+    dfns, fa, ea, off, K, c = batch
+    fa, ea = fa.copy(), ea.copy()
+    net = DeviceNetwork(fa, ea, K, c)
+    try:
+        info = net.run()
+        assert info["nit"] == 60 and info["n_unknowns"] == len(ea)
+        net.download()
+    finally:
+        net.close()
+    for r, (fb, eb) in enumerate(split(fa, ea, off)):
+        print(f"realisation {r}", flush=True)
+        f1 = dfns[r].fractures_struc_array_hpc.copy()
+        e1 = dfns[r].elements_struc_array_hpc.copy()
+        n1 = DeviceNetwork(f1, e1, K, c)
+        try:
+            n1.run()
+            n1.download()
+        finally:
+            n1.close()
+        assert _rel(eb["coef"], e1["coef"]) < 1e-10
+        assert _rel(eb["q"], e1["q"]) < 1e-10
+        assert _rel(fb["constant"], f1["constant"]) < 1e-10
+
+
+def test_batched_adjoint_matches_separate_adjoints(batch):
+    dfns, fa, ea, off, K, c = batch
+    dq = np.zeros(len(ea))
+    for r, d in enumerate(dfns):
+        dq[off[r][1] + boundary_groups(d)["x0"]] = 1.0
+    net = DeviceNetwork(fa.copy(), ea.copy(), K, c)
+    try:
+        net.run()
+        gt, gh, info = net.adjoint(dq=dq, tol=1e-12)
+        assert info["converged"]
+    finally:
+        net.close()
+    for r, d in enumerate(dfns):
+        f1 = d.fractures_struc_array_hpc.copy()
+        e1 = d.elements_struc_array_hpc.copy()
+        d1 = np.zeros(len(e1))
+        d1[boundary_groups(d)["x0"]] = 1.0
+        n1 = DeviceNetwork(f1, e1, K, c)
+        try:
+            n1.run()
+            g1, h1, i1 = n1.adjoint(dq=d1, tol=1e-12)
+        finally:
+            n1.close()
+        s = np.max(np.abs(g1))
+        assert np.max(np.abs(gt[off[r][0]:off[r + 1][0]] - g1)) <= 1e-8 * s
+        assert np.max(np.abs(gh[off[r][1]:off[r + 1][1]] - h1)) <= 1e-8 * np.max(np.abs(h1))
+
+
+def test_batched_mixed_reaches_head_tolerance(batch):
+    dfns, fa, ea, off, K, c = batch
+    net = DeviceNetwork(fa.copy(), ea.copy(), K, dict_constants(c, MAX_ITERATIONS=500))
+    try:
+        info = net.run("mixed", head_tol=1e-6)
+    finally:
+        net.close()
+    assert info["converged"] and info["head_residual"] <= 1e-6
+
+
+def dict_constants(c, **kw):
+    out = np.array(c, copy=True)
+    for k, v in kw.items():
+        out[k] = v
+    return out
