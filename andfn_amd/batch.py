@@ -11,7 +11,9 @@ realisations is at most the tolerance.
 returns per-realisation views of the combined arrays.  ``realisation_specs`` draws the batched
 benchmark family: fracture counts uniform on [500, 4000], the 200-disk law (radii [5, 50] m, pdf
 r^-2.5, log10 T ~ N(-5, 0.5)) in a cube scaled as (N / 200)^(1/3) x 100 m so that every
-realisation has the density of the 200-disk network, and 10 to 20 coefficients per element.
+realisation has the density of the 200-disk network, and 10 to 20 coefficients per element;
+elements are clipped to 95% of their fracture radius (with 97%, 4 of 64 realisations make
+AnDFN's damped iteration diverge; with 95% all converge, in 44-76 iterations on those four).
 """
 
 from dataclasses import replace
@@ -72,7 +74,7 @@ def realisation_specs(count=64, seed=2024, nmin=500, nmax=4000, cmin=10, cmax=20
         n = nmin + int(un[r] * (nmax - nmin + 1))
         m = cmin + int(um[r] * (cmax - cmin + 1))
         cube = 100.0 * (n / 200.0) ** (1.0 / 3.0)
-        specs.append(NetworkSpec(n, cube, 5.0, 50.0, seed=seed + r, ncoef=m, nint=2 * m, rim_clearance=0.03,
+        specs.append(NetworkSpec(n, cube, 5.0, 50.0, seed=seed + r, ncoef=m, nint=2 * m, rim_clearance=0.05,
                                  min_separation=0.05))
     return specs
 

@@ -376,7 +376,6 @@ class Solver {
   DBuf<double2> d_gv_, d_xv_, d_wv_;
   DBuf<double> d_V_;
   DBuf<double> d_AT_, d_linvT_, d_uinvT_, d_csrT_val_;
-  DBuf<float> d_AT32_, d_linvT32_, d_uinvT32_;
   DBuf<int> d_csrT_ptr_, d_csrT_col_;
   void adj_tables(const TargetSet& ts, bool colloc, AdjTables& t);
   template <class T>
@@ -1661,24 +1660,16 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     d_xent_.upload(xent, st_);
     nxsrc = (int)xsrc.size();
   }
-  // transposed factors, once per factorisation
-  // the adjoint uses the factors the run finished with: FP32-rounded (refined) or FP64
-  const bool a32 = solve32_;
+  // transposed FP64 factors, once per factorisation (also after a mixed-form run: one FP64 step
+  // per application is cheaper than three refinement steps with FP32-rounded factors)
   if (!adjT_valid_) {
-    if (a32) make_transposed<float>(d_A32_.p, true, d_AT32_, d_linvT32_, d_uinvT32_);
-    else make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
+    make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
     make_csr_transpose();
     adjT_valid_ = true;
   }
-  if (a32) {
-    d_x32_.alloc(n);
-    d_w32_.alloc(n);
-    d_v32_.alloc(n);
-  } else {
-    d_x_.alloc(n);
-    d_w_.alloc(n);
-    d_v_.alloc(n);
-  }
+  d_x_.alloc(n);
+  d_w_.alloc(n);
+  d_v_.alloc(n);
 
   // objective: J_c and J_y (discharge / constant weights plus the observation-head terms)
   std::vector<double> jy0(n, 0.0);
@@ -1736,8 +1727,6 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   d_wrhs_.zero(std::max<int64_t>(rhs_.npts(), 1), st_);
   d_sfx_.zero((size_t)E_ * 2 * L_.M, st_);
   d_stats_.zero(4, st_);
-  // A^-T with the run's factors: one FP64 step, or FP32 steps refined against the FP64 matrix
-  const int nref = a32 ? 3 : 1;
   // d_gc_ = B lam (+ RHS_c^T A^-T J_y when with_j) and d_mu_ = A^-T (S_y^T lam (+ J_y)), where B is
   // the fixed-point map's linear part on the coefficients: FIT^T, EVAL^T at the collocation
   // points (coefficients, discharges, constants, branch cuts), A^-T, RHS^T
@@ -1750,12 +1739,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
     aem::launch_adj_build_z(nde, F_, d_de_.p, sys(), n, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
-    for (int r = 0; r < nref; ++r) {
-      if (a32)
-        adj_mu_step<float>(d_AT32_.p, d_linvT32_.p, d_uinvT32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
-      else
-        adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
-    }
+    adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
     aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_de_pos_.p, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
     aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
                            tab_rhs_.pz.p, tab_rhs_.pelem.p, d_wrhs_.p, 0, 0, 1, d_gc_.p, d_gq_.p, st_);

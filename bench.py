@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="time only the --precision form")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--realisations", type=int, default=64, help="cfg5: number of realisations")
     ap.add_argument("--max-iterations", type=int, default=0, help="override MAX_ITERATIONS (short profiling runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
@@ -124,8 +125,73 @@ def _time_form(net, precision, lu_mode, warmup, steps, label):
     return infos, walls
 
 
+def bench_batch(args):
+    """Batched realisations: 64 seeded DFNs (fracture counts uniform on [500, 4000], the 200-disk law
+    at constant density, 10 to 20 coefficients) as one network; the mixed form to head residual
+    1e-6 and the adjoint of the summed x = 0 boundary discharge w.r.t. every log10 T."""
+    from andfn_amd.autograd import boundary_groups
+    from andfn_amd.batch import build_batch, realisation_specs
+    from andfn_amd.hpc_solve import DeviceNetwork
+
+    count = args.realisations
+    t0 = time.perf_counter()
+    specs = realisation_specs(count)
+    dfns, fa, ea, off, K, consts = build_batch(specs, dict(MAX_ERROR=1e-9, MAX_NCOEF=10, MAX_ITERATIONS=500))
+    print(f"# batch: {count} realisations, {len(fa)} fractures, {len(ea)} elements, built in "
+          f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    net = DeviceNetwork(fa, ea, K, consts)
+    dq = np.zeros(len(ea))
+    for r, d in enumerate(dfns):
+        dq[off[r][1] + boundary_groups(d)["x0"]] = 1.0
+    res = {}
+    for prec in (args.precision, "fp64" if args.precision == "mixed" else "mixed"):
+        if prec != args.precision and args.no_compare:
+            continue
+        infos, walls = _time_form(net, prec, 0, args.warmup, args.steps, prec)
+        aw = []
+        for k in range(1 + args.steps):
+            t1 = time.perf_counter()
+            gt, gh, ai = net.adjoint(dq=dq, tol=ADJOINT_TOL)
+            aw.append(time.perf_counter() - t1)
+            print(f"# {prec} adjoint {k}: {aw[-1]:.3f}s iterations={ai['nit']}", file=sys.stderr, flush=True)
+        res[prec] = {"ms_per_batch": 1e3 * sum(walls) / args.steps, "nit": int(infos[-1]["nit"]),
+                     "head_residual": infos[-1]["head_residual"], "solves_fp32": int(infos[-1]["solves_fp32"]),
+                     "nit_fp32": int(infos[-1]["nit_fp32"]), "t_factor_s": infos[-1]["t_factor"],
+                     "adjoint_ms": 1e3 * sum(aw[1:]) / args.steps, "adjoint_iterations": int(ai["nit"])}
+        res[prec]["adjoint_forward_ratio"] = res[prec]["adjoint_ms"] / res[prec]["ms_per_batch"]
+    net.close()
+    main_r = res[args.precision]
+    out = {
+        "metric": "DFN realisation solves/sec at max head residual 1e-6 (batched, one network)",
+        "value": count / (main_r["ms_per_batch"] / 1e3),
+        "unit": "realisations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_r["ms_per_batch"],
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": f"cfg5: {count} realisations, fracture counts uniform on [500, 4000], 200-disk law "
+                               f"at constant density, 10-20 coefficients",
+                   "fractures": int(len(fa)), "elements": int(len(ea))},
+        "forms": res,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
+    if args.workload == "cfg5":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only", file=sys.stderr)
+            sys.exit(2)
+        bench_batch(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)

@@ -155,8 +155,8 @@ int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_eleme
  * 0 for other elements), with the coefficient counts frozen at their converged values.  The
  * adjoint equations are the transpose of the fixed-point iteration (collocation sweep, fit,
  * discharge solve): (I - B^T) lam = g with B the iteration's linear map on the coefficients.
- * They are solved with the transposed operators (A^-T from the run's LU factors, refined against
- * the FP64 matrix) by restarted GMRES or by the forward's damped iteration.  Returns
+ * They are solved with the transposed operators (A^-T from the run's FP64 LU factors) by
+ * restarted GMRES or by the forward's damped iteration.  Returns
  * ANDFN_MI_NOT_CONVERGED if max_iterations is reached first.
  */
 typedef struct andfn_mi_objective {
