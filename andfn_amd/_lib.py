@@ -74,6 +74,8 @@ class Info(ctypes.Structure):
         ("nit_fp32", ctypes.c_int64),
         ("solves_fp32", ctypes.c_int64),
         ("refine_rate", ctypes.c_double),
+        ("sweep_points_per_lane", ctypes.c_int32),
+        ("rhs_points_per_lane", ctypes.c_int32),
     ]
 
     def as_dict(self):

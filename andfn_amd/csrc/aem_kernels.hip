@@ -102,140 +102,202 @@ __device__ __forceinline__ double2 discharge_point(const NetView& net, int e, in
 // --------------------------------------------------------------------------------------
 // Sweep: Omega_f(z) = C_f + sum over the fracture's sources (hpc_fracture.py:51-92)
 // --------------------------------------------------------------------------------------
-template <int MODE, bool MIX>
+// P target points per lane: a tile is 64 P consecutive points of one fracture, lane l owns
+// points p0 + l + 64 k.  Every source's parameters and coefficients are loaded once per tile and
+// applied to all P points (P independent arithmetic chains); each point's value is summed over
+// the sources in the same order for every P, so the result does not depend on P.
+template <int MODE, bool MIX, int P>
 __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
   const int t = blockIdx.x;
   const Tile tile = tg.tiles[t];
   const int lane = threadIdx.x;
-  const int p = tile.p0 + lane;
-  const bool valid = p < tile.pend;
   const int f = tile.frac;
 
   constexpr bool colloc_pts = (MODE == MODE_COLLOC || MODE == MODE_COLLOC_HEADS);
   constexpr bool use_q = (MODE != MODE_RHS);
   constexpr bool heads_out = (MODE == MODE_POINTS || MODE == MODE_COLLOC_HEADS || MODE == MODE_DISCH_HEADS);
-  double2 z = c2(0.0, 0.0);
-  int excl = -1;
-  int want_imag = 0;
-  if (valid) {
-    if (MODE == MODE_POINTS) {
-      z = tg.pt_z[p];
-    } else {
-      int s = find_side(tg.pt_off, tile.sbeg, tile.send, p);
-      int e = tg.side_elem[s];
-      int which = tg.side_which[s];
-      int i = p - tg.pt_off[s];
-      if (colloc_pts) {
-        int n = net.nint[e];
-        z = colloc_point(net, e, which, i, n);
-        if (MODE == MODE_COLLOC) {
-          excl = e;
-          int ty = net.etype[e];
-          want_imag = (ty == 1 || ty == 4 || ty == 5);
-        }
+  double2 z[P];
+  int excl[P], want_imag[P];
+  bool valid[P];
+  int wi_any = 0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    const int p = tile.p0 + lane + 64 * k;
+    valid[k] = p < tile.pend;
+    z[k] = c2(0.0, 0.0);
+    excl[k] = -1;
+    want_imag[k] = 0;
+    if (valid[k]) {
+      if (MODE == MODE_POINTS) {
+        z[k] = tg.pt_z[p];
       } else {
-        z = discharge_point(net, e, which, i, K);
+        const int sd = find_side(tg.pt_off, tile.sbeg, tile.send, p);
+        const int e = tg.side_elem[sd];
+        const int which = tg.side_which[sd];
+        const int i = p - tg.pt_off[sd];
+        if (colloc_pts) {
+          z[k] = colloc_point(net, e, which, i, net.nint[e]);
+          if (MODE == MODE_COLLOC) {
+            excl[k] = e;
+            const int ty = net.etype[e];
+            want_imag[k] = (ty == 1 || ty == 4 || ty == 5);
+          }
+        } else {
+          z[k] = discharge_point(net, e, which, i, K);
+        }
       }
     }
+    wi_any |= want_imag[k];
   }
-  const bool any_imag = __any(want_imag);
+  const bool any_imag = __any(wi_any);
 
-  double ore = 0.0, oim = 0.0;
-  float fre = 0.0f, fim = 0.0f;  // FP32 accumulators of the far-field terms (mixed precision)
+  double ore[P], oim[P];
+  float fre[P], fim[P];  // FP32 accumulators of the far-field terms (mixed precision)
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    ore[k] = 0.0;
+    oim[k] = 0.0;
+    fre[k] = 0.0f;
+    fim[k] = 0.0f;
+  }
   const int kbeg = net.mem_ptr[f];
   const int kend = net.mem_ptr[f + 1];
-  for (int k = kbeg; k < kend; ++k) {
-    const Slot sl = net.slots[k];
+  for (int ks = kbeg; ks < kend; ++ks) {
+    const Slot sl = net.slots[ks];
     const int src = sl.elem;
     const int m = net.ncoef[src];
     const double2* __restrict__ cf = net.coef + (size_t)src * net.M;
-    double tre = 0.0, tim = 0.0;
+    double tre[P], tim[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      tre[k] = 0.0;
+      tim[k] = 0.0;
+    }
     if (sl.kind == KIND_LINE || sl.kind == KIND_IMPL) {
-      double2 Z = c2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
-                     fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
-      if (MIX && __all(!valid || fma(Z.x, Z.x, Z.y * Z.y) >= net.far2)) {
-        // far field of this source for the whole wavefront: FP32 map, series and log
-        const float2 Zf = make_float2((float)Z.x, (float)Z.y);
-        const float2 chif = joukowsky_exterior_f(Zf);
-        const float r2f = fmaf(chif.x, chif.x, chif.y * chif.y);
-        const float invf = 1.0f / r2f;
-        const float2 wf = make_float2(chif.x * invf, -chif.y * invf);
-        const float2 hf = horner_f(net.coef32 + (size_t)src * net.M, m, wf);
-        const float sg = (float)sl.sign;
-        float gre = sg * hf.x, gim = sg * hf.y;
-        if (sl.kind == KIND_LINE && use_q) {
-          const float qq = (float)(sl.sign * net.q[src] * kInv2Pi);
-          if (qq != 0.0f) {
-            gre = fmaf(qq, 0.5f * logf(r2f), gre);
-            if (any_imag) gim = fmaf(qq, atan2f(chif.y, chif.x), gim);
-          }
+      double2 Z[P];
+      bool far = true;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        Z[k] = c2(fma(sl.alpha.x, z[k].x, fma(-sl.alpha.y, z[k].y, sl.beta.x)),
+                  fma(sl.alpha.x, z[k].y, fma(sl.alpha.y, z[k].x, sl.beta.y)));
+        far = far && (!valid[k] || fma(Z[k].x, Z[k].x, Z[k].y * Z[k].y) >= net.far2);
+      }
+      if (MIX && __all(far)) {
+        // far field of this source for every point of the tile: FP32 map, series and log
+        float2 chif[P], wf[P], hf[P];
+        float r2f[P];
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          chif[k] = joukowsky_exterior_f(make_float2((float)Z[k].x, (float)Z[k].y));
+          r2f[k] = fmaf(chif[k].x, chif[k].x, chif[k].y * chif[k].y);
+          const float invf = 1.0f / r2f[k];
+          wf[k] = make_float2(chif[k].x * invf, -chif[k].y * invf);
         }
-        if (src != excl) {
-          fre += gre;
-          fim += gim;
+        horner_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
+        const float sg = (float)sl.sign;
+        const float qq = (sl.kind == KIND_LINE && use_q) ? (float)(sl.sign * net.q[src] * kInv2Pi) : 0.0f;
+#pragma unroll
+        for (int k = 0; k < P; ++k) {
+          float gre = sg * hf[k].x, gim = sg * hf[k].y;
+          if (qq != 0.0f) {
+            gre = fmaf(qq, 0.5f * logf(r2f[k]), gre);
+            if (any_imag) gim = fmaf(qq, atan2f(chif[k].y, chif[k].x), gim);
+          }
+          if (src != excl[k]) {
+            fre[k] += gre;
+            fim[k] += gim;
+          }
         }
         continue;
       }
-      double2 chi = joukowsky_exterior(Z);
-      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
-      double inv = rcp_pos(r2);
-      double2 w = c2(chi.x * inv, -chi.y * inv);
-      double2 h = horner(cf, m, w);
-      tre = sl.sign * h.x;
-      tim = sl.sign * h.y;
-      if (sl.kind == KIND_LINE && use_q) {
-        double qq = sl.sign * net.q[src] * kInv2Pi;
+      double2 chi[P], w[P], h[P];
+      double r2[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        chi[k] = joukowsky_exterior(Z[k]);
+        r2[k] = fma(chi[k].x, chi[k].x, chi[k].y * chi[k].y);
+        const double inv = rcp_pos(r2[k]);
+        w[k] = c2(chi[k].x * inv, -chi[k].y * inv);
+      }
+      horner_p<P>(cf, m, w, h);
+      const double qq = (sl.kind == KIND_LINE && use_q) ? sl.sign * net.q[src] * kInv2Pi : 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        tre[k] = sl.sign * h[k].x;
+        tim[k] = sl.sign * h[k].y;
         if (qq != 0.0) {
-          tre = fma(qq, 0.5 * log_pos(r2), tre);
-          if (any_imag) tim = fma(qq, atan2(chi.y, chi.x), tim);
+          tre[k] = fma(qq, 0.5 * log_pos(r2[k]), tre[k]);
+          if (any_imag) tim[k] = fma(qq, atan2(chi[k].y, chi[k].x), tim[k]);
         }
       }
     } else if (sl.kind == KIND_BOUND) {
-      double2 chi = c2(sl.alpha.x * z.x, sl.alpha.x * z.y);  // z / R
-      double2 h = horner(cf, m, chi);
-      tre = h.x;
-      tim = h.y;
-    } else if (sl.kind == KIND_WELL) {
-      double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
-      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
-      double qq = use_q ? net.q[src] * kInv2Pi : 0.0;
-      if (qq != 0.0) {
-        tre = qq * (0.5 * log_pos(r2));
-        if (any_imag) tim = qq * atan2(chi.y, chi.x);
+      double2 chi[P], h[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) chi[k] = c2(sl.alpha.x * z[k].x, sl.alpha.x * z[k].y);  // z / R
+      horner_p<P>(cf, m, chi, h);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        tre[k] = h[k].x;
+        tim[k] = h[k].y;
       }
-      if (sqrt(r2) < 1.0 - 1e-12) {
-        tre = __builtin_nan("");
-        tim = __builtin_nan("");
+    } else if (sl.kind == KIND_WELL) {
+      const double qq = use_q ? net.q[src] * kInv2Pi : 0.0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 chi = c2(fma(sl.alpha.x, z[k].x, sl.beta.x), fma(sl.alpha.x, z[k].y, sl.beta.y));
+        const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+        if (qq != 0.0) {
+          tre[k] = qq * (0.5 * log_pos(r2));
+          if (any_imag) tim[k] = qq * atan2(chi.y, chi.x);
+        }
+        if (sqrt(r2) < 1.0 - 1e-12) {
+          tre[k] = __builtin_nan("");
+          tim[k] = __builtin_nan("");
+        }
       }
     } else {  // KIND_IMPC
-      double2 chi = c2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
-      double r2 = fma(chi.x, chi.x, chi.y * chi.y);
-      double inv = rcp_pos(r2);
-      double2 w = c2(chi.x * inv, -chi.y * inv);
-      double2 h = horner(cf, m, w);
-      tre = h.x;
-      tim = h.y;
-      if (sqrt(r2) < 1.0 - 1e-10) {
-        tre = __builtin_nan("");
-        tim = __builtin_nan("");
+      double2 w[P], h[P];
+      double r2[P];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const double2 chi = c2(fma(sl.alpha.x, z[k].x, sl.beta.x), fma(sl.alpha.x, z[k].y, sl.beta.y));
+        r2[k] = fma(chi.x, chi.x, chi.y * chi.y);
+        const double inv = rcp_pos(r2[k]);
+        w[k] = c2(chi.x * inv, -chi.y * inv);
+      }
+      horner_p<P>(cf, m, w, h);
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        tre[k] = h[k].x;
+        tim[k] = h[k].y;
+        if (sqrt(r2[k]) < 1.0 - 1e-10) {
+          tre[k] = __builtin_nan("");
+          tim[k] = __builtin_nan("");
+        }
       }
     }
-    if (src != excl) {
-      ore += tre;
-      oim += tim;
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+      if (src != excl[k]) {
+        ore[k] += tre[k];
+        oim[k] += tim[k];
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    if (MIX) {
+      ore[k] += (double)fre[k];
+      oim[k] += (double)fim[k];
     }
-  }
-  if (MIX) {
-    ore += (double)fre;
-    oim += (double)fim;
-  }
-  if (valid) {
-    if (MODE == MODE_COLLOC)
-      val[p] = want_imag ? oim : ore + net.constant[f];
-    else if (heads_out)
-      val[p] = (ore + net.constant[f]) / net.t[f];
-    else
-      val[p] = ore;
+    if (valid[k]) {
+      const int p = tile.p0 + lane + 64 * k;
+      if (MODE == MODE_COLLOC)
+        val[p] = want_imag[k] ? oim[k] : ore[k] + net.constant[f];
+      else if (heads_out)
+        val[p] = (ore[k] + net.constant[f]) / net.t[f];
+      else
+        val[p] = ore[k];
+    }
   }
 }
 
@@ -745,23 +807,31 @@ __global__ void init_errors_kernel(FitView fv, int ne) {
 // --------------------------------------------------------------------------------------
 // launchers
 // --------------------------------------------------------------------------------------
-void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
-                  bool mixed) {
-  if (ntiles <= 0) return;
+template <int P>
+static void launch_sweep_p(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val,
+                           hipStream_t st, bool mixed) {
+  const dim3 g(ntiles), b(64);
   if (mode == MODE_COLLOC && mixed)
-    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, true>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, true, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_COLLOC)
-    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC, false, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_RHS && mixed)
-    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, true>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, true, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_RHS)
-    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_RHS, false, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_COLLOC_HEADS)
-    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC_HEADS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC_HEADS, false, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_DISCH_HEADS)
-    hipLaunchKernelGGL((sweep_kernel<MODE_DISCH_HEADS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_DISCH_HEADS, false, P>), g, b, 0, st, net, tg, K, val);
   else
-    hipLaunchKernelGGL((sweep_kernel<MODE_POINTS, false>), dim3(ntiles), dim3(64), 0, st, net, tg, K, val);
+    hipLaunchKernelGGL((sweep_kernel<MODE_POINTS, false, P>), g, b, 0, st, net, tg, K, val);
+}
+
+void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
+                  bool mixed, int points_per_lane) {
+  if (ntiles <= 0) return;
+  if (points_per_lane == 2) launch_sweep_p<2>(mode, net, tg, ntiles, K, val, st, mixed);
+  else launch_sweep_p<1>(mode, net, tg, ntiles, K, val, st, mixed);
 }
 
 // single-precision copy of every element's coefficients (mixed-precision far field)

@@ -125,8 +125,9 @@ struct IterScalars {
   int arg_residual;
 };
 
+// points_per_lane 1 or 2: the target view's tiles must hold 64 x points_per_lane points each
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
-                  bool mixed = false);
+                  bool mixed = false, int points_per_lane = 1);
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
                 const int* pt_off, double damping, int blend, hipStream_t st, bool f32 = false);

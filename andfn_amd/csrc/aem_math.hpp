@@ -107,6 +107,30 @@ __device__ __forceinline__ double2 cmadd(double2 r, double2 w, double2 c) {
   return c2(fma(r.x, w.x, fma(-r.y, w.y, c.x)), fma(r.x, w.y, fma(r.y, w.x, c.y)));
 }
 
+// P independent Horner chains sharing every coefficient load (P target points per lane)
+template <int P>
+__device__ __forceinline__ void horner_p(const double2* __restrict__ c, int m, const double2 (&w)[P],
+                                         double2 (&r)[P]) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) r[k] = c[m - 1];
+  int j = m - 2;
+  for (; j >= 3; j -= 4) {
+    const double2 c0 = c[j], c1 = c[j - 1], c2_ = c[j - 2], c3 = c[j - 3];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      r[k] = cmadd(r[k], w[k], c0);
+      r[k] = cmadd(r[k], w[k], c1);
+      r[k] = cmadd(r[k], w[k], c2_);
+      r[k] = cmadd(r[k], w[k], c3);
+    }
+  }
+  for (; j >= 0; --j) {
+    const double2 cj = c[j];
+#pragma unroll
+    for (int k = 0; k < P; ++k) r[k] = cmadd(r[k], w[k], cj);
+  }
+}
+
 // sum_{j<m} c_j w^j by Horner; c is wave-uniform (scalar loads).  Coefficients are fetched
 // eight at a time ahead of their multiply-adds so the scalar-load latency is paid once per chunk.
 __device__ __forceinline__ double2 horner(const double2* __restrict__ c, int m, double2 w) {
@@ -152,6 +176,19 @@ __device__ __forceinline__ float2 joukowsky_exterior_f(float2 Z) {
     s.y = -s.y;
   }
   return make_float2(Z.x + s.x, Z.y + s.y);
+}
+
+// P FP32 Horner chains sharing the coefficient loads
+template <int P>
+__device__ __forceinline__ void horner_f_p(const float2* __restrict__ c, int m, const float2 (&w)[P],
+                                           float2 (&r)[P]) {
+#pragma unroll
+  for (int k = 0; k < P; ++k) r[k] = c[m - 1];
+  for (int j = m - 2; j >= 0; --j) {
+    const float2 cj = c[j];
+#pragma unroll
+    for (int k = 0; k < P; ++k) r[k] = cmaddf(r[k], w[k], cj);
+  }
 }
 
 // FP32 Horner, coefficients fetched eight at a time (wave-uniform scalar loads)

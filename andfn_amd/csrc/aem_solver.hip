@@ -157,15 +157,18 @@ double pair_flops(int kind, int m, bool with_q) {
 
 struct TargetSet {
   std::vector<int> side_elem, side_which, pt_off;
+  std::vector<Tile> tiles2;  // 128-point tiles (two points per lane)
+  DBuf<Tile> d_tiles2;
   std::vector<int> fsb;  // [F + 1]: sides of fracture f are [fsb[f], fsb[f + 1])
   std::vector<Tile> tiles;
   std::vector<int> frac_nslots;  // sources per tile's fracture, for work accounting
   DBuf<int> d_side_elem, d_side_which, d_pt_off;
   DBuf<Tile> d_tiles;
   int64_t npts() const { return pt_off.empty() ? 0 : pt_off.back(); }
-  aem::TargetView view() const {
+  int ntiles(int ppl) const { return (int)(ppl == 2 ? tiles2.size() : tiles.size()); }
+  aem::TargetView view(int ppl = 1) const {
     aem::TargetView v;
-    v.tiles = d_tiles.p;
+    v.tiles = ppl == 2 ? d_tiles2.p : d_tiles.p;
     v.side_elem = d_side_elem.p;
     v.side_which = d_side_which.p;
     v.pt_off = d_pt_off.p;
@@ -354,6 +357,11 @@ class Solver {
   }
   double head_residual(const double* b, int* row);
   double bc_residual();
+
+  // tile shape of the collocation and right-hand-side sweeps (points per lane, 1 or 2), picked on
+  // the first sweep of each kind by timing both shapes on the same inputs (same results)
+  int ppl_[2] = {0, 0};
+  void sweep(int mode, const TargetSet& ts, double* val, bool mixed);
 
   // ---- implicit adjoint
   struct AdjTables {
@@ -632,6 +640,7 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
   ts.side_which.clear();
   ts.pt_off.assign(1, 0);
   ts.tiles.clear();
+  ts.tiles2.clear();
   ts.frac_nslots.clear();
   ts.fsb.assign(F_ + 1, 0);
   for (int f = 0; f < F_; ++f) {
@@ -653,6 +662,7 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
       ts.tiles.push_back(Tile{f, sbeg, send, p0, pend});
       ts.frac_nslots.push_back(mem_ptr_[f + 1] - mem_ptr_[f]);
     }
+    for (int p0 = pbeg; p0 < pend; p0 += 128) ts.tiles2.push_back(Tile{f, sbeg, send, p0, pend});
   }
   ts.fsb[F_] = (int)ts.side_elem.size();
   // heaviest fractures first (work of a tile ~ number of sources of its fracture)
@@ -667,6 +677,9 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
   }
   ts.tiles.swap(t2);
   ts.frac_nslots.swap(n2);
+  std::stable_sort(ts.tiles2.begin(), ts.tiles2.end(), [&](const Tile& a, const Tile& b) {
+    return mem_ptr_[a.frac + 1] - mem_ptr_[a.frac] > mem_ptr_[b.frac + 1] - mem_ptr_[b.frac];
+  });
   upload_targets(ts);
 }
 
@@ -675,6 +688,7 @@ void Solver::upload_targets(TargetSet& ts) {
   ts.d_side_which.upload(ts.side_which, st_);
   ts.d_pt_off.upload(ts.pt_off, st_);
   ts.d_tiles.upload(ts.tiles, st_);
+  ts.d_tiles2.upload(ts.tiles2, st_);
 }
 
 void Solver::build_items(const TargetSet& ts, std::vector<FitItem>& items, const std::vector<char>* only) {
@@ -1060,6 +1074,31 @@ double* Solver::discharge_solve(double* b, double* x) {
   return x;
 }
 
+// Collocation / right-hand-side sweep with the tuned tile shape.  The first call of each kind runs
+// both shapes twice on the same inputs (identical values) and keeps the faster.
+void Solver::sweep(int mode, const TargetSet& ts, double* val, bool mixed) {
+  const int slot = mode == aem::MODE_RHS ? 1 : 0;
+  if (ppl_[slot] == 0 && ts.ntiles(1) > 0) {
+    // two rounds in alternating order; the best time of each shape decides
+    float best[2] = {1e30f, 1e30f};
+    for (int r = 0; r < 4; ++r) {
+      const int k = (r == 1 || r == 2) ? 1 : 0;  // shapes 1, 2, 2, 1
+      const int ppl = k + 1;
+      float ms = 0.f;
+      HIP_CHECK(hipEventRecord(ev_[6], st_));
+      aem::launch_sweep(mode, net(), ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl);
+      HIP_CHECK(hipEventRecord(ev_[7], st_));
+      HIP_CHECK(hipEventSynchronize(ev_[7]));
+      HIP_CHECK(hipEventElapsedTime(&ms, ev_[6], ev_[7]));
+      best[k] = std::min(best[k], ms);
+    }
+    ppl_[slot] = best[1] < 0.97f * best[0] ? 2 : 1;  // the two-point shape must win clearly
+    return;
+  }
+  const int ppl = ppl_[slot] ? ppl_[slot] : 1;
+  aem::launch_sweep(mode, net(), ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl);
+}
+
 // max_j |(A0 y)_j - b_j| * row_scale_j over the head equations for the current y
 double Solver::head_residual(const double* b, int* row) {
   const int n = (int)N_;
@@ -1196,7 +1235,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
 
   auto sweep_full = [&](bool blend_now) {
     HIP_CHECK(hipEventRecord(ev_[0], st_));
-    aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(), (int)full_.tiles.size(), K_, d_val_.p, st_, mixed_phase_);
+    sweep(aem::MODE_COLLOC, full_, d_val_.p, mixed_phase_);
     HIP_CHECK(hipEventRecord(ev_[1], st_));
     aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, damping, blend_now ? 1 : 0, st_,
                     mixed_phase_);
@@ -1231,7 +1270,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
     if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
     HIP_CHECK(hipEventRecord(ev_[3], st_));
-    aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_, mixed_phase_);
+    sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed_phase_);
     // only the head rows are written; the continuity and padding rows of b stay zero
     aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
     if (head_tol > 0.0 && nit >= 2) {
@@ -1323,7 +1362,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         }
         HIP_CHECK(hipEventRecord(ev_[6], st_));
         if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
-        aem::launch_sweep(aem::MODE_COLLOC, nv, grown_.view(), (int)grown_.tiles.size(), K_, d_valg_.p, st_, mixed_phase_);
+        sweep(aem::MODE_COLLOC, grown_, d_valg_.p, mixed_phase_);
         HIP_CHECK(hipEventRecord(ev_[7], st_));
         grown_timed = true;
         ++n_sweep;
@@ -1372,7 +1411,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const bool stopped_on_head = head_tol > 0.0 && resid >= 0.0 && resid <= head_tol;
   if (!stopped_on_head) {
     // right-hand side of the returned coefficients, then the residual of the returned discharges
-    aem::launch_sweep(aem::MODE_RHS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_valr_.p, st_);
+    sweep(aem::MODE_RHS, rhs_, d_valr_.p, false);
     aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
     resid = head_residual(d_b_.p, &resid_row);
   }
@@ -1410,6 +1449,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->nit_fp32 = precision_ == ANDFN_MI_MIXED ? (mixed_phase_ ? nit : nit_fp32) : 0;
     info->solves_fp32 = solves32_;
     info->refine_rate = precision_ == ANDFN_MI_MIXED ? refine_rate_ : 0.0;
+    info->sweep_points_per_lane = ppl_[0];
+    info->rhs_points_per_lane = ppl_[1];
   }
   mixed_phase_ = false;
   have_state_ = true;
@@ -1904,7 +1945,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   d_titems_.upload(titems, st_);
   d_ts_.alloc(std::max<size_t>(titems.size(), 1));
   d_val_.alloc(std::max<int64_t>(full_.npts(), 1));
-  aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(), (int)full_.tiles.size(), K_, d_val_.p, st_);
+  sweep(aem::MODE_COLLOC, full_, d_val_.p, false);
   aem::launch_adj_tgrad(nv, d_titems_.p, (int)titems.size(), d_lam_.p, d_val_.p, full_.d_pt_off.p, d_ts_.p, st_);
   d_hw_.alloc(std::max<int64_t>(rhs_.npts(), 1));
   aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_hw_.p, st_);

@@ -104,6 +104,8 @@ typedef struct andfn_mi_info {
                            * (the rest use the FP64 factors: after the refinement stopped contracting) */
   double refine_rate;     /* mixed form: measured contraction per refinement step with the FP32-rounded
                            * factors (above 0.25 the run solves with the FP64 factors throughout) */
+  int32_t sweep_points_per_lane;  /* tile shape the autotuner chose for the collocation sweep (1 or 2) */
+  int32_t rhs_points_per_lane;    /* and for the right-hand-side sweep */
 } andfn_mi_info;
 
 /*
