@@ -29,6 +29,8 @@ LU_TRSV = 0
 LU_GETRS = 1
 LU_INVERSE = 2
 
+FACTORS = {"tournament": 0, "rocsolver": 1}
+
 
 class Options(ctypes.Structure):
     _fields_ = [
@@ -42,6 +44,8 @@ class Options(ctypes.Structure):
         ("growth_record", ctypes.c_void_p),
         ("record_iterations", ctypes.c_int64),
         ("head_tol", ctypes.c_double),
+        ("factor", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

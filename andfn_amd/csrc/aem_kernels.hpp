@@ -151,6 +151,30 @@ void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y
 template <class T>
 void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st);
 void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);
+
+// Blocked LU with tournament pivoting (aem_lu.hip).  Inner panels of kLuPanel columns grouped
+// into outer panels of kLuOuter columns; tournament groups of kLuGroupRows rows.  Buffers:
+// cand[2] and cnt[2] hold ceil(n / kLuGroupRows) sets of kLuPanel rows and their counts; lu11
+// kLuPanel^2 doubles; pairs kLuPairStride ints per inner panel of an outer panel (the row
+// interchange: destinations, sources, their count, the pivots found); perm n ints (perm[i] =
+// original row now at row i, updated in place); info: first zero pivot (1-based).
+constexpr int kLuPanel = 64;
+constexpr int kLuOuter = 256;
+constexpr int kLuGroupRows = 512;
+constexpr int kLuMaxPairs = 2 * kLuPanel;
+constexpr int kLuPairStride = 2 * kLuMaxPairs + 2;
+// pivot tournament of panel [j0, j0 + bw); returns the buffer holding the winners in pivot order
+const int* launch_tslu(const double* A, int64_t lda, int64_t n, int64_t j0, int bw, int* cand[2], int* cnt[2],
+                       double* lu11, int* pairs, int* perm, int64_t* info, hipStream_t st);
+// interchange, L11\U11 and U12 of panel [j0, j0 + bw) on the outer panel's columns [c0, c1)
+void launch_calu_apply(double* A, int64_t lda, int64_t c0, int64_t c1, int64_t j0, int bw, const int* pairs,
+                       const int* win, const double* lu11, hipStream_t st);
+// L21 = A21 U11^-1 for the rows below panel [j0, j0 + bw)
+void launch_calu_l21(double* A, int64_t lda, int64_t n, int64_t j0, int bw, const double* lu11, hipStream_t st);
+// the interchanges of nlists inner panels on every column outside [skip0, skip1)
+void launch_calu_swap(double* A, int64_t lda, int64_t n, int64_t skip0, int64_t skip1, const int* pairs, int nlists,
+                      hipStream_t st);
+
 void launch_refine_rhs(const double* b, const double* ay, const int* perm, const double* rs, int n, float* x32,
                        hipStream_t st);
 void launch_refine_update(const double* y, const float* d, int n, double* x, hipStream_t st);

@@ -280,6 +280,12 @@ class Solver {
   std::vector<int> ncoef0_, nint0_;
   std::vector<double> y0_;
   int lu_mode_ = ANDFN_MI_LU_TRSV;
+  int factor_ = ANDFN_MI_FACTOR_TOURNAMENT;
+  // blocked LU with tournament pivoting (one connected part): candidate sets, the panel's
+  // L11\U11 block and its row interchange
+  DBuf<int> lu_cand_[2], lu_cnt_[2], lu_pairs_;
+  DBuf<double> lu11_;
+  void calu_factor(double* A, int64_t n);
   DBuf<Slot> d_slots_;
   DBuf<FitItem> d_items_, d_items_g_;
   DBuf<RhsItem> d_rhs_items_;
@@ -871,7 +877,10 @@ void Solver::assemble_and_factor() {
   }
   d_ipiv_.alloc(N);
   d_info_.zero(nblk, st_);
-  if (nblk == 1)
+  const bool tournament = nblk == 1 && lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_TOURNAMENT;
+  if (tournament)
+    calu_factor(d_A_.p, npad);
+  else if (nblk == 1)
     RB_CHECK(rocsolver_dgetrf_64(rb_, npad, npad, d_A_.p, npad, d_ipiv_.p, d_info_.p));
   else
     RB_CHECK(rocsolver_dgetrf_strided_batched_64(rb_, npad, npad, d_A_.p, npad, (rocblas_stride)(npad * npad),
@@ -898,7 +907,7 @@ void Solver::assemble_and_factor() {
   }
   d_x_.alloc(N);
   d_w_.alloc(N);
-  if (lu_mode_ == ANDFN_MI_LU_TRSV) {
+  if (lu_mode_ == ANDFN_MI_LU_TRSV && !tournament) {
     // LAPACK row interchanges of every block -> one gather permutation: (P b)[i] = b[perm[i]]
     std::vector<int64_t> ipiv(N);
     HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st_));
@@ -910,6 +919,9 @@ void Solver::assemble_and_factor() {
       for (int64_t i = 0; i < npad; ++i) std::swap(perm[o + i], perm[o + ipiv[o + i] - 1]);
     }
     d_perm_.upload(perm, st_);
+    HIP_CHECK(hipStreamSynchronize(st_));
+  }
+  if (lu_mode_ == ANDFN_MI_LU_TRSV) {
     if (precision_ == ANDFN_MI_MIXED) {
       prepare_blocked<float>(d_A32_.p, npad, nblk, d_linv32_, d_uinv32_);
       d_x32_.alloc(N);
@@ -939,6 +951,57 @@ void Solver::assemble_and_factor() {
     RB_CHECK(rocsolver_dgetri(rb_, (rocblas_int)n, d_A_.p, (rocblas_int)n, ip32.p, inf32.p));
     HIP_CHECK(hipStreamSynchronize(st_));
   }
+}
+
+// In-place LU of the n x n column-major matrix A with tournament pivoting (aem_lu.hip).  Outer
+// panels of kLuOuter columns; within one, per inner panel of kLuPanel columns: the pivot
+// tournament, the interchange with L11\U11 and U12 on the outer panel's columns, L21 = A21 U11^-1
+// and the update of the outer panel's remaining columns.  Then the outer panel's interchanges on
+// all other columns, its U12 and the trailing update.  Leaves the factors in LAPACK layout (unit
+// L below the diagonal, U on and above), the row permutation in d_perm_ ((P b)[i] = b[perm[i]])
+// and the first zero pivot (1-based, 0 if none) in d_info_[0].
+void Solver::calu_factor(double* A, int64_t n) {
+  const int64_t groups = (n + aem::kLuGroupRows - 1) / aem::kLuGroupRows;
+  for (int i = 0; i < 2; ++i) {
+    lu_cand_[i].alloc((size_t)(groups * aem::kLuPanel));
+    lu_cnt_[i].alloc((size_t)groups);
+  }
+  lu11_.alloc(aem::kLuPanel * aem::kLuPanel);
+  lu_pairs_.alloc((aem::kLuOuter / aem::kLuPanel) * aem::kLuPairStride);
+  {
+    std::vector<int> id((size_t)n);
+    for (int64_t i = 0; i < n; ++i) id[i] = (int)i;
+    d_perm_.upload(id, st_);
+    HIP_CHECK(hipStreamSynchronize(st_));
+  }
+  int* cand[2] = {lu_cand_[0].p, lu_cand_[1].p};
+  int* cnt[2] = {lu_cnt_[0].p, lu_cnt_[1].p};
+  const double one = 1.0, mone = -1.0;
+  for (int64_t J0 = 0; J0 < n; J0 += aem::kLuOuter) {
+    const int64_t J1 = std::min<int64_t>(n, J0 + aem::kLuOuter);
+    int nl = 0;
+    for (int64_t j0 = J0; j0 < J1; j0 += aem::kLuPanel, ++nl) {
+      const int bw = (int)std::min<int64_t>(aem::kLuPanel, J1 - j0);
+      int* pairs = lu_pairs_.p + (int64_t)nl * aem::kLuPairStride;
+      const int* win = aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, d_perm_.p, d_info_.p, st_);
+      aem::launch_calu_apply(A, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_);
+      const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
+      aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_);
+      if (rows > 0 && cols > 0)
+        RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rows, cols, bw, &mone,
+                                  A + (j0 + bw) + j0 * n, n, A + j0 + (j0 + bw) * n, n, &one,
+                                  A + (j0 + bw) + (j0 + bw) * n, n));
+    }
+    aem::launch_calu_swap(A, n, n, J0, J1, lu_pairs_.p, nl, st_);
+    const int64_t rest = n - J1;
+    if (rest > 0) {
+      RB_CHECK(rocblas_dtrsm_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                rocblas_diagonal_unit, J1 - J0, rest, &one, A + J0 + J0 * n, n, A + J0 + J1 * n, n));
+      RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rest, rest, J1 - J0, &mone,
+                                A + J1 + J0 * n, n, A + J0 + J1 * n, n, &one, A + J1 + J1 * n, n));
+    }
+  }
+  HIP_CHECK(hipGetLastError());
 }
 
 // Mixed form: contraction rate of the refinement y <- y + (LU)_32^-1 R P (b - A y) on the probe
@@ -1154,6 +1217,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const double damping = C_.DAMPING;
   const int verbose = opt ? opt->verbose : 0;
   lu_mode_ = opt ? opt->lu_mode : ANDFN_MI_LU_TRSV;
+  factor_ = opt ? opt->factor : ANDFN_MI_FACTOR_TOURNAMENT;
+  if (factor_ != ANDFN_MI_FACTOR_TOURNAMENT && factor_ != ANDFN_MI_FACTOR_ROCSOLVER)
+    fail(ANDFN_MI_EINVAL, "factor %d unknown", factor_);
   precision_ = opt ? opt->precision : ANDFN_MI_FP64;
   if (precision_ != ANDFN_MI_FP64 && precision_ != ANDFN_MI_MIXED) fail(ANDFN_MI_EINVAL, "precision %d unknown", precision_);
   if (precision_ == ANDFN_MI_MIXED && lu_mode_ != ANDFN_MI_LU_TRSV)

@@ -29,14 +29,17 @@ def last_solve_info():
 
 
 def solve(fracture_struc_array, element_struc_array, discharge_int, constants, ntype_elements=None, precision="fp64",
-          device=0, verbose=False, lu_mode=0, growth_schedule=None, record_growth=False, head_tol=0.0):
+          device=0, verbose=False, lu_mode=0, growth_schedule=None, record_growth=False, head_tol=0.0,
+          factor="tournament"):
     """Solve in place; see the module docstring.
 
     ``growth_schedule`` (int32 [iterations, n_elements], optional) replays another run's
     coefficient counts; ``record_growth`` stores this run's counts per iteration in
     ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 replaces AnDFN's stop rule: the
     solve stops as soon as the head residual (max |A y - b(c)| over the head equations, metres)
-    is at most ``head_tol`` (or at MAX_ITERATIONS).
+    is at most ``head_tol`` (or at MAX_ITERATIONS).  ``factor``: "tournament" (the library's
+    blocked LU with tournament pivoting) or "rocsolver" (rocSOLVER getrf) for a network of one
+    connected part; networks of several parts always use rocSOLVER's batched getrf.
     """
     global _last_info
     _lib.require_records(fracture_struc_array, element_struc_array)
@@ -46,8 +49,7 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
     if consts.dtype.names is None or "MAX_ERROR" not in consts.dtype.names:
         raise TypeError("constants must be a dtype_constants record")
     nte = np.zeros(6, dtype=np.int64) if ntype_elements is None else np.ascontiguousarray(ntype_elements, dtype=np.int64)
-    opt = _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
-                       int(lu_mode), 0.0, None, 0, None, 0, float(head_tol))
+    opt = _options(precision, device, verbose, lu_mode, head_tol, factor)
     sched = None
     if growth_schedule is not None:
         sched = np.ascontiguousarray(growth_schedule, dtype=np.int32)
@@ -92,9 +94,11 @@ def get_heads(fracture_struc_array, element_struc_array, frac, z, device=0):
     return out
 
 
-def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0):
+def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0, factor="tournament"):
+    if factor not in _lib.FACTORS:
+        raise ValueError(f"factor must be one of {sorted(_lib.FACTORS)}")
     return _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
-                        int(lu_mode), 0.0, None, 0, None, 0, float(head_tol))
+                        int(lu_mode), 0.0, None, 0, None, 0, float(head_tol), _lib.FACTORS[factor], 0)
 
 
 class DeviceNetwork:
@@ -118,8 +122,8 @@ class DeviceNetwork:
                                             ctypes.byref(self.h)))
         self.device = device
 
-    def run(self, precision="fp64", lu_mode=0, head_tol=0.0, verbose=False):
-        opt = _options(precision, self.device, verbose, lu_mode, head_tol)
+    def run(self, precision="fp64", lu_mode=0, head_tol=0.0, verbose=False, factor="tournament"):
+        opt = _options(precision, self.device, verbose, lu_mode, head_tol, factor)
         info = _lib.Info()
         rc = self.lib.andfn_mi_run(self.h, ctypes.byref(opt), ctypes.byref(info))
         _lib.check(rc)

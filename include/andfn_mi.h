@@ -46,6 +46,11 @@ extern "C" {
 #define ANDFN_MI_LU_GETRS 1    /* rocSOLVER getrs */
 #define ANDFN_MI_LU_INVERSE 2  /* explicit inverse formed once (getri), then one gemv per iteration */
 
+/* andfn_mi_options.factor: how a network of one connected part is factored (lu_mode 0 only;
+ * networks of several parts and the other lu_modes always use rocSOLVER) */
+#define ANDFN_MI_FACTOR_TOURNAMENT 0 /* the library's blocked LU with tournament pivoting */
+#define ANDFN_MI_FACTOR_ROCSOLVER 1  /* rocSOLVER getrf (partial pivoting) */
+
 typedef struct andfn_mi_options {
   int32_t precision;     /* ANDFN_MI_FP64 or ANDFN_MI_MIXED */
   int32_t device;        /* HIP device ordinal */
@@ -65,6 +70,8 @@ typedef struct andfn_mi_options {
    * instead of AnDFN's coefficient/discharge-change rule; it is evaluated at the start of every
    * iteration from the new right-hand side.  MAX_ITERATIONS still bounds the solve. */
   double head_tol;
+  int32_t factor;        /* ANDFN_MI_FACTOR_* */
+  int32_t reserved;      /* zero */
 } andfn_mi_options;
 
 typedef struct andfn_mi_info {

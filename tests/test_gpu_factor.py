@@ -1,0 +1,76 @@
+"""The library's blocked LU with tournament pivoting against rocSOLVER's getrf.
+
+Both factor the same equilibrated discharge matrix with different row pivoting, so the per-
+iteration discharge solves agree to rounding and so do the iterates.  Networks of the 200-disk
+law at constant density: 40 fractures (one panel, n < 64), 1,500 fractures (3,470 unknowns: two
+tournament levels) and 4,000 fractures (8,884 unknowns: three levels, 139 panels).  Fixed
+iteration count (the change-based stop off), so both runs take the same iterations.
+Tolerance: 1e-10 relative to the largest magnitude of each quantity; the head residual of the
+final iterate (still hundreds of metres after 40 iterations) agrees to 1e-10 relative.
+"""
+
+import numpy as np
+import pytest
+
+from andfn_amd import _lib
+from andfn_amd.generator import build_network, workload
+from andfn_amd.hpc_solve import DeviceNetwork
+
+pytestmark = pytest.mark.gpu
+CONSTS = dict(MAX_ERROR=0.0, MAX_ITERATIONS=40)
+
+
+def _rel(a, b):
+    s = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (s if s > 0 else 1.0))
+
+
+def _network(n):
+    if n == 40:
+        spec = workload("dfn40", constants=CONSTS)
+    else:
+        spec = workload("cfg1", n_fractures=n, cube=100.0 * (n / 200.0) ** (1.0 / 3.0), constants=CONSTS)
+    d = build_network(spec)
+    d.consolidate_dfn()
+    return d
+
+
+def _run(d, factor, precision="fp64", head_tol=0.0):
+    fa = d.fractures_struc_array_hpc.copy()
+    ea = d.elements_struc_array_hpc.copy()
+    net = DeviceNetwork(fa, ea, d.discharge_int, d.constants)
+    try:
+        info = net.run(precision, head_tol=head_tol, factor=factor)
+        net.download()
+    finally:
+        net.close()
+    return fa, ea, info
+
+
+@pytest.mark.parametrize("n", [40, 1500, 4000])
+def test_tournament_lu_matches_rocsolver(n):
+    print(f"network of {n} fractures", flush=True)
+    d = _network(n)
+    fr, er, ir = _run(d, "rocsolver")
+    ft, et, it = _run(d, "tournament")
+    print(f"unknowns {it['n_unknowns']}, factor s: rocsolver {ir['t_factor']:.3f} tournament {it['t_factor']:.3f}")
+    assert ir["nit"] == it["nit"] == CONSTS["MAX_ITERATIONS"]
+    assert _rel(et["coef"], er["coef"]) < 1e-10
+    assert _rel(et["q"], er["q"]) < 1e-10
+    assert _rel(ft["constant"], fr["constant"]) < 1e-10
+    assert abs(it["head_residual"] - ir["head_residual"]) <= 1e-10 * max(1.0, ir["head_residual"])
+
+
+def test_tournament_lu_mixed_form_reaches_head_tolerance():
+    d = _network(1500)
+    c = np.array(d.constants, copy=True)
+    c["MAX_ITERATIONS"] = 400
+    d.constants = c
+    _, _, info = _run(d, "tournament", precision="mixed", head_tol=1e-6)
+    assert info["converged"] and info["head_residual"] <= 1e-6
+
+
+def test_unknown_factor_is_rejected():
+    with pytest.raises(ValueError):
+        _run(_network(40), "qr")
+    assert _lib.FACTORS == {"tournament": 0, "rocsolver": 1}
