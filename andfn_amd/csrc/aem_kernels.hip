@@ -9,7 +9,8 @@
 //   sweep_kernel<COLLOC> potential at every collocation point of every element, all other
 //                       elements of the fracture superposed   (calc_omega inside the Cauchy fits,
 //                                                              hpc_fracture.py:51-92)
-//   fit_kernel          Cauchy/DFT fit, element update, errors, damping (hpc_math_functions.py:243-297,
+//   unwrap_kernel, dft_kernel (MFMA), fit_kernel
+//                       Cauchy/DFT fit, element update, errors, damping (hpc_math_functions.py:243-297,
 //                       537-670; element solves in hpc_intersection.py, hpc_bounding_circle.py,
 //                       hpc_const_head_line.py, hpc_imp_object.py; hpc_solve.py:383-388)
 //   grow_kernel         coefficient-growth test                 (hpc_solve.py:465-559)
@@ -322,53 +323,124 @@ void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, 
 }
 
 // --------------------------------------------------------------------------------------
-// Fit: one wavefront per element (hpc_math_functions.py:243-297, 537-670 and the element solves)
+// Fit (hpc_math_functions.py:243-297, 537-670 and the element solves), in three launches:
+//   unwrap_kernel  branch-cut correction of circle-type sides, in place on the swept values
+//   dft_kernel     the Cauchy/DFT sums of every side as a GEMM on the matrix cores: tiles of 16
+//                  sides that share (angle family, n, m); D[16 x 2m] = Phi[16 x n] W[n x 2m],
+//                  W = (cos j theta_i | sin j theta_i), v_mfma_f64_16x16x4f64 (FP32 form:
+//                  v_mfma_f32_16x16x4f32 with FP32 accumulation)
+//   fit_kernel     element solves, errors and damping from the sums, one wavefront per element
 // --------------------------------------------------------------------------------------
-template <bool F32>
-__global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
-                                                 const double* __restrict__ val, const int* __restrict__ pt_off,
-                                                 double damping, int blend) {
-  __shared__ double sv[2][kMaxPts];
+__global__ __launch_bounds__(64) void unwrap_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
+                                                    double* __restrict__ val, const int* __restrict__ pt_off) {
   __shared__ double sdpsi[kMaxPts];
+  const FitItem it = items[blockIdx.x];
+  const int e = it.elem;
+  const int ty = net.etype[e];
+  if (!(ty == 1 || ty == 4 || ty == 5)) return;
+  const int lane = threadIdx.x;
+  const int n = net.nint[e];
+  // kappa_i = sum q * sign over the cached crossings, then unwrap (get_dpsi_corr
+  // hpc_math_functions.py:406-447; cauchy_integral_domega 583-595)
+  for (int i = lane; i < n; i += 64) sdpsi[i] = 0.0;
+  __syncthreads();
+  if (lane == 0) {
+    const int cb = fv.cross_ptr[e];
+    const int cn = fv.cross_cnt[e];
+    for (int c = 0; c < cn; ++c) {
+      const int2 rec = fv.cross[cb + c];  // (position, element); sign in cross_sign
+      sdpsi[rec.x] += net.q[rec.y] * (double)fv.cross_sign[cb + c];
+    }
+    double* gd = fv.dpsi_corr + (size_t)e * 2 * net.M;
+    for (int i = 0; i < n - 1; ++i) gd[i] = sdpsi[i];
+    double* v = val + pt_off[it.s0];
+    double psi0 = v[0];
+    double prev = v[0];
+    for (int i = 1; i < n; ++i) {
+      const double cur = v[i];
+      const double d = (cur - prev) - sdpsi[i - 1];
+      prev = cur;
+      psi0 = psi0 + d;
+      v[i] = psi0;
+    }
+  }
+}
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One wavefront per tile (4 per workgroup).  MFMA 16x16x4 operand layout (lane l, measured on
+// gfx950): A[l % 16][l / 16] (side l % 16, point k0 + l / 16), B[l / 16][l % 16] (point, column);
+// result register r holds D[l / 16 + 4 r][l % 16] for the f64 instruction and
+// D[4 (l / 16) + r][l % 16] for the f32 one.
+// Column c < m is X_c = sum_i v_i cos(c theta_i), column m + c is Y_c = sum_i v_i sin(c theta_i);
+// theta_i = (2 i + 1) pi / (2 n) (family 0) or 2 pi i / n (family 1), the angles of
+// calc_thetas (hpc_math_functions.py:673-699), reduced exactly before sincospi.
+template <bool F32>
+__global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ tiles, int ntiles,
+                                                  const int* __restrict__ sides, const double* __restrict__ val,
+                                                  const int* __restrict__ pt_off, int M, double* __restrict__ dft) {
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (tile >= ntiles) return;  // the whole wavefront
+  const DftTile tl = tiles[tile];
+  const int lane = threadIdx.x & 63, r = lane & 15, kq = lane >> 4;
+  const int n = tl.n, m = tl.m, ncol = 2 * m;
+  const int side_a = r < tl.ns ? sides[tl.s0 + r] : -1;
+  const double* va = side_a >= 0 ? val + pt_off[side_a] : nullptr;
+  for (int c0 = 0; c0 < ncol; c0 += 16) {
+    const int col = c0 + r;
+    const bool colok = col < ncol;
+    const bool sinpart = col >= m;
+    const int j = sinpart ? col - m : col;
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    f32x4 accf = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < n; k0 += 4) {
+      const int i = k0 + kq;
+      const bool iok = i < n;
+      const double a = (va != nullptr && iok) ? va[i] : 0.0;
+      if (F32) {
+        float w = 0.0f;
+        if (colok && iok) {
+          float sn, cs;
+          if (tl.family == 1) sincospif(2.0f * (float)((int64_t)i * j % n) / (float)n, &sn, &cs);
+          else sincospif((float)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n)) / (float)(2 * n), &sn, &cs);
+          w = sinpart ? sn : cs;
+        }
+        accf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, w, accf, 0, 0, 0);
+      } else {
+        double w = 0.0;
+        if (colok && iok) {
+          double sn, cs;
+          if (tl.family == 1) sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &sn, &cs);
+          else sincospi((double)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n)) / (double)(2 * n), &sn, &cs);
+          w = sinpart ? sn : cs;
+        }
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, w, acc, 0, 0, 0);
+      }
+    }
+    if (colok) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = F32 ? 4 * kq + q : kq + 4 * q;
+        if (row < tl.ns) {
+          const int side = sides[tl.s0 + row];
+          dft[(size_t)side * 2 * M + (sinpart ? M + j : j)] = F32 ? (double)accf[q] : acc[q];
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
+                                                 const double* __restrict__ dft, double damping, int blend) {
   const FitItem it = items[blockIdx.x];
   const int e = it.elem;
   const int lane = threadIdx.x;
   const int ty = net.etype[e];
   const int m = net.ncoef[e];
   const int n = net.nint[e];
-  const int nsides = (ty == 0) ? 2 : 1;
-  for (int s = 0; s < nsides; ++s) {
-    const int base = pt_off[s == 0 ? it.s0 : it.s1];
-    for (int i = lane; i < n; i += 64) sv[s][i] = val[base + i];
-  }
-  const bool circle = (ty == 1 || ty == 4 || ty == 5);
-  __syncthreads();
-  if (circle) {
-    // branch-cut correction kappa_i = sum q * sign over the cached crossings, then unwrap
-    // (get_dpsi_corr hpc_math_functions.py:406-447; cauchy_integral_domega 583-595)
-    for (int i = lane; i < n; i += 64) sdpsi[i] = 0.0;
-    __syncthreads();
-    if (lane == 0) {
-      const int cb = fv.cross_ptr[e];
-      const int cn = fv.cross_cnt[e];
-      for (int c = 0; c < cn; ++c) {
-        const int2 rec = fv.cross[cb + c];  // (position, element); sign in cross_sign
-        sdpsi[rec.x] += net.q[rec.y] * (double)fv.cross_sign[cb + c];
-      }
-      double* gd = fv.dpsi_corr + (size_t)e * 2 * net.M;
-      for (int i = 0; i < n - 1; ++i) gd[i] = sdpsi[i];
-      double psi0 = sv[0][0];
-      double prev = sv[0][0];
-      for (int i = 1; i < n; ++i) {
-        double cur = sv[0][i];
-        double d = (cur - prev) - sdpsi[i - 1];
-        prev = cur;
-        psi0 = psi0 + d;
-        sv[0][i] = psi0;
-      }
-    }
-    __syncthreads();
-  }
+  const double* d0 = dft + (size_t)it.s0 * 2 * net.M;
+  const double* d1 = dft + (size_t)(ty == 0 ? it.s1 : it.s0) * 2 * net.M;
 
   // per-lane coefficients j = lane, lane + 64, ...
   double2* wc = net.wcoef + (size_t)e * net.M;
@@ -378,49 +450,7 @@ __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const 
   const double t0 = (ty == 0) ? net.t[net.efrac0[e]] : 0.0;
   const double t1 = (ty == 0) ? net.t[net.efrac1[e]] : 0.0;
   for (int j = lane; j < m; j += 64) {
-    double x0 = 0.0, y0 = 0.0, x1 = 0.0;
-    if (F32) {
-      // mixed precision: FP32 DFT with FP32 accumulation
-      float fx0 = 0.0f, fy0 = 0.0f, fx1 = 0.0f;
-      if (ty == 1) {
-        for (int i = 0; i < n; ++i) {
-          float sn, cs;
-          sincospif(2.0f * (float)((int64_t)i * j % n) / (float)n, &sn, &cs);
-          const float v = (float)sv[0][i];
-          fx0 = fmaf(v, cs, fx0);
-          fy0 = fmaf(v, sn, fy0);
-        }
-      } else {
-        const int64_t period = 4 * (int64_t)n;
-        for (int i = 0; i < n; ++i) {
-          float sn, cs;
-          sincospif((float)(((int64_t)j * (2 * i + 1)) % period) / (float)(2 * n), &sn, &cs);
-          const float v = (float)sv[0][i];
-          fx0 = fmaf(v, cs, fx0);
-          fy0 = fmaf(v, sn, fy0);
-          if (ty == 0) fx1 = fmaf((float)sv[1][i], cs, fx1);
-        }
-      }
-      x0 = fx0;
-      y0 = fy0;
-      x1 = fx1;
-    } else if (ty == 1) {
-      for (int i = 0; i < n; ++i) {
-        double s, c;
-        sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &s, &c);
-        x0 = fma(sv[0][i], c, x0);
-        y0 = fma(sv[0][i], s, y0);
-      }
-    } else {
-      const int64_t period = 4 * (int64_t)n;
-      for (int i = 0; i < n; ++i) {
-        double s, c;
-        sincospi((double)(((int64_t)j * (2 * i + 1)) % period) / (double)(2 * n), &s, &c);
-        x0 = fma(sv[0][i], c, x0);
-        y0 = fma(sv[0][i], s, y0);
-        if (ty == 0) x1 = fma(sv[1][i], c, x1);
-      }
-    }
+    const double x0 = d0[j], y0 = d0[net.M + j], x1 = ty == 0 ? d1[j] : 0.0;
     const double two_n = 2.0 / (double)n;
     double2 cnew;
     if (ty == 0) {
@@ -849,13 +879,20 @@ void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st)
   hipLaunchKernelGGL(coef_to_f32_kernel, dim3(ne), dim3(64), 0, st, net, ne, out);
 }
 
-void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
-                const int* pt_off, double damping, int blend, hipStream_t st, bool f32) {
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, double* val,
+                const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides, double* dft, double damping,
+                int blend, hipStream_t st, bool f32) {
   if (nitems <= 0) return;
-  if (f32)
-    hipLaunchKernelGGL(fit_kernel<true>, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
-  else
-    hipLaunchKernelGGL(fit_kernel<false>, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off, damping, blend);
+  hipLaunchKernelGGL(unwrap_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off);
+  if (ntiles > 0) {
+    if (f32)
+      hipLaunchKernelGGL(dft_kernel<true>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
+                         pt_off, net.M, dft);
+    else
+      hipLaunchKernelGGL(dft_kernel<false>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
+                         pt_off, net.M, dft);
+  }
+  hipLaunchKernelGGL(fit_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, dft, damping, blend);
 }
 
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st) {

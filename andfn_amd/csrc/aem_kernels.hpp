@@ -102,6 +102,14 @@ struct FitItem {
   int s0, s1;  // side indices in the target set (s1 = -1 unless intersection)
 };
 
+// The fit's DFT as a GEMM: a tile is up to 16 sides of one target set (sides[s0 .. s0 + ns) of
+// the tile list's side array) that share the point count n, the coefficient count m and the
+// angle family (0: theta_i = (i + 1/2) pi / n, lines and impermeable objects; 1: theta_i =
+// 2 pi i / n, bounding circles).
+struct DftTile {
+  int s0, ns, n, m, family;
+};
+
 struct RhsItem {
   int elem;
   int s0, s1;
@@ -129,8 +137,11 @@ struct IterScalars {
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
                   bool mixed = false, int points_per_lane = 1);
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
-void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const double* val,
-                const int* pt_off, double damping, int blend, hipStream_t st, bool f32 = false);
+// unwrap (circle-type sides, in place in val), DFT sums of every side into dft (side s: X_j at
+// s 2M + j, Y_j at s 2M + M + j), then the element solves
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, double* val,
+                const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides, double* dft, double damping,
+                int blend, hipStream_t st, bool f32 = false);
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
 void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,

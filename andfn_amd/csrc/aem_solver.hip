@@ -25,6 +25,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/andfn_mi.h"
@@ -164,6 +165,11 @@ struct TargetSet {
   std::vector<int> frac_nslots;  // sources per tile's fracture, for work accounting
   DBuf<int> d_side_elem, d_side_which, d_pt_off;
   DBuf<Tile> d_tiles;
+  // collocation sets: the fit's DFT tiles (sides grouped by angle family, n and m)
+  std::vector<aem::DftTile> dft_tiles;
+  std::vector<int> dft_sides;
+  DBuf<aem::DftTile> d_dft_tiles;
+  DBuf<int> d_dft_sides;
   int64_t npts() const { return pt_off.empty() ? 0 : pt_off.back(); }
   int ntiles(int ppl) const { return (int)(ppl == 2 ? tiles2.size() : tiles.size()); }
   aem::TargetView view(int ppl = 1) const {
@@ -235,7 +241,7 @@ class Solver {
   DBuf<int2> d_cross_;
   DBuf<double2> d_ep_, d_center_, d_coef_, d_wcoef_;
   DBuf<double> d_radius_, d_phi_, d_q_, d_t_, d_constant_, d_error_, d_error_old_, d_error_old2_, d_error_coef_;
-  DBuf<double> d_dpsi_, d_val_, d_valg_, d_valr_, d_A_, d_b_, d_y_, d_yold_, d_x_;
+  DBuf<double> d_dpsi_, d_val_, d_valg_, d_valr_, d_dft_, d_A_, d_b_, d_y_, d_yold_, d_x_;
   DBuf<int> d_perm_, d_sched_;
   DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
   DBuf<int> d_csr_ptr_, d_csr_col_;
@@ -686,6 +692,26 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
   std::stable_sort(ts.tiles2.begin(), ts.tiles2.end(), [&](const Tile& a, const Tile& b) {
     return mem_ptr_[a.frac + 1] - mem_ptr_[a.frac] > mem_ptr_[b.frac + 1] - mem_ptr_[b.frac];
   });
+  ts.dft_tiles.clear();
+  ts.dft_sides.clear();
+  if (!rhs) {
+    // sides grouped by (angle family, n, m), cut into tiles of 16
+    const int ns = (int)ts.side_elem.size();
+    auto key = [&](int sd) {
+      const int e = ts.side_elem[sd];
+      return std::make_tuple(etype_[e] == 1 ? 1 : 0, nint_[e], ncoef_[e]);
+    };
+    ts.dft_sides.resize(ns);
+    for (int i = 0; i < ns; ++i) ts.dft_sides[i] = i;
+    std::stable_sort(ts.dft_sides.begin(), ts.dft_sides.end(), [&](int a, int b) { return key(a) < key(b); });
+    for (int i = 0; i < ns;) {
+      int j = i;
+      while (j < ns && j - i < 16 && key(ts.dft_sides[j]) == key(ts.dft_sides[i])) ++j;
+      const int e = ts.side_elem[ts.dft_sides[i]];
+      ts.dft_tiles.push_back(aem::DftTile{i, j - i, nint_[e], ncoef_[e], etype_[e] == 1 ? 1 : 0});
+      i = j;
+    }
+  }
   upload_targets(ts);
 }
 
@@ -695,6 +721,8 @@ void Solver::upload_targets(TargetSet& ts) {
   ts.d_pt_off.upload(ts.pt_off, st_);
   ts.d_tiles.upload(ts.tiles, st_);
   ts.d_tiles2.upload(ts.tiles2, st_);
+  ts.d_dft_tiles.upload(ts.dft_tiles, st_);
+  ts.d_dft_sides.upload(ts.dft_sides, st_);
 }
 
 void Solver::build_items(const TargetSet& ts, std::vector<FitItem>& items, const std::vector<char>* only) {
@@ -1280,6 +1308,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   d_b_.zero(n, st_);
   d_val_.alloc(std::max<int64_t>(full_.npts(), 1));
   d_valr_.alloc(std::max<int64_t>(rhs_.npts(), 1));
+  // DFT sums of every collocation side (a grown subset indexes its own sides from 0)
+  d_dft_.alloc(std::max<size_t>(full_.side_elem.size(), 1) * 2 * (size_t)L_.M);
   aem::launch_init_errors(fv, E_, st_);
   aem::launch_cross(nv, fv, d_circles_.p, (int)circles_.size(), st_);
   HIP_CHECK(hipGetLastError());
@@ -1303,7 +1333,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipEventRecord(ev_[0], st_));
     sweep(aem::MODE_COLLOC, full_, d_val_.p, mixed_phase_);
     HIP_CHECK(hipEventRecord(ev_[1], st_));
-    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, damping, blend_now ? 1 : 0, st_,
+    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, full_.d_dft_tiles.p,
+                    (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, blend_now ? 1 : 0, st_,
                     mixed_phase_);
     aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_);
     HIP_CHECK(hipEventRecord(ev_[2], st_));
@@ -1432,8 +1463,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         HIP_CHECK(hipEventRecord(ev_[7], st_));
         grown_timed = true;
         ++n_sweep;
-        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_valg_.p, grown_.d_pt_off.p, damping, 0, st_,
-                        mixed_phase_);
+        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_valg_.p, grown_.d_pt_off.p,
+                        grown_.d_dft_tiles.p, (int)grown_.dft_tiles.size(), grown_.d_dft_sides.p, d_dft_.p, damping, 0,
+                        st_, mixed_phase_);
         int64_t pr = 0;
         fl_sweep += set_flops(grown_, true, &pr);
         pr_sweep += pr;
