@@ -170,7 +170,7 @@ void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);
 // interchange: destinations, sources, their count, the pivots found); perm n ints (perm[i] =
 // original row now at row i, updated in place); info: first zero pivot (1-based).
 constexpr int kLuPanel = 64;
-constexpr int kLuOuter = 256;
+constexpr int kLuOuter = 512;
 constexpr int kLuGroupRows = 512;
 constexpr int kLuMaxPairs = 2 * kLuPanel;
 constexpr int kLuPairStride = 2 * kLuMaxPairs + 2;

@@ -17,7 +17,9 @@
 //   swap_kernel   applies their interchanges to every column outside the outer panel, and the
 //                 outer U12 (rocBLAS trsm) and the rank-kLuOuter trailing update (rocBLAS dgemm,
 //                 MFMA FP64) follow, so the trailing matrix is read and written once per
-//                 kLuOuter columns instead of once per panel.
+//                 kLuOuter columns instead of once per panel.  The host (Solver::calu_factor)
+//                 updates the next outer panel's columns first and runs the rest of the
+//                 trailing update on a second stream, overlapping the next panel's factorisation.
 // The row permutation is kept as perm[i] = original row now at row i.
 #include <hip/hip_runtime.h>
 #include <stdint.h>

@@ -206,7 +206,11 @@ class Solver {
   ~Solver() {
     drop_graph();
     if (rb_) rocblas_destroy_handle(rb_);
+    if (rb2_) rocblas_destroy_handle(rb2_);
     for (auto& e : ev_) (void)hipEventDestroy(e);
+    if (lu_ev_[0]) (void)hipEventDestroy(lu_ev_[0]);
+    if (lu_ev_[1]) (void)hipEventDestroy(lu_ev_[1]);
+    if (st2_) (void)hipStreamDestroy(st2_);
     if (st_) (void)hipStreamDestroy(st_);
   }
 
@@ -222,6 +226,11 @@ class Solver {
   ConstantsRec C_{};
   hipStream_t st_ = nullptr;
   rocblas_handle rb_ = nullptr;
+  // LU look-ahead: the trailing update of one outer panel runs on st2_ (default priority, like
+  // st_) while st_ factors the next outer panel; created on first use
+  hipStream_t st2_ = nullptr;
+  rocblas_handle rb2_ = nullptr;
+  hipEvent_t lu_ev_[2] = {};
 
   // host mirrors (SoA)
   std::vector<int> etype_, efrac0_, efrac1_, ncoef_, nint_;
@@ -985,7 +994,8 @@ void Solver::assemble_and_factor() {
 // panels of kLuOuter columns; within one, per inner panel of kLuPanel columns: the pivot
 // tournament, the interchange with L11\U11 and U12 on the outer panel's columns, L21 = A21 U11^-1
 // and the update of the outer panel's remaining columns.  Then the outer panel's interchanges on
-// all other columns, its U12 and the trailing update.  Leaves the factors in LAPACK layout (unit
+// all other columns, its U12 and the trailing update: the next outer panel's columns on st_, the
+// rest on st2_, overlapping the next outer panel's factorisation (look-ahead).  Leaves the factors in LAPACK layout (unit
 // L below the diagonal, U on and above), the row permutation in d_perm_ ((P b)[i] = b[perm[i]])
 // and the first zero pivot (1-based, 0 if none) in d_info_[0].
 void Solver::calu_factor(double* A, int64_t n) {
@@ -1002,33 +1012,59 @@ void Solver::calu_factor(double* A, int64_t n) {
     d_perm_.upload(id, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
   }
+  if (!st2_) {
+    HIP_CHECK(hipStreamCreate(&st2_));
+    RB_CHECK(rocblas_create_handle(&rb2_));
+    RB_CHECK(rocblas_set_stream(rb2_, st2_));
+    HIP_CHECK(hipEventCreateWithFlags(&lu_ev_[0], hipEventDisableTiming));
+    HIP_CHECK(hipEventCreateWithFlags(&lu_ev_[1], hipEventDisableTiming));
+  }
   int* cand[2] = {lu_cand_[0].p, lu_cand_[1].p};
   int* cnt[2] = {lu_cnt_[0].p, lu_cnt_[1].p};
   const double one = 1.0, mone = -1.0;
+  bool pending = false;  // a trailing update on st2_ has not been waited for yet
   for (int64_t J0 = 0; J0 < n; J0 += aem::kLuOuter) {
     const int64_t J1 = std::min<int64_t>(n, J0 + aem::kLuOuter);
+    // inner panels: only columns [J0, J1) are read or written (brought up to date by the
+    // previous outer panel's update on st_), so they overlap the trailing update on st2_
     int nl = 0;
     for (int64_t j0 = J0; j0 < J1; j0 += aem::kLuPanel, ++nl) {
       const int bw = (int)std::min<int64_t>(aem::kLuPanel, J1 - j0);
       int* pairs = lu_pairs_.p + (int64_t)nl * aem::kLuPairStride;
       const int* win = aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, d_perm_.p, d_info_.p, st_);
       aem::launch_calu_apply(A, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_);
-      const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
       aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_);
+      const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
       if (rows > 0 && cols > 0)
         RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rows, cols, bw, &mone,
                                   A + (j0 + bw) + j0 * n, n, A + j0 + (j0 + bw) * n, n, &one,
                                   A + (j0 + bw) + (j0 + bw) * n, n));
+    }
+    // everything below touches columns >= J1 (and < J0): the previous trailing update must be done
+    if (pending) {
+      HIP_CHECK(hipStreamWaitEvent(st_, lu_ev_[1], 0));
+      pending = false;
     }
     aem::launch_calu_swap(A, n, n, J0, J1, lu_pairs_.p, nl, st_);
     const int64_t rest = n - J1;
     if (rest > 0) {
       RB_CHECK(rocblas_dtrsm_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                                 rocblas_diagonal_unit, J1 - J0, rest, &one, A + J0 + J0 * n, n, A + J0 + J1 * n, n));
-      RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rest, rest, J1 - J0, &mone,
+      // the next outer panel's columns first, on st_; the rest of the update on st2_
+      const int64_t J2 = std::min<int64_t>(n, J1 + aem::kLuOuter);
+      RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rest, J2 - J1, J1 - J0, &mone,
                                 A + J1 + J0 * n, n, A + J0 + J1 * n, n, &one, A + J1 + J1 * n, n));
+      if (J2 < n) {
+        HIP_CHECK(hipEventRecord(lu_ev_[0], st_));
+        HIP_CHECK(hipStreamWaitEvent(st2_, lu_ev_[0], 0));
+        RB_CHECK(rocblas_dgemm_64(rb2_, rocblas_operation_none, rocblas_operation_none, rest, n - J2, J1 - J0,
+                                  &mone, A + J1 + J0 * n, n, A + J0 + J2 * n, n, &one, A + J1 + J2 * n, n));
+        HIP_CHECK(hipEventRecord(lu_ev_[1], st2_));
+        pending = true;
+      }
     }
   }
+  if (pending) HIP_CHECK(hipStreamWaitEvent(st_, lu_ev_[1], 0));
   HIP_CHECK(hipGetLastError());
 }
 
