@@ -174,17 +174,23 @@ constexpr int kLuOuter = 512;
 constexpr int kLuGroupRows = 512;
 constexpr int kLuMaxPairs = 2 * kLuPanel;
 constexpr int kLuPairStride = 2 * kLuMaxPairs + 2;
+// Batches: nblk independent n x n matrices at A + z sA (launch grid z); per block z the
+// candidate sets, counts, L11\U11 block, pairs (at z sp), n permutation entries and the info word
+// are offset accordingly.
 // pivot tournament of panel [j0, j0 + bw); returns the buffer holding the winners in pivot order
 const int* launch_tslu(const double* A, int64_t lda, int64_t n, int64_t j0, int bw, int* cand[2], int* cnt[2],
-                       double* lu11, int* pairs, int* perm, int64_t* info, hipStream_t st);
+                       double* lu11, int* pairs, int* perm, int64_t* info, hipStream_t st, int nblk = 1,
+                       int64_t sA = 0, int64_t sp = 0);
 // interchange, L11\U11 and U12 of panel [j0, j0 + bw) on the outer panel's columns [c0, c1)
-void launch_calu_apply(double* A, int64_t lda, int64_t c0, int64_t c1, int64_t j0, int bw, const int* pairs,
-                       const int* win, const double* lu11, hipStream_t st);
+void launch_calu_apply(double* A, int64_t lda, int64_t n, int64_t c0, int64_t c1, int64_t j0, int bw,
+                       const int* pairs, const int* win, const double* lu11, hipStream_t st, int nblk = 1,
+                       int64_t sA = 0, int64_t sp = 0);
 // L21 = A21 U11^-1 for the rows below panel [j0, j0 + bw)
-void launch_calu_l21(double* A, int64_t lda, int64_t n, int64_t j0, int bw, const double* lu11, hipStream_t st);
+void launch_calu_l21(double* A, int64_t lda, int64_t n, int64_t j0, int bw, const double* lu11, hipStream_t st,
+                     int nblk = 1, int64_t sA = 0);
 // the interchanges of nlists inner panels on every column outside [skip0, skip1)
 void launch_calu_swap(double* A, int64_t lda, int64_t n, int64_t skip0, int64_t skip1, const int* pairs, int nlists,
-                      hipStream_t st);
+                      hipStream_t st, int nblk = 1, int64_t sA = 0, int64_t sp = 0);
 
 void launch_refine_rhs(const double* b, const double* ay, const int* perm, const double* rs, int n, float* x32,
                        hipStream_t st);

@@ -78,7 +78,24 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
                                                       int* __restrict__ cand_out, int* __restrict__ cnt_out,
                                                       int final_level, double* __restrict__ lu11,
                                                       int* __restrict__ pairs, int* __restrict__ perm,
-                                                      int64_t* __restrict__ info) {
+                                                      int64_t* __restrict__ info, int64_t sA, int64_t sc,
+                                                      int64_t sp) {
+  // block z of a batch: matrix at z sA, candidate sets at z sc (counts at z sc / kBW), pairs at
+  // z sp, its own L11\U11, n permutation entries and info word
+  {
+    const int64_t z = blockIdx.z;
+    A += z * sA;
+    if (cand_in != nullptr) {
+      cand_in += z * sc;
+      cnt_in += z * (sc / kBW);
+    }
+    cand_out += z * sc;
+    cnt_out += z * (sc / kBW);
+    lu11 += z * kBW * kBW;
+    pairs += z * sp;
+    perm += z * n;
+    info += z;
+  }
   __shared__ double prow[kBW];
   __shared__ double pinv;
   __shared__ unsigned long long redk[kWaves];
@@ -205,7 +222,15 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
 __global__ __launch_bounds__(256) void calu_apply_kernel(double* __restrict__ A, int64_t lda, int64_t c0, int64_t c1,
                                                          int64_t j0, int bw, const int* __restrict__ pairs,
                                                          const int* __restrict__ win,
-                                                         const double* __restrict__ lu11) {
+                                                         const double* __restrict__ lu11, int64_t sA, int64_t sc,
+                                                         int64_t sp) {
+  {
+    const int64_t z = blockIdx.z;
+    A += z * sA;
+    pairs += z * sp;
+    win += z * sc;
+    lu11 += z * kBW * kBW;
+  }
   const int lane = threadIdx.x & 63;
   const int64_t c = c0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   const bool valid = c < c1;
@@ -245,7 +270,10 @@ __global__ __launch_bounds__(256) void calu_apply_kernel(double* __restrict__ A,
 // The row interchanges of an outer panel's inner panels (lists 0 .. nlists - 1, in order) on every
 // column outside that outer panel [skip0, skip1): one wavefront per column.
 __global__ __launch_bounds__(256) void calu_swap_kernel(double* __restrict__ A, int64_t lda, int64_t n, int64_t skip0,
-                                                        int64_t skip1, const int* __restrict__ pairs, int nlists) {
+                                                        int64_t skip1, const int* __restrict__ pairs, int nlists,
+                                                        int64_t sA, int64_t sp) {
+  A += (int64_t)blockIdx.z * sA;
+  pairs += (int64_t)blockIdx.z * sp;
   const int lane = threadIdx.x & 63;
   int64_t c = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= skip0) c += skip1 - skip0;
@@ -276,7 +304,9 @@ __global__ __launch_bounds__(256) void calu_swap_kernel(double* __restrict__ A, 
 // One thread per row r >= j0 + bw: L21 row = A21 row U11^-1 by forward substitution over the
 // columns, with U11 (stored by rows) and the reciprocals of its diagonal in LDS.
 __global__ __launch_bounds__(256) void calu_l21_kernel(double* __restrict__ A, int64_t lda, int64_t n, int64_t j0,
-                                                       int bw, const double* __restrict__ lu11) {
+                                                       int bw, const double* __restrict__ lu11, int64_t sA) {
+  A += (int64_t)blockIdx.z * sA;
+  lu11 += (int64_t)blockIdx.z * kBW * kBW;
   __shared__ double ut[kBW * kBW];  // ut[c * kBW + c2] = U11(c, c2)
   __shared__ double rinv[kBW];
   for (int i = threadIdx.x; i < kBW * kBW; i += 256) {
@@ -308,43 +338,49 @@ __global__ __launch_bounds__(256) void calu_l21_kernel(double* __restrict__ A, i
 }
 
 const int* launch_tslu(const double* A, int64_t lda, int64_t n, int64_t j0, int bw, int* cand[2], int* cnt[2],
-                       double* lu11, int* pairs, int* perm, int64_t* info, hipStream_t st) {
+                       double* lu11, int* pairs, int* perm, int64_t* info, hipStream_t st, int nblk, int64_t sA,
+                       int64_t sp) {
   const int64_t m = n - j0;
+  const int64_t sc = (n + kRows - 1) / kRows * kBW;  // candidate sets per block at the leaf level
   int groups = (int)((m + kRows - 1) / kRows);
   int cur = 0;
-  hipLaunchKernelGGL(tslu_kernel, dim3(groups), dim3(kRows), 0, st, A, lda, n, j0, bw, (const int*)nullptr,
-                     (const int*)nullptr, 0, cand[cur], cnt[cur], groups == 1 ? 1 : 0, lu11, pairs, perm, info);
+  hipLaunchKernelGGL(tslu_kernel, dim3(groups, 1, nblk), dim3(kRows), 0, st, A, lda, n, j0, bw, (const int*)nullptr,
+                     (const int*)nullptr, 0, cand[cur], cnt[cur], groups == 1 ? 1 : 0, lu11, pairs, perm, info, sA, sc,
+                     sp);
   while (groups > 1) {
     const int nsets = groups;
     groups = (nsets + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(tslu_kernel, dim3(groups), dim3(kRows), 0, st, A, lda, n, j0, bw, (const int*)cand[cur],
-                       (const int*)cnt[cur], nsets, cand[cur ^ 1], cnt[cur ^ 1], groups == 1 ? 1 : 0, lu11, pairs,
-                       perm, info);
+    hipLaunchKernelGGL(tslu_kernel, dim3(groups, 1, nblk), dim3(kRows), 0, st, A, lda, n, j0, bw,
+                       (const int*)cand[cur], (const int*)cnt[cur], nsets, cand[cur ^ 1], cnt[cur ^ 1],
+                       groups == 1 ? 1 : 0, lu11, pairs, perm, info, sA, sc, sp);
     cur ^= 1;
   }
   return cand[cur];
 }
 
-void launch_calu_apply(double* A, int64_t lda, int64_t c0, int64_t c1, int64_t j0, int bw, const int* pairs,
-                       const int* win, const double* lu11, hipStream_t st) {
+void launch_calu_apply(double* A, int64_t lda, int64_t n, int64_t c0, int64_t c1, int64_t j0, int bw,
+                       const int* pairs, const int* win, const double* lu11, hipStream_t st, int nblk, int64_t sA,
+                       int64_t sp) {
   if (c1 <= c0) return;
-  hipLaunchKernelGGL(calu_apply_kernel, dim3((unsigned)((c1 - c0 + 3) / 4)), dim3(256), 0, st, A, lda, c0, c1, j0,
-                     bw, pairs, win, lu11);
+  const int64_t sc = (n + kRows - 1) / kRows * kBW;
+  hipLaunchKernelGGL(calu_apply_kernel, dim3((unsigned)((c1 - c0 + 3) / 4), 1, nblk), dim3(256), 0, st, A, lda, c0,
+                     c1, j0, bw, pairs, win, lu11, sA, sc, sp);
 }
 
 void launch_calu_swap(double* A, int64_t lda, int64_t n, int64_t skip0, int64_t skip1, const int* pairs, int nlists,
-                      hipStream_t st) {
+                      hipStream_t st, int nblk, int64_t sA, int64_t sp) {
   const int64_t cols = n - (skip1 - skip0);
   if (cols <= 0 || nlists <= 0) return;
-  hipLaunchKernelGGL(calu_swap_kernel, dim3((unsigned)((cols + 3) / 4)), dim3(256), 0, st, A, lda, n, skip0, skip1,
-                     pairs, nlists);
+  hipLaunchKernelGGL(calu_swap_kernel, dim3((unsigned)((cols + 3) / 4), 1, nblk), dim3(256), 0, st, A, lda, n, skip0,
+                     skip1, pairs, nlists, sA, sp);
 }
 
-void launch_calu_l21(double* A, int64_t lda, int64_t n, int64_t j0, int bw, const double* lu11, hipStream_t st) {
+void launch_calu_l21(double* A, int64_t lda, int64_t n, int64_t j0, int bw, const double* lu11, hipStream_t st,
+                     int nblk, int64_t sA) {
   const int64_t rows = n - j0 - bw;
   if (rows <= 0) return;
-  hipLaunchKernelGGL(calu_l21_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, st, A, lda, n, j0, bw,
-                     lu11);
+  hipLaunchKernelGGL(calu_l21_kernel, dim3((unsigned)((rows + 255) / 256), 1, nblk), dim3(256), 0, st, A, lda, n, j0,
+                     bw, lu11, sA);
 }
 
 }  // namespace aem

@@ -300,7 +300,7 @@ class Solver {
   // L11\U11 block and its row interchange
   DBuf<int> lu_cand_[2], lu_cnt_[2], lu_pairs_;
   DBuf<double> lu11_;
-  void calu_factor(double* A, int64_t n);
+  void calu_factor(double* A, int64_t n, int nblk);
   DBuf<Slot> d_slots_;
   DBuf<FitItem> d_items_, d_items_g_;
   DBuf<RhsItem> d_rhs_items_;
@@ -914,9 +914,9 @@ void Solver::assemble_and_factor() {
   }
   d_ipiv_.alloc(N);
   d_info_.zero(nblk, st_);
-  const bool tournament = nblk == 1 && lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_TOURNAMENT;
+  const bool tournament = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_TOURNAMENT;
   if (tournament)
-    calu_factor(d_A_.p, npad);
+    calu_factor(d_A_.p, npad, nblk);
   else if (nblk == 1)
     RB_CHECK(rocsolver_dgetrf_64(rb_, npad, npad, d_A_.p, npad, d_ipiv_.p, d_info_.p));
   else
@@ -990,7 +990,8 @@ void Solver::assemble_and_factor() {
   }
 }
 
-// In-place LU of the n x n column-major matrix A with tournament pivoting (aem_lu.hip).  Outer
+// In-place LU of the nblk n x n column-major matrices at A + b n^2 with tournament pivoting
+// (aem_lu.hip; one launch per step covers every block).  Outer
 // panels of kLuOuter columns; within one, per inner panel of kLuPanel columns: the pivot
 // tournament, the interchange with L11\U11 and U12 on the outer panel's columns, L21 = A21 U11^-1
 // and the update of the outer panel's remaining columns.  Then the outer panel's interchanges on
@@ -998,17 +999,19 @@ void Solver::assemble_and_factor() {
 // rest on st2_, overlapping the next outer panel's factorisation (look-ahead).  Leaves the factors in LAPACK layout (unit
 // L below the diagonal, U on and above), the row permutation in d_perm_ ((P b)[i] = b[perm[i]])
 // and the first zero pivot (1-based, 0 if none) in d_info_[0].
-void Solver::calu_factor(double* A, int64_t n) {
+void Solver::calu_factor(double* A, int64_t n, int nblk) {
   const int64_t groups = (n + aem::kLuGroupRows - 1) / aem::kLuGroupRows;
+  const int nslots = aem::kLuOuter / aem::kLuPanel;
+  const int64_t sA = n * n, sp = (int64_t)nslots * aem::kLuPairStride;
   for (int i = 0; i < 2; ++i) {
-    lu_cand_[i].alloc((size_t)(groups * aem::kLuPanel));
-    lu_cnt_[i].alloc((size_t)groups);
+    lu_cand_[i].alloc((size_t)(nblk * groups * aem::kLuPanel));
+    lu_cnt_[i].alloc((size_t)(nblk * groups));
   }
-  lu11_.alloc(aem::kLuPanel * aem::kLuPanel);
-  lu_pairs_.alloc((aem::kLuOuter / aem::kLuPanel) * aem::kLuPairStride);
+  lu11_.alloc((size_t)nblk * aem::kLuPanel * aem::kLuPanel);
+  lu_pairs_.alloc((size_t)(nblk * sp));
   {
-    std::vector<int> id((size_t)n);
-    for (int64_t i = 0; i < n; ++i) id[i] = (int)i;
+    std::vector<int> id((size_t)(n * nblk));
+    for (int64_t i = 0; i < n * nblk; ++i) id[i] = (int)i;
     d_perm_.upload(id, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
   }
@@ -1022,6 +1025,15 @@ void Solver::calu_factor(double* A, int64_t n) {
   int* cand[2] = {lu_cand_[0].p, lu_cand_[1].p};
   int* cnt[2] = {lu_cnt_[0].p, lu_cnt_[1].p};
   const double one = 1.0, mone = -1.0;
+  // C -= Ablk Bblk on every block (rocBLAS, strided-batched for several blocks)
+  auto gemm = [&](rocblas_handle h, int64_t m, int64_t nc, int64_t k, const double* a, const double* bm, double* c) {
+    if (nblk == 1)
+      RB_CHECK(rocblas_dgemm_64(h, rocblas_operation_none, rocblas_operation_none, m, nc, k, &mone, a, n, bm, n, &one,
+                                c, n));
+    else
+      RB_CHECK(rocblas_dgemm_strided_batched_64(h, rocblas_operation_none, rocblas_operation_none, m, nc, k, &mone, a,
+                                                n, sA, bm, n, sA, &one, c, n, sA, nblk));
+  };
   bool pending = false;  // a trailing update on st2_ has not been waited for yet
   for (int64_t J0 = 0; J0 < n; J0 += aem::kLuOuter) {
     const int64_t J1 = std::min<int64_t>(n, J0 + aem::kLuOuter);
@@ -1031,34 +1043,36 @@ void Solver::calu_factor(double* A, int64_t n) {
     for (int64_t j0 = J0; j0 < J1; j0 += aem::kLuPanel, ++nl) {
       const int bw = (int)std::min<int64_t>(aem::kLuPanel, J1 - j0);
       int* pairs = lu_pairs_.p + (int64_t)nl * aem::kLuPairStride;
-      const int* win = aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, d_perm_.p, d_info_.p, st_);
-      aem::launch_calu_apply(A, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_);
-      aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_);
+      const int* win =
+          aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, d_perm_.p, d_info_.p, st_, nblk, sA, sp);
+      aem::launch_calu_apply(A, n, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_, nblk, sA, sp);
+      aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_, nblk, sA);
       const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
       if (rows > 0 && cols > 0)
-        RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rows, cols, bw, &mone,
-                                  A + (j0 + bw) + j0 * n, n, A + j0 + (j0 + bw) * n, n, &one,
-                                  A + (j0 + bw) + (j0 + bw) * n, n));
+        gemm(rb_, rows, cols, bw, A + (j0 + bw) + j0 * n, A + j0 + (j0 + bw) * n, A + (j0 + bw) + (j0 + bw) * n);
     }
     // everything below touches columns >= J1 (and < J0): the previous trailing update must be done
     if (pending) {
       HIP_CHECK(hipStreamWaitEvent(st_, lu_ev_[1], 0));
       pending = false;
     }
-    aem::launch_calu_swap(A, n, n, J0, J1, lu_pairs_.p, nl, st_);
+    aem::launch_calu_swap(A, n, n, J0, J1, lu_pairs_.p, nl, st_, nblk, sA, sp);
     const int64_t rest = n - J1;
     if (rest > 0) {
-      RB_CHECK(rocblas_dtrsm_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                                rocblas_diagonal_unit, J1 - J0, rest, &one, A + J0 + J0 * n, n, A + J0 + J1 * n, n));
+      if (nblk == 1)
+        RB_CHECK(rocblas_dtrsm_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                  rocblas_diagonal_unit, J1 - J0, rest, &one, A + J0 + J0 * n, n, A + J0 + J1 * n, n));
+      else
+        RB_CHECK(rocblas_dtrsm_strided_batched_64(rb_, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                                                  rocblas_diagonal_unit, J1 - J0, rest, &one, A + J0 + J0 * n, n, sA,
+                                                  A + J0 + J1 * n, n, sA, nblk));
       // the next outer panel's columns first, on st_; the rest of the update on st2_
       const int64_t J2 = std::min<int64_t>(n, J1 + aem::kLuOuter);
-      RB_CHECK(rocblas_dgemm_64(rb_, rocblas_operation_none, rocblas_operation_none, rest, J2 - J1, J1 - J0, &mone,
-                                A + J1 + J0 * n, n, A + J0 + J1 * n, n, &one, A + J1 + J1 * n, n));
+      gemm(rb_, rest, J2 - J1, J1 - J0, A + J1 + J0 * n, A + J0 + J1 * n, A + J1 + J1 * n);
       if (J2 < n) {
         HIP_CHECK(hipEventRecord(lu_ev_[0], st_));
         HIP_CHECK(hipStreamWaitEvent(st2_, lu_ev_[0], 0));
-        RB_CHECK(rocblas_dgemm_64(rb2_, rocblas_operation_none, rocblas_operation_none, rest, n - J2, J1 - J0,
-                                  &mone, A + J1 + J0 * n, n, A + J0 + J2 * n, n, &one, A + J1 + J2 * n, n));
+        gemm(rb2_, rest, n - J2, J1 - J0, A + J1 + J0 * n, A + J0 + J2 * n, A + J1 + J2 * n);
         HIP_CHECK(hipEventRecord(lu_ev_[1], st2_));
         pending = true;
       }

@@ -38,8 +38,8 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
     ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 replaces AnDFN's stop rule: the
     solve stops as soon as the head residual (max |A y - b(c)| over the head equations, metres)
     is at most ``head_tol`` (or at MAX_ITERATIONS).  ``factor``: "tournament" (the library's
-    blocked LU with tournament pivoting) or "rocsolver" (rocSOLVER getrf) for a network of one
-    connected part; networks of several parts always use rocSOLVER's batched getrf.
+    blocked LU with tournament pivoting, all connected parts of the network in one batch) or
+    "rocsolver" (rocSOLVER getrf, strided-batched for several parts).
     """
     global _last_info
     _lib.require_records(fracture_struc_array, element_struc_array)

@@ -46,10 +46,10 @@ extern "C" {
 #define ANDFN_MI_LU_GETRS 1    /* rocSOLVER getrs */
 #define ANDFN_MI_LU_INVERSE 2  /* explicit inverse formed once (getri), then one gemv per iteration */
 
-/* andfn_mi_options.factor: how a network of one connected part is factored (lu_mode 0 only;
- * networks of several parts and the other lu_modes always use rocSOLVER) */
-#define ANDFN_MI_FACTOR_TOURNAMENT 0 /* the library's blocked LU with tournament pivoting */
-#define ANDFN_MI_FACTOR_ROCSOLVER 1  /* rocSOLVER getrf (partial pivoting) */
+/* andfn_mi_options.factor: how the discharge system is factored (lu_mode 0 only; the other
+ * lu_modes always use rocSOLVER) */
+#define ANDFN_MI_FACTOR_TOURNAMENT 0 /* the library's blocked LU with tournament pivoting (all parts of the network in one batch) */
+#define ANDFN_MI_FACTOR_ROCSOLVER 1  /* rocSOLVER getrf (partial pivoting; strided-batched for several parts) */
 
 typedef struct andfn_mi_options {
   int32_t precision;     /* ANDFN_MI_FP64 or ANDFN_MI_MIXED */

@@ -4,7 +4,8 @@ Both factor the same equilibrated discharge matrix with different row pivoting, 
 iteration discharge solves agree to rounding and so do the iterates.  Networks of the 200-disk
 law at constant density: 40 fractures (one panel, n < 64), 1,500 fractures (3,470 unknowns: two
 tournament levels) and 4,000 fractures (8,884 unknowns: three levels, 139 panels).  Fixed
-iteration count (the change-based stop off), so both runs take the same iterations.
+iteration count (the change-based stop off), so both runs take the same iterations.  A batch of
+three realisations (three discharge blocks factored in one batch) is compared the same way.
 Tolerance: 1e-10 relative to the largest magnitude of each quantity; the head residual of the
 final iterate (still hundreds of metres after 40 iterations) agrees to 1e-10 relative.
 """
@@ -68,6 +69,29 @@ def test_tournament_lu_mixed_form_reaches_head_tolerance():
     d.constants = c
     _, _, info = _run(d, "tournament", precision="mixed", head_tol=1e-6)
     assert info["converged"] and info["head_residual"] <= 1e-6
+
+
+def test_tournament_lu_batch_of_realisations_matches_rocsolver():
+    # three independent realisations: one discharge block each, factored in one batch
+    from andfn_amd.batch import build_batch, realisation_specs
+
+    specs = realisation_specs(3, seed=7, nmin=150, nmax=400)
+    _, fa, ea, off, K, c = build_batch(specs, dict(MAX_ERROR=0.0, MAX_NCOEF=10, MAX_ITERATIONS=40))
+    res = {}
+    for factor in ("rocsolver", "tournament"):
+        f1, e1 = fa.copy(), ea.copy()
+        net = DeviceNetwork(f1, e1, K, c)
+        try:
+            info = net.run("fp64", factor=factor)
+            net.download()
+        finally:
+            net.close()
+        res[factor] = (f1, e1, info)
+    (fr, er, ir), (ft, et, it) = res["rocsolver"], res["tournament"]
+    assert ir["nit"] == it["nit"] == 40
+    assert _rel(et["coef"], er["coef"]) < 1e-10
+    assert _rel(et["q"], er["q"]) < 1e-10
+    assert _rel(ft["constant"], fr["constant"]) < 1e-10
 
 
 def test_unknown_factor_is_rejected():
