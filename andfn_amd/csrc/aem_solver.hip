@@ -797,6 +797,22 @@ static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_di
   return rocblas_strmm_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, ldb, B, ldb);
 }
 
+// the same product on nb blocks: T + z sT, B + z sB
+static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
+                                            const double* T, int ldt, int64_t sT, double* B, int64_t ldb, int64_t sB,
+                                            int nb) {
+  const double one = 1.0;
+  return rocblas_dtrmm_strided_batched_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, sT,
+                                          B, ldb, sB, B, ldb, sB, nb);
+}
+static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
+                                            const float* T, int ldt, int64_t sT, float* B, int64_t ldb, int64_t sB,
+                                            int nb) {
+  const float one = 1.0f;
+  return rocblas_strmm_strided_batched_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, sT,
+                                          B, ldb, sB, B, ldb, sB, nb);
+}
+
 // Blocked triangular-solve operators from the LU factors in A (n x n, column-major, kNB blocks):
 // Linv_j / Uinv_j = inverses of the diagonal blocks of L (unit) and U, and, in place in A, the
 // panels G_j = L[below j, j] Linv_j and H_j = U[above j, j] Uinv_j.  With them, L z = c becomes
@@ -813,6 +829,31 @@ void Solver::prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& 
   const int64_t sinv = nb * kNB * kNB;  // inverse blocks of one system
   linv.zero((size_t)(nblk * sinv), st_);
   uinv.zero((size_t)(nblk * sinv), st_);
+  if (nblk > 1) {
+    // several systems of the same (padded) size: per diagonal block j, one call covers them all
+    const int64_t sA = n * n;
+    for (int64_t j = 0; j < nb; ++j) {
+      const int64_t r0 = j * kNB;
+      const int bs = (int)std::min<int64_t>(kNB, n - r0);
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, bs, A + r0 * (n + 1), n, sA, linv.p + j * kNB * kNB, kNB,
+                                sinv, nblk));
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, bs, A + r0 * (n + 1), n, sA, uinv.p + j * kNB * kNB, kNB,
+                                sinv, nblk));
+    }
+    aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)(nb * nblk), kNB, st_);
+    for (int64_t j = 0; j < nb; ++j) {
+      const int64_t r0 = j * kNB;
+      const int bs = (int)std::min<int64_t>(kNB, n - r0);
+      const int rest = (int)(n - r0 - bs);
+      if (rest > 0)
+        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + j * kNB * kNB, kNB, sinv,
+                                       A + r0 * n + r0 + bs, n, sA, nblk));
+      if (r0 > 0)
+        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + j * kNB * kNB, kNB, sinv,
+                                       A + r0 * n, n, sA, nblk));
+    }
+    return;
+  }
   for (int b = 0; b < nblk; ++b) {
     T* Ab = A + (int64_t)b * n * n;
     T* li = linv.p + b * sinv;
@@ -1712,7 +1753,23 @@ void Solver::make_transposed(const T* A, bool blocked, DBuf<T>& AT, DBuf<T>& lin
     DBuf<T> tmp;
     tmp.alloc(tot);
     HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * tot, hipMemcpyDeviceToDevice, st_));
-    for (int b = 0; blocked && b < nblk_; ++b) {
+    if (blocked && nblk_ > 1) {
+      // several systems of the same (padded) size: one call per panel covers them all
+      const int64_t sA = n * n;
+      for (int64_t j = 0; j < nb; ++j) {
+        const int64_t r0 = j * kNB;
+        const int bs = (int)std::min<int64_t>(kNB, n - r0);
+        const int rest = (int)(n - r0 - bs);
+        const T* djj = tmp.p + r0 * (n + 1);
+        if (rest > 0)
+          RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n, sA,
+                                         tmp.p + r0 * n + r0 + bs, n, sA, nblk_));
+        if (r0 > 0)
+          RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
+                                         sA, tmp.p + r0 * n, n, sA, nblk_));
+      }
+    }
+    for (int b = 0; blocked && nblk_ == 1 && b < nblk_; ++b) {
       T* Ab = tmp.p + (int64_t)b * n * n;
       for (int64_t j = 0; j < nb; ++j) {
         const int64_t r0 = j * kNB;
