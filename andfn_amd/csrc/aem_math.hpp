@@ -108,21 +108,40 @@ __device__ __forceinline__ double2 cmadd(double2 r, double2 w, double2 c) {
 }
 
 // P independent Horner chains sharing every coefficient load (P target points per lane)
+// r = sum_{j<m} c_j w^j for P target points (P per lane) by Horner; P independent chains share
+// every coefficient load (wave-uniform, scalar loads).
 template <int P>
 __device__ __forceinline__ void horner_p(const double2* __restrict__ c, int m, const double2 (&w)[P],
                                          double2 (&r)[P]) {
 #pragma unroll
   for (int k = 0; k < P; ++k) r[k] = c[m - 1];
   int j = m - 2;
-  for (; j >= 3; j -= 4) {
-    const double2 c0 = c[j], c1 = c[j - 1], c2_ = c[j - 2], c3 = c[j - 3];
+  if (j >= 3) {
+    // software-pipelined: the next four coefficients are requested before the current four are
+    // used, so the scalar-load latency overlaps the multiply-adds (same arithmetic order)
+    double2 n0 = c[j], n1 = c[j - 1], n2 = c[j - 2], n3 = c[j - 3];
+    for (; j >= 7; j -= 4) {
+      const double2 c0 = n0, c1 = n1, c2_ = n2, c3 = n3;
+      n0 = c[j - 4];
+      n1 = c[j - 5];
+      n2 = c[j - 6];
+      n3 = c[j - 7];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        r[k] = cmadd(r[k], w[k], c0);
+        r[k] = cmadd(r[k], w[k], c1);
+        r[k] = cmadd(r[k], w[k], c2_);
+        r[k] = cmadd(r[k], w[k], c3);
+      }
+    }
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      r[k] = cmadd(r[k], w[k], c0);
-      r[k] = cmadd(r[k], w[k], c1);
-      r[k] = cmadd(r[k], w[k], c2_);
-      r[k] = cmadd(r[k], w[k], c3);
+      r[k] = cmadd(r[k], w[k], n0);
+      r[k] = cmadd(r[k], w[k], n1);
+      r[k] = cmadd(r[k], w[k], n2);
+      r[k] = cmadd(r[k], w[k], n3);
     }
+    j -= 4;
   }
   for (; j >= 0; --j) {
     const double2 cj = c[j];

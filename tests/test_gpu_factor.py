@@ -28,7 +28,9 @@ def _rel(a, b):
 
 def _network(n):
     if n == 40:
-        spec = workload("dfn40", constants=CONSTS)
+        # coefficient growth off, as in the other networks: the comparison is of the factorisation,
+        # not of a growth trajectory (whose threshold tests amplify rounding-level differences)
+        spec = workload("dfn40", constants=dict(CONSTS, MAX_NCOEF=10))
     else:
         spec = workload("cfg1", n_fractures=n, cube=100.0 * (n / 200.0) ** (1.0 / 3.0), constants=CONSTS)
     d = build_network(spec)
