@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "aem_kernels.hpp"
 #include "aem_math.hpp"
 
@@ -776,23 +778,54 @@ __device__ __forceinline__ void nanmax_merge(double& a, int& ia, double b, int i
 }
 
 // one block of 1024 threads: max error (with argmax), max error_coef, max |y - y_old|, max |y_old|
-__global__ __launch_bounds__(1024) void reduce_kernel(FitView fv, int ne, const double* __restrict__ y,
-                                                      const double* __restrict__ yold, int n,
-                                                      IterScalars* __restrict__ out) {
-  __shared__ double se[1024], sc[1024], sd[1024], so[1024];
-  __shared__ int si[1024];
-  const int tid = threadIdx.x;
+// Two-stage reductions: each workgroup reduces a contiguous chunk and writes its partial; the
+// last workgroup to finish (atomic counter, reset by that workgroup for the next launch) merges
+// the partials in increasing chunk order, so ties resolve to the first index as in np.argmax.
+constexpr int kRedThreads = 256;
+constexpr int kRedMaxGroups = 128;
+struct RedPart {
+  double e, c, d, o;
+  int ie, pad;
+};
+struct RedScratch {
+  unsigned count_reduce, count_residual, pad[2];
+  RedPart parts[kRedMaxGroups];
+};
+
+__device__ __forceinline__ void absmax_merge(double& a, double b) { a = (a != a) ? a : ((b != b || b > a) ? b : a); }
+
+__device__ __forceinline__ bool last_group(unsigned* counter) {
+  __shared__ bool last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this group's partial is visible before the count
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (last) __threadfence();  // every other group's partial is visible to this one
+  return last;
+}
+
+__global__ __launch_bounds__(kRedThreads) void reduce_kernel(FitView fv, int ne, const double* __restrict__ y,
+                                                             const double* __restrict__ yold, int n,
+                                                             RedScratch* __restrict__ rs,
+                                                             IterScalars* __restrict__ out) {
+  __shared__ double se[kRedThreads], sc[kRedThreads], sd[kRedThreads], so[kRedThreads];
+  __shared__ int si[kRedThreads];
+  const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
   double em = -__builtin_inf(), cm = -__builtin_inf(), dm = 0.0, om = 0.0;
   int ie = 0x7fffffff, dummy = 0;
-  for (int e = tid; e < ne; e += 1024) {
-    nanmax_merge(em, ie, fv.error[e], e);
-    nanmax_merge(cm, dummy, fv.error_coef[e], e);
-  }
-  for (int k = tid; k < n; k += 1024) {
-    double d = fabs(y[k] - yold[k]);
-    dm = (d != d || d > dm) ? d : dm;
-    double o = fabs(yold[k]);
-    om = (o != o || o > om) ? o : om;
+  {
+    const int per = (ne + G - 1) / G, e0 = g * per, e1 = min(ne, e0 + per);
+    for (int e = e0 + tid; e < e1; e += kRedThreads) {
+      nanmax_merge(em, ie, fv.error[e], e);
+      nanmax_merge(cm, dummy, fv.error_coef[e], e);
+    }
+    const int pn = (n + G - 1) / G, k0 = g * pn, k1 = min(n, k0 + pn);
+    for (int k = k0 + tid; k < k1; k += kRedThreads) {
+      absmax_merge(dm, fabs(y[k] - yold[k]));
+      absmax_merge(om, fabs(yold[k]));
+    }
   }
   se[tid] = em;
   si[tid] = ie;
@@ -800,7 +833,7 @@ __global__ __launch_bounds__(1024) void reduce_kernel(FitView fv, int ne, const 
   sd[tid] = dm;
   so[tid] = om;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
     if (tid < s) {
       double a = se[tid];
       int ia = si[tid];
@@ -811,19 +844,33 @@ __global__ __launch_bounds__(1024) void reduce_kernel(FitView fv, int ne, const 
       double c = sc[tid];
       nanmax_merge(c, d0, sc[tid + s], 0);
       sc[tid] = c;
-      double d = sd[tid], d2 = sd[tid + s];
-      sd[tid] = (d != d) ? d : ((d2 != d2 || d2 > d) ? d2 : d);
-      double o = so[tid], o2 = so[tid + s];
-      so[tid] = (o != o) ? o : ((o2 != o2 || o2 > o) ? o2 : o);
+      double d = sd[tid];
+      absmax_merge(d, sd[tid + s]);
+      sd[tid] = d;
+      double o = so[tid];
+      absmax_merge(o, so[tid + s]);
+      so[tid] = o;
     }
     __syncthreads();
   }
+  if (tid == 0) rs->parts[g] = RedPart{se[0], sc[0], sd[0], so[0], si[0], 0};
+  if (!last_group(&rs->count_reduce)) return;
   if (tid == 0) {
-    out->error = se[0];
-    out->argmax = si[0];
-    out->error_coef = sc[0];
-    out->dq_max = sd[0];
-    out->yold_max = so[0];
+    RedPart r = rs->parts[0];
+    for (int k = 1; k < G; ++k) {
+      const RedPart q = rs->parts[k];
+      nanmax_merge(r.e, r.ie, q.e, q.ie);
+      int d0 = 0;
+      nanmax_merge(r.c, d0, q.c, 0);
+      absmax_merge(r.d, q.d);
+      absmax_merge(r.o, q.o);
+    }
+    out->error = r.e;
+    out->argmax = r.ie;
+    out->error_coef = r.c;
+    out->dq_max = r.d;
+    out->yold_max = r.o;
+    rs->count_reduce = 0;
   }
 }
 
@@ -879,11 +926,11 @@ void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st)
   hipLaunchKernelGGL(coef_to_f32_kernel, dim3(ne), dim3(64), 0, st, net, ne, out);
 }
 
-void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, double* val,
-                const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides, double* dft, double damping,
-                int blend, hipStream_t st, bool f32) {
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const FitItem* citems,
+                int ncitems, double* val, const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides,
+                double* dft, double damping, int blend, hipStream_t st, bool f32) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(unwrap_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, val, pt_off);
+  if (ncitems > 0) hipLaunchKernelGGL(unwrap_kernel, dim3(ncitems), dim3(64), 0, st, net, fv, citems, val, pt_off);
   if (ntiles > 0) {
     if (f32)
       hipLaunchKernelGGL(dft_kernel<true>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
@@ -1199,9 +1246,11 @@ void launch_scatter(const NetView& net, const int* de, int nde, int nf, const Sy
   hipLaunchKernelGGL(scatter_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, net, de, nde, nf, sys, y);
 }
 
-void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
-                   hipStream_t st) {
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, fv, ne, y, yold, n, out);
+void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, void* scratch,
+                   IterScalars* out, hipStream_t st) {
+  const int groups = std::max(1, std::min(kRedMaxGroups, (std::max(ne, n) + 1023) / 1024));
+  hipLaunchKernelGGL(reduce_kernel, dim3(groups), dim3(kRedThreads), 0, st, fv, ne, y, yold, n,
+                     (RedScratch*)scratch, out);
 }
 
 void launch_init_errors(const FitView& fv, int ne, hipStream_t st) {
@@ -1212,16 +1261,19 @@ void launch_init_errors(const FitView& fv, int ne, hipStream_t st) {
 // max over the head equations of |(A y)_j - b_j| in metres (constant-head/well rows are in
 // potential units and are divided by t of their fracture via row_scale, which is 0 on the
 // continuity and padding rows)
-__global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict__ ay, const double* __restrict__ b,
-                                                        const double* __restrict__ row_scale, int nrows,
-                                                        IterScalars* __restrict__ out) {
-  __shared__ double sr[1024];
-  __shared__ int si[1024];
-  const int tid = threadIdx.x;
+__global__ __launch_bounds__(kRedThreads) void residual_kernel(const double* __restrict__ ay,
+                                                               const double* __restrict__ b,
+                                                               const double* __restrict__ row_scale, int nrows,
+                                                               RedScratch* __restrict__ rs,
+                                                               IterScalars* __restrict__ out) {
+  __shared__ double sr[kRedThreads];
+  __shared__ int si[kRedThreads];
+  const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
   double r = 0.0;
   int ir = 0x7fffffff;
-  for (int j = tid; j < nrows; j += 1024) {
-    double v = fabs(ay[j] - b[j]) * row_scale[j];  // row_scale 0 outside the head equations
+  const int per = (nrows + G - 1) / G, j0 = g * per, j1 = min(nrows, j0 + per);
+  for (int j = j0 + tid; j < j1; j += kRedThreads) {
+    const double v = fabs(ay[j] - b[j]) * row_scale[j];  // row_scale 0 outside the head equations
     if (v != v || v > r) {
       r = v;
       ir = j;
@@ -1230,9 +1282,9 @@ __global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict
   sr[tid] = r;
   si[tid] = ir;
   __syncthreads();
-  for (int s = 512; s > 0; s >>= 1) {
+  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
     if (tid < s) {
-      double a = sr[tid], c = sr[tid + s];
+      const double a = sr[tid], c = sr[tid + s];
       if (a == a && (c != c || c > a || (c == a && si[tid + s] < si[tid]))) {
         sr[tid] = c;
         si[tid] = si[tid + s];
@@ -1240,16 +1292,33 @@ __global__ __launch_bounds__(1024) void residual_kernel(const double* __restrict
     }
     __syncthreads();
   }
+  if (tid == 0) rs->parts[g] = RedPart{sr[0], 0.0, 0.0, 0.0, si[0], 0};
+  if (!last_group(&rs->count_residual)) return;
   if (tid == 0) {
-    out->residual = sr[0];
-    out->arg_residual = si[0];
+    double m = rs->parts[0].e;
+    int im = rs->parts[0].ie;
+    for (int k = 1; k < G; ++k) {
+      const double c = rs->parts[k].e;
+      const int ic = rs->parts[k].ie;
+      if (m == m && (c != c || c > m || (c == m && ic < im))) {
+        m = c;
+        im = ic;
+      }
+    }
+    out->residual = m;
+    out->arg_residual = im;
+    rs->count_residual = 0;
   }
 }
 
-void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, IterScalars* out,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(residual_kernel, dim3(1), dim3(1024), 0, st, ay, b, row_scale, nde, out);
+void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, void* scratch,
+                     IterScalars* out, hipStream_t st) {
+  const int groups = std::max(1, std::min(kRedMaxGroups, (nde + 1023) / 1024));
+  hipLaunchKernelGGL(residual_kernel, dim3(groups), dim3(kRedThreads), 0, st, ay, b, row_scale, nde,
+                     (RedScratch*)scratch, out);
 }
+
+size_t reduce_scratch_bytes() { return sizeof(RedScratch); }
 
 // pointwise boundary-condition residual in metres: head continuity at every collocation point of
 // every intersection, prescribed head at every collocation point of every constant-head line and

@@ -139,7 +139,9 @@ void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 // unwrap (circle-type sides, in place in val), DFT sums of every side into dft (side s: X_j at
 // s 2M + j, Y_j at s 2M + M + j), then the element solves
-void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, double* val,
+// citems: the circle-type items among items (types 1, 4, 5), whose sides are unwrapped
+void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const FitItem* citems,
+                int ncitems, double* val,
                 const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides, double* dft, double damping,
                 int blend, hipStream_t st, bool f32 = false);
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
@@ -207,11 +209,14 @@ void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const
 void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st);
 void launch_scatter(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* y,
                     hipStream_t st);
-void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, IterScalars* out,
-                   hipStream_t st);
+// scratch: reduce_scratch_bytes() of device memory, zeroed once before the first launch (the
+// launches leave their counters at zero)
+size_t reduce_scratch_bytes();
+void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, void* scratch,
+                   IterScalars* out, hipStream_t st);
 void launch_init_errors(const FitView& fv, int ne, hipStream_t st);
-void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, IterScalars* out,
-                     hipStream_t st);
+void launch_residual(const double* ay, const double* b, const double* row_scale, int nde, void* scratch,
+                     IterScalars* out, hipStream_t st);
 void launch_bc_residual(const NetView& net, const FitItem* items, int nitems, const double* val, const int* pt_off,
                         const RhsItem* witems, int nw, const double* wval, const int* wpt_off, int K, double* out_max,
                         hipStream_t st);

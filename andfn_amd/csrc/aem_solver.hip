@@ -303,9 +303,19 @@ class Solver {
   void calu_factor(double* A, int64_t n, int nblk);
   DBuf<Slot> d_slots_;
   DBuf<FitItem> d_items_, d_items_g_;
+  DBuf<FitItem> d_citems_, d_citems_g_;  // the circle-type items (branch-cut unwrap)
+  int ncitems_ = 0, ncitems_g_ = 0;
+  int upload_circle_items(const std::vector<FitItem>& items, DBuf<FitItem>& d) {
+    std::vector<FitItem> c;
+    for (const FitItem& it : items)
+      if (is_circle_like(etype_[it.elem])) c.push_back(it);
+    d.upload(c, st_);
+    return (int)c.size();
+  }
   DBuf<RhsItem> d_rhs_items_;
   DBuf<int64_t> d_ipiv_, d_info_;  // 64-bit LAPACK interface: n^2 exceeds 2^31 from ~46k unknowns
   DBuf<aem::IterScalars> d_scal_;
+  DBuf<unsigned char> d_red_;  // scratch of the two-stage reductions (counters reset by the kernels)
 
   TargetSet full_, grown_, rhs_;
   std::vector<FitItem> items_, items_g_;
@@ -606,6 +616,7 @@ void Solver::upload() {
   d_status_.zero(1, st_);
   d_grow_.zero(E_, st_);
   d_scal_.alloc(1);
+  d_red_.zero(aem::reduce_scratch_bytes(), st_);
   for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
 }
 
@@ -1137,7 +1148,7 @@ double Solver::refine_probe() {
   double r[4] = {0, 0, 0, 0};
   for (int k = 0; k < 4; ++k) {
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, y.p, d_ay_.p, st_);
-    aem::launch_residual(d_ay_.p, b0.p, d_rowscale_.p, n, d_scal_.p, st_);
+    aem::launch_residual(d_ay_.p, b0.p, d_rowscale_.p, n, d_red_.p, d_scal_.p, st_);
     aem::IterScalars sc{};
     HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
@@ -1285,7 +1296,7 @@ void Solver::sweep(int mode, const TargetSet& ts, double* val, bool mixed) {
 double Solver::head_residual(const double* b, int* row) {
   const int n = (int)N_;
   aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
-  aem::launch_residual(d_ay_.p, b, d_rowscale_.p, n, d_scal_.p, st_);
+  aem::launch_residual(d_ay_.p, b, d_rowscale_.p, n, d_red_.p, d_scal_.p, st_);
   aem::IterScalars sc{};
   HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
@@ -1382,6 +1393,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   build_targets(full_, nullptr, false);
   build_items(full_, items_, nullptr);
   d_items_.upload(items_, st_);
+  ncitems_ = upload_circle_items(items_, d_citems_);
   build_targets(rhs_, nullptr, true);
   {
     std::vector<int> s0(E_, -1), s1(E_, -1);
@@ -1424,7 +1436,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipEventRecord(ev_[0], st_));
     sweep(aem::MODE_COLLOC, full_, d_val_.p, mixed_phase_);
     HIP_CHECK(hipEventRecord(ev_[1], st_));
-    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_val_.p, full_.d_pt_off.p, full_.d_dft_tiles.p,
+    aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_citems_.p, ncitems_, d_val_.p, full_.d_pt_off.p,
+                    full_.d_dft_tiles.p,
                     (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, blend_now ? 1 : 0, st_,
                     mixed_phase_);
     aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_);
@@ -1543,6 +1556,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         build_targets(grown_, &grown, false);
         build_items(grown_, items_g_, &grown);
         d_items_g_.upload(items_g_, st_);
+        ncitems_g_ = upload_circle_items(items_g_, d_citems_g_);
         d_valg_.alloc(std::max<int64_t>(grown_.npts(), 1));
         if (!gcirc.empty()) {
           d_grown_circles_.upload(gcirc, st_);
@@ -1554,7 +1568,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         HIP_CHECK(hipEventRecord(ev_[7], st_));
         grown_timed = true;
         ++n_sweep;
-        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_valg_.p, grown_.d_pt_off.p,
+        aem::launch_fit(nv, fv, d_items_g_.p, (int)items_g_.size(), d_citems_g_.p, ncitems_g_, d_valg_.p,
+                        grown_.d_pt_off.p,
                         grown_.d_dft_tiles.p, (int)grown_.dft_tiles.size(), grown_.d_dft_sides.p, d_dft_.p, damping, 0,
                         st_, mixed_phase_);
         int64_t pr = 0;
@@ -1566,7 +1581,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       }
       aem::launch_blend(nv, E_, damping, st_);
     }
-    aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_scal_.p, st_);
+    aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_red_.p, d_scal_.p, st_);
     HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
     HIP_CHECK(hipGetLastError());
