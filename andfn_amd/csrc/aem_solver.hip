@@ -1379,7 +1379,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     fail(ANDFN_MI_ECAPACITY, "MAX_NCOEF %d + COEF_INCREASE %d exceeds the coefficient capacity %lld", max_coef,
          coef_increase, (long long)L_.M);
   for (int e = 0; e < E_; ++e)
-    if (etype_[e] != 2 && is_circle_like(etype_[e]) && radius_[e] <= 0)
+    if ((etype_[e] == 1 || etype_[e] == 4) && radius_[e] <= 0)
       fail(ANDFN_MI_EINVAL, "element %d: circle radius must be positive", e);
 
   have_state_ = false;

@@ -91,6 +91,15 @@ def case_template_t():
     return g
 
 
+def case_template_imp():
+    """Template geometry with an impermeable circle (fracture 1) and an impermeable line
+    (fracture 2), placed clear of the intersections and inside the bounding circles."""
+    g = case_template()
+    g["imp_circles"] = [(1, 0.15, -0.45 + 0.3j)]
+    g["imp_lines"] = [(2, [-0.1 - 0.3j, 0.2 - 0.2j])]
+    return g
+
+
 def case_synthetic(name):
     from andfn_amd.generator import build_network, workload
 
@@ -116,6 +125,7 @@ def case_synthetic(name):
 CASES = {
     "template": case_template,
     "template_t": case_template_t,
+    "template_imp": case_template_imp,
     "dfn40": lambda: case_synthetic("dfn40"),
     "dfn40w": lambda: case_synthetic("dfn40w"),
     "dfn200": lambda: case_synthetic("dfn200"),
@@ -131,6 +141,10 @@ def build_reference(andfn, g):
     ref.add_fracture(fr)
     for fi, ends, head in g["chl"]:
         andfn.ConstantHeadLine(f"chl{fi}", np.array(ends, dtype=complex), head, fr[fi], g["ncoef"], g["nint"])
+    for fi, r, c in g.get("imp_circles", []):
+        andfn.ImpermeableCircle(f"ic{fi}", r, c, fr[fi], g["ncoef"], g["nint"])
+    for fi, ends in g.get("imp_lines", []):
+        andfn.ImpermeableLine(f"il{fi}", np.array(ends, dtype=complex), fr[fi], g["ncoef"], g["nint"])
     ref.get_fracture_intersections(g["ncoef"], g["nint"])
     for c, n, r, h, lab in g["planes"]:
         ref.set_constant_head_boundary(np.array(c), np.array(n), r, h, label=lab, ncoef=g["ncoef"], nint=g["nint"])
