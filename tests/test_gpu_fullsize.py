@@ -1,0 +1,111 @@
+"""The cfg2 workload at full size (5,000 disks generated, 4,656 kept, 19,661 unknowns), checked
+through properties that do not depend on its size.
+
+The oracle's full solve of this network takes over a minute and a half (SuperLU alone ~6 s, 160
+sweeps of ~0.5 s), so the GPU's converged state is checked instead:
+
+* the head residual, recomputed by the oracle from the GPU's coefficients, discharges and
+  fracture constants (its own discharge matrix and right-hand-side sweep), is within the stop
+  tolerance and agrees with the residual the GPU reported;
+* the continuity rows of the discharge system (every fracture's discharges sum to zero) hold to
+  rounding;
+* two FP64 runs give bit-identical results (every reduction has a fixed order);
+* the mixed form stops at the same head residual within the errors stated for it.
+"""
+
+import numpy as np
+import pytest
+
+from andfn_amd.generator import build_network, workload
+from andfn_amd.hpc_solve import DeviceNetwork
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+HEAD_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    print("cfg2: building the network", flush=True)
+    d = build_network(workload("cfg2"))
+    d.consolidate_dfn()
+    return d
+
+
+def _run(d, precision):
+    fa = d.fractures_struc_array_hpc.copy()
+    ea = d.elements_struc_array_hpc.copy()
+    print(f"cfg2: {precision} solve", flush=True)
+    net = DeviceNetwork(fa, ea, d.discharge_int, d.constants)
+    info = net.run(precision=precision, head_tol=HEAD_TOL)
+    net.download()
+    net.close()
+    return fa, ea, info
+
+
+@pytest.fixture(scope="module")
+def fp64(cfg2):
+    return _run(cfg2, "fp64")
+
+
+def _oracle_rows(d, fa, ea):
+    """A y - b of the oracle's discharge system at the state held by the records, with the head
+    rows scaled to metres (hpc_solve.py:832-887 for A, 155-174 for b)."""
+    O = oracle.OracleHandle(fa.copy(), ea.copy(), d.discharge_int)
+    y = np.zeros(O.n)
+    O.lib.oracle_old_discharges(O.h, oracle._ptr(y))
+    A = O.discharge_matrix()
+    b = O.head_rhs()
+    nde = O.nde
+    O.close()
+    r = A @ y - b
+    de = np.nonzero(np.isin(ea["_type"], (0, 2, 3)))[0]
+    scale = np.ones(nde)
+    chl = ea["_type"][de] != 0
+    scale[chl] = 1.0 / fa["t"][ea["frac0"][de[chl]]]
+    return np.abs(r[:nde]) * scale, r[nde:], y
+
+
+def test_head_residual_recomputed_by_the_oracle(cfg2, fp64):
+    fa, ea, info = fp64
+    assert info["converged"] and info["head_residual"] <= HEAD_TOL
+    head, cont, y = _oracle_rows(cfg2, fa, ea)
+    print(f"cfg2 fp64: GPU head residual {info['head_residual']:.6e}, oracle {head.max():.6e}")
+    assert head.max() <= HEAD_TOL * (1 + 1e-6)
+    assert abs(head.max() - info["head_residual"]) <= 1e-3 * HEAD_TOL
+
+
+def test_fracture_discharges_balance(cfg2, fp64):
+    fa, ea, _ = fp64
+    _, cont, y = _oracle_rows(cfg2, fa, ea)
+    qmax = np.max(np.abs(ea["q"]))
+    print(f"cfg2 fp64: continuity rows max {np.max(np.abs(cont)):.3e} (max |q| {qmax:.3e})")
+    assert np.max(np.abs(cont)) <= 1e-10 * qmax
+
+
+def test_fp64_runs_are_bit_identical(cfg2, fp64):
+    fa, ea, info = fp64
+    fa2, ea2, info2 = _run(cfg2, "fp64")
+    assert info2["nit"] == info["nit"]
+    assert np.array_equal(ea2["ncoef"], ea["ncoef"])
+    assert np.array_equal(ea2["coef"], ea["coef"])
+    assert np.array_equal(ea2["q"], ea["q"])
+    assert np.array_equal(fa2["constant"], fa["constant"])
+
+
+def test_mixed_form_against_fp64(cfg2, fp64):
+    fa, ea, info = fp64
+    fam, eam, infom = _run(cfg2, "mixed")
+    assert infom["converged"] and infom["head_residual"] <= HEAD_TOL
+    assert infom["nit_fp32"] > 0
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    errs = dict(coef=rel(eam["coef"], ea["coef"]), q=rel(eam["q"], ea["q"]),
+                constant=rel(fam["constant"], fa["constant"]))
+    print(f"cfg2 mixed vs fp64: {errs}")
+    assert errs["coef"] < 1e-7 and errs["q"] < 1e-7 and errs["constant"] < 1e-9
+    head, _, _ = _oracle_rows(cfg2, fam, eam)
+    assert head.max() <= HEAD_TOL * (1 + 1e-6)
