@@ -720,14 +720,14 @@ __global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg
       acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log_pos(r2);
     }
     acc = wave_sum(acc);
-    if (lane == 0) atomicAdd(&A[mat_at(row, sys.de_pos[src], sys.npad)], rs * acc / (double)K);
+    if (lane == 0) atomicAdd(&A[mat_at(row, sys.de_pos[src], sys)], rs * acc / (double)K);
   }
   if (lane == 0) {
     if (inter) {
-      if (which == 0) atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys.npad)], 1.0 / net.t[net.efrac0[e]]);
-      else atomicAdd(&A[mat_at(row, sys.fpos[net.efrac1[e]], sys.npad)], -1.0 / net.t[net.efrac1[e]]);
+      if (which == 0) atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys)], 1.0 / net.t[net.efrac0[e]]);
+      else atomicAdd(&A[mat_at(row, sys.fpos[net.efrac1[e]], sys)], -1.0 / net.t[net.efrac1[e]]);
     } else {
-      atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys.npad)], 1.0);
+      atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys)], 1.0);
     }
   }
 }
@@ -742,7 +742,7 @@ __global__ void continuity_kernel(NetView net, SysLayout sys, int nf, double* __
     const int ty = net.etype[e];
     if (!(ty == 0 || ty == 2 || ty == 3)) continue;
     const double v = (ty != 0 || net.efrac0[e] == f) ? 1.0 : -1.0;
-    A[mat_at(row, sys.de_pos[e], sys.npad)] += v;
+    A[mat_at(row, sys.de_pos[e], sys)] += v;
   }
 }
 
@@ -1016,17 +1016,17 @@ void launch_scale(const double* b, const double* rs, int n, double* x, hipStream
 }
 
 // CSR copy of the assembled (unscaled, unfactored) dense matrix: val[k] = A[row, col[k]]
-__global__ void csr_gather_kernel(const double* __restrict__ A, int64_t npad, int64_t nrows,
+__global__ void csr_gather_kernel(const double* __restrict__ A, SysLayout sys, int64_t nrows,
                                   const int* __restrict__ row_ptr, const int* __restrict__ col,
                                   double* __restrict__ val) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
-  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) val[k] = A[mat_at(r, col[k], npad)];
+  for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) val[k] = A[mat_at(r, col[k], sys)];
 }
 
-void launch_csr_gather(const double* A, int64_t npad, int64_t nrows, const int* row_ptr, const int* col, double* val,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, A, npad, nrows,
+void launch_csr_gather(const double* A, const SysLayout& sys, int64_t nrows, const int* row_ptr, const int* col,
+                       double* val, hipStream_t st) {
+  hipLaunchKernelGGL(csr_gather_kernel, dim3((unsigned)((nrows + 255) / 256)), dim3(256), 0, st, A, sys, nrows,
                      row_ptr, col, val);
 }
 

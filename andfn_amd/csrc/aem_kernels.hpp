@@ -76,17 +76,27 @@ struct FitView {
 // Unknowns of the discharge system: nblk independent blocks (connected components of the
 // network) of npad unknowns each, real unknowns first and identity padding after.  Unknown
 // g = b npad + local; block b's matrix is the column-major npad x npad slab at b npad^2.
+// Discharge system of one or more independent parts (connected components).  Each part is a
+// dense column-major block padded to its size class: the parts are grouped into at most
+// kMaxSysClasses classes of similar size, class k holding blocks of npad[k] unknowns stored one
+// after the other from matrix entry a0[k] and unknown u0[k] (u0[ncls] = total unknowns).
+constexpr int kMaxSysClasses = 8;
 struct SysLayout {
-  int64_t npad;
-  int nblk;
+  int ncls;
+  int64_t u0[kMaxSysClasses + 1];
+  int64_t a0[kMaxSysClasses];
+  int64_t npad[kMaxSysClasses];
   const int* de_pos;  // [E] unknown of each discharge element (-1 for other elements)
   const int* fpos;    // [F] unknown of each fracture constant
 };
 
-// position of (row r, column c) of the same block in the block-slab storage
-__host__ __device__ __forceinline__ int64_t mat_at(int64_t r, int64_t c, int64_t npad) {
-  const int64_t b = r / npad;
-  return b * npad * npad + (c - b * npad) * npad + (r - b * npad);
+// position of (row r, column c) of the same block in the block storage
+__host__ __device__ __forceinline__ int64_t mat_at(int64_t r, int64_t c, const SysLayout& s) {
+  int k = 0;
+  while (k + 1 < s.ncls && r >= s.u0[k + 1]) ++k;
+  const int64_t np = s.npad[k];
+  const int64_t base = s.u0[k] + (r - s.u0[k]) / np * np;
+  return s.a0[k] + (base - s.u0[k]) * np + (c - base) * np + (r - base);
 }
 
 struct TargetView {
@@ -148,8 +158,8 @@ void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
 void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
                  hipStream_t st);
-void launch_csr_gather(const double* A, int64_t npad, int64_t nrows, const int* row_ptr, const int* col, double* val,
-                       hipStream_t st);
+void launch_csr_gather(const double* A, const SysLayout& sys, int64_t nrows, const int* row_ptr, const int* col,
+                       double* val, hipStream_t st);
 void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, const double* x, double* y,
                  hipStream_t st);
 // batch > 1: the same product on `batch` slabs, A + z sA, x + z sx, y + z sx

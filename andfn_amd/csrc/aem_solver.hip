@@ -255,27 +255,47 @@ class Solver {
   DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
   DBuf<int> d_csr_ptr_, d_csr_col_;
   int64_t nnz_A_ = 0;
-  // discharge-system layout: one dense block per connected component of the network, padded to
-  // npad_ unknowns (discharge elements in element order, then fracture constants, then padding)
-  int64_t npad_ = 0, N_ = 0;
-  int nblk_ = 1;
+  // discharge-system layout: one dense block per connected component of the network (discharge
+  // elements in element order, then fracture constants, then padding), the blocks grouped into
+  // size classes of a common padded size (a class's smallest part has at least kClassRatio of
+  // its largest part's unknowns), largest class first
+  static constexpr double kClassRatio = 0.7;
+  struct SysClass {
+    int nblk, b0;      // blocks of the class and the global index of its first block
+    int64_t npad;      // unknowns per block
+    int64_t u0, a0;    // first unknown and first matrix entry of the class
+    int64_t inv0;      // first entry of the class's diagonal-block inverses
+  };
+  std::vector<SysClass> cls_;
+  int64_t N_ = 0, Atot_ = 0, invtot_ = 0;
+  int nblk_ = 1;  // blocks over all classes
   std::vector<int> fpos_, blk_n_;
   DBuf<int> d_fpos_, d_blk_n_;
   aem::SysLayout sys() const {
-    aem::SysLayout l;
-    l.npad = npad_;
-    l.nblk = nblk_;
+    aem::SysLayout l{};
+    l.ncls = (int)cls_.size();
+    for (int k = 0; k < l.ncls; ++k) {
+      l.u0[k] = cls_[k].u0;
+      l.a0[k] = cls_[k].a0;
+      l.npad[k] = cls_[k].npad;
+    }
+    l.u0[l.ncls] = N_;
     l.de_pos = d_de_pos_.p;
     l.fpos = d_fpos_.p;
     return l;
   }
   // inverses of the diagonal blocks of L (unit lower) and U, for the blocked triangular solves
   static constexpr int kNB = 1024;
+  static int64_t inv_entries(int64_t npad) { return (npad + kNB - 1) / kNB * kNB * kNB; }
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   template <class T>
-  void prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
+  void prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit);
   template <class T>
-  void blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v);
+  void prepare_blocked_all(T* A, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
+  template <class T>
+  void blocked_solve_all(const T* A, const T* linv, const T* uinv, T* x, T* w, T* v);
+  template <class T>
+  void blocked_solve(const T* A, int n, int nblk, const T* linv, const T* uinv, T* x, T* w, T* v);
   // mixed precision: single-precision factors and blocked-solve buffers
   DBuf<float> d_A32_, d_linv32_, d_uinv32_, d_x32_, d_w32_, d_v32_;
   DBuf<float2> d_coef32_;
@@ -300,7 +320,8 @@ class Solver {
   // L11\U11 block and its row interchange
   DBuf<int> lu_cand_[2], lu_cnt_[2], lu_pairs_;
   DBuf<double> lu11_;
-  void calu_factor(double* A, int64_t n, int nblk);
+  void calu_reserve(int64_t n, int nblk);
+  void calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* info);
   DBuf<Slot> d_slots_;
   DBuf<FitItem> d_items_, d_items_g_;
   DBuf<FitItem> d_citems_, d_citems_g_;  // the circle-type items (branch-cut unwrap)
@@ -553,14 +574,40 @@ void Solver::parse() {
     std::vector<int> cnt(nblk_, 0), local_e(E_, -1), local_f(F_);
     for (int e : de_) local_e[e] = cnt[bid[efrac0_[e]]]++;
     for (int f = 0; f < F_; ++f) local_f[f] = cnt[bid[f]]++;
-    npad_ = 0;
-    for (int b = 0; b < nblk_; ++b) npad_ = std::max<int64_t>(npad_, cnt[b]);
-    blk_n_ = cnt;
-    N_ = npad_ * nblk_;
+    // blocks by size, largest first (ties in part order); a new class starts below kClassRatio
+    // of the current class's size while classes remain
+    std::vector<int> order(nblk_);
+    for (int b = 0; b < nblk_; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+    std::vector<int> slot(nblk_);  // block index in class order
+    cls_.clear();
+    blk_n_.assign(nblk_, 0);
+    for (int i = 0; i < nblk_; ++i) {
+      const int b = order[i];
+      if (cls_.empty() || ((double)cnt[b] < kClassRatio * (double)cls_.back().npad &&
+                           (int)cls_.size() < aem::kMaxSysClasses))
+        cls_.push_back(SysClass{0, i, std::max<int64_t>(cnt[b], 1), 0, 0, 0});
+      cls_.back().nblk++;
+      slot[b] = i;
+      blk_n_[i] = cnt[b];
+    }
+    N_ = Atot_ = invtot_ = 0;
+    for (auto& c : cls_) {
+      c.u0 = N_;
+      c.a0 = Atot_;
+      c.inv0 = invtot_;
+      N_ += c.nblk * c.npad;
+      Atot_ += c.nblk * c.npad * c.npad;
+      invtot_ += c.nblk * inv_entries(c.npad);
+    }
     if (N_ > INT32_MAX) fail(ANDFN_MI_ECAPACITY, "discharge system of %lld padded unknowns", (long long)N_);
+    // first unknown of every block
+    std::vector<int64_t> base(nblk_);
+    for (const auto& c : cls_)
+      for (int k = 0; k < c.nblk; ++k) base[c.b0 + k] = c.u0 + k * c.npad;
     fpos_.assign(F_, 0);
-    for (int e : de_) de_pos_[e] = (int)(bid[efrac0_[e]] * npad_ + local_e[e]);
-    for (int f = 0; f < F_; ++f) fpos_[f] = (int)(bid[f] * npad_ + local_f[f]);
+    for (int e : de_) de_pos_[e] = (int)(base[slot[bid[efrac0_[e]]]] + local_e[e]);
+    for (int f = 0; f < F_; ++f) fpos_[f] = (int)(base[slot[bid[f]]] + local_f[f]);
   }
   cross_ptr_.assign(E_, 0);
   cross_cap_.assign(E_, 0);
@@ -832,43 +879,43 @@ static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, ro
 // a single batched launch.
 // lower_unit: L has the unit diagonal (LU factors); false for the transposed factors U^T L^T,
 // whose lower triangle carries the diagonal and whose upper triangle is unit.
+// For the nblk blocks of one size class (n unknowns each) at A; linv / uinv hold the class's
+// inverse blocks (zeroed by the caller).
 template <class T>
-void Solver::prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
+void Solver::prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit) {
   const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
   const rocblas_diagonal dl = lower_unit ? rocblas_diagonal_unit : rocblas_diagonal_non_unit;
   const rocblas_diagonal du = lower_unit ? rocblas_diagonal_non_unit : rocblas_diagonal_unit;
   const int64_t sinv = nb * kNB * kNB;  // inverse blocks of one system
-  linv.zero((size_t)(nblk * sinv), st_);
-  uinv.zero((size_t)(nblk * sinv), st_);
   if (nblk > 1) {
     // several systems of the same (padded) size: per diagonal block j, one call covers them all
     const int64_t sA = n * n;
     for (int64_t j = 0; j < nb; ++j) {
       const int64_t r0 = j * kNB;
       const int bs = (int)std::min<int64_t>(kNB, n - r0);
-      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, bs, A + r0 * (n + 1), n, sA, linv.p + j * kNB * kNB, kNB,
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, bs, A + r0 * (n + 1), n, sA, linv + j * kNB * kNB, kNB,
                                 sinv, nblk));
-      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, bs, A + r0 * (n + 1), n, sA, uinv.p + j * kNB * kNB, kNB,
+      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, bs, A + r0 * (n + 1), n, sA, uinv + j * kNB * kNB, kNB,
                                 sinv, nblk));
     }
-    aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)(nb * nblk), kNB, st_);
+    aem::launch_unit_diag<T>(lower_unit ? linv : uinv, (int)(nb * nblk), kNB, st_);
     for (int64_t j = 0; j < nb; ++j) {
       const int64_t r0 = j * kNB;
       const int bs = (int)std::min<int64_t>(kNB, n - r0);
       const int rest = (int)(n - r0 - bs);
       if (rest > 0)
-        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + j * kNB * kNB, kNB, sinv,
+        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, dl, rest, bs, linv + j * kNB * kNB, kNB, sinv,
                                        A + r0 * n + r0 + bs, n, sA, nblk));
       if (r0 > 0)
-        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + j * kNB * kNB, kNB, sinv,
+        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv + j * kNB * kNB, kNB, sinv,
                                        A + r0 * n, n, sA, nblk));
     }
     return;
   }
   for (int b = 0; b < nblk; ++b) {
     T* Ab = A + (int64_t)b * n * n;
-    T* li = linv.p + b * sinv;
-    T* ui = uinv.p + b * sinv;
+    T* li = linv + b * sinv;
+    T* ui = uinv + b * sinv;
     if (nfull > 0) {
       RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, Ab, n, kNB * (n + 1), li, kNB, (int64_t)kNB * kNB,
                                 (int)nfull));
@@ -884,7 +931,7 @@ void Solver::prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& 
     }
   }
   // a unit-triangular inverse is written without its diagonal
-  aem::launch_unit_diag<T>(lower_unit ? linv.p : uinv.p, (int)(nb * nblk), kNB, st_);
+  aem::launch_unit_diag<T>(lower_unit ? linv : uinv, (int)(nb * nblk), kNB, st_);
   for (int b = 0; b < nblk; ++b) {
     T* Ab = A + (int64_t)b * n * n;
     for (int64_t j = 0; j < nb; ++j) {
@@ -892,27 +939,36 @@ void Solver::prepare_blocked(T* A, int64_t n, int nblk, DBuf<T>& linv, DBuf<T>& 
       const int bs = (int)std::min<int64_t>(kNB, n - r0);
       const int rest = (int)(n - r0 - bs);
       if (rest > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv.p + b * sinv + j * kNB * kNB, kNB,
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv + b * sinv + j * kNB * kNB, kNB,
                                Ab + r0 * n + r0 + bs, n));
       if (r0 > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv.p + b * sinv + j * kNB * kNB, kNB,
+        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv + b * sinv + j * kNB * kNB, kNB,
                                Ab + r0 * n, n));
     }
   }
 }
 
+// prepare_blocked for every size class; linv / uinv sized and zeroed here
+template <class T>
+void Solver::prepare_blocked_all(T* A, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
+  linv.zero((size_t)invtot_, st_);
+  uinv.zero((size_t)invtot_, st_);
+  for (const auto& c : cls_)
+    prepare_blocked<T>(A + c.a0, c.npad, c.nblk, linv.p + c.inv0, uinv.p + c.inv0, lower_unit);
+}
+
 void Solver::assemble_and_factor() {
   drop_graph();
   adjT_valid_ = false;
-  const int64_t npad = npad_, N = N_;
+  const int64_t N = N_;
   const int nblk = nblk_;
   if (nblk > 1 && lu_mode_ != ANDFN_MI_LU_TRSV)
     fail(ANDFN_MI_EINVAL, "a network of %d independent parts is solved with the blocked triangular solves (lu_mode 0)",
          nblk);
-  d_A_.zero((size_t)(N * npad), st_);
+  d_A_.zero((size_t)Atot_, st_);
   aem::TargetView tv = rhs_.view();
   aem::launch_assemble(net(), tv, (int)rhs_.side_elem.size(), sys(), K_, F_, d_A_.p, st_);
-  aem::launch_pad_identity(d_A_.p, npad, nblk, d_blk_n_.p, st_);
+  for (const auto& c : cls_) aem::launch_pad_identity(d_A_.p + c.a0, c.npad, c.nblk, d_blk_n_.p + c.b0, st_);
   HIP_CHECK(hipGetLastError());
   // a CSR copy of the unfactored matrix (rows in unknown order) for the head residual A y - b;
   // row_scale: 1 on intersection rows, 1/t on constant-head and well rows, 0 elsewhere
@@ -949,7 +1005,7 @@ void Solver::assemble_and_factor() {
     d_csr_ptr_.upload(rp, st_);
     d_csr_col_.upload(cols, st_);
     d_csr_val_.alloc(cols.size());
-    aem::launch_csr_gather(d_A_.p, npad, N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+    aem::launch_csr_gather(d_A_.p, sys(), N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
     d_rowscale_.upload(rs, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
     nnz_A_ = (int64_t)cols.size();
@@ -960,20 +1016,31 @@ void Solver::assemble_and_factor() {
   {
     DBuf<unsigned long long> mx;
     mx.alloc(N);
-    aem::launch_row_scale(d_A_.p, npad, nblk, d_req_.p, mx.p, st_);
+    for (const auto& c : cls_) aem::launch_row_scale(d_A_.p + c.a0, c.npad, c.nblk, d_req_.p + c.u0, mx.p + c.u0, st_);
     HIP_CHECK(hipGetLastError());
     HIP_CHECK(hipStreamSynchronize(st_));
   }
   d_ipiv_.alloc(N);
   d_info_.zero(nblk, st_);
   const bool tournament = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_TOURNAMENT;
-  if (tournament)
-    calu_factor(d_A_.p, npad, nblk);
-  else if (nblk == 1)
-    RB_CHECK(rocsolver_dgetrf_64(rb_, npad, npad, d_A_.p, npad, d_ipiv_.p, d_info_.p));
-  else
-    RB_CHECK(rocsolver_dgetrf_strided_batched_64(rb_, npad, npad, d_A_.p, npad, (rocblas_stride)(npad * npad),
-                                                 d_ipiv_.p, (rocblas_stride)npad, d_info_.p, nblk));
+  if (tournament) {
+    std::vector<int> id((size_t)N);
+    for (int64_t i = 0; i < N; ++i) id[i] = (int)i;
+    d_perm_.upload(id, st_);
+    // work buffers sized for the largest class first: calu_factor's allocations are then no-ops
+    // (no buffer is released while an earlier class's launches may still use it)
+    for (const auto& c : cls_) calu_reserve(c.npad, c.nblk);
+    for (const auto& c : cls_) calu_factor(d_A_.p + c.a0, c.npad, c.nblk, d_perm_.p + c.u0, d_info_.p + c.b0);
+  } else {
+    for (const auto& c : cls_) {
+      if (c.nblk == 1)
+        RB_CHECK(rocsolver_dgetrf_64(rb_, c.npad, c.npad, d_A_.p + c.a0, c.npad, d_ipiv_.p + c.u0, d_info_.p + c.b0));
+      else
+        RB_CHECK(rocsolver_dgetrf_strided_batched_64(rb_, c.npad, c.npad, d_A_.p + c.a0, c.npad,
+                                                     (rocblas_stride)(c.npad * c.npad), d_ipiv_.p + c.u0,
+                                                     (rocblas_stride)c.npad, d_info_.p + c.b0, c.nblk));
+    }
+  }
   {
     std::vector<int64_t> info(nblk, 0);
     HIP_CHECK(hipMemcpyAsync(info.data(), d_info_.p, sizeof(int64_t) * nblk, hipMemcpyDeviceToHost, st_));
@@ -990,8 +1057,8 @@ void Solver::assemble_and_factor() {
     // of the CSR copy drives the refinement, so the iteration's fixed point is the FP64 one.
     // (rocSOLVER's sgetrf_64 in this ROCm reports spurious zero pivots at n = 65,690, on random
     // matrices too, and is no faster than dgetrf below that size, so the factorisation is FP64.)
-    d_A32_.alloc((size_t)(N * npad));
-    aem::launch_to_f32(d_A_.p, N * npad, d_A32_.p, st_);
+    d_A32_.alloc((size_t)Atot_);
+    aem::launch_to_f32(d_A_.p, Atot_, d_A32_.p, st_);
     solve32_ = true;
   }
   d_x_.alloc(N);
@@ -1002,17 +1069,18 @@ void Solver::assemble_and_factor() {
     HIP_CHECK(hipMemcpyAsync(ipiv.data(), d_ipiv_.p, sizeof(int64_t) * N, hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
     std::vector<int> perm(N);
-    for (int b = 0; b < nblk; ++b) {
-      const int64_t o = (int64_t)b * npad;
-      for (int64_t i = 0; i < npad; ++i) perm[o + i] = (int)(o + i);
-      for (int64_t i = 0; i < npad; ++i) std::swap(perm[o + i], perm[o + ipiv[o + i] - 1]);
-    }
+    for (const auto& c : cls_)
+      for (int b = 0; b < c.nblk; ++b) {
+        const int64_t o = c.u0 + (int64_t)b * c.npad;
+        for (int64_t i = 0; i < c.npad; ++i) perm[o + i] = (int)(o + i);
+        for (int64_t i = 0; i < c.npad; ++i) std::swap(perm[o + i], perm[o + ipiv[o + i] - 1]);
+      }
     d_perm_.upload(perm, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
   }
   if (lu_mode_ == ANDFN_MI_LU_TRSV) {
     if (precision_ == ANDFN_MI_MIXED) {
-      prepare_blocked<float>(d_A32_.p, npad, nblk, d_linv32_, d_uinv32_);
+      prepare_blocked_all<float>(d_A32_.p, d_linv32_, d_uinv32_);
       d_x32_.alloc(N);
       d_w32_.alloc(N);
       d_v32_.alloc(N);
@@ -1021,7 +1089,7 @@ void Solver::assemble_and_factor() {
       refine_rate_ = refine_probe();
       if (!(refine_rate_ <= 0.25)) switch_to_fp64_solves();
     } else {
-      prepare_blocked<double>(d_A_.p, npad, nblk, d_linv_, d_uinv_);
+      prepare_blocked_all<double>(d_A_.p, d_linv_, d_uinv_);
       a64_blocked_ = true;
     }
     d_w_.alloc(N);
@@ -1029,7 +1097,7 @@ void Solver::assemble_and_factor() {
     HIP_CHECK(hipStreamSynchronize(st_));
   } else if (lu_mode_ == ANDFN_MI_LU_INVERSE) {
     // getri has only the 32-bit interface here (this mode is for comparisons at moderate n)
-    const int64_t n = npad;
+    const int64_t n = N;  // one block (checked above)
     std::vector<int64_t> p64(n);
     HIP_CHECK(hipMemcpyAsync(p64.data(), d_ipiv_.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
@@ -1049,24 +1117,25 @@ void Solver::assemble_and_factor() {
 // and the update of the outer panel's remaining columns.  Then the outer panel's interchanges on
 // all other columns, its U12 and the trailing update: the next outer panel's columns on st_, the
 // rest on st2_, overlapping the next outer panel's factorisation (look-ahead).  Leaves the factors in LAPACK layout (unit
-// L below the diagonal, U on and above), the row permutation in d_perm_ ((P b)[i] = b[perm[i]])
-// and the first zero pivot (1-based, 0 if none) in d_info_[0].
-void Solver::calu_factor(double* A, int64_t n, int nblk) {
+// L below the diagonal, U on and above), the row permutation in perm ((P b)[i] = b[perm[i]]; it
+// holds the identity on entry, n entries per block) and the first zero pivot (1-based, 0 if none)
+// in info[b] for every block b.
+// work buffers of calu_factor for nblk blocks of n unknowns (only ever grown)
+void Solver::calu_reserve(int64_t n, int nblk) {
   const int64_t groups = (n + aem::kLuGroupRows - 1) / aem::kLuGroupRows;
-  const int nslots = aem::kLuOuter / aem::kLuPanel;
-  const int64_t sA = n * n, sp = (int64_t)nslots * aem::kLuPairStride;
+  const int64_t sp = (int64_t)(aem::kLuOuter / aem::kLuPanel) * aem::kLuPairStride;
   for (int i = 0; i < 2; ++i) {
     lu_cand_[i].alloc((size_t)(nblk * groups * aem::kLuPanel));
     lu_cnt_[i].alloc((size_t)(nblk * groups));
   }
   lu11_.alloc((size_t)nblk * aem::kLuPanel * aem::kLuPanel);
   lu_pairs_.alloc((size_t)(nblk * sp));
-  {
-    std::vector<int> id((size_t)(n * nblk));
-    for (int64_t i = 0; i < n * nblk; ++i) id[i] = (int)i;
-    d_perm_.upload(id, st_);
-    HIP_CHECK(hipStreamSynchronize(st_));
-  }
+}
+
+void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* info) {
+  const int nslots = aem::kLuOuter / aem::kLuPanel;
+  const int64_t sA = n * n, sp = (int64_t)nslots * aem::kLuPairStride;
+  calu_reserve(n, nblk);
   if (!st2_) {
     HIP_CHECK(hipStreamCreate(&st2_));
     RB_CHECK(rocblas_create_handle(&rb2_));
@@ -1096,7 +1165,7 @@ void Solver::calu_factor(double* A, int64_t n, int nblk) {
       const int bw = (int)std::min<int64_t>(aem::kLuPanel, J1 - j0);
       int* pairs = lu_pairs_.p + (int64_t)nl * aem::kLuPairStride;
       const int* win =
-          aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, d_perm_.p, d_info_.p, st_, nblk, sA, sp);
+          aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, perm, info, st_, nblk, sA, sp);
       aem::launch_calu_apply(A, n, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_, nblk, sA, sp);
       aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_, nblk, sA);
       const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
@@ -1155,7 +1224,7 @@ double Solver::refine_probe() {
     r[k] = sc.residual;
     if (k == 3) break;
     aem::launch_refine_rhs(b0.p, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    blocked_solve<float>(d_A32_.p, (int)npad_, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+    blocked_solve_all<float>(d_A32_.p, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
     aem::launch_refine_update(y.p, d_v32_.p, n, x.p, st_);
     HIP_CHECK(hipMemcpyAsync(y.p, x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   }
@@ -1169,7 +1238,7 @@ double Solver::refine_probe() {
 void Solver::switch_to_fp64_solves() {
   if (!solve32_) return;
   if (!a64_blocked_) {
-    prepare_blocked<double>(d_A_.p, npad_, nblk_, d_linv_, d_uinv_);
+    prepare_blocked_all<double>(d_A_.p, d_linv_, d_uinv_);
     d_w_.alloc(N_);
     d_v_.alloc(N_);
     a64_blocked_ = true;
@@ -1216,24 +1285,31 @@ void Solver::solve_rhs() {
   }
 }
 
-// v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten), for all nblk_
-// blocks of n unknowns at once (every launch covers the same step of every block)
+// v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten), for the nblk
+// blocks of n unknowns of one size class at once (every launch covers the same step of every block)
 template <class T>
-void Solver::blocked_solve(const T* A, int n, const T* linv, const T* uinv, T* x, T* w, T* v) {
+void Solver::blocked_solve(const T* A, int n, int nblk, const T* linv, const T* uinv, T* x, T* w, T* v) {
   const int nb = (n + kNB - 1) / kNB;
   const int64_t sA = (int64_t)n * n, sinv = (int64_t)nb * kNB * kNB;
   for (int j = 0; j < nb; ++j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0), rest = n - r0 - bs;
     if (rest > 0)
-      aem::launch_gemv_cm<T>(A + (size_t)r0 * n + r0 + bs, n, rest, bs, x + r0, x + r0 + bs, T(-1), T(1), st_, nblk_,
+      aem::launch_gemv_cm<T>(A + (size_t)r0 * n + r0 + bs, n, rest, bs, x + r0, x + r0 + bs, T(-1), T(1), st_, nblk,
                              sA, n);
   }
-  aem::launch_gemv_diag<T>(linv, kNB, n, 1, x, w, st_, nblk_, sinv);
+  aem::launch_gemv_diag<T>(linv, kNB, n, 1, x, w, st_, nblk, sinv);
   for (int j = nb - 1; j > 0; --j) {
     const int r0 = j * kNB, bs = std::min(kNB, n - r0);
-    aem::launch_gemv_cm<T>(A + (size_t)r0 * n, n, r0, bs, w + r0, w, T(-1), T(1), st_, nblk_, sA, n);
+    aem::launch_gemv_cm<T>(A + (size_t)r0 * n, n, r0, bs, w + r0, w, T(-1), T(1), st_, nblk, sA, n);
   }
-  aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_, nblk_, sinv);
+  aem::launch_gemv_diag<T>(uinv, kNB, n, 2, w, v, st_, nblk, sinv);
+}
+
+// blocked_solve for every size class (the class's slices of A, the inverses and the vectors)
+template <class T>
+void Solver::blocked_solve_all(const T* A, const T* linv, const T* uinv, T* x, T* w, T* v) {
+  for (const auto& c : cls_)
+    blocked_solve<T>(A + c.a0, (int)c.npad, c.nblk, linv + c.inv0, uinv + c.inv0, x + c.u0, w + c.u0, v + c.u0);
 }
 
 // x = A^-1 b with the factors (or inverse) of the discharge matrix; returns the buffer holding x
@@ -1254,7 +1330,7 @@ double* Solver::discharge_solve(double* b, double* x) {
     // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
     aem::launch_refine_rhs(b, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    blocked_solve<float>(d_A32_.p, (int)npad_, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+    blocked_solve_all<float>(d_A32_.p, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
     float* v32 = d_v32_.p;
     aem::launch_refine_update(d_y_.p, v32, n, x, st_);
     return x;
@@ -1262,7 +1338,7 @@ double* Solver::discharge_solve(double* b, double* x) {
   // R P b, then the blocked triangular solves
   aem::launch_permute(b, d_perm_.p, d_req_.p, n, x, st_);
   double* v = d_v_.p;
-  blocked_solve<double>(d_A_.p, (int)npad_, d_linv_.p, d_uinv_.p, x, d_w_.p, v);
+  blocked_solve_all<double>(d_A_.p, d_linv_.p, d_uinv_.p, x, d_w_.p, v);
   HIP_CHECK(hipMemcpyAsync(x, v, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   return x;
 }
@@ -1761,49 +1837,43 @@ void Solver::adj_tables(const TargetSet& ts, bool colloc, AdjTables& t) {
 // blocked solves with a non-unit lower (U^T) and a unit upper (L^T) triangle.
 template <class T>
 void Solver::make_transposed(const T* A, bool blocked, DBuf<T>& AT, DBuf<T>& linvT, DBuf<T>& uinvT) {
-  const int64_t n = npad_;
-  const int64_t nb = (n + kNB - 1) / kNB;
-  const size_t tot = (size_t)(n * n * nblk_);
+  const size_t tot = (size_t)Atot_;
   {
     DBuf<T> tmp;
     tmp.alloc(tot);
     HIP_CHECK(hipMemcpyAsync(tmp.p, A, sizeof(T) * tot, hipMemcpyDeviceToDevice, st_));
-    if (blocked && nblk_ > 1) {
-      // several systems of the same (padded) size: one call per panel covers them all
-      const int64_t sA = n * n;
-      for (int64_t j = 0; j < nb; ++j) {
-        const int64_t r0 = j * kNB;
-        const int bs = (int)std::min<int64_t>(kNB, n - r0);
-        const int rest = (int)(n - r0 - bs);
-        const T* djj = tmp.p + r0 * (n + 1);
-        if (rest > 0)
-          RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n, sA,
-                                         tmp.p + r0 * n + r0 + bs, n, sA, nblk_));
-        if (r0 > 0)
-          RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
-                                         sA, tmp.p + r0 * n, n, sA, nblk_));
-      }
-    }
-    for (int b = 0; blocked && nblk_ == 1 && b < nblk_; ++b) {
-      T* Ab = tmp.p + (int64_t)b * n * n;
-      for (int64_t j = 0; j < nb; ++j) {
-        const int64_t r0 = j * kNB;
-        const int bs = (int)std::min<int64_t>(kNB, n - r0);
-        const int rest = (int)(n - r0 - bs);
-        const T* djj = Ab + r0 * (n + 1);
-        if (rest > 0)
-          RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n,
-                                 Ab + r0 * n + r0 + bs, n));
-        if (r0 > 0)
-          RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
-                                 Ab + r0 * n, n));
-      }
-    }
     AT.alloc(tot);
-    aem::launch_transpose<T>(tmp.p, n, AT.p, st_, nblk_);
+    for (const auto& c : cls_) {
+      const int64_t n = c.npad, nb = (n + kNB - 1) / kNB, sA = n * n;
+      T* tc = tmp.p + c.a0;
+      for (int64_t j = 0; blocked && j < nb; ++j) {
+        const int64_t r0 = j * kNB;
+        const int bs = (int)std::min<int64_t>(kNB, n - r0);
+        const int rest = (int)(n - r0 - bs);
+        if (c.nblk > 1) {
+          // several systems of the same (padded) size: one call per panel covers them all
+          const T* djj = tc + r0 * (n + 1);
+          if (rest > 0)
+            RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n, sA,
+                                           tc + r0 * n + r0 + bs, n, sA, c.nblk));
+          if (r0 > 0)
+            RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj,
+                                           (int)n, sA, tc + r0 * n, n, sA, c.nblk));
+        } else {
+          const T* djj = tc + r0 * (n + 1);
+          if (rest > 0)
+            RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, rocblas_diagonal_unit, rest, bs, djj, (int)n,
+                                   tc + r0 * n + r0 + bs, n));
+          if (r0 > 0)
+            RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, rocblas_diagonal_non_unit, (int)r0, bs, djj, (int)n,
+                                   tc + r0 * n, n));
+        }
+      }
+      aem::launch_transpose<T>(tc, n, AT.p + c.a0, st_, c.nblk);
+    }
     HIP_CHECK(hipStreamSynchronize(st_));
   }
-  prepare_blocked<T>(AT.p, n, nblk_, linvT, uinvT, false);
+  prepare_blocked_all<T>(AT.p, linvT, uinvT, false);
   HIP_CHECK(hipStreamSynchronize(st_));
 }
 
@@ -1838,7 +1908,7 @@ void Solver::adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w
   const int n = (int)N_;
   aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
   aem::launch_adj_refine_in<T>(d_z_.p, d_aty_.p, n, x, st_);
-  blocked_solve<T>(AT, (int)npad_, linvT, uinvT, x, w, v);
+  blocked_solve_all<T>(AT, linvT, uinvT, x, w, v);
   aem::launch_adj_refine_out<T>(v, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
 }
 
