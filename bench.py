@@ -5,7 +5,8 @@ A step is one complete solve of the seeded 5,000-fracture network (500 m cube, r
 LU factorisation, the fixed-point iterations with coefficient growth, and the final head
 residual, with the network already resident in device memory when the timed region starts.
 The solve stops as soon as the head residual (max |A y - b(c)| over the head equations) is at
-most 1e-6 m.  ``value`` is the FP64 form (``--precision``); ``forms`` adds the other form (the
+most 1e-6 m.  ``value`` is the FP64 form (``--precision``; the batched cfg5 workload defaults
+to its specified mixed form); ``forms`` adds the other form (the
 mixed-precision form: FP32 factors refined in FP64, FP32 far field and fits) with its error
 against the FP64 result; ``adjoint`` times the implicit adjoint (gradient of the x = 0 boundary
 discharge plus 100 observation heads with respect to every log10 transmissivity and both
@@ -39,7 +40,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cfg2")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="form timed for value (default: fp64; mixed for cfg5, which is specified mixed)")
     ap.add_argument("--lu-mode", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="time only the --precision form")
@@ -49,7 +51,10 @@ def parse():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle iterations in the CPU sample")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "cfg5" else "fp64"
+    return args
 
 
 def traffic_from_profiles(workload):
