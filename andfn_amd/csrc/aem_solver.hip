@@ -260,6 +260,9 @@ class Solver {
   // size classes of a common padded size (a class's smallest part has at least kClassRatio of
   // its largest part's unknowns), largest class first
   static constexpr double kClassRatio = 0.7;
+  // block size rounded up to 32 unknowns: every column of a block (leading dimension npad) then
+  // starts on a 128-byte line in FP64 and FP32, so a 64-row segment is whole lines
+  static constexpr int64_t kPadAlign = 32;
   struct SysClass {
     int nblk, b0;      // blocks of the class and the global index of its first block
     int64_t npad;      // unknowns per block
@@ -586,7 +589,7 @@ void Solver::parse() {
       const int b = order[i];
       if (cls_.empty() || ((double)cnt[b] < kClassRatio * (double)cls_.back().npad &&
                            (int)cls_.size() < aem::kMaxSysClasses))
-        cls_.push_back(SysClass{0, i, std::max<int64_t>(cnt[b], 1), 0, 0, 0});
+        cls_.push_back(SysClass{0, i, (cnt[b] + kPadAlign - 1) / kPadAlign * kPadAlign, 0, 0, 0});
       cls_.back().nblk++;
       slot[b] = i;
       blk_n_[i] = cnt[b];

@@ -12,9 +12,10 @@ against the FP64 result; ``adjoint`` times the implicit adjoint (gradient of the
 discharge plus 100 observation heads with respect to every log10 transmissivity and both
 boundary heads) against one forward solve.
 
-Prints one JSON line.  ``roofline`` describes the collocation sweep kernel (FP64 vector ALU
-bound); ``cpu_baseline`` times the CPU FP64 oracle (a restatement of AnDFN's Numba path with
-SciPy's SuperLU) on a bounded sample of the same solve on this host.
+Prints one JSON line.  ``roofline`` describes the dominant kernel chain, the blocked triangular
+solves (HBM bound; one per iteration); ``roofline_sweep`` the collocation sweep kernel (FP64
+vector ALU bound); ``cpu_baseline`` times the CPU FP64 oracle (a restatement of AnDFN's Numba
+path with SciPy's SuperLU) on a bounded sample of the same solve on this host.
 """
 
 import argparse
@@ -30,6 +31,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), flop/s
+PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 HEAD_TOL = 1e-6
 ADJOINT_TOL = 1e-10
 
@@ -57,12 +59,14 @@ def parse():
     return args
 
 
-def traffic_from_profiles(workload):
-    """HBM bytes per sweep launch measured with rocprofv3 PMC counters (profiles/*.json), if present."""
-    p = ROOT / "profiles" / f"sweep_{workload}_traffic.json"
+def traffic_from_profiles(name, workload):
+    """HBM bytes per launch (``sweep``) or per solve (``solve``) measured with rocprofv3 PMC
+    counters (profiles/<name>_<workload>_traffic.json), if present."""
+    p = ROOT / "profiles" / f"{name}_{workload}_traffic.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("bytes_per_launch")
+            d = json.loads(p.read_text())
+            return d.get("bytes_per_launch", d.get("bytes_per_solve"))
         except Exception:
             return None
     return None
@@ -286,6 +290,15 @@ def main():
     sweep_ms = last["k_sweep_ms"] / max(last["n_sweep"], 1)
     flops_launch = last["flops_sweep"] / max(last["n_sweep"], 1)
     achieved = flops_launch / (sweep_ms * 1e-3) if sweep_ms > 0 else 0.0
+    # the dominant kernel chain: the blocked triangular solves (one HIP graph per solve), timed
+    # by HIP events around every graph launch.  One connected part, so one dense block of n
+    # unknowns: the chain reads the strictly triangular panels of L and U and the triangles of
+    # the diagonal-block inverses, n^2 + n values.
+    n_solve = max(int(last["n_rhs"]), 1)
+    solve_ms = last["k_lu_ms"] / n_solve
+    n_unk = int(last["n_unknowns"])
+    solve_bytes = (8.0 if args.precision == "fp64" else 4.0) * (n_unk * n_unk + n_unk)
+    solve_bw = solve_bytes / (solve_ms * 1e-3) if solve_ms > 0 else 0.0
     out = {
         "metric": "DFN solves/sec at max head residual 1e-6 (FP64 vs mixed GPU); adjoint/forward ratio",
         "value": args.steps / total,
@@ -321,13 +334,28 @@ def main():
         "forms": forms,
         "adjoint": adjoint,
         "roofline": {
+            "bound": "hbm",
+            "kernel": "blocked triangular solves (gemv_cm_kernel + gemv_diag_kernel chain, one HIP graph per "
+                      "solve)",
+            "share_of_step": (last["k_lu_ms"] / (1e3 * total / args.steps)) if total > 0 else None,
+            "achieved": solve_bw / 1e9,
+            "peak": PEAK_HBM / 1e9,
+            "unit": "GB/s",
+            "frac": solve_bw / PEAK_HBM,
+            "traffic": traffic_from_profiles("solve", args.workload) if args.precision == "fp64" else None,
+            "avg_launch_ms": solve_ms,
+            "bytes_per_launch": solve_bytes,
+            "launches": n_solve,
+            "unit_of_launch": "one triangular-solve chain (graph launch)",
+        },
+        "roofline_sweep": {
             "bound": "mfma",
             "kernel": "sweep_kernel<MODE_COLLOC> (FP64 vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
-            "traffic": traffic_from_profiles(args.workload),
+            "traffic": traffic_from_profiles("sweep", args.workload),
             "avg_launch_ms": sweep_ms,
             "flops_per_launch": flops_launch,
             "launches": int(last["n_sweep"]),
