@@ -1081,9 +1081,22 @@ __device__ __forceinline__ void gemv_cm_block(const T* __restrict__ A, int64_t l
   T s = 0;
   if (row < m && c0 < c1) {
     const T* a = A + (int64_t)c0 * lda + row;
-    // eight independent loads in flight per lane
+    // sixteen independent loads in flight per lane (a short panel has few workgroups, so the
+    // bytes in flight per workgroup set its streaming rate)
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     int c = c0;
+    for (; c + 16 <= c1; c += 16, a += 16 * lda) {
+      T v[16];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) v[k] = a[k * lda];
+#pragma unroll
+      for (int k = 0; k < 16; k += 4) {
+        s0 = fma(v[k], sx[c + k], s0);
+        s1 = fma(v[k + 1], sx[c + k + 1], s1);
+        s2 = fma(v[k + 2], sx[c + k + 2], s2);
+        s3 = fma(v[k + 3], sx[c + k + 3], s3);
+      }
+    }
     for (; c + 8 <= c1; c += 8, a += 8 * lda) {
       const T a0 = a[0], a1 = a[lda], a2 = a[2 * lda], a3 = a[3 * lda];
       const T a4 = a[4 * lda], a5 = a[5 * lda], a6 = a[6 * lda], a7 = a[7 * lda];
