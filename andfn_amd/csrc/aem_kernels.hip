@@ -1057,64 +1057,63 @@ void launch_spmv(int n, const int* row_ptr, const int* col, const double* val, c
 }
 
 // y[0:m] = beta * y + alpha * A[0:m, 0:ncols] x for a column-major block (lda) with ncols <= 1024:
-// 64 rows per workgroup (lane = row, coalesced column reads), the columns split over 16 wavefronts
-// and their partial sums reduced in LDS in a fixed order.  Sixteen wavefronts with eight loads in
-// flight each keep enough bytes in flight per CU to stream a panel at HBM rate.  ``tri`` skips the
-// zero half of a triangular block (1: lower, column c <= row; 2: upper, column c >= row).
-// Used for the blocked triangular solves.
+// y = beta y + alpha A x for one block of gemv_rows<T> rows of the column-major A (ncols <= 1024
+// columns, x staged in LDS).  A 128-byte line holds gemv_rows<T> = 16 doubles (32 floats): lane
+// l of a wavefront takes row l % gemv_rows and column group l / gemv_rows, so one load
+// instruction reads whole lines (4 column segments in FP64, 2 in FP32).  The columns are split
+// over 16 wavefronts, a lane keeps sixteen loads in flight, and the partial sums are combined in
+// a fixed order (lane exchanges across the column groups, then the wavefronts through LDS).
+// Short panels (the tail of the forward pass, the start of the backward pass) have few rows;
+// line-high workgroups give them four (FP64) or two (FP32) times the workgroups, and so the
+// streaming rate, of 64-row ones.  ``tri`` skips the zero half of a triangular block (1: lower,
+// column c <= row; 2: upper, column c >= row).  Used for the blocked triangular solves.
 constexpr int kGemvCols = 1024;
 constexpr int kGemvWaves = 16;
+template <class T>
+constexpr int gemv_rows() {
+  return 128 / (int)sizeof(T);
+}
 template <class T>
 __device__ __forceinline__ void gemv_cm_block(const T* __restrict__ A, int64_t lda, int m, int ncols,
                                               const T* __restrict__ x, T* __restrict__ y, T alpha, T beta, int tri,
                                               int rblk) {
+  constexpr int RG = gemv_rows<T>(), CG = 64 / RG;
   __shared__ T sx[kGemvCols];
-  __shared__ T part[kGemvWaves][64];
+  __shared__ T part[kGemvWaves][RG];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   for (int c = threadIdx.x; c < ncols; c += 64 * kGemvWaves) sx[c] = x[c];
   __syncthreads();
-  const int row = rblk * 64 + lane;
+  const int r = lane % RG, cg = lane / RG;
+  const int row = rblk * RG + r;
   const int per = (ncols + kGemvWaves - 1) / kGemvWaves;
   int c0 = wv * per, c1 = min(ncols, c0 + per);
-  if (tri == 1) c1 = min(c1, rblk * 64 + 64);  // lower: columns beyond the block's last row are zero
-  if (tri == 2) c0 = max(c0, rblk * 64);       // upper: columns before the block's first row are zero
+  if (tri == 1) c1 = min(c1, rblk * RG + RG);  // lower: columns beyond the block's last row are zero
+  if (tri == 2) c0 = max(c0, rblk * RG);       // upper: columns before the block's first row are zero
   T s = 0;
-  if (row < m && c0 < c1) {
-    const T* a = A + (int64_t)c0 * lda + row;
-    // sixteen independent loads in flight per lane (a short panel has few workgroups, so the
-    // bytes in flight per workgroup set its streaming rate)
+  if (row < m && c0 + cg < c1) {
+    int c = c0 + cg;
+    const T* a = A + (int64_t)c * lda + row;
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int c = c0;
-    for (; c + 16 <= c1; c += 16, a += 16 * lda) {
+    for (; c + 15 * CG < c1; c += 16 * CG, a += 16 * CG * lda) {
       T v[16];
 #pragma unroll
-      for (int k = 0; k < 16; ++k) v[k] = a[k * lda];
+      for (int k = 0; k < 16; ++k) v[k] = a[CG * k * lda];
 #pragma unroll
       for (int k = 0; k < 16; k += 4) {
-        s0 = fma(v[k], sx[c + k], s0);
-        s1 = fma(v[k + 1], sx[c + k + 1], s1);
-        s2 = fma(v[k + 2], sx[c + k + 2], s2);
-        s3 = fma(v[k + 3], sx[c + k + 3], s3);
+        s0 = fma(v[k], sx[c + CG * k], s0);
+        s1 = fma(v[k + 1], sx[c + CG * (k + 1)], s1);
+        s2 = fma(v[k + 2], sx[c + CG * (k + 2)], s2);
+        s3 = fma(v[k + 3], sx[c + CG * (k + 3)], s3);
       }
     }
-    for (; c + 8 <= c1; c += 8, a += 8 * lda) {
-      const T a0 = a[0], a1 = a[lda], a2 = a[2 * lda], a3 = a[3 * lda];
-      const T a4 = a[4 * lda], a5 = a[5 * lda], a6 = a[6 * lda], a7 = a[7 * lda];
-      s0 = fma(a0, sx[c], s0);
-      s1 = fma(a1, sx[c + 1], s1);
-      s2 = fma(a2, sx[c + 2], s2);
-      s3 = fma(a3, sx[c + 3], s3);
-      s0 = fma(a4, sx[c + 4], s0);
-      s1 = fma(a5, sx[c + 5], s1);
-      s2 = fma(a6, sx[c + 6], s2);
-      s3 = fma(a7, sx[c + 7], s3);
-    }
-    for (; c < c1; ++c, a += lda) s0 = fma(*a, sx[c], s0);
+    for (; c < c1; c += CG, a += CG * lda) s0 = fma(*a, sx[c], s0);
     s = (s0 + s1) + (s2 + s3);
   }
-  part[wv][lane] = s;
+#pragma unroll
+  for (int o = RG; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+  if (lane < RG) part[wv][lane] = s;
   __syncthreads();
-  if (wv == 0 && row < m) {
+  if (wv == 0 && lane < RG && row < m) {
     T t = 0;
 #pragma unroll
     for (int k = 0; k < kGemvWaves; ++k) t += part[k][lane];
@@ -1140,7 +1139,7 @@ __global__ __launch_bounds__(64 * kGemvWaves) void gemv_diag_kernel(const T* __r
   const int j = blockIdx.y;
   const int r0 = j * nbk;
   const int bs = min(nbk, n - r0);
-  if ((int)blockIdx.x * 64 >= bs) return;  // uniform per workgroup, before any barrier
+  if ((int)blockIdx.x * gemv_rows<T>() >= bs) return;  // uniform per workgroup, before any barrier
   const int64_t z = blockIdx.z;
   gemv_cm_block<T>(B + z * sB + (size_t)j * nbk * nbk, nbk, bs, bs, x + z * n + r0, y + z * n + r0, T(1), T(0), tri,
                    blockIdx.x);
@@ -1150,8 +1149,9 @@ template <class T>
 void launch_gemv_cm(const T* A, int64_t lda, int m, int ncols, const T* x, T* y, T alpha, T beta, hipStream_t st,
                     int batch, int64_t sA, int64_t sx) {
   if (m <= 0 || ncols <= 0 || batch <= 0) return;
-  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + 63) / 64, 1, batch), dim3(64 * kGemvWaves), 0, st, A, lda, m, ncols,
-                     x, y, alpha, beta, sA, sx);
+  constexpr int RG = gemv_rows<T>();
+  hipLaunchKernelGGL(gemv_cm_kernel<T>, dim3((m + RG - 1) / RG, 1, batch), dim3(64 * kGemvWaves), 0, st, A, lda, m,
+                     ncols, x, y, alpha, beta, sA, sx);
 }
 template void launch_gemv_cm<double>(const double*, int64_t, int, int, const double*, double*, double, double,
                                      hipStream_t, int, int64_t, int64_t);
@@ -1163,8 +1163,9 @@ void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y
                       int64_t sB) {
   if (n <= 0 || batch <= 0) return;
   const int nb = (n + nbk - 1) / nbk;
-  hipLaunchKernelGGL(gemv_diag_kernel<T>, dim3((nbk + 63) / 64, nb, batch), dim3(64 * kGemvWaves), 0, st, blocks, nbk,
-                     n, tri, x, y, sB);
+  constexpr int RG = gemv_rows<T>();
+  hipLaunchKernelGGL(gemv_diag_kernel<T>, dim3((nbk + RG - 1) / RG, nb, batch), dim3(64 * kGemvWaves), 0, st, blocks,
+                     nbk, n, tri, x, y, sB);
 }
 template void launch_gemv_diag<double>(const double*, int, int, int, const double*, double*, hipStream_t, int,
                                        int64_t);
