@@ -208,6 +208,8 @@ class Solver {
     if (rb_) rocblas_destroy_handle(rb_);
     if (rb2_) rocblas_destroy_handle(rb2_);
     for (auto& e : ev_) (void)hipEventDestroy(e);
+    for (auto& e : rhs_ev_)
+      if (e) (void)hipEventDestroy(e);
     if (lu_ev_[0]) (void)hipEventDestroy(lu_ev_[0]);
     if (lu_ev_[1]) (void)hipEventDestroy(lu_ev_[1]);
     if (st2_) (void)hipStreamDestroy(st2_);
@@ -231,6 +233,7 @@ class Solver {
   hipStream_t st2_ = nullptr;
   rocblas_handle rb2_ = nullptr;
   hipEvent_t lu_ev_[2] = {};
+  hipEvent_t rhs_ev_[2] = {};  // start of the right-hand-side phase, alternating by iteration
 
   // host mirrors (SoA)
   std::vector<int> etype_, efrac0_, efrac1_, ncoef_, nint_;
@@ -668,6 +671,7 @@ void Solver::upload() {
   d_scal_.alloc(1);
   d_red_.zero(aem::reduce_scratch_bytes(), st_);
   for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
+  for (auto& e : rhs_ev_) HIP_CHECK(hipEventCreate(&e));
 }
 
 template <class T>
@@ -1527,7 +1531,16 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     ++n_sweep;
   };
   bool grown_timed = false;
-  auto add_times = [&]() {
+  // With a head tolerance (and no per-iteration log) the iteration scalars are not read back
+  // every iteration: nothing launched depends on them (the stop test is the head residual, read
+  // at the top of the next iteration), so the stream runs on and the scalars and the kernel
+  // times of an iteration are collected at the next synchronisation.  The start event of the
+  // right-hand side alternates between two events because the next iteration records its own
+  // before that synchronisation.
+  const bool lazy = head_tol > 0.0 && !verbose;
+  int times_pending = 0;  // iteration whose kernel times are still to be collected (0: none)
+  auto add_times = [&](int it) {
+    hipEvent_t rhs0 = rhs_ev_[it & 1];
     float a = 0, b = 0, c = 0, d = 0;
     if (grown_timed) {
       float g = 0;
@@ -1537,7 +1550,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     }
     HIP_CHECK(hipEventElapsedTime(&a, ev_[0], ev_[1]));
     HIP_CHECK(hipEventElapsedTime(&b, ev_[1], ev_[2]));
-    HIP_CHECK(hipEventElapsedTime(&c, ev_[3], ev_[4]));
+    HIP_CHECK(hipEventElapsedTime(&c, rhs0, ev_[4]));
     HIP_CHECK(hipEventElapsedTime(&d, ev_[4], ev_[5]));
     ms_sweep += a;
     ms_fit += b;
@@ -1549,13 +1562,17 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     ++nit;
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
     if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
-    HIP_CHECK(hipEventRecord(ev_[3], st_));
+    HIP_CHECK(hipEventRecord(rhs_ev_[nit & 1], st_));
     sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed_phase_);
     // only the head rows are written; the continuity and padding rows of b stay zero
     aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
     if (head_tol > 0.0 && nit >= 2) {
       // head residual of the previous iterate against the right-hand side of its coefficients
-      resid = head_residual(d_b_.p, &resid_row);
+      resid = head_residual(d_b_.p, &resid_row);  // synchronises the stream
+      if (times_pending) {
+        add_times(times_pending);
+        times_pending = 0;
+      }
       if (solve32_) {
         // the FP32-factor refinement must contract: a residual 4x above the best so far (or not
         // finite) hands the solve to the FP64 factors for the rest of the run
@@ -1615,9 +1632,17 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         d_sched_.upload(sched_row_, st_);
         dsched = d_sched_.p;
       }
-      aem::launch_grow(nv, fv, E_, gp, dsched, d_grow_.p, st_);
-      HIP_CHECK(hipMemcpyAsync(grow_h.data(), d_grow_.p, sizeof(int) * E_, hipMemcpyDeviceToHost, st_));
-      HIP_CHECK(hipStreamSynchronize(st_));
+      // an element grows only below MAX_NCOEF; when none is, the test (and its device-to-host
+      // round trip) is skipped, with the same result
+      int growable = 0;
+      for (int e = 0; e < E_; ++e) growable += etype_[e] != 2 && ncoef_[e] < max_coef;
+      if (growable > 0) {
+        aem::launch_grow(nv, fv, E_, gp, dsched, d_grow_.p, st_);
+        HIP_CHECK(hipMemcpyAsync(grow_h.data(), d_grow_.p, sizeof(int) * E_, hipMemcpyDeviceToHost, st_));
+        HIP_CHECK(hipStreamSynchronize(st_));
+      } else {
+        std::fill(grow_h.begin(), grow_h.end(), 0);
+      }
       int ng = 0;
       std::vector<int>& gcirc = gcirc_;
       gcirc.clear();
@@ -1661,11 +1686,15 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       aem::launch_blend(nv, E_, damping, st_);
     }
     aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_red_.p, d_scal_.p, st_);
+    if (record && nit - 1 < record_n) std::copy(ncoef_.begin(), ncoef_.end(), record + (size_t)(nit - 1) * E_);
+    if (lazy && nit < max_iterations) {
+      times_pending = nit;
+      continue;
+    }
     HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
     HIP_CHECK(hipStreamSynchronize(st_));
     HIP_CHECK(hipGetLastError());
-    add_times();
-    if (record && nit - 1 < record_n) std::copy(ncoef_.begin(), ncoef_.end(), record + (size_t)(nit - 1) * E_);
+    add_times(nit);
     error = sc.error;
     argmax = sc.argmax;
     error_coef = sc.error_coef;
@@ -1690,6 +1719,18 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     }
   }
   HIP_CHECK(hipStreamSynchronize(st_));
+  if (times_pending) {
+    // the last complete iteration's kernel times and scalars (deferred, see above)
+    add_times(times_pending);
+    HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    HIP_CHECK(hipGetLastError());
+    error = sc.error;
+    argmax = sc.argmax;
+    error_coef = sc.error_coef;
+    error_q = sc.dq_max / (sc.yold_max + 1e-16);
+    times_pending = 0;
+  }
   auto t_loop = clk::now();
   const bool stopped_on_head = head_tol > 0.0 && resid >= 0.0 && resid <= head_tol;
   if (!stopped_on_head) {
