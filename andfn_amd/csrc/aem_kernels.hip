@@ -1188,6 +1188,46 @@ void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st) {
 template void launch_unit_diag<double>(double*, int, int, hipStream_t);
 template void launch_unit_diag<float>(float*, int, int, hipStream_t);
 
+// zero the triangle opposite to `lower` (strictly upper when lower, strictly lower otherwise) of
+// nb stacked nbk x nbk blocks
+template <class T>
+__global__ void tri_zero_kernel(T* __restrict__ blocks, int64_t nbk, int lower) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nbk * nbk) return;
+  const int64_t c = i / nbk, r = i - c * nbk;
+  if (lower ? c > r : r > c) blocks[(int64_t)blockIdx.y * nbk * nbk + i] = T(0);
+}
+
+template <class T>
+void launch_tri_zero(T* blocks, int nb, int nbk, int lower, hipStream_t st) {
+  if (nb <= 0) return;
+  hipLaunchKernelGGL(tri_zero_kernel<T>, dim3((unsigned)(((int64_t)nbk * nbk + 255) / 256), nb), dim3(256), 0, st,
+                     blocks, (int64_t)nbk, lower);
+}
+template void launch_tri_zero<double>(double*, int, int, int, hipStream_t);
+template void launch_tri_zero<float>(float*, int, int, int, hipStream_t);
+
+// B_z = W_z for nb column-major m x nc matrices: W packed (leading dimension m, stride m nc),
+// B with leading dimension ldb and stride sB
+template <class T>
+__global__ void copy_panels_kernel(const T* __restrict__ w, int64_t m, int64_t nc, T* __restrict__ b, int64_t ldb,
+                                   int64_t sB) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m * nc) return;
+  const int64_t z = blockIdx.y, c = i / m, r = i - c * m;
+  b[z * sB + c * ldb + r] = w[z * m * nc + i];
+}
+
+template <class T>
+void launch_copy_panels(const T* w, int64_t m, int64_t nc, int nb, T* b, int64_t ldb, int64_t sB, hipStream_t st) {
+  if (m <= 0 || nc <= 0 || nb <= 0) return;
+  hipLaunchKernelGGL(copy_panels_kernel<T>, dim3((unsigned)((m * nc + 255) / 256), nb), dim3(256), 0, st, w, m, nc, b,
+                     ldb, sB);
+}
+template void launch_copy_panels<double>(const double*, int64_t, int64_t, int, double*, int64_t, int64_t,
+                                         hipStream_t);
+template void launch_copy_panels<float>(const float*, int64_t, int64_t, int, float*, int64_t, int64_t, hipStream_t);
+
 // single-precision copy of a matrix
 __global__ void to_f32_kernel(const double* __restrict__ a, int64_t n, float* __restrict__ b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -173,6 +173,12 @@ void launch_gemv_diag(const T* blocks, int nbk, int n, int tri, const T* x, T* y
                       int64_t sB = 0);
 template <class T>
 void launch_unit_diag(T* blocks, int nb, int nbk, hipStream_t st);
+// zero the strictly upper (lower = 1) or strictly lower (lower = 0) triangle of nb stacked blocks
+template <class T>
+void launch_tri_zero(T* blocks, int nb, int nbk, int lower, hipStream_t st);
+// B_z = W_z for nb m x nc matrices (W packed, B with leading dimension ldb and stride sB)
+template <class T>
+void launch_copy_panels(const T* w, int64_t m, int64_t nc, int nb, T* b, int64_t ldb, int64_t sB, hipStream_t st);
 void launch_to_f32(const double* a, int64_t n, float* b, hipStream_t st);
 
 // Blocked LU with tournament pivoting (aem_lu.hip).  Inner panels of kLuPanel columns grouped

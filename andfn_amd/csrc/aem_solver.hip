@@ -295,7 +295,7 @@ class Solver {
   static int64_t inv_entries(int64_t npad) { return (npad + kNB - 1) / kNB * kNB * kNB; }
   DBuf<double> d_linv_, d_uinv_, d_w_, d_v_, d_req_;
   template <class T>
-  void prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit);
+  void prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit, T* ws);
   template <class T>
   void prepare_blocked_all(T* A, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit = true);
   template <class T>
@@ -856,10 +856,21 @@ static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_di
   const double one = 1.0;
   return rocblas_dtrmm_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, ldb, B, ldb);
 }
-static rocblas_status rb_trmm_right(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
-                                    const float* T, int ldt, float* B, int64_t ldb) {
-  const float one = 1.0f;
-  return rocblas_strmm_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, B, ldb, B, ldb);
+
+// C_z = A_z B_z (m x nc, inner dimension k) on nb blocks, column-major, strided
+static rocblas_status rb_gemm_nn(rocblas_handle h, int64_t m, int64_t nc, int64_t k, const double* A, int64_t lda,
+                                 int64_t sA, const double* B, int64_t ldb, int64_t sB, double* C, int64_t ldc,
+                                 int64_t sC, int nb) {
+  const double one = 1.0, zero = 0.0;
+  return rocblas_dgemm_strided_batched_64(h, rocblas_operation_none, rocblas_operation_none, m, nc, k, &one, A, lda, sA,
+                                          B, ldb, sB, &zero, C, ldc, sC, nb);
+}
+static rocblas_status rb_gemm_nn(rocblas_handle h, int64_t m, int64_t nc, int64_t k, const float* A, int64_t lda,
+                                 int64_t sA, const float* B, int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC,
+                                 int nb) {
+  const float one = 1.0f, zero = 0.0f;
+  return rocblas_sgemm_strided_batched_64(h, rocblas_operation_none, rocblas_operation_none, m, nc, k, &one, A, lda, sA,
+                                          B, ldb, sB, &zero, C, ldc, sC, nb);
 }
 
 // the same product on nb blocks: T + z sT, B + z sB
@@ -868,13 +879,6 @@ static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, ro
                                             int nb) {
   const double one = 1.0;
   return rocblas_dtrmm_strided_batched_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, sT,
-                                          B, ldb, sB, B, ldb, sB, nb);
-}
-static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, rocblas_diagonal d, int m, int nc,
-                                            const float* T, int ldt, int64_t sT, float* B, int64_t ldb, int64_t sB,
-                                            int nb) {
-  const float one = 1.0f;
-  return rocblas_strmm_strided_batched_64(h, rocblas_side_right, f, rocblas_operation_none, d, m, nc, &one, T, ldt, sT,
                                           B, ldb, sB, B, ldb, sB, nb);
 }
 
@@ -889,7 +893,7 @@ static rocblas_status rb_trmm_right_batched(rocblas_handle h, rocblas_fill f, ro
 // For the nblk blocks of one size class (n unknowns each) at A; linv / uinv hold the class's
 // inverse blocks (zeroed by the caller).
 template <class T>
-void Solver::prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit) {
+void Solver::prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool lower_unit, T* ws) {
   const int64_t nb = (n + kNB - 1) / kNB, nfull = n / kNB;
   const rocblas_diagonal dl = lower_unit ? rocblas_diagonal_unit : rocblas_diagonal_non_unit;
   const rocblas_diagonal du = lower_unit ? rocblas_diagonal_non_unit : rocblas_diagonal_unit;
@@ -905,52 +909,47 @@ void Solver::prepare_blocked(T* A, int64_t n, int nblk, T* linv, T* uinv, bool l
       RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, bs, A + r0 * (n + 1), n, sA, uinv + j * kNB * kNB, kNB,
                                 sinv, nblk));
     }
-    aem::launch_unit_diag<T>(lower_unit ? linv : uinv, (int)(nb * nblk), kNB, st_);
-    for (int64_t j = 0; j < nb; ++j) {
-      const int64_t r0 = j * kNB;
-      const int bs = (int)std::min<int64_t>(kNB, n - r0);
-      const int rest = (int)(n - r0 - bs);
-      if (rest > 0)
-        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_lower, dl, rest, bs, linv + j * kNB * kNB, kNB, sinv,
-                                       A + r0 * n + r0 + bs, n, sA, nblk));
-      if (r0 > 0)
-        RB_CHECK(rb_trmm_right_batched(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv + j * kNB * kNB, kNB, sinv,
-                                       A + r0 * n, n, sA, nblk));
-    }
-    return;
-  }
-  for (int b = 0; b < nblk; ++b) {
-    T* Ab = A + (int64_t)b * n * n;
-    T* li = linv + b * sinv;
-    T* ui = uinv + b * sinv;
-    if (nfull > 0) {
-      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, Ab, n, kNB * (n + 1), li, kNB, (int64_t)kNB * kNB,
-                                (int)nfull));
-      RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, kNB, Ab, n, kNB * (n + 1), ui, kNB, (int64_t)kNB * kNB,
-                                (int)nfull));
-    }
-    if (nb > nfull) {
-      const int64_t r0 = nfull * kNB;
-      const int rem = (int)(n - r0);
-      const T* blk = Ab + r0 * (n + 1);
-      RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, dl, rem, blk, n, li + nfull * kNB * kNB, kNB));
-      RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, du, rem, blk, n, ui + nfull * kNB * kNB, kNB));
+  } else {
+    for (int b = 0; b < nblk; ++b) {
+      T* Ab = A + (int64_t)b * n * n;
+      T* li = linv + b * sinv;
+      T* ui = uinv + b * sinv;
+      if (nfull > 0) {
+        RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_lower, dl, kNB, Ab, n, kNB * (n + 1), li, kNB, (int64_t)kNB * kNB,
+                                  (int)nfull));
+        RB_CHECK(rb_trtri_batched(rb_, rocblas_fill_upper, du, kNB, Ab, n, kNB * (n + 1), ui, kNB, (int64_t)kNB * kNB,
+                                  (int)nfull));
+      }
+      if (nb > nfull) {
+        const int64_t r0 = nfull * kNB;
+        const int rem = (int)(n - r0);
+        const T* blk = Ab + r0 * (n + 1);
+        RB_CHECK(rb_trtri(rb_, rocblas_fill_lower, dl, rem, blk, n, li + nfull * kNB * kNB, kNB));
+        RB_CHECK(rb_trtri(rb_, rocblas_fill_upper, du, rem, blk, n, ui + nfull * kNB * kNB, kNB));
+      }
     }
   }
-  // a unit-triangular inverse is written without its diagonal
+  // the inverses as dense blocks: the opposite triangles zero, and a unit-triangular inverse
+  // (written without its diagonal) given its ones
+  aem::launch_tri_zero<T>(linv, (int)(nb * nblk), kNB, 1, st_);
+  aem::launch_tri_zero<T>(uinv, (int)(nb * nblk), kNB, 0, st_);
   aem::launch_unit_diag<T>(lower_unit ? linv : uinv, (int)(nb * nblk), kNB, st_);
-  for (int b = 0; b < nblk; ++b) {
-    T* Ab = A + (int64_t)b * n * n;
-    for (int64_t j = 0; j < nb; ++j) {
-      const int64_t r0 = j * kNB;
-      const int bs = (int)std::min<int64_t>(kNB, n - r0);
-      const int rest = (int)(n - r0 - bs);
-      if (rest > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_lower, dl, rest, bs, linv + b * sinv + j * kNB * kNB, kNB,
-                               Ab + r0 * n + r0 + bs, n));
-      if (r0 > 0)
-        RB_CHECK(rb_trmm_right(rb_, rocblas_fill_upper, du, (int)r0, bs, uinv + b * sinv + j * kNB * kNB, kNB,
-                               Ab + r0 * n, n));
+  // panels: G_j = L[below j, j] Linv_j and H_j = U[above j, j] Uinv_j as dense products (MFMA
+  // GEMM) into the workspace ws (at least (n - 1) kNB nblk entries), copied back in place
+  const int64_t sA = n * n;
+  for (int64_t j = 0; j < nb; ++j) {
+    const int64_t r0 = j * kNB;
+    const int bs = (int)std::min<int64_t>(kNB, n - r0);
+    const int64_t rest = n - r0 - bs;
+    if (rest > 0) {
+      T* P = A + r0 * n + r0 + bs;
+      RB_CHECK(rb_gemm_nn(rb_, rest, bs, bs, P, n, sA, linv + j * kNB * kNB, kNB, sinv, ws, rest, rest * bs, nblk));
+      aem::launch_copy_panels<T>(ws, rest, bs, nblk, P, n, sA, st_);
+    }
+    if (r0 > 0) {
+      T* P = A + r0 * n;
+      RB_CHECK(rb_gemm_nn(rb_, r0, bs, bs, P, n, sA, uinv + j * kNB * kNB, kNB, sinv, ws, r0, r0 * bs, nblk));
+      aem::launch_copy_panels<T>(ws, r0, bs, nblk, P, n, sA, st_);
     }
   }
 }
@@ -960,8 +959,13 @@ template <class T>
 void Solver::prepare_blocked_all(T* A, DBuf<T>& linv, DBuf<T>& uinv, bool lower_unit) {
   linv.zero((size_t)invtot_, st_);
   uinv.zero((size_t)invtot_, st_);
+  size_t wsn = 1;
+  for (const auto& c : cls_) wsn = std::max(wsn, (size_t)(c.npad * kNB * c.nblk));
+  DBuf<T> ws;
+  ws.alloc(wsn);
   for (const auto& c : cls_)
-    prepare_blocked<T>(A + c.a0, c.npad, c.nblk, linv.p + c.inv0, uinv.p + c.inv0, lower_unit);
+    prepare_blocked<T>(A + c.a0, c.npad, c.nblk, linv.p + c.inv0, uinv.p + c.inv0, lower_unit, ws.p);
+  HIP_CHECK(hipStreamSynchronize(st_));  // the workspace is released on return
 }
 
 void Solver::assemble_and_factor() {
