@@ -258,6 +258,7 @@ class Solver {
   DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
   DBuf<int> d_csr_ptr_, d_csr_col_;
   int64_t nnz_A_ = 0;
+  bool csr_ready_ = false;  // the CSR pattern of A (d_csr_ptr_, d_csr_col_) is built
   // discharge-system layout: one dense block per connected component of the network (discharge
   // elements in element order, then fracture constants, then padding), the blocks grouped into
   // size classes of a common padded size (a class's smallest part has at least kClassRatio of
@@ -984,42 +985,47 @@ void Solver::assemble_and_factor() {
   // a CSR copy of the unfactored matrix (rows in unknown order) for the head residual A y - b;
   // row_scale: 1 on intersection rows, 1/t on constant-head and well rows, 0 elsewhere
   {
-    std::vector<int> row_e(N, -1), row_f(N, -1);
-    for (int e : de_) row_e[de_pos_[e]] = e;
-    for (int f = 0; f < F_; ++f) row_f[fpos_[f]] = f;
-    std::vector<int> rp(1, 0), cols, tmp;
-    std::vector<double> rs(N, 0.0);
-    for (int64_t g = 0; g < N; ++g) {
-      tmp.clear();
-      if (row_e[g] >= 0) {
-        const int e = row_e[g];
-        const int sides = etype_[e] == 0 ? 2 : 1;
-        for (int sd = 0; sd < sides; ++sd) {
-          const int f = sd ? efrac1_[e] : efrac0_[e];
-          for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
-            const int src = slots_[k].elem;
-            if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+    if (!csr_ready_) {
+      // the sparsity pattern depends on the network only: built once per handle
+      std::vector<int> row_e(N, -1), row_f(N, -1);
+      for (int e : de_) row_e[de_pos_[e]] = e;
+      for (int f = 0; f < F_; ++f) row_f[fpos_[f]] = f;
+      std::vector<int> rp(1, 0), cols, tmp;
+      for (int64_t g = 0; g < N; ++g) {
+        tmp.clear();
+        if (row_e[g] >= 0) {
+          const int e = row_e[g];
+          const int sides = etype_[e] == 0 ? 2 : 1;
+          for (int sd = 0; sd < sides; ++sd) {
+            const int f = sd ? efrac1_[e] : efrac0_[e];
+            for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
+              const int src = slots_[k].elem;
+              if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+            }
+            tmp.push_back(fpos_[f]);
           }
-          tmp.push_back(fpos_[f]);
+        } else if (row_f[g] >= 0) {
+          const int f = row_f[g];
+          for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+            if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
         }
-        rs[g] = etype_[e] == 0 ? 1.0 : 1.0 / t_[efrac0_[e]];
-      } else if (row_f[g] >= 0) {
-        const int f = row_f[g];
-        for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-          if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        cols.insert(cols.end(), tmp.begin(), tmp.end());
+        rp.push_back((int)cols.size());
       }
-      std::sort(tmp.begin(), tmp.end());
-      tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-      cols.insert(cols.end(), tmp.begin(), tmp.end());
-      rp.push_back((int)cols.size());
+      d_csr_ptr_.upload(rp, st_);
+      d_csr_col_.upload(cols, st_);
+      d_csr_val_.alloc(cols.size());
+      nnz_A_ = (int64_t)cols.size();
+      csr_ready_ = true;
     }
-    d_csr_ptr_.upload(rp, st_);
-    d_csr_col_.upload(cols, st_);
-    d_csr_val_.alloc(cols.size());
+    // the row scale follows the transmissivities (which set_parameters may change)
+    std::vector<double> rs(N, 0.0);
+    for (int e : de_) rs[de_pos_[e]] = etype_[e] == 0 ? 1.0 : 1.0 / t_[efrac0_[e]];
     aem::launch_csr_gather(d_A_.p, sys(), N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
     d_rowscale_.upload(rs, st_);
     HIP_CHECK(hipStreamSynchronize(st_));
-    nnz_A_ = (int64_t)cols.size();
   }
   d_ay_.alloc(N);
   // row equilibration (rows of intersections scale with 1/t), as SuperLU equilibrates by default
