@@ -686,6 +686,142 @@ __global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsIt
 }
 
 // --------------------------------------------------------------------------------------
+// Right-hand side as a linear map of the coefficients.  With q = C = 0 the potential at a
+// discharge point is the sum of the sources' series, so the mean over the K points of side d is
+//   om_d = sum over the sources s of the side's fracture, j < ncoef_s, of Re(c_sj W_dsj),
+//   W_dsj = sign_s / K sum_k w_s(z_dk)^j
+// (w_s the source's series variable at the point, as in the sweep).  W depends on the geometry
+// only: rhs_basis_kernel builds it once per handle, rhs_eval_kernel applies it every iteration
+// and forms b as rhs_reduce_kernel does from the swept values.  A side with a point inside a
+// well or an impermeable circle gets NaN, as the sweep gives there.
+// --------------------------------------------------------------------------------------
+constexpr int kRhsMaxQ = 8;  // discharge points per lane (K <= 512)
+
+__global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView tg, int K,
+                                                       const int64_t* __restrict__ wptr, const int* __restrict__ mcap,
+                                                       double2* __restrict__ W, int2* __restrict__ widx,
+                                                       int* __restrict__ side_nan) {
+  const int d = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int e = tg.side_elem[d];
+  const int which = tg.side_which[d];
+  const int f = which ? net.efrac1[e] : net.efrac0[e];
+  double2 z[kRhsMaxQ];
+  bool ok[kRhsMaxQ];
+#pragma unroll
+  for (int q = 0; q < kRhsMaxQ; ++q) {
+    const int i = lane + 64 * q;
+    ok[q] = i < K;
+    z[q] = ok[q] ? discharge_point(net, e, which, i, K) : c2(0.0, 0.0);
+  }
+  const double invK = 1.0 / (double)K;
+  int inside = 0;
+  int64_t o = wptr[d];
+  for (int ks = net.mem_ptr[f]; ks < net.mem_ptr[f + 1]; ++ks) {
+    const Slot sl = net.slots[ks];
+    const int src = sl.elem;
+    if (sl.kind == KIND_WELL) {
+#pragma unroll
+      for (int q = 0; q < kRhsMaxQ; ++q)
+        if (ok[q]) {
+          const double2 chi = c2(fma(sl.alpha.x, z[q].x, sl.beta.x), fma(sl.alpha.x, z[q].y, sl.beta.y));
+          if (sqrt(fma(chi.x, chi.x, chi.y * chi.y)) < 1.0 - 1e-12) inside = 1;
+        }
+      continue;
+    }
+    double2 w[kRhsMaxQ], pw[kRhsMaxQ];
+#pragma unroll
+    for (int q = 0; q < kRhsMaxQ; ++q) {
+      w[q] = c2(0.0, 0.0);
+      pw[q] = c2(ok[q] ? 1.0 : 0.0, 0.0);
+      if (!ok[q]) continue;
+      if (sl.kind == KIND_BOUND) {
+        w[q] = c2(sl.alpha.x * z[q].x, sl.alpha.x * z[q].y);  // z / R
+      } else if (sl.kind == KIND_IMPC) {
+        const double2 chi = c2(fma(sl.alpha.x, z[q].x, sl.beta.x), fma(sl.alpha.x, z[q].y, sl.beta.y));
+        const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+        if (sqrt(r2) < 1.0 - 1e-10) inside = 1;
+        const double inv = rcp_pos(r2);
+        w[q] = c2(chi.x * inv, -chi.y * inv);
+      } else {  // KIND_LINE, KIND_IMPL
+        const double2 Z = c2(fma(sl.alpha.x, z[q].x, fma(-sl.alpha.y, z[q].y, sl.beta.x)),
+                             fma(sl.alpha.x, z[q].y, fma(sl.alpha.y, z[q].x, sl.beta.y)));
+        const double2 chi = joukowsky_exterior(Z);
+        const double inv = rcp_pos(fma(chi.x, chi.x, chi.y * chi.y));
+        w[q] = c2(chi.x * inv, -chi.y * inv);
+      }
+    }
+    const double sg = (sl.kind == KIND_LINE || sl.kind == KIND_IMPL) ? sl.sign * invK : invK;
+    const int m = mcap[src];
+    for (int j = 0; j < m; ++j) {
+      double ar = 0.0, ai = 0.0;
+#pragma unroll
+      for (int q = 0; q < kRhsMaxQ; ++q) {
+        ar += pw[q].x;
+        ai += pw[q].y;
+        const double2 t = pw[q];
+        pw[q] = c2(fma(t.x, w[q].x, -t.y * w[q].y), fma(t.x, w[q].y, t.y * w[q].x));
+      }
+      ar = wave_sum(ar);
+      ai = wave_sum(ai);
+      if (lane == 0) {
+        W[o + j] = c2(sg * ar, sg * ai);
+        widx[o + j] = make_int2(src, j);
+      }
+    }
+    o += m;
+  }
+  inside = __any(inside) ? 1 : 0;
+  if (lane == 0) side_nan[d] = inside;
+}
+
+__device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const int64_t* __restrict__ wptr,
+                                                const double2* __restrict__ W, const int2* __restrict__ widx,
+                                                const int* __restrict__ side_nan) {
+  double acc = 0.0;
+  for (int64_t t = wptr[d] + threadIdx.x; t < wptr[d + 1]; t += 64) {
+    const int2 q = widx[t];
+    if (q.y < net.ncoef[q.x]) {
+      const double2 c = net.coef[(size_t)q.x * net.M + q.y];
+      const double2 w = W[t];
+      acc = fma(c.x, w.x, fma(-c.y, w.y, acc));
+    }
+  }
+  acc = wave_sum(acc);
+  return side_nan[d] ? __builtin_nan("") : acc;
+}
+
+__global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
+                                                      const int64_t* __restrict__ wptr,
+                                                      const double2* __restrict__ W, const int2* __restrict__ widx,
+                                                      const int* __restrict__ side_nan, const int* __restrict__ de_pos,
+                                                      double* __restrict__ b) {
+  const int j = blockIdx.x;
+  if (j >= nde) return;
+  const RhsItem it = items[j];
+  const double om0 = rhs_side_mean(net, it.s0, wptr, W, widx, side_nan);
+  const double om1 = it.s1 >= 0 ? rhs_side_mean(net, it.s1, wptr, W, widx, side_nan) : 0.0;
+  if (threadIdx.x == 0) {
+    const int e = it.elem;
+    if (net.etype[e] == 0) b[de_pos[e]] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
+    else b[de_pos[e]] = net.phi[e] - om0;
+  }
+}
+
+void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
+                      const int* mcap, double2* W, int2* widx, int* side_nan, hipStream_t st) {
+  if (nsides <= 0) return;
+  hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, widx, side_nan);
+}
+
+void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
+                     const int2* widx, const int* side_nan, const int* de_pos, double* b, hipStream_t st) {
+  if (nde <= 0) return;
+  hipLaunchKernelGGL(rhs_eval_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, widx, side_nan, de_pos,
+                     b);
+}
+
+// --------------------------------------------------------------------------------------
 // Discharge matrix (fill_discharge_matrix, hpc_solve.py:1117-1201; discharge terms
 // hpc_intersection.py:22-46, hpc_well.py:14-35, hpc_const_head_line.py:14-35)
 // one wavefront per (discharge row, side); dense column-major A (lda = n)

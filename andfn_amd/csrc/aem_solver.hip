@@ -341,6 +341,15 @@ class Solver {
     return (int)c.size();
   }
   DBuf<RhsItem> d_rhs_items_;
+  // right-hand side as a linear map of the coefficients (rhs_basis_kernel), built once per handle
+  DBuf<double2> d_rbW_;
+  DBuf<int2> d_rbIdx_;
+  DBuf<int64_t> d_rbPtr_;
+  DBuf<int> d_rbNan_, d_rbMcap_;
+  std::vector<int> rb_mcap_;
+  bool rb_ok_ = false;
+  int64_t rb_entries_ = 0;
+  bool ensure_rhs_basis(int max_coef, int coef_increase);
   DBuf<int64_t> d_ipiv_, d_info_;  // 64-bit LAPACK interface: n^2 exceeds 2^31 from ~46k unknowns
   DBuf<aem::IterScalars> d_scal_;
   DBuf<unsigned char> d_red_;  // scratch of the two-stage reductions (counters reset by the kernels)
@@ -1429,6 +1438,46 @@ double Solver::bc_residual() {
   return r;
 }
 
+// Basis of the right-hand side (rhs_basis_kernel) for the current discharge target set: every
+// source gets as many powers as its coefficient count can reach (MAX_NCOEF + COEF_INCREASE - 1
+// while it may still grow).  Returns false (the iteration then sweeps the discharge points) when
+// the discharge points exceed what the kernel takes or the basis would exceed 16 GB.
+bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
+  if (K_ > aem::kRhsMaxK) return false;
+  std::vector<int> mcap(E_, 0);
+  for (int e = 0; e < E_; ++e)
+    if (etype_[e] != 2)
+      mcap[e] = ncoef_[e] < max_coef ? (int)std::min<int64_t>(L_.M, max_coef + coef_increase - 1) : ncoef_[e];
+  if (rb_ok_ && mcap == rb_mcap_) return true;
+  const int ns = (int)rhs_.side_elem.size();
+  std::vector<int64_t> ptr(ns + 1, 0);
+  for (int d = 0; d < ns; ++d) {
+    const int e = rhs_.side_elem[d];
+    const int f = rhs_.side_which[d] ? efrac1_[e] : efrac0_[e];
+    int64_t cnt = 0;
+    for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+      if (slots_[k].kind != aem::KIND_WELL) cnt += mcap[slots_[k].elem];
+    ptr[d + 1] = ptr[d] + cnt;
+  }
+  const int64_t entries = ptr[ns];
+  if (entries * (int64_t)(sizeof(double2) + sizeof(int2)) > (int64_t)16 << 30) {
+    rb_ok_ = false;
+    return false;
+  }
+  d_rbPtr_.upload(ptr, st_);
+  d_rbMcap_.upload(mcap, st_);
+  d_rbW_.alloc((size_t)std::max<int64_t>(entries, 1));
+  d_rbIdx_.alloc((size_t)std::max<int64_t>(entries, 1));
+  d_rbNan_.alloc((size_t)std::max(ns, 1));
+  aem::launch_rhs_basis(net(), rhs_.view(), ns, K_, d_rbPtr_.p, d_rbMcap_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, st_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipStreamSynchronize(st_));
+  rb_mcap_ = mcap;
+  rb_entries_ = entries;
+  rb_ok_ = true;
+  return true;
+}
+
 int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
@@ -1521,9 +1570,25 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   std::vector<int> grow_h(E_);
   std::vector<char> grown(E_);
   aem::IterScalars sc{};
+  // head rows of b from the current coefficients: the basis product, or a sweep of the discharge
+  // points when no basis could be built
+  const bool use_basis = ensure_rhs_basis(max_coef, coef_increase);
   int64_t pr_tmp = 0;
-  const double rhs_flops = set_flops(rhs_, false, &pr_tmp);
-  const int64_t rhs_pairs = pr_tmp;
+  double rhs_flops = set_flops(rhs_, false, &pr_tmp);
+  int64_t rhs_pairs = pr_tmp;
+  if (use_basis) {
+    rhs_flops = 4.0 * (double)rb_entries_;  // two multiply-adds per basis entry
+    rhs_pairs = rb_entries_;
+  }
+  auto rhs_head = [&](bool mixed) {
+    if (use_basis) {
+      aem::launch_rhs_eval(nv, d_rhs_items_.p, nde, d_rbPtr_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, d_de_pos_.p, d_b_.p,
+                           st_);
+    } else {
+      sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed);
+      aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
+    }
+  };
 
   auto sweep_full = [&](bool blend_now) {
     HIP_CHECK(hipEventRecord(ev_[0], st_));
@@ -1573,9 +1638,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
     if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
     HIP_CHECK(hipEventRecord(rhs_ev_[nit & 1], st_));
-    sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed_phase_);
     // only the head rows are written; the continuity and padding rows of b stay zero
-    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
+    rhs_head(mixed_phase_);
     if (head_tol > 0.0 && nit >= 2) {
       // head residual of the previous iterate against the right-hand side of its coefficients
       resid = head_residual(d_b_.p, &resid_row);  // synchronises the stream
@@ -1745,8 +1809,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const bool stopped_on_head = head_tol > 0.0 && resid >= 0.0 && resid <= head_tol;
   if (!stopped_on_head) {
     // right-hand side of the returned coefficients, then the residual of the returned discharges
-    sweep(aem::MODE_RHS, rhs_, d_valr_.p, false);
-    aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
+    rhs_head(false);
     resid = head_residual(d_b_.p, &resid_row);
   }
   const double bcres = bc_residual();
