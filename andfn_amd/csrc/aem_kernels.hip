@@ -778,17 +778,28 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
 __device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const int64_t* __restrict__ wptr,
                                                 const double2* __restrict__ W, const int2* __restrict__ widx,
                                                 const int* __restrict__ side_nan) {
-  double acc = 0.0;
-  for (int64_t t = wptr[d] + threadIdx.x; t < wptr[d + 1]; t += 64) {
-    const int2 q = widx[t];
-    if (q.y < net.ncoef[q.x]) {
-      const double2 c = net.coef[(size_t)q.x * net.M + q.y];
-      const double2 w = W[t];
-      acc = fma(c.x, w.x, fma(-c.y, w.y, acc));
+  // four entries per lane in flight (index, count and coefficient loads are a dependent chain)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t t1 = wptr[d + 1];
+  for (int64_t t0 = wptr[d] + threadIdx.x; t0 < t1; t0 += 256) {
+    int2 q[4];
+    double2 w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t t = t0 + 64 * u;
+      q[u] = t < t1 ? widx[t] : make_int2(0, 0x7fffffff);
+      w[u] = t < t1 ? W[t] : c2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (q[u].y < net.ncoef[q[u].x]) {
+        const double2 c = net.coef[(size_t)q[u].x * net.M + q[u].y];
+        acc[u] = fma(c.x, w[u].x, fma(-c.y, w[u].y, acc[u]));
+      }
     }
   }
-  acc = wave_sum(acc);
-  return side_nan[d] ? __builtin_nan("") : acc;
+  double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  return side_nan[d] ? __builtin_nan("") : a;
 }
 
 __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
