@@ -87,15 +87,15 @@ typedef struct andfn_mi_info {
   double t_iterate;       /* seconds: the fixed-point iterations */
   double t_total;         /* seconds: whole call, records in to records out */
   double k_sweep_ms;      /* summed device time of the collocation sweep kernel */
-  double k_rhs_ms;        /* summed device time of the head right-hand-side sweep kernel */
+  double k_rhs_ms;        /* summed device time of the head right-hand side (basis product or sweep) and residual */
   double k_fit_ms;        /* summed device time of the fit kernel */
   double k_lu_ms;         /* summed device time of the per-iteration discharge solve */
   int64_t n_sweep;        /* launches of the collocation sweep kernel */
   int64_t n_rhs;          /* launches of the right-hand-side sweep kernel */
   int64_t pairs_sweep;    /* target-point x source pairs evaluated by all collocation sweeps */
-  int64_t pairs_rhs;      /* target-point x source pairs evaluated by all right-hand-side sweeps */
+  int64_t pairs_rhs;      /* right-hand-side basis entries applied (or point x source pairs swept) */
   double flops_sweep;     /* algorithmic flops of all collocation sweeps (line 8m+61, circle 8m-6, well 24) */
-  double flops_rhs;       /* algorithmic flops of all right-hand-side sweeps */
+  double flops_rhs;       /* algorithmic flops of all right-hand-side evaluations */
   /* head residual of the returned state, metres: max over the discharge (head) equations of
    * |A y - b(c)|, i.e. how far the returned discharges and constants are from satisfying the
    * averaged head conditions of the returned coefficients (intersection rows are head
@@ -112,7 +112,7 @@ typedef struct andfn_mi_info {
   double refine_rate;     /* mixed form: measured contraction per refinement step with the FP32-rounded
                            * factors (above 0.25 the run solves with the FP64 factors throughout) */
   int32_t sweep_points_per_lane;  /* tile shape the autotuner chose for the collocation sweep (1 or 2) */
-  int32_t rhs_points_per_lane;    /* and for the right-hand-side sweep */
+  int32_t rhs_points_per_lane;    /* and for the right-hand-side sweep (0: the basis product ran instead) */
 } andfn_mi_info;
 
 /*
