@@ -1444,6 +1444,9 @@ double Solver::bc_residual() {
 // the discharge points exceed what the kernel takes or the basis would exceed 16 GB.
 bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
   if (K_ > aem::kRhsMaxK) return false;
+  // ANDFN_MI_RHS_SWEEP set: sweep the discharge points every iteration (the fallback path; tests
+  // compare the two)
+  if (std::getenv("ANDFN_MI_RHS_SWEEP")) return false;
   std::vector<int> mcap(E_, 0);
   for (int e = 0; e < E_; ++e)
     if (etype_[e] != 2)
