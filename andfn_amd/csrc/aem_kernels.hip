@@ -699,7 +699,8 @@ constexpr int kRhsMaxQ = 8;  // discharge points per lane (K <= 512)
 
 __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView tg, int K,
                                                        const int64_t* __restrict__ wptr, const int* __restrict__ mcap,
-                                                       double2* __restrict__ W, int* __restrict__ side_nan) {
+                                                       double2* __restrict__ W, int2* __restrict__ widx,
+                                                       int* __restrict__ side_nan) {
   const int d = blockIdx.x;
   const int lane = threadIdx.x;
   const int e = tg.side_elem[d];
@@ -763,7 +764,10 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);
-      if (lane == 0) W[o + j] = c2(sg * ar, sg * ai);
+      if (lane == 0) {
+        W[o + j] = c2(sg * ar, sg * ai);
+        widx[o + j] = make_int2(src, j);
+      }
     }
     o += m;
   }
@@ -771,83 +775,60 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
   if (lane == 0) side_nan[d] = inside;
 }
 
-// One workgroup per block of up to kRhsBlockSides discharge sides of one fracture: the
-// fracture's coefficients (zero beyond each source's count) are staged in LDS a chunk at a time,
-// and every side of the block reads its basis row once against them (coalesced, no gathers);
-// om[d] is the mean of side d.  Blocks keep the work of large fractures spread over CUs.  All sides of a
-// fracture have the same entries (its sources' powers), and each side is summed by one
-// wavefront in a fixed order.
-constexpr int kRhsChunk = 2048;
-__global__ __launch_bounds__(256) void rhs_eval_kernel(NetView net, const int4* __restrict__ blocks,
-                                                       const int64_t* __restrict__ wptr, const int* __restrict__ soff,
-                                                       const int* __restrict__ mcap, const double2* __restrict__ W,
-                                                       double* __restrict__ om) {
-  const int4 bk = blocks[blockIdx.x];  // (fracture, first side, end side, -)
-  const int f = bk.x, d0 = bk.y, d1 = bk.z;
-  __shared__ double2 cs[kRhsChunk];
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int64_t L = wptr[d0 + 1] - wptr[d0];
-  if (L == 0) {  // no source with a series: the means are zero
-    for (int d = d0 + tid; d < d1; d += 256) om[d] = 0.0;
-    return;
-  }
-  for (int64_t e0 = 0; e0 < L; e0 += kRhsChunk) {
-    const int64_t e1 = e0 + kRhsChunk < L ? e0 + kRhsChunk : L;
-    for (int k = net.mem_ptr[f] + tid; k < net.mem_ptr[f + 1]; k += 256) {
-      const int o = soff[k];
-      if (o < 0) continue;  // a well: no series
-      const int src = net.slots[k].elem;
-      const int m = mcap[src], nc = net.ncoef[src];
-      const int64_t jb = e0 - o > 0 ? e0 - o : 0, je = e1 - o < m ? e1 - o : m;
-      for (int64_t jj = jb; jj < je; ++jj)
-        cs[o + jj - e0] = jj < nc ? net.coef[(size_t)src * net.M + jj] : c2(0.0, 0.0);
+__device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const int64_t* __restrict__ wptr,
+                                                const double2* __restrict__ W, const int2* __restrict__ widx,
+                                                const int* __restrict__ side_nan) {
+  // four entries per lane in flight (index, count and coefficient loads are a dependent chain)
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int64_t t1 = wptr[d + 1];
+  for (int64_t t0 = wptr[d] + threadIdx.x; t0 < t1; t0 += 256) {
+    int2 q[4];
+    double2 w[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t t = t0 + 64 * u;
+      q[u] = t < t1 ? widx[t] : make_int2(0, 0x7fffffff);
+      w[u] = t < t1 ? W[t] : c2(0.0, 0.0);
     }
-    __syncthreads();
-    for (int d = d0 + wv; d < d1; d += 4) {
-      const double2* __restrict__ w = W + wptr[d];
-      double acc = 0.0;
-      for (int64_t e = e0 + lane; e < e1; e += 64) {
-        const double2 c = cs[e - e0], x = w[e];
-        acc = fma(c.x, x.x, fma(-c.y, x.y, acc));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (q[u].y < net.ncoef[q[u].x]) {
+        const double2 c = net.coef[(size_t)q[u].x * net.M + q[u].y];
+        acc[u] = fma(c.x, w[u].x, fma(-c.y, w[u].y, acc[u]));
       }
-      acc = wave_sum(acc);
-      if (lane == 0) om[d] = (e0 == 0 ? 0.0 : om[d]) + acc;
     }
-    __syncthreads();
   }
+  double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  return side_nan[d] ? __builtin_nan("") : a;
 }
 
-// b from the side means (rhs_reduce_kernel's formula), one thread per discharge element
-__global__ void rhs_form_kernel(NetView net, const RhsItem* __restrict__ items, int nde, const double* __restrict__ om,
-                                const int* __restrict__ side_nan, const int* __restrict__ de_pos,
-                                double* __restrict__ b) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
+                                                      const int64_t* __restrict__ wptr,
+                                                      const double2* __restrict__ W, const int2* __restrict__ widx,
+                                                      const int* __restrict__ side_nan, const int* __restrict__ de_pos,
+                                                      double* __restrict__ b) {
+  const int j = blockIdx.x;
   if (j >= nde) return;
   const RhsItem it = items[j];
-  const int e = it.elem;
-  const double nan = __builtin_nan("");
-  const double om0 = side_nan[it.s0] ? nan : om[it.s0];
-  if (net.etype[e] == 0) {
-    const double om1 = side_nan[it.s1] ? nan : om[it.s1];
-    b[de_pos[e]] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
-  } else {
-    b[de_pos[e]] = net.phi[e] - om0;
+  const double om0 = rhs_side_mean(net, it.s0, wptr, W, widx, side_nan);
+  const double om1 = it.s1 >= 0 ? rhs_side_mean(net, it.s1, wptr, W, widx, side_nan) : 0.0;
+  if (threadIdx.x == 0) {
+    const int e = it.elem;
+    if (net.etype[e] == 0) b[de_pos[e]] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
+    else b[de_pos[e]] = net.phi[e] - om0;
   }
 }
 
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int* side_nan, hipStream_t st) {
+                      const int* mcap, double2* W, int2* widx, int* side_nan, hipStream_t st) {
   if (nsides <= 0) return;
-  hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, side_nan);
+  hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, widx, side_nan);
 }
 
-void launch_rhs_eval(const NetView& net, int nblocks, const int4* blocks, const int64_t* wptr, const int* soff,
-                     const int* mcap, const double2* W, double* om, const RhsItem* items, int nde,
-                     const int* side_nan, const int* de_pos, double* b, hipStream_t st) {
+void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
+                     const int2* widx, const int* side_nan, const int* de_pos, double* b, hipStream_t st) {
   if (nde <= 0) return;
-  if (nblocks > 0)
-    hipLaunchKernelGGL(rhs_eval_kernel, dim3(nblocks), dim3(256), 0, st, net, blocks, wptr, soff, mcap, W, om);
-  hipLaunchKernelGGL(rhs_form_kernel, dim3((nde + 255) / 256), dim3(256), 0, st, net, items, nde, om, side_nan, de_pos,
+  hipLaunchKernelGGL(rhs_eval_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, widx, side_nan, de_pos,
                      b);
 }
 

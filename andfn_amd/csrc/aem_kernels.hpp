@@ -218,16 +218,12 @@ void launch_row_scale(double* A, int64_t npad, int nblk, double* rs, unsigned lo
 void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st);
 void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st);
 // right-hand side as a linear map of the coefficients: basis W (entries [wptr[d], wptr[d + 1]) of
-// discharge side d: the powers 0 .. mcap[s] - 1 of every non-well source s of the side's
-// fracture, in slot order, source s's first at offset soff[slot]) built once, applied per
-// iteration (blocks (f, d0, d1): sides [d0, d1) of fracture f; om receives the side means)
+// discharge side d, with their (source, power) in widx) built once, applied per iteration
 constexpr int kRhsMaxK = 512;
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int* side_nan, hipStream_t st);
-constexpr int kRhsBlockSides = 16;  // discharge sides per workgroup of the product
-void launch_rhs_eval(const NetView& net, int nblocks, const int4* blocks, const int64_t* wptr, const int* soff,
-                     const int* mcap, const double2* W, double* om, const RhsItem* items, int nde,
-                     const int* side_nan, const int* de_pos, double* b, hipStream_t st);
+                      const int* mcap, double2* W, int2* widx, int* side_nan, hipStream_t st);
+void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
+                     const int2* widx, const int* side_nan, const int* de_pos, double* b, hipStream_t st);
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
                        const int* de_pos, double* b, hipStream_t st);
 void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,

@@ -343,11 +343,9 @@ class Solver {
   DBuf<RhsItem> d_rhs_items_;
   // right-hand side as a linear map of the coefficients (rhs_basis_kernel), built once per handle
   DBuf<double2> d_rbW_;
+  DBuf<int2> d_rbIdx_;
   DBuf<int64_t> d_rbPtr_;
-  DBuf<int> d_rbNan_, d_rbMcap_, d_rbSoff_;
-  DBuf<int4> d_rbBlk_;
-  int rb_nblk_ = 0;
-  DBuf<double> d_rbOm_;
+  DBuf<int> d_rbNan_, d_rbMcap_;
   std::vector<int> rb_mcap_;
   bool rb_ok_ = false;
   int64_t rb_entries_ = 0;
@@ -1465,37 +1463,16 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
     ptr[d + 1] = ptr[d] + cnt;
   }
   const int64_t entries = ptr[ns];
-  if (entries * (int64_t)sizeof(double2) > (int64_t)16 << 30) {
+  if (entries * (int64_t)(sizeof(double2) + sizeof(int2)) > (int64_t)16 << 30) {
     rb_ok_ = false;
     return false;
   }
-  // offset of every non-well slot's powers within its fracture's entries (-1 for a well)
-  std::vector<int> soff(slots_.size(), -1);
-  for (int f = 0; f < F_; ++f) {
-    int o = 0;
-    for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-      if (slots_[k].kind != aem::KIND_WELL) {
-        soff[k] = o;
-        o += mcap[slots_[k].elem];
-      }
-  }
   d_rbPtr_.upload(ptr, st_);
   d_rbMcap_.upload(mcap, st_);
-  d_rbSoff_.upload(soff, st_);
-  // workgroups of the product: up to kRhsBlockSides sides of one fracture each, largest rows first
-  std::vector<int4> blk;
-  for (int f = 0; f < F_; ++f)
-    for (int d = rhs_.fsb[f]; d < rhs_.fsb[f + 1]; d += aem::kRhsBlockSides)
-      blk.push_back(make_int4(f, d, std::min(d + aem::kRhsBlockSides, rhs_.fsb[f + 1]), 0));
-  std::stable_sort(blk.begin(), blk.end(), [&](const int4& a, const int4& b) {
-    return (ptr[a.y + 1] - ptr[a.y]) * (a.z - a.y) > (ptr[b.y + 1] - ptr[b.y]) * (b.z - b.y);
-  });
-  d_rbBlk_.upload(blk, st_);
-  rb_nblk_ = (int)blk.size();
   d_rbW_.alloc((size_t)std::max<int64_t>(entries, 1));
+  d_rbIdx_.alloc((size_t)std::max<int64_t>(entries, 1));
   d_rbNan_.alloc((size_t)std::max(ns, 1));
-  d_rbOm_.alloc((size_t)std::max(ns, 1));
-  aem::launch_rhs_basis(net(), rhs_.view(), ns, K_, d_rbPtr_.p, d_rbMcap_.p, d_rbW_.p, d_rbNan_.p, st_);
+  aem::launch_rhs_basis(net(), rhs_.view(), ns, K_, d_rbPtr_.p, d_rbMcap_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, st_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(st_));
   rb_mcap_ = mcap;
@@ -1608,8 +1585,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   }
   auto rhs_head = [&](bool mixed) {
     if (use_basis) {
-      aem::launch_rhs_eval(nv, rb_nblk_, d_rbBlk_.p, d_rbPtr_.p, d_rbSoff_.p, d_rbMcap_.p, d_rbW_.p, d_rbOm_.p,
-                           d_rhs_items_.p, nde, d_rbNan_.p, d_de_pos_.p, d_b_.p, st_);
+      aem::launch_rhs_eval(nv, d_rhs_items_.p, nde, d_rbPtr_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, d_de_pos_.p, d_b_.p,
+                           st_);
     } else {
       sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed);
       aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);
