@@ -118,45 +118,80 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
   // pivot key: float bits of |a[k]| (monotone for non-negative values; a NaN ranks above every
   // number, so a NaN row is still taken and the winner list stays full) over the inverted
   // thread index (ties go to the lower thread); 0 marks a row that is already a pivot.
-  // Fully unrolled so that the row stays in registers with static indices.
-#pragma clang loop unroll(full)
-  for (int k = 0; k < kBW; ++k) {
-    if (k < bw) {
-      unsigned long long key = 0ull;
-      if (active)
-        key = ((unsigned long long)__float_as_uint((float)fabs(a[k])) << 32) |
-              (unsigned long long)(0xFFFFFFFFu - (unsigned)t);
-      key = wave_key_max(key);
-      if ((t & 63) == 0) redk[t >> 6] = key;
-      __syncthreads();
-      unsigned long long best = redk[0];
+  // The 64 steps run as a loop over 8 blocks of 8 columns: the block's columns are copied into
+  // cur[] and its 8 steps are unrolled, so every register index is static (the row stays in
+  // registers) while the code stays ~2k instructions; fully unrolled, the 64 steps take ~140 KB
+  // of code, more than the instruction cache, and every step waits on instruction fetch.  Each
+  // element receives the same multiply-adds in the same order either way.
+  constexpr int kCB = 8;
+#pragma clang loop unroll(disable)
+  for (int kb = 0; kb < kBW / kCB; ++kb) {
+    if (kb * kCB >= bw) break;
+    double cur[kCB];
 #pragma unroll
-      for (int w = 1; w < kWaves; ++w) {
-        const unsigned long long wk = redk[w];
-        best = wk > best ? wk : best;
+    for (int cb = 0; cb < kBW / kCB; ++cb)
+      if (cb == kb) {
+#pragma unroll
+        for (int q = 0; q < kCB; ++q) cur[q] = a[cb * kCB + q];
       }
-      if (best != 0ull) {
-        npiv = k + 1;
-        if (t == (int)(0xFFFFFFFFu - (unsigned)best)) {
 #pragma unroll
-          for (int c = k; c < kBW; ++c) prow[c] = a[c];
-          pinv = a[k] != 0.0 ? 1.0 / a[k] : 0.0;
-          swin[k] = (int)row;
-          mypiv = k;
-          active = false;
-        }
-      }
-      __syncthreads();
-      if (best != 0ull) {
-        if (active) {
-          const double l = a[k] * pinv;
-          a[k] = l;
+    for (int q = 0; q < kCB; ++q) {
+      const int k = kb * kCB + q;
+      if (k < bw) {
+        unsigned long long key = 0ull;
+        if (active)
+          key = ((unsigned long long)__float_as_uint((float)fabs(cur[q])) << 32) |
+                (unsigned long long)(0xFFFFFFFFu - (unsigned)t);
+        key = wave_key_max(key);
+        if ((t & 63) == 0) redk[t >> 6] = key;
+        __syncthreads();
+        unsigned long long best = redk[0];
 #pragma unroll
-          for (int c = k + 1; c < kBW; ++c) a[c] = fma(-l, prow[c], a[c]);
+        for (int w = 1; w < kWaves; ++w) {
+          const unsigned long long wk = redk[w];
+          best = wk > best ? wk : best;
         }
-        if (final_level && t == 0 && prow[k] == 0.0 && info[0] == 0) info[0] = j0 + k + 1;
+        if (best != 0ull) {
+          npiv = k + 1;
+          if (t == (int)(0xFFFFFFFFu - (unsigned)best)) {
+#pragma unroll
+            for (int q2 = 0; q2 < kCB; ++q2) prow[kb * kCB + q2] = cur[q2];
+#pragma unroll
+            for (int cb = 0; cb < kBW / kCB; ++cb)
+              if (cb > kb) {
+#pragma unroll
+                for (int q2 = 0; q2 < kCB; ++q2) prow[cb * kCB + q2] = a[cb * kCB + q2];
+              }
+            pinv = cur[q] != 0.0 ? 1.0 / cur[q] : 0.0;
+            swin[k] = (int)row;
+            mypiv = k;
+            active = false;
+          }
+        }
+        __syncthreads();
+        if (best != 0ull) {
+          if (active) {
+            const double l = cur[q] * pinv;
+            cur[q] = l;
+#pragma unroll
+            for (int q2 = q + 1; q2 < kCB; ++q2) cur[q2] = fma(-l, prow[kb * kCB + q2], cur[q2]);
+#pragma unroll
+            for (int cb = 0; cb < kBW / kCB; ++cb)
+              if (cb > kb) {
+#pragma unroll
+                for (int q2 = 0; q2 < kCB; ++q2) a[cb * kCB + q2] = fma(-l, prow[cb * kCB + q2], a[cb * kCB + q2]);
+              }
+          }
+          if (final_level && t == 0 && prow[k] == 0.0 && info[0] == 0) info[0] = j0 + k + 1;
+        }
       }
     }
+#pragma unroll
+    for (int cb = 0; cb < kBW / kCB; ++cb)
+      if (cb == kb) {
+#pragma unroll
+        for (int q = 0; q < kCB; ++q) a[cb * kCB + q] = cur[q];
+      }
   }
   __syncthreads();
   if (t < kBW && t < npiv) cand_out[(int64_t)g * kBW + t] = swin[t];

@@ -210,8 +210,9 @@ class Solver {
     for (auto& e : ev_) (void)hipEventDestroy(e);
     for (auto& e : rhs_ev_)
       if (e) (void)hipEventDestroy(e);
-    if (lu_ev_[0]) (void)hipEventDestroy(lu_ev_[0]);
-    if (lu_ev_[1]) (void)hipEventDestroy(lu_ev_[1]);
+    for (auto& e : lu_ev_)
+      if (e) (void)hipEventDestroy(e);
+    if (st3_) (void)hipStreamDestroy(st3_);
     if (st2_) (void)hipStreamDestroy(st2_);
     if (st_) (void)hipStreamDestroy(st_);
   }
@@ -232,7 +233,10 @@ class Solver {
   // st_) while st_ factors the next outer panel; created on first use
   hipStream_t st2_ = nullptr;
   rocblas_handle rb2_ = nullptr;
-  hipEvent_t lu_ev_[2] = {};
+  // st2_ is kept off a few CUs and st3_ has only those CUs (three streams in all: with more,
+  // two can share one of the process's hardware queues and its CU mask)
+  hipStream_t st3_ = nullptr;
+  hipEvent_t lu_ev_[4] = {};
   hipEvent_t rhs_ev_[2] = {};  // start of the right-hand-side phase, alternating by iteration
 
   // host mirrors (SoA)
@@ -1163,11 +1167,35 @@ void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* inf
   const int64_t sA = n * n, sp = (int64_t)nslots * aem::kLuPairStride;
   calu_reserve(n, nblk);
   if (!st2_) {
-    HIP_CHECK(hipStreamCreate(&st2_));
+    // A trailing update fills every CU for a while; the panel kernels of the next outer panel,
+    // sharing CUs with it, run at half speed, which matters once the update is shorter than
+    // the panel chain.  st2_ is kept off `reserve` CUs of each of the 8 XCDs (mask bit i lies in
+    // block g = i / (cu / 8), position p = i % (cu / 8); bits with p % 8 == g and p / 8 < reserve
+    // are cleared, which spreads the cleared bits over the XCDs whether the mask enumerates CUs
+    // XCD-major or interleaved) and st3_ has only those CUs.  st_ keeps every CU.
+    int reserve = 4;
+    if (const char* e = std::getenv("ANDFN_MI_LU_RESERVE")) reserve = std::atoi(e);
+    int dev = 0, cu = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per = cu / 8;
+    if (reserve > 0 && cu % 8 == 0 && per % 8 == 0 && reserve <= per / 8) {
+      std::vector<uint32_t> mask((cu + 31) / 32, 0u), kept((cu + 31) / 32, 0u);
+      for (int i = 0; i < cu; ++i) {
+        const int g = i / per, p = i % per;
+        if (!(p % 8 == g && p / 8 < reserve))
+          mask[i / 32] |= 1u << (i % 32);
+        else
+          kept[i / 32] |= 1u << (i % 32);
+      }
+      HIP_CHECK(hipExtStreamCreateWithCUMask(&st2_, (uint32_t)mask.size(), mask.data()));
+      HIP_CHECK(hipExtStreamCreateWithCUMask(&st3_, (uint32_t)kept.size(), kept.data()));
+    } else {
+      HIP_CHECK(hipStreamCreate(&st2_));
+    }
     RB_CHECK(rocblas_create_handle(&rb2_));
     RB_CHECK(rocblas_set_stream(rb2_, st2_));
-    HIP_CHECK(hipEventCreateWithFlags(&lu_ev_[0], hipEventDisableTiming));
-    HIP_CHECK(hipEventCreateWithFlags(&lu_ev_[1], hipEventDisableTiming));
+    for (auto& e : lu_ev_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   int* cand[2] = {lu_cand_[0].p, lu_cand_[1].p};
   int* cnt[2] = {lu_cnt_[0].p, lu_cnt_[1].p};
@@ -1181,7 +1209,22 @@ void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* inf
       RB_CHECK(rocblas_dgemm_strided_batched_64(h, rocblas_operation_none, rocblas_operation_none, m, nc, k, &mone, a,
                                                 n, sA, bm, n, sA, &one, c, n, sA, nblk));
   };
+  // Where the work of an outer panel runs, set when the previous outer panel issues its trailing
+  // update (which overlaps this outer panel's factorisation):
+  //   kSerial  everything on st_ (no trailing update pending: the first outer panel);
+  //   kNarrow  a small update (below narrow_flops, one block) on st2_, the panel kernels on st3_
+  //            and the inner updates on st_;
+  //   kWide    a large update on st_ itself, which keeps every CU for it, and the whole panel
+  //            chain on st2_ (the chain waits on the update for CUs either way; there it keeps
+  //            the reserved CUs).
+  // narrow_flops: 400 GFLOP (measured: on a 19.7k block every outer panel runs faster narrow;
+  // on a 65.7k block the updates above it run faster wide).
+  enum { kSerial, kNarrow, kWide } mode = kSerial;
   bool pending = false;  // a trailing update on st2_ has not been waited for yet
+  static const double narrow_flops = [] {
+    const char* e = std::getenv("ANDFN_MI_LU_NARROW_GFLOP");
+    return (e ? std::atof(e) : 400.0) * 1e9;
+  }();
   for (int64_t J0 = 0; J0 < n; J0 += aem::kLuOuter) {
     const int64_t J1 = std::min<int64_t>(n, J0 + aem::kLuOuter);
     // inner panels: only columns [J0, J1) are read or written (brought up to date by the
@@ -1190,13 +1233,28 @@ void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* inf
     for (int64_t j0 = J0; j0 < J1; j0 += aem::kLuPanel, ++nl) {
       const int bw = (int)std::min<int64_t>(aem::kLuPanel, J1 - j0);
       int* pairs = lu_pairs_.p + (int64_t)nl * aem::kLuPairStride;
+      hipStream_t pst = mode == kWide ? st2_ : st_;  // panel kernels
+      rocblas_handle ib = mode == kWide ? rb2_ : rb_;  // inner update
+      if (mode == kNarrow) {
+        HIP_CHECK(hipEventRecord(lu_ev_[2], st_));
+        HIP_CHECK(hipStreamWaitEvent(st3_, lu_ev_[2], 0));
+        pst = st3_;
+      }
       const int* win =
-          aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, perm, info, st_, nblk, sA, sp);
-      aem::launch_calu_apply(A, n, n, J0, J1, j0, bw, pairs, win, lu11_.p, st_, nblk, sA, sp);
-      aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, st_, nblk, sA);
+          aem::launch_tslu(A, n, n, j0, bw, cand, cnt, lu11_.p, pairs, perm, info, pst, nblk, sA, sp);
+      aem::launch_calu_apply(A, n, n, J0, J1, j0, bw, pairs, win, lu11_.p, pst, nblk, sA, sp);
+      aem::launch_calu_l21(A, n, n, j0, bw, lu11_.p, pst, nblk, sA);
+      if (mode == kNarrow) {
+        HIP_CHECK(hipEventRecord(lu_ev_[3], st3_));
+        HIP_CHECK(hipStreamWaitEvent(st_, lu_ev_[3], 0));
+      }
       const int64_t rows = n - j0 - bw, cols = J1 - j0 - bw;
       if (rows > 0 && cols > 0)
-        gemm(rb_, rows, cols, bw, A + (j0 + bw) + j0 * n, A + j0 + (j0 + bw) * n, A + (j0 + bw) + (j0 + bw) * n);
+        gemm(ib, rows, cols, bw, A + (j0 + bw) + j0 * n, A + j0 + (j0 + bw) * n, A + (j0 + bw) + (j0 + bw) * n);
+    }
+    if (mode == kWide) {  // the chain on st2_ ran beside the update on st_
+      HIP_CHECK(hipEventRecord(lu_ev_[3], st2_));
+      HIP_CHECK(hipStreamWaitEvent(st_, lu_ev_[3], 0));
     }
     // everything below touches columns >= J1 (and < J0): the previous trailing update must be done
     if (pending) {
@@ -1216,12 +1274,19 @@ void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* inf
       // the next outer panel's columns first, on st_; the rest of the update on st2_
       const int64_t J2 = std::min<int64_t>(n, J1 + aem::kLuOuter);
       gemm(rb_, rest, J2 - J1, J1 - J0, A + J1 + J0 * n, A + J0 + J1 * n, A + J1 + J1 * n);
+      mode = kSerial;
       if (J2 < n) {
+        mode = st3_ != nullptr && nblk == 1 && 2.0 * rest * (n - J2) * (J1 - J0) < narrow_flops ? kNarrow : kWide;
+        // the next outer panel's columns are up to date: its chain may start
         HIP_CHECK(hipEventRecord(lu_ev_[0], st_));
         HIP_CHECK(hipStreamWaitEvent(st2_, lu_ev_[0], 0));
-        gemm(rb2_, rest, n - J2, J1 - J0, A + J1 + J0 * n, A + J0 + J2 * n, A + J1 + J2 * n);
-        HIP_CHECK(hipEventRecord(lu_ev_[1], st2_));
-        pending = true;
+        if (mode == kNarrow) {
+          gemm(rb2_, rest, n - J2, J1 - J0, A + J1 + J0 * n, A + J0 + J2 * n, A + J1 + J2 * n);
+          HIP_CHECK(hipEventRecord(lu_ev_[1], st2_));
+          pending = true;
+        } else {
+          gemm(rb_, rest, n - J2, J1 - J0, A + J1 + J0 * n, A + J0 + J2 * n, A + J1 + J2 * n);
+        }
       }
     }
   }
