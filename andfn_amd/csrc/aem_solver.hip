@@ -1188,11 +1188,16 @@ void Solver::calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* inf
         else
           kept[i / 32] |= 1u << (i % 32);
       }
-      HIP_CHECK(hipExtStreamCreateWithCUMask(&st2_, (uint32_t)mask.size(), mask.data()));
-      HIP_CHECK(hipExtStreamCreateWithCUMask(&st3_, (uint32_t)kept.size(), kept.data()));
-    } else {
-      HIP_CHECK(hipStreamCreate(&st2_));
+      // a device or runtime that refuses CU masks gets plain streams (and no narrow split)
+      if (hipExtStreamCreateWithCUMask(&st2_, (uint32_t)mask.size(), mask.data()) != hipSuccess ||
+          hipExtStreamCreateWithCUMask(&st3_, (uint32_t)kept.size(), kept.data()) != hipSuccess) {
+        (void)hipGetLastError();
+        if (st2_) (void)hipStreamDestroy(st2_);
+        if (st3_) (void)hipStreamDestroy(st3_);
+        st2_ = st3_ = nullptr;
+      }
     }
+    if (!st2_) HIP_CHECK(hipStreamCreate(&st2_));
     RB_CHECK(rocblas_create_handle(&rb2_));
     RB_CHECK(rocblas_set_stream(rb2_, st2_));
     for (auto& e : lu_ev_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));

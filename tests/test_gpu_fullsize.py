@@ -94,6 +94,18 @@ def test_fp64_runs_are_bit_identical(cfg2, fp64):
     assert np.array_equal(fa2["constant"], fa["constant"])
 
 
+def test_lu_stream_placement_is_bit_identical(cfg2, fp64, monkeypatch):
+    """The LU's panel kernels on reserved CUs (the default) or on plain streams compute the same
+    arithmetic in the same order: results identical to the bit."""
+    fa, ea, info = fp64
+    monkeypatch.setenv("ANDFN_MI_LU_RESERVE", "0")
+    fa2, ea2, info2 = _run(cfg2, "fp64")
+    assert info2["nit"] == info["nit"]
+    assert np.array_equal(ea2["coef"], ea["coef"])
+    assert np.array_equal(ea2["q"], ea["q"])
+    assert np.array_equal(fa2["constant"], fa["constant"])
+
+
 def test_mixed_form_against_fp64(cfg2, fp64):
     fa, ea, info = fp64
     fam, eam, infom = _run(cfg2, "mixed")
