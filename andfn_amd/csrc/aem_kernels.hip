@@ -911,6 +911,20 @@ __global__ void scatter_kernel(NetView net, const int* __restrict__ de, int nde,
   else if (k < nde + nf) net.constant_w[k - nde] = y[sys.fpos[k - nde]];
 }
 
+// accept a discharge solve x: y_old <- y, y <- x, and q and the fracture constants from x (the
+// scatter of hpc_solve.py:720-761), one launch
+__global__ void accept_kernel(NetView net, const int* __restrict__ de, int nde, int nf, SysLayout sys,
+                              const double* __restrict__ x, double* __restrict__ y, double* __restrict__ yold,
+                              int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    yold[k] = y[k];
+    y[k] = x[k];
+  }
+  if (k < nde) net.q_w[de[k]] = x[sys.de_pos[de[k]]];
+  else if (k < nde + nf) net.constant_w[k - nde] = x[sys.fpos[k - nde]];
+}
+
 __device__ __forceinline__ void nanmax_merge(double& a, int& ia, double b, int ib) {
   // np.max / np.argmax semantics: NaN wins, first index on ties
   const bool an = a != a, bn = b != b;
@@ -1438,6 +1452,14 @@ void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hi
   const int64_t tot = npad * nblk;
   if (tot <= 0) return;
   hipLaunchKernelGGL(pad_identity_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, A, npad, nblk, blk_n);
+}
+
+void launch_accept(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* x, double* y,
+                   double* yold, int64_t n, hipStream_t st) {
+  const int64_t tot = std::max<int64_t>(n, (int64_t)nde + nf);
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(accept_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, net, de, nde, nf, sys, x, y,
+                     yold, n);
 }
 
 void launch_scatter(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* y,

@@ -230,6 +230,9 @@ void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const
                      double* A, hipStream_t st);
 // A[b][i][i] = 1 for the padding unknowns i >= blk_n[b] of every block
 void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st);
+// y_old <- y, y <- x, q and fracture constants <- x (n >= nde + nf unknowns)
+void launch_accept(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* x, double* y,
+                   double* yold, int64_t n, hipStream_t st);
 void launch_scatter(const NetView& net, const int* de, int nde, int nf, const SysLayout& sys, const double* y,
                     hipStream_t st);
 // scratch: reduce_scratch_bytes() of device memory, zeroed once before the first launch (the

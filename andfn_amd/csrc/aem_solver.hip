@@ -210,6 +210,11 @@ class Solver {
     for (auto& e : ev_) (void)hipEventDestroy(e);
     for (auto& e : rhs_ev_)
       if (e) (void)hipEventDestroy(e);
+    for (auto& p : sol_ev_)
+      for (auto& e : p)
+        if (e) (void)hipEventDestroy(e);
+    if (res_ev_) (void)hipEventDestroy(res_ev_);
+    if (h_scal_) (void)hipHostFree(h_scal_);
     for (auto& e : lu_ev_)
       if (e) (void)hipEventDestroy(e);
     if (st3_) (void)hipStreamDestroy(st3_);
@@ -238,6 +243,9 @@ class Solver {
   hipStream_t st3_ = nullptr;
   hipEvent_t lu_ev_[4] = {};
   hipEvent_t rhs_ev_[2] = {};  // start of the right-hand-side phase, alternating by iteration
+  hipEvent_t sol_ev_[2][2] = {};  // start / end of the discharge solve, alternating by iteration
+  hipEvent_t res_ev_ = nullptr;   // head residual copied to h_scal_
+  aem::IterScalars* h_scal_ = nullptr;  // pinned
 
   // host mirrors (SoA)
   std::vector<int> etype_, efrac0_, efrac1_, ncoef_, nint_;
@@ -428,6 +436,7 @@ class Solver {
     lu_graph_tried_ = false;
   }
   double head_residual(const double* b, int* row);
+  void head_residual_async(const double* b);
   double bc_residual();
 
   // tile shape of the collocation and right-hand-side sweeps (points per lane, 1 or 2), picked on
@@ -686,6 +695,10 @@ void Solver::upload() {
   d_red_.zero(aem::reduce_scratch_bytes(), st_);
   for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
   for (auto& e : rhs_ev_) HIP_CHECK(hipEventCreate(&e));
+  for (auto& p : sol_ev_)
+    for (auto& e : p) HIP_CHECK(hipEventCreate(&e));
+  HIP_CHECK(hipEventCreateWithFlags(&res_ev_, hipEventDisableTiming));
+  HIP_CHECK(hipHostMalloc((void**)&h_scal_, sizeof(aem::IterScalars), hipHostMallocDefault));
 }
 
 template <class T>
@@ -1465,6 +1478,15 @@ void Solver::sweep(int mode, const TargetSet& ts, double* val, bool mixed) {
 }
 
 // max_j |(A0 y)_j - b_j| * row_scale_j over the head equations for the current y
+// head_residual without waiting: the scalars land in h_scal_ (pinned) when res_ev_ completes
+void Solver::head_residual_async(const double* b) {
+  const int n = (int)N_;
+  aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
+  aem::launch_residual(d_ay_.p, b, d_rowscale_.p, n, d_red_.p, d_scal_.p, st_);
+  HIP_CHECK(hipMemcpyAsync(h_scal_, d_scal_.p, sizeof(aem::IterScalars), hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipEventRecord(res_ev_, st_));
+}
+
 double Solver::head_residual(const double* b, int* row) {
   const int n = (int)N_;
   aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
@@ -1698,14 +1720,26 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     }
     HIP_CHECK(hipEventElapsedTime(&a, ev_[0], ev_[1]));
     HIP_CHECK(hipEventElapsedTime(&b, ev_[1], ev_[2]));
-    HIP_CHECK(hipEventElapsedTime(&c, rhs0, ev_[4]));
-    HIP_CHECK(hipEventElapsedTime(&d, ev_[4], ev_[5]));
+    HIP_CHECK(hipEventElapsedTime(&c, rhs0, sol_ev_[it & 1][0]));
+    HIP_CHECK(hipEventElapsedTime(&d, sol_ev_[it & 1][0], sol_ev_[it & 1][1]));
     ms_sweep += a;
     ms_fit += b;
     ms_rhs += c;
     ms_lu += d;
   };
 
+  // d_x_ = A^-1 d_b_, then y_old <- y, y <- x and q, constants <- x (one launch)
+  auto solve_and_accept = [&](int it) {
+    HIP_CHECK(hipEventRecord(sol_ev_[it & 1][0], st_));
+    solve_rhs();
+    HIP_CHECK(hipEventRecord(sol_ev_[it & 1][1], st_));
+    aem::launch_accept(nv, d_de_.p, nde, F_, sys(), d_x_.p, d_y_.p, d_yold_.p, n, st_);
+  };
+  // drop an accepted solve: y <- y_old, q and constants from it
+  auto undo_accept = [&]() {
+    HIP_CHECK(hipMemcpyAsync(d_y_.p, d_yold_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
+    aem::launch_scatter(nv, d_de_.p, nde, F_, sys(), d_y_.p, st_);
+  };
   while (cnt_error < 2 && nit < max_iterations) {
     ++nit;
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
@@ -1713,9 +1747,20 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipEventRecord(rhs_ev_[nit & 1], st_));
     // only the head rows are written; the continuity and padding rows of b stay zero
     rhs_head(mixed_phase_);
+    bool early = false, early32 = false;
     if (head_tol > 0.0 && nit >= 2) {
-      // head residual of the previous iterate against the right-hand side of its coefficients
-      resid = head_residual(d_b_.p, &resid_row);  // synchronises the stream
+      // head residual of the previous iterate against the right-hand side of its coefficients.
+      // This iteration's discharge solve is enqueued behind it before the host waits for the
+      // value, so the GPU does not idle over the round trip; when the value ends the run or
+      // hands the solve to the FP64 factors, the solve is dropped (y restored from y_old).
+      head_residual_async(d_b_.p);
+      early32 = solve32_;
+      if (early32) ++solves32_;
+      solve_and_accept(nit);
+      early = true;
+      HIP_CHECK(hipEventSynchronize(res_ev_));
+      resid = h_scal_->residual;
+      resid_row = h_scal_->arg_residual;
       if (times_pending) {
         add_times(times_pending);
         times_pending = 0;
@@ -1727,6 +1772,11 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
           if (verbose)
             fprintf(stderr, "mixed form: head residual %.3e at iteration %d (best %.3e); FP64 factors from here\n",
                     resid, nit, best_resid);
+          if (early) {
+            undo_accept();
+            if (early32) --solves32_;
+            early = false;
+          }
           switch_to_fp64_solves();
           if (mixed_phase_) {
             mixed_phase_ = false;
@@ -1751,19 +1801,18 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
           nit_fp32 = nit - 1;
         }
       } else if (resid <= head_tol) {
+        if (early) {
+          undo_accept();
+          if (early32) --solves32_;
+        }
         --nit;
-        HIP_CHECK(hipEventRecord(ev_[4], st_));
-        HIP_CHECK(hipEventRecord(ev_[5], st_));
         break;
       }
     }
-    HIP_CHECK(hipEventRecord(ev_[4], st_));
-    if (solve32_) ++solves32_;
-    solve_rhs();  // d_x_ = A^-1 d_b_
-    HIP_CHECK(hipEventRecord(ev_[5], st_));
-    HIP_CHECK(hipMemcpyAsync(d_yold_.p, d_y_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
-    HIP_CHECK(hipMemcpyAsync(d_y_.p, d_x_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
-    aem::launch_scatter(nv, d_de_.p, nde, F_, sys(), d_y_.p, st_);
+    if (!early) {
+      if (solve32_) ++solves32_;
+      solve_and_accept(nit);
+    }
     fl_rhs += rhs_flops;
     pr_rhs += rhs_pairs;
     ++n_rhs;
