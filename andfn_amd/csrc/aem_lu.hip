@@ -142,6 +142,9 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
         if (active)
           key = ((unsigned long long)__float_as_uint((float)fabs(cur[q])) << 32) |
                 (unsigned long long)(0xFFFFFFFFu - (unsigned)t);
+        // every row's reciprocal is formed while the pivot search runs, so the winner only
+        // stores its own (the division is off the path between the two barriers)
+        const double rinv = cur[q] != 0.0 ? 1.0 / cur[q] : 0.0;
         key = wave_key_max(key);
         if ((t & 63) == 0) redk[t >> 6] = key;
         __syncthreads();
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
 #pragma unroll
                 for (int q2 = 0; q2 < kCB; ++q2) prow[cb * kCB + q2] = a[cb * kCB + q2];
               }
-            pinv = cur[q] != 0.0 ? 1.0 / cur[q] : 0.0;
+            pinv = rinv;
             swin[k] = (int)row;
             mypiv = k;
             active = false;
