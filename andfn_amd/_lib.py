@@ -46,7 +46,12 @@ class Options(ctypes.Structure):
         ("head_tol", ctypes.c_double),
         ("factor", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("iteration_log", ctypes.c_void_p),
+        ("log_iterations", ctypes.c_int64),
     ]
+
+
+LOG_FIELDS = 6  # ANDFN_MI_LOG_FIELDS
 
 
 class Info(ctypes.Structure):
@@ -80,6 +85,8 @@ class Info(ctypes.Structure):
         ("refine_rate", ctypes.c_double),
         ("sweep_points_per_lane", ctypes.c_int32),
         ("rhs_points_per_lane", ctypes.c_int32),
+        ("nnz_discharge", ctypes.c_int64),
+        ("nnz_factors", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -104,6 +111,8 @@ class AdjointOptions(ctypes.Structure):
         ("damping", ctypes.c_double),
         ("verbose", ctypes.c_int32),
         ("method", ctypes.c_int32),
+        ("accept_unconverged", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -170,7 +179,13 @@ def load():
     return lib
 
 
+class CapacityError(AndfnMiError, IndexError):
+    """A record capacity was exceeded (AnDFN raises IndexError there: andfn/dfn.py:790-791)."""
+
+
 def check(rc):
+    if rc == ECAPACITY:
+        raise CapacityError(rc, load().andfn_mi_last_error().decode())
     if rc < 0:
         raise AndfnMiError(rc, load().andfn_mi_last_error().decode())
     return rc

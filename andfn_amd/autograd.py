@@ -41,10 +41,15 @@ class DFNProblem:
 
     groups: ordered mapping name -> element indices of constant-head elements sharing one head
     obs_frac, obs_z: observation points (fracture index, local complex coordinate)
+    head_tol: the forward solves stop at this head residual (metres); a solve that reaches
+        MAX_ITERATIONS first raises, since the implicit adjoint is the gradient of the fixed point
+    fixed_iterations: instead run every forward solve for exactly MAX_ITERATIONS iterations (the
+        stop rules off; for studies that converge the iteration to rounding, such as finite-
+        difference checks) and accept that state for the adjoint
     """
 
     def __init__(self, dfn, groups, obs_frac=None, obs_z=None, precision="fp64", head_tol=1e-6, adjoint_tol=1e-10,
-                 device=0):
+                 device=0, fixed_iterations=False):
         if dfn.fractures_struc_array_hpc is None:
             dfn.consolidate_dfn()
         self.fa = dfn.fractures_struc_array_hpc
@@ -57,6 +62,9 @@ class DFNProblem:
         self.precision = precision
         self.head_tol = head_tol
         self.adjoint_tol = adjoint_tol
+        self.fixed_iterations = bool(fixed_iterations)
+        if self.fixed_iterations and float(np.asarray(dfn.constants)["MAX_ERROR"].reshape(-1)[0]) != 0.0:
+            raise ValueError("fixed_iterations needs MAX_ERROR = 0 (AnDFN's change-based stop rule off)")
         self.t0 = self.fa["t"].astype(np.float64).copy()
         ty = self.ea["_type"]
         fr0 = self.ea["frac0"]
@@ -78,7 +86,12 @@ class DFNProblem:
         log10_t = np.asarray(log10_t, dtype=np.float64)
         group_heads = np.asarray(group_heads, dtype=np.float64)
         self.net.set_parameters(t=10.0 ** log10_t, head=self._heads(group_heads))
-        self.last_info = self.net.run(self.precision, head_tol=self.head_tol)
+        self.last_info = self.net.run(self.precision, head_tol=0.0 if self.fixed_iterations else self.head_tol)
+        if not self.last_info["converged"] and not self.fixed_iterations:
+            self._key = None
+            raise RuntimeError(f"forward solve stopped at MAX_ITERATIONS ({self.last_info['nit']}) with head residual "
+                               f"{self.last_info['head_residual']:.3e} m above head_tol {self.head_tol:g}: the implicit "
+                               f"adjoint needs the fixed point")
         q, _ = self.net.state_discharges()
         Q = np.array([q[idx].sum() for idx in self.groups])
         h = self.net.state_heads(self.obs_frac, self.obs_z) if len(self.obs_frac) else np.zeros(0)
@@ -98,7 +111,7 @@ class DFNProblem:
         gt, gh, info = self.net.adjoint(dq=dq, obs_frac=self.obs_frac if len(self.obs_frac) else None,
                                         obs_z=self.obs_z if len(self.obs_frac) else None,
                                         dhead=np.asarray(gH, dtype=np.float64) if len(self.obs_frac) else None,
-                                        tol=self.adjoint_tol)
+                                        tol=self.adjoint_tol, accept_unconverged=self.fixed_iterations)
         self.last_adjoint_info = info
         g_log10_t = gt * (10.0 ** log10_t) * LN10
         g_heads = np.array([gh[idx].sum() for idx in self.groups])

@@ -25,6 +25,7 @@
 #include <stdint.h>
 
 #include "aem_kernels.hpp"
+#include "lu_pivot_key.hpp"
 
 namespace aem {
 
@@ -32,6 +33,7 @@ namespace {
 
 constexpr int kBW = kLuPanel;       // panel width
 constexpr int kRows = kLuGroupRows; // rows per tournament group (one per thread)
+static_assert(kRows <= (1 << (kPivotIdxBits - 1)), "the pivot key holds the thread index of at most 512 rows");
 constexpr int kWaves = kRows / 64;
 
 // y = max(x, x of the lane that DPP control CTRL selects), on 64-bit keys
@@ -115,9 +117,9 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
 #pragma unroll
   for (int c = 0; c < kBW; ++c) a[c] = (active && c < bw) ? A[row + (j0 + c) * lda] : 0.0;
   int npiv = 0, mypiv = -1;
-  // pivot key: float bits of |a[k]| (monotone for non-negative values; a NaN ranks above every
-  // number, so a NaN row is still taken and the winner list stays full) over the inverted
-  // thread index (ties go to the lower thread); 0 marks a row that is already a pivot.
+  // pivot key (lu_pivot_key.hpp): the FP64 bits of |a[k]| with the inverted thread index in
+  // the low bits, so one 64-bit max picks the largest magnitude (ties to the lower thread);
+  // 0 marks a row that is already a pivot.
   // The 64 steps run as a loop over 8 blocks of 8 columns: the block's columns are copied into
   // cur[] and its 8 steps are unrolled, so every register index is static (the row stays in
   // registers) while the code stays ~2k instructions; fully unrolled, the 64 steps take ~140 KB
@@ -139,9 +141,7 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
       const int k = kb * kCB + q;
       if (k < bw) {
         unsigned long long key = 0ull;
-        if (active)
-          key = ((unsigned long long)__float_as_uint((float)fabs(cur[q])) << 32) |
-                (unsigned long long)(0xFFFFFFFFu - (unsigned)t);
+        if (active) key = lu_pivot_key(cur[q], t);
         // every row's reciprocal is formed while the pivot search runs, so the winner only
         // stores its own (the division is off the path between the two barriers)
         const double rinv = cur[q] != 0.0 ? 1.0 / cur[q] : 0.0;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kRows) void tslu_kernel(const double* __restrict__ 
         }
         if (best != 0ull) {
           npiv = k + 1;
-          if (t == (int)(0xFFFFFFFFu - (unsigned)best)) {
+          if (t == lu_pivot_thread(best)) {
 #pragma unroll
             for (int q2 = 0; q2 < kCB; ++q2) prow[kb * kCB + q2] = cur[q2];
 #pragma unroll

@@ -270,6 +270,7 @@ class Solver {
   DBuf<double> d_rowscale_, d_ay_, d_bcmax_, d_csr_val_;
   DBuf<int> d_csr_ptr_, d_csr_col_;
   int64_t nnz_A_ = 0;
+  int64_t nnz_factors_ = 0;  // stored entries of the LU factors
   bool csr_ready_ = false;  // the CSR pattern of A (d_csr_ptr_, d_csr_col_) is built
   // discharge-system layout: one dense block per connected component of the network (discharge
   // elements in element order, then fracture constants, then padding), the blocks grouped into
@@ -451,6 +452,8 @@ class Solver {
     DBuf<int> fpb, fpe;  // [F] point range of every fracture
   };
   bool have_state_ = false, adjT_valid_ = false;
+  bool converged_ = false;  // the last run met its stop rule (head tolerance or AnDFN's two-hit rule)
+  DBuf<aem::IterScalars> d_hist_;  // per-iteration scalars for the iteration log
   std::vector<double> head_;  // prescribed head of constant-head elements (phi / t)
   AdjTables tab_full_, tab_rhs_, tab_obs_;
   DBuf<int> d_eslot_ptr_, d_gitems_, d_xsrc_, d_xptr_;
@@ -1093,6 +1096,7 @@ void Solver::assemble_and_factor() {
         fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (block %d, zero pivot at %lld)", b,
              (long long)info[b]);
   }
+  nnz_factors_ = Atot_;  // dense blocks: L and U share the n^2 entries of each block
   solve32_ = false;
   a64_blocked_ = false;
   if (precision_ == ANDFN_MI_MIXED) {
@@ -1604,6 +1608,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const int64_t sched_n = opt ? opt->schedule_iterations : 0;
   int32_t* record = opt ? opt->growth_record : nullptr;
   const int64_t record_n = opt ? opt->record_iterations : 0;
+  double* log_out = opt ? opt->iteration_log : nullptr;
+  const int64_t log_n = log_out ? std::min<int64_t>(opt->log_iterations, max_iterations) : 0;
+  std::vector<std::pair<int, int>> grow_events;  // (element, iteration) of every growth, for the log
   const double head_tol = opt ? opt->head_tol : 0.0;
   double resid = -1.0;
   int resid_row = -1;
@@ -1620,7 +1627,9 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       fail(ANDFN_MI_EINVAL, "element %d: circle radius must be positive", e);
 
   have_state_ = false;
+  converged_ = false;
   restore();
+  if (log_n > 0) d_hist_.alloc((size_t)log_n);
   const int nde = (int)de_.size();
   const int n = (int)N_;  // unknowns of the block layout (padding included)
   aem::NetView nv = net();
@@ -1846,6 +1855,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
         grown[e] = (char)grow_h[e];
         if (grow_h[e]) {
           ++ng;
+          if (log_n > 0) grow_events.emplace_back(e, nit);
           ncoef_[e] += coef_increase;
           nint_[e] = 2 * ncoef_[e];
           if (is_circle_like(etype_[e])) gcirc.push_back(e);
@@ -1882,6 +1892,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       aem::launch_blend(nv, E_, damping, st_);
     }
     aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_red_.p, d_scal_.p, st_);
+    if (nit - 1 < log_n)
+      HIP_CHECK(hipMemcpyAsync(d_hist_.p + (nit - 1), d_scal_.p, sizeof(aem::IterScalars), hipMemcpyDeviceToDevice, st_));
     if (record && nit - 1 < record_n) std::copy(ncoef_.begin(), ncoef_.end(), record + (size_t)(nit - 1) * E_);
     if (lazy && nit < max_iterations) {
       times_pending = nit;
@@ -1939,6 +1951,29 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   HIP_CHECK(hipMemcpyAsync(&status, d_status_.p, sizeof(int), hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
   if (status & 1) fail(ANDFN_MI_ECAPACITY, "branch-cut crossing list overflow");
+  if (log_n > 0) {
+    // AnDFN's log line per iteration: errors, the element of the largest error and its
+    // coefficient count at that iteration (initial count plus the growths up to it)
+    const int64_t rows = std::min<int64_t>(nit, log_n);
+    std::vector<aem::IterScalars> hist((size_t)std::max<int64_t>(rows, 1));
+    if (rows > 0)
+      HIP_CHECK(hipMemcpyAsync(hist.data(), d_hist_.p, sizeof(aem::IterScalars) * rows, hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    std::sort(grow_events.begin(), grow_events.end());
+    for (int64_t k = 0; k < rows; ++k) {
+      const aem::IterScalars& h = hist[k];
+      const int e = std::min(std::max(h.argmax, 0), E_ - 1);
+      const auto lo = std::lower_bound(grow_events.begin(), grow_events.end(), std::make_pair(e, 0));
+      const auto hi = std::upper_bound(grow_events.begin(), grow_events.end(), std::make_pair(e, (int)k + 1));
+      double* row = log_out + k * ANDFN_MI_LOG_FIELDS;
+      row[0] = h.error;
+      row[1] = h.error_coef;
+      row[2] = h.dq_max / (h.yold_max + 1e-16);
+      row[3] = e;
+      row[4] = ncoef0_[e] + (double)coef_increase * (double)(hi - lo);
+      row[5] = etype_[e];
+    }
+  }
   auto t_end = clk::now();
   if (info) {
     info->nit = nit;
@@ -1970,10 +2005,13 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->refine_rate = precision_ == ANDFN_MI_MIXED ? refine_rate_ : 0.0;
     info->sweep_points_per_lane = ppl_[0];
     info->rhs_points_per_lane = ppl_[1];
+    info->nnz_discharge = nnz_A_;
+    info->nnz_factors = nnz_factors_;
   }
   mixed_phase_ = false;
   have_state_ = true;
-  return (cnt_error < 2 && !stopped_on_head) ? ANDFN_MI_NOT_CONVERGED : ANDFN_MI_OK;
+  converged_ = cnt_error >= 2 || stopped_on_head;
+  return converged_ ? ANDFN_MI_OK : ANDFN_MI_NOT_CONVERGED;
 }
 
 // copy the device state back into the caller's records (fields the reference mutates)
@@ -2156,6 +2194,10 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
   if (!have_state_) fail(ANDFN_MI_EINVAL, "no solved state on this handle: run a solve first");
+  if (!converged_ && !(opt && opt->accept_unconverged))
+    fail(ANDFN_MI_EINVAL,
+         "the last run stopped at MAX_ITERATIONS without meeting its stop rule: the implicit adjoint is the gradient "
+         "of the fixed point (set accept_unconverged to use this state anyway)");
   if (lu_mode_ != ANDFN_MI_LU_TRSV) fail(ANDFN_MI_EINVAL, "the adjoint uses the blocked triangular solves (lu_mode 0)");
   if (!grad_t || !grad_head) fail(ANDFN_MI_EINVAL, "null gradient pointer");
   const double tol = (opt && opt->tol > 0) ? opt->tol : 1e-10;

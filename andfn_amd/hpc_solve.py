@@ -23,6 +23,31 @@ logger = logging.getLogger("andfn")
 _last_info = None
 
 
+def constant(constants, name):
+    """Scalar field ``name`` of a dtype_constants record (0-d, or a one-element array)."""
+    return np.asarray(constants)[name].reshape(-1)[0].item()
+
+
+def _log_solve(nf, info, log_rows, error_q_final):
+    """AnDFN's log lines of one solve on the ``andfn`` logger (hpc_solve.py:140-141, 198-216, 242-255)."""
+    logger.info(f"Number of fractures: {nf}")
+    logger.info(f"Number of entries in discharge matrix: {info.nnz_discharge}")
+    for k, row in enumerate(log_rows):
+        nit = k + 1
+        error, error_coef, error_q, el, nc, ty = row
+        it = f"0{nit}" if nit < 10 else f"{nit}"
+        logger.info(f"Iteration: {it}, Error E: {error:.3e}, Coef: {error_coef:.3e}, Q: {error_q:.3e}, "
+                    f"Element: {int(el)}, N of Coef: {int(nc)}, Type: {int(ty)}")
+    logger.info("---------------------------------------")
+    logger.info("Solver results")
+    logger.info("---------------------------------------")
+    logger.info(f"Iterations: {info.nit}, Error E: {info.error:.3e}, Q: {error_q_final:.3e}, ")
+    days, rem = divmod(info.t_total, 86400)
+    hours, rem = divmod(rem, 3600)
+    minutes, seconds = divmod(rem, 60)
+    logger.info(f"Solve time: {int(days)} days, {int(hours)} hours, {int(minutes)} minutes, {seconds:.2f} seconds\n")
+
+
 def last_solve_info():
     """Dict of the last solve's iteration count, errors, timings and kernel counters."""
     return dict(_last_info) if _last_info is not None else None
@@ -57,11 +82,17 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
             raise ValueError("growth_schedule must have shape (iterations, n_elements)")
         opt.growth_schedule = sched.ctypes.data
         opt.schedule_iterations = sched.shape[0]
+    max_iterations = int(constant(consts, "MAX_ITERATIONS"))
     rec = None
     if record_growth:
-        rec = np.zeros((int(consts["MAX_ITERATIONS"]), len(element_struc_array)), dtype=np.int32)
+        rec = np.zeros((max(max_iterations, 0), len(element_struc_array)), dtype=np.int32)
         opt.growth_record = rec.ctypes.data
         opt.record_iterations = rec.shape[0]
+    log = None
+    if logger.isEnabledFor(logging.INFO):
+        log = np.zeros((max(max_iterations, 0), _lib.LOG_FIELDS), dtype=np.float64)
+        opt.iteration_log = log.ctypes.data
+        opt.log_iterations = log.shape[0]
     info = _lib.Info()
     rc = lib.andfn_mi_solve(_lib.ptr(fracture_struc_array), len(fracture_struc_array), me,
                             _lib.ptr(element_struc_array), len(element_struc_array), m, int(discharge_int),
@@ -73,9 +104,13 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
     _last_info["converged"] = rc == _lib.OK
     if rec is not None:
         _last_info["ncoef_history"] = rec[: info.nit].copy()
+    if log is not None:
+        # AnDFN sets error_q to 1e30 on each stop-rule hit (hpc_solve.py:237-239), so a run ended
+        # by that rule logs 1e30
+        eq = 1e30 if (rc == _lib.OK and head_tol <= 0.0) else info.error_q
+        _log_solve(len(fracture_struc_array), info, log[: info.nit], eq)
     if rc == _lib.NOT_CONVERGED:
         logger.warning(f"solve stopped at MAX_ITERATIONS ({info.nit}) without meeting the tolerance")
-    logger.info(f"Iterations: {info.nit}, Error E: {info.error:.3e}, Q: {info.error_q:.3e}, ")
     return element_struc_array
 
 
@@ -98,7 +133,7 @@ def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0,
     if factor not in _lib.FACTORS:
         raise ValueError(f"factor must be one of {sorted(_lib.FACTORS)}")
     return _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
-                        int(lu_mode), 0.0, None, 0, None, 0, float(head_tol), _lib.FACTORS[factor], 0)
+                        int(lu_mode), 0.0, None, 0, None, 0, float(head_tol), _lib.FACTORS[factor], 0, None, 0)
 
 
 class DeviceNetwork:
@@ -162,10 +197,12 @@ class DeviceNetwork:
         return out
 
     def adjoint(self, dq=None, dconstant=None, obs_frac=None, obs_z=None, dhead=None, tol=1e-10,
-                max_iterations=5000, damping=0.75, verbose=False, method="gmres"):
+                max_iterations=5000, damping=0.75, verbose=False, method="gmres", accept_unconverged=False):
         """Gradient of J = dq.q + dconstant.C + dhead.h(obs) at the last run's state with respect
         to every fracture transmissivity and every prescribed head (implicit adjoint of the
-        fixed point).  Returns (dJ/dt [F], dJ/dhead [E], info)."""
+        fixed point).  Returns (dJ/dt [F], dJ/dhead [E], info).  A run that stopped at
+        MAX_ITERATIONS without meeting its stop rule is refused unless ``accept_unconverged``
+        (the adjoint is the gradient of the fixed point, not of the iterate)."""
         keep = []
 
         def arr(a, dtype, n=None):
@@ -185,7 +222,8 @@ class DeviceNetwork:
         if method not in ("gmres", "fixed_point"):
             raise ValueError("method must be 'gmres' or 'fixed_point'")
         opt = _lib.AdjointOptions(float(tol), int(max_iterations), float(damping), 1 if verbose else 0,
-                                  _lib.ADJ_GMRES if method == "gmres" else _lib.ADJ_FIXED_POINT)
+                                  _lib.ADJ_GMRES if method == "gmres" else _lib.ADJ_FIXED_POINT,
+                                  1 if accept_unconverged else 0, 0)
         gt = np.zeros(len(self.fa))
         gh = np.zeros(len(self.ea))
         info = _lib.AdjointInfo()

@@ -72,7 +72,14 @@ typedef struct andfn_mi_options {
   double head_tol;
   int32_t factor;        /* ANDFN_MI_FACTOR_* */
   int32_t reserved;      /* zero */
+  /* Optional output: the fields of AnDFN's per-iteration log line (hpc_solve.py:198-216), one row of
+   * ANDFN_MI_LOG_FIELDS doubles per iteration for log_iterations rows: error, error_coef, error_q,
+   * the element of the largest error, its ncoef at that iteration and its type. */
+  double* iteration_log;
+  int64_t log_iterations;
 } andfn_mi_options;
+
+#define ANDFN_MI_LOG_FIELDS 6
 
 typedef struct andfn_mi_info {
   int64_t nit;            /* outer iterations taken (hpc_solve.py:152) */
@@ -113,6 +120,8 @@ typedef struct andfn_mi_info {
                            * factors (above 0.25 the run solves with the FP64 factors throughout) */
   int32_t sweep_points_per_lane;  /* tile shape the autotuner chose for the collocation sweep (1 or 2) */
   int32_t rhs_points_per_lane;    /* and for the right-hand-side sweep (0: the basis product ran instead) */
+  int64_t nnz_discharge;  /* structural nonzeros of the discharge matrix (hpc_solve.py:140-141 logs getnnz()) */
+  int64_t nnz_factors;    /* entries of the stored LU factors of the discharge matrix (L and U together) */
 } andfn_mi_info;
 
 /*
@@ -186,6 +195,11 @@ typedef struct andfn_mi_adjoint_options {
   double damping;           /* fixed point: 0 means 0.75, the forward's effective damping (hpc_solve.py:383-388, 561-575) */
   int32_t verbose;          /* >0: progress lines on stderr */
   int32_t method;           /* ANDFN_MI_ADJ_* */
+  /* 0: refuse (ANDFN_MI_EINVAL) a state whose run did not meet its stop rule, since the implicit
+   * adjoint is the gradient of the fixed point only; nonzero: accept it (e.g. a run of a fixed
+   * iteration count known to be converged) */
+  int32_t accept_unconverged;
+  int32_t reserved;         /* zero */
 } andfn_mi_adjoint_options;
 
 typedef struct andfn_mi_adjoint_info {

@@ -31,6 +31,9 @@ from pathlib import Path
 
 import numpy as np
 
+# importing the reference must leave its checkout as it is: no __pycache__ written there
+sys.dont_write_bytecode = True
+
 HERE = Path(__file__).resolve().parent
 sys.path.insert(0, str(HERE.parents[1]))
 

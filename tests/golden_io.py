@@ -49,3 +49,39 @@ def compare(g, fa, ea):
     out["q"] = float(np.max(np.abs(ea["q"] - g["out_q"])) / np.max(np.abs(g["out_q"])))
     out["constant"] = float(np.max(np.abs(fa["constant"] - g["out_constant"])) / np.max(np.abs(g["out_constant"])))
     return out
+
+
+def compare_errors(g, ea):
+    """The in-place error outputs of a solve against a golden run (hpc_math_functions.py:611-670,
+    406-447):
+
+    * ``error_coef`` (mean |c_j - c_old_j| per element) and ``dpsi_corr`` (branch-cut corrections of
+      the bounding circles): max absolute difference relative to the largest golden coefficient and
+      the largest golden correction; both are differences or sums of coefficient-sized terms, so
+      their rounding floor is set by those magnitudes;
+    * ``error`` = | |sum c[:0.9m]| / |sum c| - 1 | per element: the largest ratio of the difference
+      to its first-order conditioning bound m * delta * (|S| + |H|) / |S|^2 at a coefficient
+      agreement delta = 1e-9 of the largest golden coefficient (S = sum c, H = sum c[:0.9m] of the
+      golden coefficients); below 1 means the difference is what 1e-9 coefficient agreement allows.
+    """
+    gc = g["out_coef"]
+    m = g["out_ncoef"]
+    scale = float(np.max(np.abs(gc)))
+    out = {}
+    out["error_coef"] = float(np.max(np.abs(ea["error_coef"] - g["out_error_coef"])) / scale)
+    gd = g["out_dpsi_corr"]
+    dscale = float(np.max(np.abs(gd)))
+    out["dpsi_corr"] = float(np.max(np.abs(ea["dpsi_corr"][:, : gd.shape[1]] - gd)) / (dscale if dscale > 0 else 1.0))
+    worst = 0.0
+    ge = g["out_error"]
+    for i in range(len(ge)):
+        mi = int(m[i])
+        c = gc[i, :mi]
+        S = c.sum()
+        H = c[: int(0.9 * mi)].sum()
+        if mi < 1 or abs(S) == 0.0:
+            continue
+        bound = 1e-9 * scale * mi * (abs(S) + abs(H)) / abs(S) ** 2
+        worst = max(worst, abs(float(ea["error"][i]) - float(ge[i])) / bound)
+    out["error_ratio"] = worst
+    return out

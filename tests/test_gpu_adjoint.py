@@ -51,13 +51,14 @@ def problem():
     d = _build("cfg1", MAX_ERROR=0.0, MAX_ITERATIONS=3000)
     groups = boundary_groups(d)
     of, oz = _obs(d.fractures_struc_array_hpc, NOBS, 5)
-    p = DFNProblem(d, groups, of, oz, head_tol=0.0, adjoint_tol=1e-11)
+    p = DFNProblem(d, groups, of, oz, head_tol=0.0, adjoint_tol=1e-11, fixed_iterations=True)
     lt = np.log10(p.t0)
     gh = np.array([1.0, 0.0])
     Q, h = p.evaluate(lt, gh)
     dq = np.zeros(len(p.ea))
     dq[p.groups[0]] = 1.0
-    gt, ghd, info = p.net.adjoint(dq=dq, obs_frac=of, obs_z=oz, dhead=np.ones(NOBS), tol=1e-11)
+    gt, ghd, info = p.net.adjoint(dq=dq, obs_frac=of, obs_z=oz, dhead=np.ones(NOBS), tol=1e-11,
+                                  accept_unconverged=True)
     assert info["converged"], info
     yield dict(p=p, lt=lt, gh=gh, Q=Q, h=h, dq=dq, gt=gt, ghd=ghd, info=info, of=of, oz=oz)
     p.close()
@@ -77,7 +78,7 @@ def test_gmres_matches_transposed_fixed_point(problem):
     p = problem["p"]
     p.evaluate(problem["lt"], problem["gh"])
     gt2, gh2, info = p.net.adjoint(dq=problem["dq"], obs_frac=problem["of"], obs_z=problem["oz"],
-                                   dhead=np.ones(NOBS), tol=1e-12, method="fixed_point")
+                                   dhead=np.ones(NOBS), tol=1e-12, method="fixed_point", accept_unconverged=True)
     assert info["converged"]
     s = np.max(np.abs(problem["gt"]))
     assert np.max(np.abs(gt2 - problem["gt"])) <= 1e-8 * s
@@ -163,3 +164,28 @@ def test_adjoint_needs_a_solved_state():
             net.adjoint(dq=np.ones(len(d.elements_struc_array_hpc)))
     finally:
         net.close()
+
+
+def test_unconverged_forward_is_refused():
+    """A forward run that stops at MAX_ITERATIONS above its head tolerance is not the fixed point:
+    the C ABI refuses its adjoint (unless the caller accepts that state) and the torch problem
+    raises."""
+    print("adjoint refusal dfn40, 3 iterations", flush=True)
+    d = _build("dfn40", MAX_ITERATIONS=3)
+    fa, ea = d.fractures_struc_array_hpc, d.elements_struc_array_hpc
+    net = DeviceNetwork(fa, ea, d.discharge_int, d.constants)
+    try:
+        info = net.run(head_tol=1e-9)
+        assert not info["converged"]
+        with pytest.raises(_lib.AndfnMiError, match="fixed point"):
+            net.adjoint(dq=np.ones(len(ea)))
+        g, h, ai = net.adjoint(dq=np.ones(len(ea)), accept_unconverged=True)
+        assert np.all(np.isfinite(g))
+    finally:
+        net.close()
+    p = DFNProblem(d, boundary_groups(d), head_tol=1e-9)
+    try:
+        with pytest.raises(RuntimeError, match="fixed point"):
+            p.evaluate(np.log10(p.t0), np.array([1.0, 0.0]))
+    finally:
+        p.close()

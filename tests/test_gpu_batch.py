@@ -66,7 +66,7 @@ def test_batched_adjoint_matches_separate_adjoints(batch):
     net = DeviceNetwork(fa.copy(), ea.copy(), K, c)
     try:
         net.run()
-        gt, gh, info = net.adjoint(dq=dq, tol=1e-12)
+        gt, gh, info = net.adjoint(dq=dq, tol=1e-12, accept_unconverged=True)
         assert info["converged"]
     finally:
         net.close()
@@ -78,7 +78,7 @@ def test_batched_adjoint_matches_separate_adjoints(batch):
         n1 = DeviceNetwork(f1, e1, K, c)
         try:
             n1.run()
-            g1, h1, i1 = n1.adjoint(dq=d1, tol=1e-12)
+            g1, h1, i1 = n1.adjoint(dq=d1, tol=1e-12, accept_unconverged=True)
         finally:
             n1.close()
         s = np.max(np.abs(g1))

@@ -10,14 +10,18 @@ sweeps of ~0.5 s), so the GPU's converged state is checked instead:
 * the continuity rows of the discharge system (every fracture's discharges sum to zero) hold to
   rounding;
 * two FP64 runs give bit-identical results (every reduction has a fixed order);
-* the mixed form stops at the same head residual within the errors stated for it.
+* the mixed form stops at the same head residual within the errors stated for it;
+* values: 160 iterations at the fixed coefficient counts (AnDFN's change-based stop off), the
+  iteration count the head-residual stop takes on this network, give coefficients, discharges
+  and fracture constants within 1e-9 of the oracle's 160 iterations (relative to the largest
+  magnitude of each quantity), with identical counts.
 """
 
 import numpy as np
 import pytest
 
 from andfn_amd.generator import build_network, workload
-from andfn_amd.hpc_solve import DeviceNetwork
+from andfn_amd.hpc_solve import DeviceNetwork, last_solve_info, solve
 from oracle import oracle
 
 pytestmark = pytest.mark.gpu
@@ -121,3 +125,29 @@ def test_mixed_form_against_fp64(cfg2, fp64):
     assert errs["coef"] < 1e-7 and errs["q"] < 1e-7 and errs["constant"] < 1e-9
     head, _, _ = _oracle_rows(cfg2, fam, eam)
     assert head.max() <= HEAD_TOL * (1 + 1e-6)
+
+
+def test_fixed_count_values_match_the_oracle(cfg2):
+    consts = np.array(cfg2.constants, copy=True)
+    consts["MAX_ERROR"] = 0.0
+    consts["MAX_ITERATIONS"] = 160
+    fa = cfg2.fractures_struc_array_hpc.copy()
+    ea = cfg2.elements_struc_array_hpc.copy()
+    fo, eo = fa.copy(), ea.copy()
+    print("cfg2: GPU, 160 iterations", flush=True)
+    solve(fa, ea, cfg2.discharge_int, consts)
+    info = last_solve_info()
+    print("cfg2: oracle, 160 iterations", flush=True)
+    oinfo = {}
+    oracle.solve(fo, eo, cfg2.discharge_int, consts, info=oinfo)
+    assert info["nit"] == oinfo["nit"] == 160
+    assert np.array_equal(ea["ncoef"], eo["ncoef"]) and np.array_equal(ea["nint"], eo["nint"])
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+    errs = dict(coef=rel(ea["coef"], eo["coef"]), q=rel(ea["q"], eo["q"]),
+                constant=rel(fa["constant"], fo["constant"]))
+    print(f"cfg2 160 iterations, GPU vs oracle: {errs}; head residual GPU {info['head_residual']:.3e}, "
+          f"oracle {oinfo['head_residual']:.3e}")
+    assert errs["coef"] < 1e-9 and errs["q"] < 1e-9 and errs["constant"] < 1e-9, errs

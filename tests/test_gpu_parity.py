@@ -48,6 +48,10 @@ def test_gpu_replays_andfn_golden(case):
         assert np.array_equal(info["ncoef_history"], sched)
     assert d["ncoef_equal"] and d["nint_equal"], d
     assert d["coef"] < TOL and d["q"] < TOL and d["constant"] < TOL, d
+    # the in-place error outputs (hpc_math_functions.py:611-670) and branch-cut corrections (406-447)
+    e = golden_io.compare_errors(g, ea)
+    print(f"{case}: error fields {e}")
+    assert e["error_coef"] < TOL and e["dpsi_corr"] < TOL and e["error_ratio"] < 1.0, e
 
 
 @pytest.mark.parametrize("name", ["dfn40", "dfn40w", "dfn200"])

@@ -38,6 +38,8 @@ def test_oracle_replays_andfn_trajectory(case):
     d = golden_io.compare(g, fa, ea)
     assert d["ncoef_equal"] and d["nint_equal"]
     assert d["coef"] < 1e-9 and d["q"] < 1e-9 and d["constant"] < 1e-9, d
+    e = golden_io.compare_errors(g, ea)
+    assert e["error_coef"] < 1e-9 and e["dpsi_corr"] < 1e-9 and e["error_ratio"] < 1.0, e
 
 
 @pytest.mark.parametrize("case", CASES)
