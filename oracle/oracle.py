@@ -108,13 +108,17 @@ class OracleHandle:
 
 
 def solve(fracture_struc_array, element_struc_array, discharge_int, constants, ntype_elements=None, info=None,
-          growth_schedule=None):
+          growth_schedule=None, refine=0):
     """CPU restatement of ``andfn.hpc.hpc_solve.solve``; mutates and returns the element array.
 
     ``info`` (a dict, optional) receives nit, the per-iteration (error, error_coef, error_q)
     history, every element's ncoef after each iteration, and phase timings.
     ``growth_schedule`` (int32 [iterations, n_elements], optional) replays a recorded
     coefficient-count trajectory instead of applying the growth test.
+    ``refine`` > 0 follows every SuperLU solve with that many steps of iterative refinement
+    (x += LU^-1 (b - A x), residual in FP64): the discharges of the exact solve, to separate the
+    iteration's arithmetic from SuperLU's own solve error (not part of the reference, which uses
+    ``lu.solve`` alone: hpc_solve.py:651).
     """
     import scipy.sparse.linalg as spla
 
@@ -151,6 +155,8 @@ def solve(fracture_struc_array, element_struc_array, discharge_int, constants, n
             y_old = y.copy()
             b = O.head_rhs()
             y = lu.solve(b)
+            for _ in range(int(refine)):
+                y = y + lu.solve(b - A @ y)
             O.set_discharges(y)
             error_q = float(np.max(np.abs(y - y_old) / (np.max(np.abs(y_old)) + 1e-16)))
         force = None
