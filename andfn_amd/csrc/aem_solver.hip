@@ -19,6 +19,7 @@
 #include <cmath>
 #include <complex>
 #include <functional>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -195,6 +196,7 @@ class Solver {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) fail(ANDFN_MI_ENODEV, "no HIP device");
     if (dev < 0 || dev >= ndev) fail(ANDFN_MI_EINVAL, "device %d out of range (%d devices)", dev, ndev);
     HIP_CHECK(hipSetDevice(dev));
+    dev_ = dev;
     HIP_CHECK(hipStreamCreate(&st_));
     parse();
     upload();
@@ -225,8 +227,12 @@ class Solver {
   int solve(const andfn_mi_options* opt, andfn_mi_info* info);
   void write_back();
   void heads(int64_t npts, const int64_t* frac, const double* z, double* out);
+  // heads() with every device buffer in the caller-provided scratch region (no allocation)
+  void heads_noalloc(int64_t npts, const int64_t* frac, const double* z, double* out, unsigned char* scratch_dev);
+  int device() const { return dev_; }
 
  private:
+  int dev_ = 0;
   Layout L_;
   char* fr_;
   char* el_;
@@ -2091,6 +2097,60 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
   for (int64_t k = 0; k < npts; ++k) out[order[k]] = v[k];
 }
 
+// ---- heads at points without allocation (andfn_mi_state_heads_noalloc) ----------------------
+// Device scratch: the points' coordinates, their heads and the 64-point tiles (at most one tile
+// per point), in a region given by the caller of heads_noalloc; the host staging arrays below are
+// static (.bss), so nothing is allocated on either side.  One call at a time (a mutex in the entry
+// point): every call shares them.
+constexpr int64_t kNoAllocMaxPoints = 65536;
+constexpr size_t kNoAllocVal = 16 * kNoAllocMaxPoints;                          // double2 z[max] at 0
+constexpr size_t kNoAllocTiles = kNoAllocVal + 8 * kNoAllocMaxPoints;           // double val[max]
+constexpr size_t kNoAllocBytes = kNoAllocTiles + sizeof(Tile) * kNoAllocMaxPoints;  // Tile tiles[max]
+static_assert(kNoAllocVal % 256 == 0 && kNoAllocTiles % 256 == 0, "scratch sections stay 256-byte aligned");
+int64_t g_na_order[kNoAllocMaxPoints];
+double2 g_na_z[kNoAllocMaxPoints];
+Tile g_na_tiles[kNoAllocMaxPoints];
+double g_na_val[kNoAllocMaxPoints];
+
+void Solver::heads_noalloc(int64_t npts, const int64_t* frac, const double* z, double* out,
+                           unsigned char* scratch_dev) {
+  if (npts < 0 || npts > kNoAllocMaxPoints)
+    fail(ANDFN_MI_ECAPACITY, "%lld points exceed the allocation-free limit of %lld per call", (long long)npts,
+         (long long)kNoAllocMaxPoints);
+  if (npts == 0) return;
+  for (int64_t i = 0; i < npts; ++i) {
+    if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "point %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
+    g_na_order[i] = i;
+  }
+  // points grouped by fracture, in input order within a fracture (the order heads() uses); std::sort
+  // on the (fracture, index) pair needs no buffer, unlike a stable sort
+  std::sort(g_na_order, g_na_order + npts,
+            [&](int64_t a, int64_t b) { return frac[a] != frac[b] ? frac[a] < frac[b] : a < b; });
+  int ntiles = 0;
+  for (int64_t k = 0; k < npts;) {
+    const int64_t f = frac[g_na_order[k]];
+    const int64_t b = k;
+    while (k < npts && frac[g_na_order[k]] == f) {
+      g_na_z[k] = make_double2(z[2 * g_na_order[k]], z[2 * g_na_order[k] + 1]);
+      ++k;
+    }
+    for (int64_t p0 = b; p0 < k; p0 += 64) g_na_tiles[ntiles++] = Tile{(int)f, 0, 0, (int)p0, (int)k};
+  }
+  double2* dz = (double2*)scratch_dev;
+  double* dv = (double*)(scratch_dev + kNoAllocVal);
+  Tile* dt = (Tile*)(scratch_dev + kNoAllocTiles);
+  HIP_CHECK(hipMemcpyAsync(dz, g_na_z, sizeof(double2) * npts, hipMemcpyHostToDevice, st_));
+  HIP_CHECK(hipMemcpyAsync(dt, g_na_tiles, sizeof(Tile) * ntiles, hipMemcpyHostToDevice, st_));
+  aem::TargetView tv{};
+  tv.tiles = dt;
+  tv.pt_z = dz;
+  aem::launch_sweep(aem::MODE_POINTS, net(), tv, ntiles, 1, dv, st_);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(g_na_val, dv, sizeof(double) * npts, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipStreamSynchronize(st_));
+  for (int64_t k = 0; k < npts; ++k) out[g_na_order[k]] = g_na_val[k];
+}
+
 // ---------------------------------------------------------------------------------------------
 // Implicit adjoint
 // ---------------------------------------------------------------------------------------------
@@ -2628,6 +2688,29 @@ struct andfn_mi_handle {
   Solver* s;
 };
 
+// The allocation-free entry point's device scratch: one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself to a kernel or copy (its host-side name is not a device
+// address): its address on each device comes from hipGetSymbolAddress, taken once per device with
+// that device current, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+unsigned char* g_scratch_ptr[kNoAllocMaxDevices] = {};
+std::mutex g_noalloc_mu;
+
+unsigned char* scratch_on_device(int dev) {
+  if (dev < 0 || dev >= kNoAllocMaxDevices) fail(ANDFN_MI_EINVAL, "device %d out of range for the scratch table", dev);
+  if (!g_scratch_ptr[dev]) {
+    HIP_CHECK(hipSetDevice(dev));
+    void* p = nullptr;
+    HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) fail(ANDFN_MI_EHIP, "no device address for the heads scratch on device %d", dev);
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" {
 
 int andfn_mi_create(void* fractures, int64_t n_fractures, int64_t max_elements, void* elements, int64_t n_elements,
@@ -2713,6 +2796,18 @@ int andfn_mi_state_heads(andfn_mi_handle* h, int64_t n_points, const int64_t* fr
     if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
     if (n_points > 0 && (!frac || !z || !heads)) fail(ANDFN_MI_EINVAL, "null pointer");
     if (n_points > 0) h->s->heads(n_points, frac, z, heads);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_heads_noalloc(andfn_mi_handle* h, int64_t n_points, const int64_t* frac, const double* z,
+                                 double* heads) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (n_points > 0 && (!frac || !z || !heads)) fail(ANDFN_MI_EINVAL, "null pointer");
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    unsigned char* ptr = scratch_on_device(h->s->device());
+    h->s->heads_noalloc(n_points, frac, z, heads, ptr);
     return ANDFN_MI_OK;
   });
 }

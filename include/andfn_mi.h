@@ -229,6 +229,16 @@ int andfn_mi_set_parameters(andfn_mi_handle* handle, const double* t, const doub
 int andfn_mi_state_heads(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
                          double* heads);
 
+/* andfn_mi_state_heads without allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The points, their heads and the point tiles live in a fixed device
+ * scratch region of the library (2.75 MiB per GPU, held until the process ends; no hipMalloc,
+ * hipFree or allocating library call is made), so one call takes at most 65,536 points
+ * (ANDFN_MI_ECAPACITY beyond).  Calls are serialised by the library; the scratch keeps the last
+ * call's contents. */
+int andfn_mi_state_heads_noalloc(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
+                                 double* heads);
+
 /* Discharges q[n_elements] and fracture constants constant[n_fractures] of the handle's current
  * state (either pointer may be NULL), without writing the records. */
 int andfn_mi_state_discharges(andfn_mi_handle* handle, double* q, double* constant);
