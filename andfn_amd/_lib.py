@@ -29,7 +29,7 @@ LU_TRSV = 0
 LU_GETRS = 1
 LU_INVERSE = 2
 
-FACTORS = {"tournament": 0, "rocsolver": 1}
+FACTORS = {"sparse": 0, "rocsolver": 1, "tournament": 2}
 
 
 class Options(ctypes.Structure):
@@ -87,6 +87,8 @@ class Info(ctypes.Structure):
         ("rhs_points_per_lane", ctypes.c_int32),
         ("nnz_discharge", ctypes.c_int64),
         ("nnz_factors", ctypes.c_int64),
+        ("factor_used", ctypes.c_int32),
+        ("factor_rounds", ctypes.c_int32),
     ]
 
     def as_dict(self):

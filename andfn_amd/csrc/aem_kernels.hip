@@ -835,10 +835,33 @@ void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const in
 // --------------------------------------------------------------------------------------
 // Discharge matrix (fill_discharge_matrix, hpc_solve.py:1117-1201; discharge terms
 // hpc_intersection.py:22-46, hpc_well.py:14-35, hpc_const_head_line.py:14-35)
-// one wavefront per (discharge row, side); dense column-major A (lda = n)
+// one wavefront per (discharge row, side), into the dense column-major blocks or into the values
+// of the matrix's CSR pattern (the same pattern the head residual uses)
 // --------------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg, SysLayout sys, int K,
-                                                      double* __restrict__ A) {
+struct DenseTarget {
+  double* A;
+  SysLayout sys;
+  __device__ double* at(int64_t r, int64_t c) const { return A + mat_at(r, c, sys); }
+};
+
+struct CsrTarget {
+  double* val;
+  const int* ptr;
+  const int* col;
+  // (r, c) is in the pattern by construction: a binary search over the row's sorted columns
+  __device__ double* at(int64_t r, int64_t c) const {
+    int lo = ptr[r], hi = ptr[r + 1];
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (col[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    return val + lo;
+  }
+};
+
+template <class Tg>
+__global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg, SysLayout sys, int K, Tg A) {
   const int s = blockIdx.x;
   const int e = tg.side_elem[s];
   const int which = tg.side_which[s];
@@ -867,20 +890,23 @@ __global__ __launch_bounds__(64) void assemble_kernel(NetView net, TargetView tg
       acc += (sl.kind == KIND_LINE ? sl.sign : 1.0) * kInv2Pi * 0.5 * log_pos(r2);
     }
     acc = wave_sum(acc);
-    if (lane == 0) atomicAdd(&A[mat_at(row, sys.de_pos[src], sys)], rs * acc / (double)K);
+    // at most two contributions meet in one entry (the two sides of an intersection that shares
+    // both fractures with the source), and two-term sums are exact in either order
+    if (lane == 0) atomicAdd(A.at(row, sys.de_pos[src]), rs * acc / (double)K);
   }
   if (lane == 0) {
     if (inter) {
-      if (which == 0) atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys)], 1.0 / net.t[net.efrac0[e]]);
-      else atomicAdd(&A[mat_at(row, sys.fpos[net.efrac1[e]], sys)], -1.0 / net.t[net.efrac1[e]]);
+      if (which == 0) atomicAdd(A.at(row, sys.fpos[net.efrac0[e]]), 1.0 / net.t[net.efrac0[e]]);
+      else atomicAdd(A.at(row, sys.fpos[net.efrac1[e]]), -1.0 / net.t[net.efrac1[e]]);
     } else {
-      atomicAdd(&A[mat_at(row, sys.fpos[net.efrac0[e]], sys)], 1.0);
+      atomicAdd(A.at(row, sys.fpos[net.efrac0[e]]), 1.0);
     }
   }
 }
 
 // fracture continuity rows: sum_e s_e q_e = 0 (hpc_solve.py:1187-1201)
-__global__ void continuity_kernel(NetView net, SysLayout sys, int nf, double* __restrict__ A) {
+template <class Tg>
+__global__ void continuity_kernel(NetView net, SysLayout sys, int nf, Tg A) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   const int64_t row = sys.fpos[f];
@@ -889,8 +915,18 @@ __global__ void continuity_kernel(NetView net, SysLayout sys, int nf, double* __
     const int ty = net.etype[e];
     if (!(ty == 0 || ty == 2 || ty == 3)) continue;
     const double v = (ty != 0 || net.efrac0[e] == f) ? 1.0 : -1.0;
-    A[mat_at(row, sys.de_pos[e], sys)] += v;
+    *A.at(row, sys.de_pos[e]) += v;
   }
+}
+
+// row equilibration scales of a CSR matrix: rs[r] = 1 / max_k |A[r, k]| (1 for an empty row)
+__global__ void csr_row_scale_kernel(int64_t n, const int* __restrict__ ptr, const double* __restrict__ val,
+                                     double* __restrict__ rs) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double m = 0.0;
+  for (int q = ptr[r]; q < ptr[r + 1]; ++q) m = fmax(m, fabs(val[q]));
+  rs[r] = m > 0.0 ? 1.0 / m : 1.0;
 }
 
 // identity rows/columns of the padding unknowns of every block
@@ -1444,8 +1480,23 @@ void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const 
 
 void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
                      double* A, hipStream_t st) {
-  if (nsides > 0) hipLaunchKernelGGL(assemble_kernel, dim3(nsides), dim3(64), 0, st, net, tg, sys, K, A);
-  if (nf > 0) hipLaunchKernelGGL(continuity_kernel, dim3((nf + 255) / 256), dim3(256), 0, st, net, sys, nf, A);
+  const DenseTarget tgt{A, sys};
+  if (nsides > 0) hipLaunchKernelGGL(assemble_kernel<DenseTarget>, dim3(nsides), dim3(64), 0, st, net, tg, sys, K, tgt);
+  if (nf > 0)
+    hipLaunchKernelGGL(continuity_kernel<DenseTarget>, dim3((nf + 255) / 256), dim3(256), 0, st, net, sys, nf, tgt);
+}
+
+void launch_assemble_csr(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
+                         const int* ptr, const int* col, double* val, hipStream_t st) {
+  const CsrTarget tgt{val, ptr, col};
+  if (nsides > 0) hipLaunchKernelGGL(assemble_kernel<CsrTarget>, dim3(nsides), dim3(64), 0, st, net, tg, sys, K, tgt);
+  if (nf > 0)
+    hipLaunchKernelGGL(continuity_kernel<CsrTarget>, dim3((nf + 255) / 256), dim3(256), 0, st, net, sys, nf, tgt);
+}
+
+void launch_csr_row_scale(int64_t n, const int* ptr, const double* val, double* rs, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(csr_row_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ptr, val, rs);
 }
 
 void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st) {

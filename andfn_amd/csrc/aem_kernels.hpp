@@ -228,6 +228,11 @@ void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const 
                        const int* de_pos, double* b, hipStream_t st);
 void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
                      double* A, hipStream_t st);
+// the same entries into the values of the matrix's CSR pattern (zeroed by the caller)
+void launch_assemble_csr(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,
+                         const int* ptr, const int* col, double* val, hipStream_t st);
+// rs[r] = 1 / max |row r| of a CSR matrix (1 for an empty row)
+void launch_csr_row_scale(int64_t n, const int* ptr, const double* val, double* rs, hipStream_t st);
 // A[b][i][i] = 1 for the padding unknowns i >= blk_n[b] of every block
 void launch_pad_identity(double* A, int64_t npad, int nblk, const int* blk_n, hipStream_t st);
 // y_old <- y, y <- x, q and fracture constants <- x (n >= nde + nf unknowns)

@@ -31,6 +31,7 @@
 
 #include "../../include/andfn_mi.h"
 #include "aem_kernels.hpp"
+#include "aem_sparse.hpp"
 
 namespace {
 
@@ -341,7 +342,20 @@ class Solver {
   std::vector<int> ncoef0_, nint0_;
   std::vector<double> y0_;
   int lu_mode_ = ANDFN_MI_LU_TRSV;
-  int factor_ = ANDFN_MI_FACTOR_TOURNAMENT;
+  int factor_ = ANDFN_MI_FACTOR_SPARSE;
+  // multifrontal sparse LU (factor_ == ANDFN_MI_FACTOR_SPARSE): its analysis is built once per
+  // handle from the CSR pattern and the fracture centres; a network the analysis cannot take
+  // (a front beyond the solve kernels' LDS) is factored densely instead
+  aem::MultifrontalLU mf_;
+  bool mf_failed_ = false;
+  bool sparse_ = false;  // the current factors are the multifrontal ones
+  std::string mf_note_;
+  std::vector<int> mf_delay_;            // levels each unknown is lifted (pivots found unstable)
+  int mf_rounds_ = 0;                    // factorisations of the last run (1 + analyses redone)
+  void mf_analyse_now();
+  std::vector<double> fcenter_;          // [F][3] fracture centres (fracture record offset 24)
+  std::vector<int> h_csr_ptr_, h_csr_col_;  // host copy of the CSR pattern
+  void ensure_csr_pattern();
   // blocked LU with tournament pivoting (one connected part): candidate sets, the panel's
   // L11\U11 block and its row interchange
   DBuf<int> lu_cand_[2], lu_cnt_[2], lu_pairs_;
@@ -481,6 +495,7 @@ class Solver {
   void make_csr_transpose();
   template <class T>
   void adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v);
+  void adj_mu_step_sparse();
 
  public:
   int adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_options* opt, double* grad_t, double* grad_head,
@@ -536,12 +551,14 @@ void Solver::parse() {
   t_.resize(F_);
   constant_.resize(F_);
   frac_id_.resize(F_);
+  fcenter_.resize(3 * (size_t)F_);
   mem_ptr_.assign(F_ + 1, 0);
   slots_.clear();
   for (int f = 0; f < F_; ++f) {
     const char* p = fr_ + (size_t)f * L_.fsz;
     frac_id_[f] = rd<int64_t>(p, 0);
     t_[f] = rd<double>(p, 8);
+    for (int a = 0; a < 3; ++a) fcenter_[3 * (size_t)f + a] = rd<double>(p, 24 + 8 * a);
     constant_[f] = rd<double>(p, L_.f_const);
     int64_t nel = rd<int64_t>(p, L_.f_nel);
     if (nel < 0 || nel > L_.ME) fail(ANDFN_MI_ECAPACITY, "fracture %d: %lld elements exceed capacity %lld", f, (long long)nel, (long long)L_.ME);
@@ -1004,6 +1021,69 @@ void Solver::prepare_blocked_all(T* A, DBuf<T>& linv, DBuf<T>& uinv, bool lower_
   HIP_CHECK(hipStreamSynchronize(st_));  // the workspace is released on return
 }
 
+// The CSR pattern of A (rows and columns in the unknown layout), built once per handle on the host
+// and uploaded: the head residual's product, the FP64 refinement and the sparse factorisation use it.
+void Solver::ensure_csr_pattern() {
+  if (csr_ready_) return;
+  const int64_t N = N_;
+  std::vector<int> row_e(N, -1), row_f(N, -1);
+  for (int e : de_) row_e[de_pos_[e]] = e;
+  for (int f = 0; f < F_; ++f) row_f[fpos_[f]] = f;
+  std::vector<int> rp(1, 0), cols, tmp;
+  for (int64_t g = 0; g < N; ++g) {
+    tmp.clear();
+    if (row_e[g] >= 0) {
+      const int e = row_e[g];
+      const int sides = etype_[e] == 0 ? 2 : 1;
+      for (int sd = 0; sd < sides; ++sd) {
+        const int f = sd ? efrac1_[e] : efrac0_[e];
+        for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
+          const int src = slots_[k].elem;
+          if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
+        }
+        tmp.push_back(fpos_[f]);
+      }
+    } else if (row_f[g] >= 0) {
+      const int f = row_f[g];
+      for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+        if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
+    }
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    cols.insert(cols.end(), tmp.begin(), tmp.end());
+    rp.push_back((int)cols.size());
+  }
+  d_csr_ptr_.upload(rp, st_);
+  d_csr_col_.upload(cols, st_);
+  d_csr_val_.alloc(cols.size());
+  nnz_A_ = (int64_t)cols.size();
+  h_csr_ptr_.swap(rp);
+  h_csr_col_.swap(cols);
+  csr_ready_ = true;
+}
+
+void Solver::mf_analyse_now() {
+  aem::MfInput in;
+  in.F = F_;
+  in.E = E_;
+  in.etype = etype_.data();
+  in.efrac0 = efrac0_.data();
+  in.efrac1 = efrac1_.data();
+  in.de_pos = de_pos_.data();
+  in.fpos = fpos_.data();
+  in.center = fcenter_.data();
+  in.N = N_;
+  in.csr_ptr = h_csr_ptr_.data();
+  in.csr_col = h_csr_col_.data();
+  in.delay = mf_delay_.empty() ? nullptr : mf_delay_.data();
+  try {
+    mf_.analyse(in, st_);
+  } catch (const std::exception& e) {
+    mf_failed_ = true;
+    mf_note_ = e.what();
+  }
+}
+
 void Solver::assemble_and_factor() {
   drop_graph();
   adjT_valid_ = false;
@@ -1012,56 +1092,88 @@ void Solver::assemble_and_factor() {
   if (nblk > 1 && lu_mode_ != ANDFN_MI_LU_TRSV)
     fail(ANDFN_MI_EINVAL, "a network of %d independent parts is solved with the blocked triangular solves (lu_mode 0)",
          nblk);
+  ensure_csr_pattern();
+  // the row scale of the head residual follows the transmissivities (which set_parameters may change):
+  // 1 on intersection rows, 1/t on constant-head and well rows, 0 elsewhere
+  {
+    std::vector<double> rs(N, 0.0);
+    for (int e : de_) rs[de_pos_[e]] = etype_[e] == 0 ? 1.0 : 1.0 / t_[efrac0_[e]];
+    d_rowscale_.upload(rs, st_);
+  }
+  sparse_ = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_SPARSE;
+  if (sparse_ && !mf_.ready() && !mf_failed_) mf_analyse_now();
+  if (sparse_ && mf_failed_) sparse_ = false;  // the dense tournament LU below instead
+  if (sparse_) {
+    // A assembled straight into its CSR values, row-equilibrated, factored by the multifrontal LU
+    HIP_CHECK(hipMemsetAsync(d_csr_val_.p, 0, sizeof(double) * std::max<int64_t>(nnz_A_, 1), st_));
+    aem::TargetView tv = rhs_.view();
+    aem::launch_assemble_csr(net(), tv, (int)rhs_.side_elem.size(), sys(), K_, F_, d_csr_ptr_.p, d_csr_col_.p,
+                             d_csr_val_.p, st_);
+    d_req_.alloc(N);
+    aem::launch_csr_row_scale(N, d_csr_ptr_.p, d_csr_val_.p, d_req_.p, st_);
+    HIP_CHECK(hipGetLastError());
+    d_ay_.alloc(N);
+    // Pivots are searched within a front's own rows only.  A pivot whose L21 column grows beyond
+    // kGrowthLimit (threshold pivoting at 1e-3 of its column) or that is zero is delayed: its
+    // column and row unknowns are lifted one level, the tree is analysed again and A refactored.
+    // The lifts persist on the handle.  After kMaxRounds the dense tournament LU takes over.
+    constexpr double kGrowthLimit = 1e3;
+    constexpr int kMaxRounds = 8;
+    bool root_zero = false;
+    mf_rounds_ = 0;
+    for (;;) {
+      int bad = 0;
+      std::vector<int> lift;
+      try {
+        bad = mf_.factor(d_csr_val_.p, d_csr_ptr_.p, d_req_.p, st_, rb_);
+        ++mf_rounds_;
+        mf_.unstable_pivots(kGrowthLimit, lift, root_zero, st_);
+      } catch (const std::exception& e) {
+        fail(ANDFN_MI_EHIP, "multifrontal factorisation: %s", e.what());
+      }
+      if (lift.empty()) {
+        if (bad || root_zero) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot in front %d)", bad - 1);
+        break;
+      }
+      if (mf_rounds_ >= kMaxRounds) {
+        mf_failed_ = true;
+        mf_note_ = "pivots still unstable after repeated delays";
+        sparse_ = false;
+        break;
+      }
+      if (mf_delay_.empty()) mf_delay_.assign(N, 0);
+      for (int u : lift) ++mf_delay_[u];
+      mf_analyse_now();
+      if (mf_failed_) {
+        sparse_ = false;
+        break;
+      }
+    }
+  }
+  if (sparse_) {
+    nnz_factors_ = mf_.nnz_factors();
+    solve32_ = false;
+    a64_blocked_ = false;
+    // the solves write the analysed unknowns only: the padding of x, w and v stays zero
+    d_x_.zero(N, st_);
+    d_w_.zero(N, st_);
+    d_v_.zero(N, st_);
+    {
+      std::vector<int> id((size_t)N);
+      for (int64_t i = 0; i < N; ++i) id[i] = (int)i;
+      d_perm_.upload(id, st_);
+    }
+    HIP_CHECK(hipStreamSynchronize(st_));
+    return;
+  }
   d_A_.zero((size_t)Atot_, st_);
   aem::TargetView tv = rhs_.view();
   aem::launch_assemble(net(), tv, (int)rhs_.side_elem.size(), sys(), K_, F_, d_A_.p, st_);
   for (const auto& c : cls_) aem::launch_pad_identity(d_A_.p + c.a0, c.npad, c.nblk, d_blk_n_.p + c.b0, st_);
   HIP_CHECK(hipGetLastError());
-  // a CSR copy of the unfactored matrix (rows in unknown order) for the head residual A y - b;
-  // row_scale: 1 on intersection rows, 1/t on constant-head and well rows, 0 elsewhere
-  {
-    if (!csr_ready_) {
-      // the sparsity pattern depends on the network only: built once per handle
-      std::vector<int> row_e(N, -1), row_f(N, -1);
-      for (int e : de_) row_e[de_pos_[e]] = e;
-      for (int f = 0; f < F_; ++f) row_f[fpos_[f]] = f;
-      std::vector<int> rp(1, 0), cols, tmp;
-      for (int64_t g = 0; g < N; ++g) {
-        tmp.clear();
-        if (row_e[g] >= 0) {
-          const int e = row_e[g];
-          const int sides = etype_[e] == 0 ? 2 : 1;
-          for (int sd = 0; sd < sides; ++sd) {
-            const int f = sd ? efrac1_[e] : efrac0_[e];
-            for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k) {
-              const int src = slots_[k].elem;
-              if (src != e && de_pos_[src] >= 0) tmp.push_back(de_pos_[src]);
-            }
-            tmp.push_back(fpos_[f]);
-          }
-        } else if (row_f[g] >= 0) {
-          const int f = row_f[g];
-          for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-            if (de_pos_[slots_[k].elem] >= 0) tmp.push_back(de_pos_[slots_[k].elem]);
-        }
-        std::sort(tmp.begin(), tmp.end());
-        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
-        cols.insert(cols.end(), tmp.begin(), tmp.end());
-        rp.push_back((int)cols.size());
-      }
-      d_csr_ptr_.upload(rp, st_);
-      d_csr_col_.upload(cols, st_);
-      d_csr_val_.alloc(cols.size());
-      nnz_A_ = (int64_t)cols.size();
-      csr_ready_ = true;
-    }
-    // the row scale follows the transmissivities (which set_parameters may change)
-    std::vector<double> rs(N, 0.0);
-    for (int e : de_) rs[de_pos_[e]] = etype_[e] == 0 ? 1.0 : 1.0 / t_[efrac0_[e]];
-    aem::launch_csr_gather(d_A_.p, sys(), N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
-    d_rowscale_.upload(rs, st_);
-    HIP_CHECK(hipStreamSynchronize(st_));
-  }
+  // a CSR copy of the unfactored matrix for the head residual A y - b
+  aem::launch_csr_gather(d_A_.p, sys(), N, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+  HIP_CHECK(hipStreamSynchronize(st_));
   d_ay_.alloc(N);
   // row equilibration (rows of intersections scale with 1/t), as SuperLU equilibrates by default
   d_req_.alloc(N);
@@ -1074,7 +1186,8 @@ void Solver::assemble_and_factor() {
   }
   d_ipiv_.alloc(N);
   d_info_.zero(nblk, st_);
-  const bool tournament = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_TOURNAMENT;
+  // the tournament LU also stands in for a multifrontal analysis that could not be built
+  const bool tournament = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ != ANDFN_MI_FACTOR_ROCSOLVER;
   if (tournament) {
     std::vector<int> id((size_t)N);
     for (int64_t i = 0; i < N; ++i) id[i] = (int)i;
@@ -1445,6 +1558,10 @@ double* Solver::discharge_solve(double* b, double* x) {
     RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, n, n, &one, d_A_.p, n, d_w_.p, 1, &zero, x, 1));
     return x;
   }
+  if (sparse_) {
+    mf_.solve_refined(b, d_req_.p, x, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, st_);
+    return x;
+  }
   if (solve32_) {
     // x = y + (L U)^-1 R P (b - A y): one refinement step per iteration, FP32 factors
     aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
@@ -1594,9 +1711,11 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const double damping = C_.DAMPING;
   const int verbose = opt ? opt->verbose : 0;
   lu_mode_ = opt ? opt->lu_mode : ANDFN_MI_LU_TRSV;
-  factor_ = opt ? opt->factor : ANDFN_MI_FACTOR_TOURNAMENT;
-  if (factor_ != ANDFN_MI_FACTOR_TOURNAMENT && factor_ != ANDFN_MI_FACTOR_ROCSOLVER)
+  factor_ = opt ? opt->factor : ANDFN_MI_FACTOR_SPARSE;
+  if (factor_ != ANDFN_MI_FACTOR_SPARSE && factor_ != ANDFN_MI_FACTOR_TOURNAMENT && factor_ != ANDFN_MI_FACTOR_ROCSOLVER)
     fail(ANDFN_MI_EINVAL, "factor %d unknown", factor_);
+  // the multifrontal factors serve the blocked-solve mode only; the other lu_modes are dense
+  if (factor_ == ANDFN_MI_FACTOR_SPARSE && lu_mode_ != ANDFN_MI_LU_TRSV) factor_ = ANDFN_MI_FACTOR_ROCSOLVER;
   precision_ = opt ? opt->precision : ANDFN_MI_FP64;
   if (precision_ != ANDFN_MI_FP64 && precision_ != ANDFN_MI_MIXED) fail(ANDFN_MI_EINVAL, "precision %d unknown", precision_);
   if (precision_ == ANDFN_MI_MIXED && lu_mode_ != ANDFN_MI_LU_TRSV)
@@ -2013,6 +2132,11 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->rhs_points_per_lane = ppl_[1];
     info->nnz_discharge = nnz_A_;
     info->nnz_factors = nnz_factors_;
+    info->factor_used = sparse_ ? ANDFN_MI_FACTOR_SPARSE
+                                : (lu_mode_ == ANDFN_MI_LU_TRSV && factor_ != ANDFN_MI_FACTOR_ROCSOLVER
+                                       ? ANDFN_MI_FACTOR_TOURNAMENT
+                                       : ANDFN_MI_FACTOR_ROCSOLVER);
+    info->factor_rounds = sparse_ ? mf_rounds_ : 0;
   }
   mixed_phase_ = false;
   have_state_ = true;
@@ -2239,6 +2363,16 @@ void Solver::make_csr_transpose() {
   d_csrT_val_.upload(tv, st_);
 }
 
+// mu += A^-T (z - A^T mu) with the multifrontal factors: A^-T = R (R A)^-T, the solve returning
+// (R A)^-T r and the update applying R
+void Solver::adj_mu_step_sparse() {
+  const int n = (int)N_;
+  aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
+  aem::launch_adj_refine_in<double>(d_z_.p, d_aty_.p, n, d_x_.p, st_);
+  mf_.solve_t(d_x_.p, nullptr, d_v_.p, st_);
+  aem::launch_adj_refine_out<double>(d_v_.p, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
+}
+
 // mu += A^-T (z - A^T mu) with the transposed factors in precision T
 template <class T>
 void Solver::adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w, T* v) {
@@ -2335,13 +2469,14 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   // transposed FP64 factors, once per factorisation (also after a mixed-form run: one FP64 step
   // per application is cheaper than three refinement steps with FP32-rounded factors)
   if (!adjT_valid_) {
-    make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
+    if (!sparse_) make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
     make_csr_transpose();
     adjT_valid_ = true;
   }
   d_x_.alloc(n);
   d_w_.alloc(n);
   d_v_.alloc(n);
+  if (sparse_) HIP_CHECK(hipMemsetAsync(d_v_.p, 0, sizeof(double) * n, st_));  // padding stays zero
 
   // objective: J_c and J_y (discharge / constant weights plus the observation-head terms)
   std::vector<double> jy0(n, 0.0);
@@ -2411,7 +2546,12 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
     aem::launch_adj_build_z(nde, F_, d_de_.p, sys(), n, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
-    adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
+    if (sparse_) {
+      adj_mu_step_sparse();
+      adj_mu_step_sparse();  // one step of refinement (the pivot blocks act through their inverses)
+    } else {
+      adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
+    }
     aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_de_pos_.p, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
     aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
                            tab_rhs_.pz.p, tab_rhs_.pelem.p, d_wrhs_.p, 0, 0, 1, d_gc_.p, d_gq_.p, st_);

@@ -1,0 +1,819 @@
+// Multifrontal sparse LU of the discharge system: ordering, symbolic analysis, numeric
+// factorisation and the per-iteration solves (declarations and overview in aem_sparse.hpp).
+#include "aem_sparse.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+#include "aem_kernels.hpp"
+#include "lu_pivot_key.hpp"
+
+namespace aem {
+
+namespace {
+
+
+#define MF_HIP(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t _e = (x);                                                                            \
+    if (_e != hipSuccess) throw std::runtime_error(std::string(#x " failed: ") + hipGetErrorString(_e)); \
+  } while (0)
+#define MF_RB(x)                                                                                  \
+  do {                                                                                            \
+    rocblas_status _s = (x);                                                                      \
+    if (_s != rocblas_status_success) throw std::runtime_error(std::string(#x " failed: ") + std::to_string((int)_s)); \
+  } while (0)
+
+template <class T>
+void dev_upload(T*& d, const std::vector<T>& h, hipStream_t st) {
+  MF_HIP(hipMalloc((void**)&d, std::max<size_t>(h.size(), 1) * sizeof(T)));
+  if (!h.empty()) MF_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+template <class T>
+void dev_alloc(T*& d, size_t n) {
+  MF_HIP(hipMalloc((void**)&d, std::max<size_t>(n, 1) * sizeof(T)));
+}
+template <class T>
+void dev_free(T*& d) {
+  if (d) (void)hipFree(d);
+  d = nullptr;
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(x, o, 64);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---- factorisation kernels --------------------------------------------------------------------
+
+// row-equilibrated A into the fronts: F[asm[q]] = req[row] * val[q] (one thread per row)
+__global__ void mf_scatter_kernel(int nrows, const int* __restrict__ ptr, const double* __restrict__ val,
+                                  const double* __restrict__ req, const int64_t* __restrict__ asmap,
+                                  double* __restrict__ F) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  const double s = req[r];
+  for (int q = ptr[r]; q < ptr[r + 1]; ++q) F[asmap[q]] = s * val[q];
+}
+
+// extend-add: item (child front, first column): four columns of the child's Schur complement
+// (a wavefront each) added into its parent at the parent-local positions rel
+__global__ __launch_bounds__(256) void mf_extend_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ items,
+                                                        const int* __restrict__ rel, double* __restrict__ F) {
+  const int2 it = items[blockIdx.x];
+  const MfFront c = fr[it.x];
+  const MfFront p = fr[c.parent];
+  const int b = c.m - c.k;
+  const int t = it.y + (threadIdx.x >> 6);
+  if (t >= b) return;
+  const int lane = threadIdx.x & 63;
+  const int* rl = rel + c.rel;
+  const double* src = F + c.off + (int64_t)(c.k + t) * c.m + c.k;
+  double* dst = F + p.off + (int64_t)rl[t] * p.m;
+  for (int r = lane; r < b; r += 64) dst[rl[r]] += src[r];
+}
+
+// In-LDS triangular solves of the small-front kernel on an X block (column-major, leading
+// dimension LD, nc columns of k rows): X = L11^-1 X (unit lower, L11 below the diagonal of L) and
+// X = U11^-1 X (U11 on and above it, rd its reciprocal diagonal); one barrier per elimination step
+template <int LD>
+__device__ void lds_lower_unit_solve(const double* L, double* X, int k, int nc) {
+  for (int j = 0; j + 1 < k; ++j) {
+    const int ni = k - j - 1;
+    for (int e = threadIdx.x; e < ni * nc; e += blockDim.x) {
+      const int i = j + 1 + e % ni, c = e / ni;
+      X[c * LD + i] = fma(-L[j * LD + i], X[c * LD + j], X[c * LD + i]);
+    }
+    __syncthreads();
+  }
+}
+
+template <int LD>
+__device__ void lds_upper_solve(const double* L, const double* rd, double* X, int k, int nc) {
+  for (int j = k - 1; j >= 0; --j) {
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) X[c * LD + j] *= rd[j];
+    __syncthreads();
+    for (int e = threadIdx.x; e < j * nc; e += blockDim.x) {
+      const int i = e % j, c = e / j;
+      X[c * LD + i] = fma(-L[j * LD + i], X[c * LD + j], X[c * LD + i]);
+    }
+    __syncthreads();
+  }
+}
+
+// Partial factorisation of a front with k <= KM pivots, one workgroup: LU of the pivot block with
+// partial pivoting over the front's own k rows, then in tiles of KM columns (rows) staged in LDS:
+// U12 = L11^-1 P F12, L21 = F21 U11^-1, and L11^-1, U11^-1.  The Schur update F22 -= L21 U12
+// follows in mf_schur_kernel.
+template <int KM>
+__global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __restrict__ fr,
+                                                              const int* __restrict__ list, double* __restrict__ F,
+                                                              double* __restrict__ INV, int* __restrict__ piv,
+                                                              int* __restrict__ status,
+                                                              unsigned long long* __restrict__ growth) {
+  constexpr int LD = KM + 1;
+  __shared__ double L[KM * LD];  // L[c * LD + r] = (r, c) of the pivot block
+  __shared__ double X[KM * LD];  // a tile of U12, L21 (transposed) or an inverse
+  __shared__ double rdiag[KM];
+  __shared__ unsigned long long gmax[KM];  // bits of max |L21(:, j)| (inf: zero pivot)
+  __shared__ int prm[KM];
+  __shared__ int spiv;
+  const int fi = list[blockIdx.x];
+  const MfFront f = fr[fi];
+  const int m = f.m, k = f.k, b = m - k;
+  double* A = F + f.off;
+  const int t = threadIdx.x;
+  for (int i = t; i < k * k; i += 256) {
+    const int r = i % k, c = i / k;
+    L[c * LD + r] = A[r + (int64_t)c * m];
+  }
+  if (t < k) {
+    prm[t] = t;
+    gmax[t] = 0ull;
+  }
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (t < 64) {
+      unsigned long long key = 0ull;
+      if (t >= j && t < k) key = lu_pivot_key(L[j * LD + t], t);
+      key = wave_max_u64(key);
+      if (t == 0) spiv = lu_pivot_thread(key);
+    }
+    __syncthreads();
+    const int p = spiv;
+    if (p != j) {
+      for (int c = t; c < k; c += 256) {
+        const double a = L[c * LD + j];
+        L[c * LD + j] = L[c * LD + p];
+        L[c * LD + p] = a;
+      }
+      if (t == 0) {
+        const int a = prm[j];
+        prm[j] = prm[p];
+        prm[p] = a;
+      }
+    }
+    __syncthreads();
+    const double d = L[j * LD + j];
+    if (d == 0.0 && t == 0) {
+      atomicMin(status, fi);
+      gmax[j] = 0x7ff0000000000000ull;  // +inf
+    }
+    const double rd = d != 0.0 ? 1.0 / d : 0.0;
+    for (int r = j + 1 + t; r < k; r += 256) L[j * LD + r] *= rd;
+    __syncthreads();
+    const int n2 = k - j - 1;
+    for (int i = t; i < n2 * n2; i += 256) {
+      const int r = j + 1 + i % n2, c = j + 1 + i / n2;
+      L[c * LD + r] = fma(-L[j * LD + r], L[c * LD + j], L[c * LD + r]);
+    }
+    __syncthreads();
+  }
+  if (t < k) rdiag[t] = L[t * LD + t] != 0.0 ? 1.0 / L[t * LD + t] : 0.0;
+  for (int i = t; i < k * k; i += 256) {
+    const int r = i % k, c = i / k;
+    A[r + (int64_t)c * m] = L[c * LD + r];
+  }
+  if (t < k) piv[f.piv + t] = prm[t];
+  __syncthreads();
+  // U12 = L11^-1 P F12, KM columns at a time
+  for (int c0 = 0; c0 < b; c0 += KM) {
+    const int nc = min(KM, b - c0);
+    for (int e = t; e < k * nc; e += 256) {
+      const int r = e % k, c = e / k;
+      X[c * LD + r] = A[prm[r] + (int64_t)(k + c0 + c) * m];
+    }
+    __syncthreads();
+    lds_lower_unit_solve<LD>(L, X, k, nc);
+    for (int e = t; e < k * nc; e += 256) {
+      const int r = e % k, c = e / k;
+      A[r + (int64_t)(k + c0 + c) * m] = X[c * LD + r];
+    }
+    __syncthreads();
+  }
+  // L21 = F21 U11^-1, KM rows at a time: X holds the rows' transposes, so U11^T X = F21^T is the
+  // same upper solve from the left once transposed: solved here as forward substitution over columns
+  for (int r0 = 0; r0 < b; r0 += KM) {
+    const int nr = min(KM, b - r0);
+    for (int e = t; e < k * nr; e += 256) {
+      const int r = e % nr, c = e / nr;
+      X[c * LD + r] = A[(k + r0 + r) + (int64_t)c * m];
+    }
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+      for (int r = t; r < nr; r += 256) X[j * LD + r] *= rdiag[j];
+      __syncthreads();
+      const int nc2 = k - j - 1;
+      for (int e = t; e < nr * nc2; e += 256) {
+        const int r = e % nr, c = j + 1 + e / nr;
+        X[c * LD + r] = fma(-X[j * LD + r], L[c * LD + j], X[c * LD + r]);
+      }
+      __syncthreads();
+    }
+    for (int e = t; e < k * nr; e += 256) {
+      const int r = e % nr, c = e / nr;
+      A[(k + r0 + r) + (int64_t)c * m] = X[c * LD + r];
+    }
+    __syncthreads();
+  }
+  // growth of every pivot: max |L21(:, j)| (pivots within the front's own rows keep |L11| <= 1)
+  for (int e = t; e < k * b; e += 256) {
+    const int r = e % b, j = e / b;
+    atomicMax(&gmax[j], (unsigned long long)__double_as_longlong(fabs(A[(k + r) + (int64_t)j * m])));
+  }
+  __syncthreads();
+  if (t < k) growth[f.piv + t] = gmax[t];
+  // L11^-1 and U11^-1 against the identity
+  double* Li = INV + f.inv;
+  double* Ui = Li + (int64_t)k * k;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int e = t; e < k * k; e += 256) {
+      const int r = e % k, c = e / k;
+      X[c * LD + r] = r == c ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (pass == 0) lds_lower_unit_solve<LD>(L, X, k, k);
+    else lds_upper_solve<LD>(L, rdiag, X, k, k);
+    double* out = pass == 0 ? Li : Ui;
+    for (int e = t; e < k * k; e += 256) {
+      const int r = e % k, c = e / k;
+      out[r + (int64_t)c * k] = X[c * LD + r];
+    }
+    __syncthreads();
+  }
+}
+
+// Schur update of small fronts: tile (front, first boundary row, first boundary column) of 64 x 64
+// entries of F22 -= L21 U12 (k <= 64), L21 and U12 panels staged in LDS, 4 x 4 outputs per thread
+__global__ __launch_bounds__(256) void mf_schur_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ tiles,
+                                                       double* __restrict__ F) {
+  __shared__ double Ls[kMfSmallMax * 65];
+  __shared__ double Us[kMfSmallMax * 65];
+  const int4 tl = tiles[blockIdx.x];
+  const MfFront f = fr[tl.x];
+  const int m = f.m, k = f.k, b = m - k, i0 = tl.y, j0 = tl.z;
+  double* A = F + f.off;
+  const int t = threadIdx.x;
+  for (int e = t; e < k * 64; e += 256) {
+    const int i = e & 63, c = e >> 6;
+    Ls[c * 65 + i] = (i0 + i < b) ? A[(k + i0 + i) + (int64_t)c * m] : 0.0;
+  }
+  for (int e = t; e < k * 64; e += 256) {
+    const int c = e % k, j = e / k;
+    Us[c * 65 + j] = (j0 + j < b) ? A[c + (int64_t)(k + j0 + j) * m] : 0.0;
+  }
+  __syncthreads();
+  const int ti = t & 15, tj = t >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[q][r] = 0.0;
+  for (int c = 0; c < k; ++c) {
+    double a[4], u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      a[q] = Ls[c * 65 + ti + 16 * q];
+      u[q] = Us[c * 65 + tj + 16 * q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[q][r] = fma(a[q], u[r], acc[q][r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int j = j0 + tj + 16 * r;
+    if (j >= b) continue;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + ti + 16 * q;
+      if (i < b) A[(k + i) + (int64_t)(k + j) * m] -= acc[q][r];
+    }
+  }
+}
+
+// growth of a large front's pivots: a wavefront per pivot column j, max |L21(:, j)| over the
+// boundary rows (inf for a zero pivot)
+__global__ __launch_bounds__(256) void mf_growth_kernel(const double* __restrict__ A, int m, int k,
+                                                        unsigned long long* __restrict__ growth) {
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= k) return;
+  const int lane = threadIdx.x & 63;
+  const double* col = A + (int64_t)j * m;
+  double g = 0.0;
+  for (int r = k + lane; r < m; r += 64) g = fmax(g, fabs(col[r]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) g = fmax(g, __shfl_xor(g, o, 64));
+  if (lane == 0) growth[j] = col[j] == 0.0 ? 0x7ff0000000000000ull : (unsigned long long)__double_as_longlong(g);
+}
+
+__global__ void mf_axpy_kernel(int64_t n, const double* __restrict__ x0, const double* __restrict__ dx,
+                               double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x0[i] + dx[i];
+}
+
+// identity in the L11^-1 and U11^-1 blocks of front f (before the triangular solves against it)
+__global__ void mf_eye_kernel(double* __restrict__ inv, int k) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t kk = (int64_t)k * k;
+  if (i >= 2 * kk) return;
+  const int64_t e = i % kk;
+  inv[i] = (e % k == e / k) ? 1.0 : 0.0;
+}
+
+// ---- solve kernels ----------------------------------------------------------------------------
+// chunk (front, first row): 64 rows of one front per workgroup.  N-form products (lanes over the
+// rows of a column-major block, the four wavefronts over interleaved columns, a fixed-order sum)
+// for A x = b; T-form (a wavefront per output, lanes over the column, a shuffle-tree sum) for
+// A^T y = c.  Children's update vectors reach a front through its gather lists.
+
+__device__ __forceinline__ double gathered(const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                           const double* __restrict__ vec, int at) {
+  double s = 0.0;
+  for (int q = gptr[at]; q < gptr[at + 1]; ++q) s += vec[gsrc[q]];
+  return s;
+}
+
+// N-form row block: out[lane] = sum over columns c in [c0(r), c1(r)) of M[r + c ld] x[c]
+template <int TRI>  // 0: full; 1: lower (c <= r); 2: upper (c >= r)
+__device__ __forceinline__ double nform_row(const double* __restrict__ M, int64_t ld, int r, int ncols,
+                                            const double* __restrict__ x, int w) {
+  int c0 = w, c1 = ncols;
+  if (TRI == 1) c1 = r + 1;
+  if (TRI == 2) c0 = r + ((w - r) & 3);  // first column >= r with c % 4 == w
+  double s0 = 0.0, s1 = 0.0;
+  int c = c0;
+  for (; c + 4 < c1; c += 8) {
+    s0 = fma(M[r + (int64_t)c * ld], x[c], s0);
+    s1 = fma(M[r + (int64_t)(c + 4) * ld], x[c + 4], s1);
+  }
+  for (; c < c1; c += 4) s0 = fma(M[r + (int64_t)c * ld], x[c], s0);
+  return s0 + s1;
+}
+
+__device__ __forceinline__ double sum4(double (*part)[64], int lane) {
+  return (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+}
+
+// forward 1: v_U = L11^-1 P (R b + children)_U
+__global__ __launch_bounds__(256) void mfs_fwd1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                       const int* __restrict__ idx, const int* __restrict__ piv,
+                                                       const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                                       const double* __restrict__ INV, const double* __restrict__ b,
+                                                       const double* __restrict__ req, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  __shared__ double part[4][64];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k;
+  for (int i = threadIdx.x; i < k; i += 256) {
+    const int j = piv[f.piv + i];
+    const int g = idx[f.idx + j];
+    sh[i] = fma(b[g], req[g], gathered(gptr, gsrc, vec, f.gat + j));
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
+  part[w][lane] = r < k ? nform_row<1>(INV + f.inv, k, r, k, sh, w) : 0.0;
+  __syncthreads();
+  if (w == 0 && r < k) vec[f.vec + r] = sum4(part, lane);
+}
+
+// forward 2: u_B = (children)_B - L21 v_U
+__global__ __launch_bounds__(256) void mfs_fwd2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ cb,
+                                                       const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                                       const double* __restrict__ F, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  __shared__ double part[4][64];
+  const int2 cc = cb[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k, b = f.m - f.k;
+  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
+  part[w][lane] = r < b ? nform_row<0>(F + f.off + k, f.m, r, k, sh, w) : 0.0;
+  __syncthreads();
+  if (w == 0 && r < b) vec[f.vec + k + r] = gathered(gptr, gsrc, vec, f.gat + k + r) - sum4(part, lane);
+}
+
+// backward 1: w_U = v_U - U12 x_B (in place of v_U)
+__global__ __launch_bounds__(256) void mfs_bwd1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                       const int* __restrict__ idx, const double* __restrict__ F,
+                                                       const double* __restrict__ x, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  __shared__ double part[4][64];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k, b = f.m - f.k;
+  if (b == 0) return;  // uniform per workgroup
+  for (int i = threadIdx.x; i < b; i += 256) sh[i] = x[idx[f.idx + k + i]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
+  part[w][lane] = r < k ? nform_row<0>(F + f.off + (int64_t)k * f.m, f.m, r, b, sh, w) : 0.0;
+  __syncthreads();
+  if (w == 0 && r < k) vec[f.vec + r] -= sum4(part, lane);
+}
+
+// backward 2: x_U = U11^-1 w_U, scattered to the solver's layout
+__global__ __launch_bounds__(256) void mfs_bwd2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                       const int* __restrict__ idx, const double* __restrict__ INV,
+                                                       const double* __restrict__ vec, double* __restrict__ x) {
+  extern __shared__ double sh[];
+  __shared__ double part[4][64];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k;
+  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
+  part[w][lane] = r < k ? nform_row<2>(INV + f.inv + (int64_t)k * k, k, r, k, sh, w) : 0.0;
+  __syncthreads();
+  if (w == 0 && r < k) x[idx[f.idx + r]] = sum4(part, lane);
+}
+
+// T-form: wavefront w of the workgroup takes outputs o0 + 16 w .. o0 + 16 w + 15 of the chunk;
+// output o = sum over rows q in [q0(o), q1(o)) of M[q + o ld] x[q]
+template <int TRI>  // 0: rows [0, nrows); 1: rows [0, o] (upper triangle); 2: rows [o, nrows) (lower)
+__device__ __forceinline__ double tform_col(const double* __restrict__ M, int64_t ld, int o, int nrows,
+                                            const double* __restrict__ x, int lane) {
+  const int q0 = TRI == 2 ? o : 0, q1 = TRI == 1 ? o + 1 : nrows;
+  const double* col = M + (int64_t)o * ld;
+  double s = 0.0;
+  for (int q = q0 + lane; q < q1; q += 64) s = fma(col[q], x[q], s);
+  return wave_sum_d(s);
+}
+
+// transposed 1: z = U11^-T (c + children)_U
+__global__ __launch_bounds__(256) void mft_1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                    const int* __restrict__ idx, const int* __restrict__ gptr,
+                                                    const int* __restrict__ gsrc, const double* __restrict__ INV,
+                                                    const double* __restrict__ c, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k;
+  for (int i = threadIdx.x; i < k; i += 256) sh[i] = c[idx[f.idx + i]] + gathered(gptr, gsrc, vec, f.gat + i);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = 0; j < 16; ++j) {
+    const int o = cc.y + 16 * w + j;
+    if (o >= k) break;  // uniform per wavefront
+    const double s = tform_col<1>(INV + f.inv + (int64_t)k * k, k, o, k, sh, lane);
+    if (lane == 0) vec[f.vec + o] = s;
+  }
+}
+
+// transposed 2: u_B = (children)_B - U12^T z
+__global__ __launch_bounds__(256) void mft_2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ cb,
+                                                    const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                                    const double* __restrict__ F, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  const int2 cc = cb[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k, b = f.m - f.k;
+  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = 0; j < 16; ++j) {
+    const int o = cc.y + 16 * w + j;
+    if (o >= b) break;
+    const double s = tform_col<0>(F + f.off + (int64_t)k * f.m, f.m, o, k, sh, lane);
+    if (lane == 0) vec[f.vec + k + o] = gathered(gptr, gsrc, vec, f.gat + k + o) - s;
+  }
+}
+
+// transposed 3: t = z - L21^T y'_B (in place of z)
+__global__ __launch_bounds__(256) void mft_3_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                    const int* __restrict__ idx, const double* __restrict__ F,
+                                                    const double* __restrict__ ys, double* __restrict__ vec) {
+  extern __shared__ double sh[];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k, b = f.m - f.k;
+  if (b == 0) return;
+  for (int i = threadIdx.x; i < b; i += 256) sh[i] = ys[idx[f.idx + k + i]];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = 0; j < 16; ++j) {
+    const int o = cc.y + 16 * w + j;
+    if (o >= k) break;
+    const double s = tform_col<0>(F + f.off + k, f.m, o, b, sh, lane);
+    if (lane == 0) vec[f.vec + o] -= s;
+  }
+}
+
+// transposed 4: w = L11^-T t; y'[row piv[i]] = w_i
+__global__ __launch_bounds__(256) void mft_4_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
+                                                    const int* __restrict__ idx, const int* __restrict__ piv,
+                                                    const double* __restrict__ INV, const double* __restrict__ vec,
+                                                    double* __restrict__ ys) {
+  extern __shared__ double sh[];
+  const int2 cc = ck[blockIdx.x];
+  const MfFront f = fr[cc.x];
+  const int k = f.k;
+  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = 0; j < 16; ++j) {
+    const int o = cc.y + 16 * w + j;
+    if (o >= k) break;
+    const double s = tform_col<2>(INV + f.inv, k, o, k, sh, lane);
+    if (lane == 0) ys[idx[f.idx + piv[f.piv + o]]] = s;
+  }
+}
+
+__global__ void mft_scale_kernel(int n, const int* __restrict__ unk, const double* __restrict__ req,
+                                 const double* __restrict__ ys, double* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const int g = unk[i];
+    y[g] = req ? req[g] * ys[g] : ys[g];
+  }
+}
+
+}  // namespace
+
+MultifrontalLU::~MultifrontalLU() { release(); }
+
+void MultifrontalLU::release() {
+  dev_free(d_fr_);
+  dev_free(d_idx_);
+  dev_free(d_rel_);
+  dev_free(d_piv_);
+  dev_free(d_gptr_);
+  dev_free(d_gsrc_);
+  dev_free(d_small_);
+  dev_free(d_tiles_);
+  dev_free(d_ck_);
+  dev_free(d_cb_);
+  dev_free(d_ext_);
+  dev_free(d_asm_);
+  dev_free(d_status_);
+  dev_free(d_F_);
+  dev_free(d_inv_);
+  dev_free(d_vec_);
+  dev_free(d_ys_);
+  dev_free(d_unk_);
+  for (int i = 0; i < 2; ++i) {
+    dev_free(d_cand_[i]);
+    dev_free(d_cnt_[i]);
+  }
+  dev_free(d_lu11_);
+  dev_free(d_pairs_);
+  dev_free(d_info_);
+  dev_free(d_growth_);
+  dev_free(d_r_);
+  dev_free(d_ay_);
+  dev_free(d_x0_);
+  dev_free(d_dx_);
+  analysed_ = false;
+}
+
+void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
+  release();
+  mf_analyse(in, sym_);
+  N_ = in.N;
+  nrows_csr_ = (int)in.N;
+  const MfSymbolic& s = sym_;
+  dev_upload(d_fr_, s.fronts, st);
+  dev_upload(d_idx_, s.idx, st);
+  dev_upload(d_rel_, s.rel, st);
+  dev_upload(d_piv_, s.piv0, st);
+  dev_upload(d_gptr_, s.gptr, st);
+  dev_upload(d_gsrc_, s.gsrc, st);
+  dev_upload(d_small_, s.small_list, st);
+  {
+    static_assert(sizeof(MfI4) == sizeof(int4) && sizeof(MfI2) == sizeof(int2), "device vector layouts");
+    int4* t = nullptr;
+    dev_alloc(t, s.tiles.size());
+    if (!s.tiles.empty()) MF_HIP(hipMemcpyAsync(t, s.tiles.data(), sizeof(int4) * s.tiles.size(), hipMemcpyHostToDevice, st));
+    d_tiles_ = t;
+    const std::vector<MfI2>* src2[3] = {&s.ck, &s.cb, &s.ext};
+    int2** dst2[3] = {&d_ck_, &d_cb_, &d_ext_};
+    for (int i = 0; i < 3; ++i) {
+      dev_alloc(*dst2[i], src2[i]->size());
+      if (!src2[i]->empty())
+        MF_HIP(hipMemcpyAsync(*dst2[i], src2[i]->data(), sizeof(int2) * src2[i]->size(), hipMemcpyHostToDevice, st));
+    }
+  }
+  dev_upload(d_asm_, s.asmap, st);
+  dev_upload(d_unk_, s.unk, st);
+  dev_alloc(d_status_, 1);
+  dev_alloc(d_F_, (size_t)s.front_total);
+  dev_alloc(d_inv_, (size_t)s.inv_total);
+  dev_alloc(d_vec_, (size_t)s.nvec);
+  dev_alloc(d_ys_, (size_t)N_);
+  MF_HIP(hipMemsetAsync(d_ys_, 0, sizeof(double) * std::max<int64_t>(N_, 1), st));
+  const int64_t groups = (s.big_k + kLuGroupRows - 1) / kLuGroupRows;
+  for (int i = 0; i < 2; ++i) {
+    dev_alloc(d_cand_[i], (size_t)std::max<int64_t>(groups, 1) * kLuPanel);
+    dev_alloc(d_cnt_[i], (size_t)std::max<int64_t>(groups, 1));
+  }
+  dev_alloc(d_lu11_, (size_t)kLuPanel * kLuPanel);
+  dev_alloc(d_pairs_, (size_t)(kLuOuter / kLuPanel) * kLuPairStride);
+  dev_alloc(d_info_, std::max<size_t>(s.fronts.size(), 1));
+  dev_alloc(d_growth_, s.piv0.size());
+  dev_alloc(d_r_, (size_t)N_);
+  dev_alloc(d_ay_, (size_t)N_);
+  dev_alloc(d_x0_, (size_t)N_);
+  dev_alloc(d_dx_, (size_t)N_);
+  MF_HIP(hipMemsetAsync(d_x0_, 0, sizeof(double) * std::max<int64_t>(N_, 1), st));
+  MF_HIP(hipMemsetAsync(d_dx_, 0, sizeof(double) * std::max<int64_t>(N_, 1), st));
+  MF_HIP(hipStreamSynchronize(st));
+  analysed_ = true;
+}
+
+void MultifrontalLU::factor_big(int fi, hipStream_t st, rocblas_handle rb) {
+  const MfFront& f = sym_.fronts[fi];
+  const int64_t m = f.m, k = f.k;
+  double* A = d_F_ + f.off;
+  int* perm = d_piv_ + f.piv;
+  int64_t* info = d_info_ + fi;
+  const double one = 1.0, mone = -1.0;
+  int* cand[2] = {d_cand_[0], d_cand_[1]};
+  int* cnt[2] = {d_cnt_[0], d_cnt_[1]};
+  // the first k steps of the front's LU, pivot search over its first k rows only
+  for (int64_t J0 = 0; J0 < k; J0 += kLuOuter) {
+    const int64_t J1 = std::min<int64_t>(k, J0 + kLuOuter);
+    int nl = 0;
+    for (int64_t j0 = J0; j0 < J1; j0 += kLuPanel, ++nl) {
+      const int bw = (int)std::min<int64_t>(kLuPanel, J1 - j0);
+      int* pairs = d_pairs_ + (int64_t)nl * kLuPairStride;
+      const int* win = launch_tslu(A, m, k, j0, bw, cand, cnt, d_lu11_, pairs, perm, info, st);
+      launch_calu_apply(A, m, k, J0, J1, j0, bw, pairs, win, d_lu11_, st);
+      launch_calu_l21(A, m, m, j0, bw, d_lu11_, st);
+      const int64_t rows = m - j0 - bw, cols = J1 - j0 - bw;
+      if (rows > 0 && cols > 0)
+        MF_RB(rocblas_dgemm_64(rb, rocblas_operation_none, rocblas_operation_none, rows, cols, bw, &mone,
+                               A + (j0 + bw) + j0 * m, m, A + j0 + (j0 + bw) * m, m, &one,
+                               A + (j0 + bw) + (j0 + bw) * m, m));
+    }
+    launch_calu_swap(A, m, m, J0, J1, d_pairs_, nl, st);
+    const int64_t rest = m - J1;
+    if (rest > 0) {
+      MF_RB(rocblas_dtrsm_64(rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit,
+                             J1 - J0, rest, &one, A + J0 + J0 * m, m, A + J0 + J1 * m, m));
+      MF_RB(rocblas_dgemm_64(rb, rocblas_operation_none, rocblas_operation_none, rest, rest, J1 - J0, &mone,
+                             A + J1 + J0 * m, m, A + J0 + J1 * m, m, &one, A + J1 + J1 * m, m));
+    }
+  }
+  // L11^-1 and U11^-1: triangular solves against the identity
+  double* Li = d_inv_ + f.inv;
+  double* Ui = Li + k * k;
+  hipLaunchKernelGGL(mf_eye_kernel, dim3((unsigned)((2 * k * k + 255) / 256)), dim3(256), 0, st, Li, (int)k);
+  MF_RB(rocblas_dtrsm_64(rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, k,
+                         k, &one, A, m, Li, k));
+  MF_RB(rocblas_dtrsm_64(rb, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                         rocblas_diagonal_non_unit, k, k, &one, A, m, Ui, k));
+  hipLaunchKernelGGL(mf_growth_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, st, A, (int)m, (int)k,
+                     d_growth_ + f.piv);
+}
+
+int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st,
+                           rocblas_handle rb) {
+  if (!analysed_) throw std::runtime_error("multifrontal factor before analysis");
+  const int nf = (int)sym_.fronts.size();
+  MF_HIP(hipMemsetAsync(d_F_, 0, sizeof(double) * std::max<int64_t>(sym_.front_total, 1), st));
+  if (!sym_.piv0.empty())
+    MF_HIP(hipMemcpyAsync(d_piv_, sym_.piv0.data(), sizeof(int) * sym_.piv0.size(), hipMemcpyHostToDevice, st));
+  MF_HIP(hipMemsetAsync(d_info_, 0, sizeof(int64_t) * std::max(nf, 1), st));
+  const int big = 0x7fffffff;
+  MF_HIP(hipMemcpyAsync(d_status_, &big, sizeof(int), hipMemcpyHostToDevice, st));
+  if (nrows_csr_ > 0)
+    hipLaunchKernelGGL(mf_scatter_kernel, dim3((nrows_csr_ + 255) / 256), dim3(256), 0, st, nrows_csr_, csr_ptr,
+                       csr_val, req, d_asm_, d_F_);
+  for (const MfLevel& L : sym_.levels) {
+    for (size_t s = 0; s < L.ext_beg.size(); ++s) {
+      const int n = L.ext_end[s] - L.ext_beg[s];
+      if (n > 0)
+        hipLaunchKernelGGL(mf_extend_kernel, dim3(n), dim3(256), 0, st, d_fr_, d_ext_ + L.ext_beg[s], d_rel_, d_F_);
+    }
+    const int n0 = L.small_end[0] - L.small_beg[0], n1 = L.small_end[1] - L.small_beg[1],
+              n2 = L.small_end[2] - L.small_beg[2];
+    if (n0 > 0)
+      hipLaunchKernelGGL(mf_small_factor_kernel<16>, dim3(n0), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[0],
+                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (n1 > 0)
+      hipLaunchKernelGGL(mf_small_factor_kernel<32>, dim3(n1), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[1],
+                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (n2 > 0)
+      hipLaunchKernelGGL(mf_small_factor_kernel<64>, dim3(n2), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[2],
+                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (L.tile_end > L.tile_beg)
+      hipLaunchKernelGGL(mf_schur_kernel, dim3(L.tile_end - L.tile_beg), dim3(256), 0, st, d_fr_,
+                         d_tiles_ + L.tile_beg, d_F_);
+    for (int fi : L.big) factor_big(fi, st, rb);
+  }
+  MF_HIP(hipGetLastError());
+  int status = 0;
+  std::vector<int64_t> info(std::max(nf, 1), 0);
+  MF_HIP(hipMemcpyAsync(&status, d_status_, sizeof(int), hipMemcpyDeviceToHost, st));
+  MF_HIP(hipMemcpyAsync(info.data(), d_info_, sizeof(int64_t) * std::max(nf, 1), hipMemcpyDeviceToHost, st));
+  MF_HIP(hipStreamSynchronize(st));
+  int bad = status != big ? status : -1;
+  for (int i = 0; i < nf && bad < 0; ++i)
+    if (info[i] > 0) bad = i;
+  return bad < 0 ? 0 : 1 + bad;
+}
+
+void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStream_t st) {
+  const int H = (int)sym_.levels.size();
+  for (int h = 0; h < H; ++h) {
+    const MfLevel& L = sym_.levels[h];
+    if (L.ck_end > L.ck_beg)
+      hipLaunchKernelGGL(mfs_fwd1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_piv_, d_gptr_, d_gsrc_, d_inv_, b, req, d_vec_);
+    if (L.cb_end > L.cb_beg)
+      hipLaunchKernelGGL(mfs_fwd2_kernel, dim3(L.cb_end - L.cb_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_cb_ + L.cb_beg, d_gptr_, d_gsrc_, d_F_, d_vec_);
+  }
+  for (int h = H - 1; h >= 0; --h) {
+    const MfLevel& L = sym_.levels[h];
+    if (L.ck_end > L.ck_beg && L.maxb > 0)
+      hipLaunchKernelGGL(mfs_bwd1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxb, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_F_, x, d_vec_);
+    if (L.ck_end > L.ck_beg)
+      hipLaunchKernelGGL(mfs_bwd2_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_inv_, d_vec_, x);
+  }
+}
+
+void MultifrontalLU::solve_t(const double* c, const double* req, double* y, hipStream_t st) {
+  const int H = (int)sym_.levels.size();
+  for (int h = 0; h < H; ++h) {
+    const MfLevel& L = sym_.levels[h];
+    if (L.ck_end > L.ck_beg)
+      hipLaunchKernelGGL(mft_1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_gptr_, d_gsrc_, d_inv_, c, d_vec_);
+    if (L.cb_end > L.cb_beg)
+      hipLaunchKernelGGL(mft_2_kernel, dim3(L.cb_end - L.cb_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_cb_ + L.cb_beg, d_gptr_, d_gsrc_, d_F_, d_vec_);
+  }
+  for (int h = H - 1; h >= 0; --h) {
+    const MfLevel& L = sym_.levels[h];
+    if (L.ck_end > L.ck_beg && L.maxb > 0)
+      hipLaunchKernelGGL(mft_3_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxb, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_F_, d_ys_, d_vec_);
+    if (L.ck_end > L.ck_beg)
+      hipLaunchKernelGGL(mft_4_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
+                         d_ck_ + L.ck_beg, d_idx_, d_piv_, d_inv_, d_vec_, d_ys_);
+  }
+  if (sym_.nunk > 0)
+    hipLaunchKernelGGL(mft_scale_kernel, dim3((unsigned)((sym_.nunk + 255) / 256)), dim3(256), 0, st, (int)sym_.nunk, d_unk_,
+                       req, d_ys_, y);
+}
+
+bool MultifrontalLU::unstable_pivots(double limit, std::vector<int>& lift, bool& root_zero, hipStream_t st) {
+  const size_t np = sym_.piv0.size();
+  std::vector<unsigned long long> g(np);
+  std::vector<int> pv(np);
+  if (np) {
+    MF_HIP(hipMemcpyAsync(g.data(), d_growth_, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, st));
+    MF_HIP(hipMemcpyAsync(pv.data(), d_piv_, sizeof(int) * np, hipMemcpyDeviceToHost, st));
+  }
+  MF_HIP(hipStreamSynchronize(st));
+  lift.clear();
+  root_zero = false;
+  for (const MfFront& f : sym_.fronts)
+    for (int j = 0; j < f.k; ++j) {
+      double v;
+      std::memcpy(&v, &g[f.piv + j], sizeof(v));
+      if (v <= limit) continue;  // (a NaN fails the test too)
+      if (f.parent < 0) {
+        root_zero = root_zero || !(v < HUGE_VAL);
+        continue;
+      }
+      lift.push_back(sym_.idx[f.idx + j]);              // the pivot's column
+      lift.push_back(sym_.idx[f.idx + pv[f.piv + j]]);  // and the row it was taken from
+    }
+  std::sort(lift.begin(), lift.end());
+  lift.erase(std::unique(lift.begin(), lift.end()), lift.end());
+  return !lift.empty();
+}
+
+void MultifrontalLU::solve_refined(const double* b, const double* req, double* x, const int* csr_ptr,
+                                   const int* csr_col, const double* csr_val, hipStream_t st) {
+  solve(b, req, d_x0_, st);
+  launch_spmv((int)N_, csr_ptr, csr_col, csr_val, d_x0_, d_ay_, st);
+  launch_adj_refine_in<double>(b, d_ay_, (int)N_, d_r_, st);
+  solve(d_r_, req, d_dx_, st);
+  hipLaunchKernelGGL(mf_axpy_kernel, dim3((unsigned)((N_ + 255) / 256)), dim3(256), 0, st, N_, d_x0_, d_dx_, x);
+}
+
+}  // namespace aem

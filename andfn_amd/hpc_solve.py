@@ -55,16 +55,17 @@ def last_solve_info():
 
 def solve(fracture_struc_array, element_struc_array, discharge_int, constants, ntype_elements=None, precision="fp64",
           device=0, verbose=False, lu_mode=0, growth_schedule=None, record_growth=False, head_tol=0.0,
-          factor="tournament"):
+          factor="sparse"):
     """Solve in place; see the module docstring.
 
     ``growth_schedule`` (int32 [iterations, n_elements], optional) replays another run's
     coefficient counts; ``record_growth`` stores this run's counts per iteration in
     ``last_solve_info()["ncoef_history"]``.  ``head_tol`` > 0 replaces AnDFN's stop rule: the
     solve stops as soon as the head residual (max |A y - b(c)| over the head equations, metres)
-    is at most ``head_tol`` (or at MAX_ITERATIONS).  ``factor``: "tournament" (the library's
-    blocked LU with tournament pivoting, all connected parts of the network in one batch) or
-    "rocsolver" (rocSOLVER getrf, strided-batched for several parts).
+    is at most ``head_tol`` (or at MAX_ITERATIONS).  ``factor``: "sparse" (the default: the
+    library's multifrontal LU of the sparse discharge matrix, as the reference factors it with
+    SuperLU), or the dense "tournament" (blocked LU with tournament pivoting, all connected parts
+    of the network in one batch) or "rocsolver" (rocSOLVER getrf).
     """
     global _last_info
     _lib.require_records(fracture_struc_array, element_struc_array)
@@ -129,7 +130,7 @@ def get_heads(fracture_struc_array, element_struc_array, frac, z, device=0):
     return out
 
 
-def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0, factor="tournament"):
+def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0, factor="sparse"):
     if factor not in _lib.FACTORS:
         raise ValueError(f"factor must be one of {sorted(_lib.FACTORS)}")
     return _lib.Options(_lib.MIXED if precision == "mixed" else _lib.FP64, int(device), 1 if verbose else 0,
@@ -157,7 +158,7 @@ class DeviceNetwork:
                                             ctypes.byref(self.h)))
         self.device = device
 
-    def run(self, precision="fp64", lu_mode=0, head_tol=0.0, verbose=False, factor="tournament"):
+    def run(self, precision="fp64", lu_mode=0, head_tol=0.0, verbose=False, factor="sparse"):
         opt = _options(precision, self.device, verbose, lu_mode, head_tol, factor)
         info = _lib.Info()
         rc = self.lib.andfn_mi_run(self.h, ctypes.byref(opt), ctypes.byref(info))

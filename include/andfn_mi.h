@@ -47,9 +47,14 @@ extern "C" {
 #define ANDFN_MI_LU_INVERSE 2  /* explicit inverse formed once (getri), then one gemv per iteration */
 
 /* andfn_mi_options.factor: how the discharge system is factored (lu_mode 0 only; the other
- * lu_modes always use rocSOLVER) */
-#define ANDFN_MI_FACTOR_TOURNAMENT 0 /* the library's blocked LU with tournament pivoting (all parts of the network in one batch) */
-#define ANDFN_MI_FACTOR_ROCSOLVER 1  /* rocSOLVER getrf (partial pivoting; strided-batched for several parts) */
+ * lu_modes always use rocSOLVER).  The reference factors the sparse matrix with SuperLU
+ * (hpc_solve.py:121, splu of the csc_matrix of :885). */
+#define ANDFN_MI_FACTOR_SPARSE 0     /* multifrontal sparse LU on a nested-dissection ordering of the fracture graph
+                                      * (the default; falls back to the tournament LU for a network its solve
+                                      * kernels cannot take) */
+#define ANDFN_MI_FACTOR_ROCSOLVER 1  /* dense: rocSOLVER getrf (partial pivoting; strided-batched for several parts) */
+#define ANDFN_MI_FACTOR_TOURNAMENT 2 /* dense: the library's blocked LU with tournament pivoting (all parts of the
+                                      * network in one batch) */
 
 typedef struct andfn_mi_options {
   int32_t precision;     /* ANDFN_MI_FP64 or ANDFN_MI_MIXED */
@@ -122,6 +127,9 @@ typedef struct andfn_mi_info {
   int32_t rhs_points_per_lane;    /* and for the right-hand-side sweep (0: the basis product ran instead) */
   int64_t nnz_discharge;  /* structural nonzeros of the discharge matrix (hpc_solve.py:140-141 logs getnnz()) */
   int64_t nnz_factors;    /* entries of the stored LU factors of the discharge matrix (L and U together) */
+  int32_t factor_used;    /* ANDFN_MI_FACTOR_* that factored the last run's matrix */
+  int32_t factor_rounds;  /* multifrontal LU: factorisations of the last run (more than one when unstable
+                           * pivots were delayed to parent fronts and the tree analysed again) */
 } andfn_mi_info;
 
 /*

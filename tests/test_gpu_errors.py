@@ -16,7 +16,7 @@ from andfn_amd.hpc_solve import last_solve_info, solve
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("factor", ["tournament", "rocsolver"])
+@pytest.mark.parametrize("factor", ["sparse", "tournament", "rocsolver"])
 def test_exactly_singular_discharge_matrix_raises(factor):
     fa, ea, consts, K = errcases.with_isolated_fracture()
     with pytest.raises(RuntimeError, match="singular"):

@@ -1,7 +1,8 @@
-"""The library's blocked LU with tournament pivoting against rocSOLVER's getrf.
+"""The library's multifrontal sparse LU (the default) and its dense blocked LU with tournament
+pivoting, against rocSOLVER's dense getrf.
 
-Both factor the same equilibrated discharge matrix with different row pivoting, so the per-
-iteration discharge solves agree to rounding and so do the iterates.  Networks of the 200-disk
+All three factor the same equilibrated discharge matrix with different orderings and row
+pivoting, so the per-iteration discharge solves agree to rounding and so do the iterates.  Networks of the 200-disk
 law at constant density: 40 fractures (one panel, n < 64), 1,500 fractures (3,470 unknowns: two
 tournament levels) and 4,000 fractures (8,884 unknowns: three levels, 139 panels).  Fixed
 iteration count (the change-based stop off), so both runs take the same iterations.  A batch of
@@ -50,13 +51,15 @@ def _run(d, factor, precision="fp64", head_tol=0.0):
     return fa, ea, info
 
 
+@pytest.mark.parametrize("factor", ["tournament", "sparse"])
 @pytest.mark.parametrize("n", [40, 1500, 4000])
-def test_tournament_lu_matches_rocsolver(n):
-    print(f"network of {n} fractures", flush=True)
+def test_lu_matches_rocsolver(n, factor):
+    print(f"network of {n} fractures, {factor}", flush=True)
     d = _network(n)
     fr, er, ir = _run(d, "rocsolver")
-    ft, et, it = _run(d, "tournament")
-    print(f"unknowns {it['n_unknowns']}, factor s: rocsolver {ir['t_factor']:.3f} tournament {it['t_factor']:.3f}")
+    ft, et, it = _run(d, factor)
+    print(f"unknowns {it['n_unknowns']}, factor s: rocsolver {ir['t_factor']:.3f} {factor} {it['t_factor']:.3f}, "
+          f"stored factor entries {it['nnz_factors']}")
     assert ir["nit"] == it["nit"] == CONSTS["MAX_ITERATIONS"]
     assert _rel(et["coef"], er["coef"]) < 1e-10
     assert _rel(et["q"], er["q"]) < 1e-10
@@ -64,32 +67,34 @@ def test_tournament_lu_matches_rocsolver(n):
     assert abs(it["head_residual"] - ir["head_residual"]) <= 1e-10 * max(1.0, ir["head_residual"])
 
 
-def test_tournament_lu_mixed_form_reaches_head_tolerance():
+@pytest.mark.parametrize("factor", ["tournament", "sparse"])
+def test_mixed_form_reaches_head_tolerance(factor):
     d = _network(1500)
     c = np.array(d.constants, copy=True)
     c["MAX_ITERATIONS"] = 400
     d.constants = c
-    _, _, info = _run(d, "tournament", precision="mixed", head_tol=1e-6)
+    _, _, info = _run(d, factor, precision="mixed", head_tol=1e-6)
     assert info["converged"] and info["head_residual"] <= 1e-6
 
 
-def test_tournament_lu_batch_of_realisations_matches_rocsolver():
+@pytest.mark.parametrize("factor", ["tournament", "sparse"])
+def test_batch_of_realisations_matches_rocsolver(factor):
     # three independent realisations: one discharge block each, factored in one batch
     from andfn_amd.batch import build_batch, realisation_specs
 
     specs = realisation_specs(3, seed=7, nmin=150, nmax=400)
     _, fa, ea, off, K, c = build_batch(specs, dict(MAX_ERROR=0.0, MAX_NCOEF=10, MAX_ITERATIONS=40))
     res = {}
-    for factor in ("rocsolver", "tournament"):
+    for fct in ("rocsolver", factor):
         f1, e1 = fa.copy(), ea.copy()
         net = DeviceNetwork(f1, e1, K, c)
         try:
-            info = net.run("fp64", factor=factor)
+            info = net.run("fp64", factor=fct)
             net.download()
         finally:
             net.close()
-        res[factor] = (f1, e1, info)
-    (fr, er, ir), (ft, et, it) = res["rocsolver"], res["tournament"]
+        res[fct] = (f1, e1, info)
+    (fr, er, ir), (ft, et, it) = res["rocsolver"], res[factor]
     assert ir["nit"] == it["nit"] == 40
     assert _rel(et["coef"], er["coef"]) < 1e-10
     assert _rel(et["q"], er["q"]) < 1e-10
@@ -99,4 +104,4 @@ def test_tournament_lu_batch_of_realisations_matches_rocsolver():
 def test_unknown_factor_is_rejected():
     with pytest.raises(ValueError):
         _run(_network(40), "qr")
-    assert _lib.FACTORS == {"tournament": 0, "rocsolver": 1}
+    assert _lib.FACTORS == {"sparse": 0, "rocsolver": 1, "tournament": 2}

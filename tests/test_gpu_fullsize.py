@@ -42,12 +42,12 @@ def cfg2():
     return d
 
 
-def _run(d, precision):
+def _run(d, precision, factor="sparse"):
     fa = d.fractures_struc_array_hpc.copy()
     ea = d.elements_struc_array_hpc.copy()
-    print(f"cfg2: {precision} solve", flush=True)
+    print(f"cfg2: {precision} solve ({factor} LU)", flush=True)
     net = DeviceNetwork(fa, ea, d.discharge_int, d.constants)
-    info = net.run(precision=precision, head_tol=HEAD_TOL)
+    info = net.run(precision=precision, head_tol=HEAD_TOL, factor=factor)
     net.download()
     net.close()
     return fa, ea, info
@@ -103,12 +103,12 @@ def test_fp64_runs_are_bit_identical(cfg2, fp64):
     assert np.array_equal(fa2["constant"], fa["constant"])
 
 
-def test_lu_stream_placement_is_bit_identical(cfg2, fp64, monkeypatch):
-    """The LU's panel kernels on reserved CUs (the default) or on plain streams compute the same
-    arithmetic in the same order: results identical to the bit."""
-    fa, ea, info = fp64
+def test_lu_stream_placement_is_bit_identical(cfg2, monkeypatch):
+    """The dense tournament LU's panel kernels on reserved CUs (the default) or on plain streams
+    compute the same arithmetic in the same order: results identical to the bit."""
+    fa, ea, info = _run(cfg2, "fp64", "tournament")
     monkeypatch.setenv("ANDFN_MI_LU_RESERVE", "0")
-    fa2, ea2, info2 = _run(cfg2, "fp64")
+    fa2, ea2, info2 = _run(cfg2, "fp64", "tournament")
     assert info2["nit"] == info["nit"]
     assert np.array_equal(ea2["coef"], ea["coef"])
     assert np.array_equal(ea2["q"], ea["q"])
