@@ -1114,10 +1114,10 @@ void Solver::assemble_and_factor() {
     HIP_CHECK(hipGetLastError());
     d_ay_.alloc(N);
     // Pivots are searched within a front's own rows only.  A pivot whose L21 column grows beyond
-    // kGrowthLimit (threshold pivoting at 1e-3 of its column) or that is zero is delayed: its
+    // kGrowthLimit (threshold pivoting at 1e-5 of its column; the solves refine once) or that is zero is delayed: its
     // column and row unknowns are lifted one level, the tree is analysed again and A refactored.
     // The lifts persist on the handle.  After kMaxRounds the dense tournament LU takes over.
-    constexpr double kGrowthLimit = 1e3;
+    constexpr double kGrowthLimit = 1e5;
     constexpr int kMaxRounds = 8;
     bool root_zero = false;
     mf_rounds_ = 0;
