@@ -351,100 +351,155 @@ __device__ __forceinline__ double gathered(const int* __restrict__ gptr, const i
   return s;
 }
 
-// N-form row block: out[lane] = sum over columns c in [c0(r), c1(r)) of M[r + c ld] x[c]
-template <int TRI>  // 0: full; 1: lower (c <= r); 2: upper (c >= r)
-__device__ __forceinline__ double nform_row(const double* __restrict__ M, int64_t ld, int r, int ncols,
-                                            const double* __restrict__ x, int w) {
-  int c0 = w, c1 = ncols;
-  if (TRI == 1) c1 = r + 1;
-  if (TRI == 2) c0 = r + ((w - r) & 3);  // first column >= r with c % 4 == w
-  double s0 = 0.0, s1 = 0.0;
-  int c = c0;
-  for (; c + 4 < c1; c += 8) {
-    s0 = fma(M[r + (int64_t)c * ld], x[c], s0);
-    s1 = fma(M[r + (int64_t)(c + 4) * ld], x[c + 4], s1);
+// The block of a solve item: kind 0 L11^-1 (lower), 1 L21, 2 U12, 3 U11^-1 (upper).
+template <int KIND>
+__device__ __forceinline__ void seg_block(const MfFront& f, const double* __restrict__ F,
+                                          const double* __restrict__ INV, const double*& M, int64_t& ld,
+                                          int& rows, int& cols) {
+  const int k = f.k, nb = f.m - f.k;
+  rows = KIND == 1 ? nb : k;
+  cols = KIND == 2 ? nb : k;
+  if (KIND == 0) { M = INV + f.inv; ld = k; }
+  else if (KIND == 1) { M = F + f.off + k; ld = f.m; }
+  else if (KIND == 2) { M = F + f.off + (int64_t)k * f.m; ld = f.m; }
+  else { M = INV + f.inv + (int64_t)k * k; ld = k; }
+}
+
+// After a factorisation: copy each item's block into its tile.  Tile layout: the value of row
+// r0 + l and column c0 + 2p + e at tile[(p * 64 + l) * 2 + e]; rows beyond the block, the odd
+// column of an odd segment and entries outside a triangular block's triangle hold zeros.
+template <int KIND>
+__global__ __launch_bounds__(256) void mfs_pack_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
+                                                       const int64_t* __restrict__ toff, const double* __restrict__ F,
+                                                       const double* __restrict__ INV, double* __restrict__ T) {
+  const int4 it = items[blockIdx.x];
+  const MfFront f = fr[it.x];
+  const double* M;
+  int64_t ld;
+  int rows, cols;
+  seg_block<KIND>(f, F, INV, M, ld, rows, cols);
+  const int r0 = it.y, c0 = it.z & ~(kMfSeg - 1);
+  const int ncol = min(kMfSeg, cols - c0), npair = (ncol + 1) >> 1;
+  double2* out = reinterpret_cast<double2*>(T + toff[blockIdx.x]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = r0 + lane;
+  for (int p = w; p < npair; p += 4) {
+    double v[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = c0 + 2 * p + e;
+      bool ok = r < rows && 2 * p + e < ncol;
+      if (KIND == 0) ok = ok && c <= r;
+      if (KIND == 3) ok = ok && c >= r;
+      v[e] = ok ? M[r + (int64_t)c * ld] : 0.0;
+    }
+    out[p * 64 + lane] = make_double2(v[0], v[1]);
   }
-  for (; c < c1; c += 4) s0 = fma(M[r + (int64_t)c * ld], x[c], s0);
-  return s0 + s1;
 }
 
-__device__ __forceinline__ double sum4(double (*part)[64], int lane) {
-  return (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+// The epilogue of a 64-row chunk's product t (row r of the kind's block):
+//   kind 0  vec_U  = L11^-1 P (R b + children)_U
+//   kind 1  vec_B  = (children)_B - L21 vec_U
+//   kind 2  vec_U -= U12 x_B
+//   kind 3  x_U    = U11^-1 vec_U   (scattered to the solver's layout)
+template <int KIND>
+__device__ __forceinline__ void seg_epilogue(const MfFront& f, int r, double t, const int* __restrict__ idx,
+                                             const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                             double* __restrict__ x, double* __restrict__ vec) {
+  if (KIND == 0) vec[f.vec + r] = t;
+  else if (KIND == 1) vec[f.vec + f.k + r] = gathered(gptr, gsrc, vec, f.gat + f.k + r) - t;
+  else if (KIND == 2) vec[f.vec + r] -= t;
+  else x[idx[f.idx + r]] = t;
 }
 
-// forward 1: v_U = L11^-1 P (R b + children)_U
-__global__ __launch_bounds__(256) void mfs_fwd1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                       const int* __restrict__ idx, const int* __restrict__ piv,
-                                                       const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                       const double* __restrict__ INV, const double* __restrict__ b,
-                                                       const double* __restrict__ req, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  __shared__ double part[4][64];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k;
-  for (int i = threadIdx.x; i < k; i += 256) {
-    const int j = piv[f.piv + i];
-    const int g = idx[f.idx + j];
-    sh[i] = fma(b[g], req[g], gathered(gptr, gsrc, vec, f.gat + j));
+// N-form products for A x = b, one item (front, first row r0, first column c0 plus the chunk's
+// segment count, first item of the chunk) per workgroup: the item's 64 x (at most kMfSeg) tile
+// times the kind's input over columns c0.. (lanes over the rows, the four wavefronts over
+// interleaved column pairs, sixteen 16-byte loads in flight per wavefront, four accumulators, a
+// fixed-order sum).  A chunk of one segment applies its epilogue here; the segments of a longer
+// chunk write their partials to part[item], summed in segment order by mfs_sum_kernel in the next
+// launch (the kernel boundary orders the two; a cross-workgroup handshake inside one launch would
+// need device-scope fences, which write back and invalidate the L2 of the XCD on every use).
+template <int KIND>
+__global__ __launch_bounds__(256) void mfs_seg_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
+                                                      int base, const int64_t* __restrict__ toff,
+                                                      const double* __restrict__ T, const int* __restrict__ idx,
+                                                      const int* __restrict__ piv, const int* __restrict__ gptr,
+                                                      const int* __restrict__ gsrc, const double* __restrict__ b,
+                                                      const double* __restrict__ req, double* __restrict__ x,
+                                                      double* __restrict__ vec, double* __restrict__ part) {
+  __shared__ double xs[kMfSeg];
+  __shared__ double ps[4][64];
+  const int4 it = items[blockIdx.x];
+  const MfFront f = fr[it.x];
+  const int k = f.k, nb = f.m - f.k;
+  const int rows = KIND == 1 ? nb : k, cols = KIND == 2 ? nb : k;
+  const int r0 = it.y, c0 = it.z & ~(kMfSeg - 1), nseg = it.z & (kMfSeg - 1);
+  const int ncol = min(kMfSeg, cols - c0), npair = (ncol + 1) >> 1;
+  for (int j = threadIdx.x; j < kMfSeg; j += 256) {
+    const int c = c0 + j;
+    double v = 0.0;  // beyond the segment: zero (the tile's padding column meets it)
+    if (j < ncol) {
+      if (KIND == 0) {
+        const int pj = piv[f.piv + c];
+        const int g = idx[f.idx + pj];
+        v = fma(b[g], req[g], gathered(gptr, gsrc, vec, f.gat + pj));
+      } else if (KIND == 2) {
+        v = x[idx[f.idx + k + c]];
+      } else {
+        v = vec[f.vec + c];
+      }
+    }
+    xs[j] = v;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
-  part[w][lane] = r < k ? nform_row<1>(INV + f.inv, k, r, k, sh, w) : 0.0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = r0 + lane;
+  const double2* T2 = reinterpret_cast<const double2*>(T + toff[base + blockIdx.x]) + lane;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int p = w;
+  for (; p + 60 < npair; p += 64) {
+    double2 m[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) m[u] = T2[(p + 4 * u) * 64];
+#pragma unroll
+    for (int u = 0; u < 16; u += 2) {
+      const int j = 2 * (p + 4 * u);
+      a0 = fma(m[u].x, xs[j], a0);
+      a1 = fma(m[u].y, xs[j + 1], a1);
+      a2 = fma(m[u + 1].x, xs[j + 8], a2);
+      a3 = fma(m[u + 1].y, xs[j + 9], a3);
+    }
+  }
+  for (; p < npair; p += 4) {
+    const double2 m = T2[p * 64];
+    a0 = fma(m.x, xs[2 * p], a0);
+    a1 = fma(m.y, xs[2 * p + 1], a1);
+  }
+  ps[w][lane] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  if (w == 0 && r < k) vec[f.vec + r] = sum4(part, lane);
+  if (w != 0) return;
+  const double t = (ps[0][lane] + ps[1][lane]) + (ps[2][lane] + ps[3][lane]);
+  if (nseg > 1) part[(int64_t)(base + blockIdx.x) * 64 + lane] = t;
+  else if (r < rows) seg_epilogue<KIND>(f, r, t, idx, gptr, gsrc, x, vec);
 }
 
-// forward 2: u_B = (children)_B - L21 v_U
-__global__ __launch_bounds__(256) void mfs_fwd2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ cb,
-                                                       const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                       const double* __restrict__ F, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  __shared__ double part[4][64];
-  const int2 cc = cb[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k, b = f.m - f.k;
-  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
-  part[w][lane] = r < b ? nform_row<0>(F + f.off + k, f.m, r, k, sh, w) : 0.0;
-  __syncthreads();
-  if (w == 0 && r < b) vec[f.vec + k + r] = gathered(gptr, gsrc, vec, f.gat + k + r) - sum4(part, lane);
-}
-
-// backward 1: w_U = v_U - U12 x_B (in place of v_U)
-__global__ __launch_bounds__(256) void mfs_bwd1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                       const int* __restrict__ idx, const double* __restrict__ F,
-                                                       const double* __restrict__ x, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  __shared__ double part[4][64];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k, b = f.m - f.k;
-  if (b == 0) return;  // uniform per workgroup
-  for (int i = threadIdx.x; i < b; i += 256) sh[i] = x[idx[f.idx + k + i]];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
-  part[w][lane] = r < k ? nform_row<0>(F + f.off + (int64_t)k * f.m, f.m, r, b, sh, w) : 0.0;
-  __syncthreads();
-  if (w == 0 && r < k) vec[f.vec + r] -= sum4(part, lane);
-}
-
-// backward 2: x_U = U11^-1 w_U, scattered to the solver's layout
-__global__ __launch_bounds__(256) void mfs_bwd2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                       const int* __restrict__ idx, const double* __restrict__ INV,
-                                                       const double* __restrict__ vec, double* __restrict__ x) {
-  extern __shared__ double sh[];
-  __shared__ double part[4][64];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k;
-  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = cc.y + lane;
-  part[w][lane] = r < k ? nform_row<2>(INV + f.inv + (int64_t)k * k, k, r, k, sh, w) : 0.0;
-  __syncthreads();
-  if (w == 0 && r < k) x[idx[f.idx + r]] = sum4(part, lane);
+// the chunks of more than one segment: 64 threads per chunk sum its partials in segment order and
+// apply the epilogue
+template <int KIND>
+__global__ __launch_bounds__(256) void mfs_sum_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
+                                                      const int* __restrict__ chunks, int nchunks,
+                                                      const double* __restrict__ part, const int* __restrict__ idx,
+                                                      const int* __restrict__ gptr, const int* __restrict__ gsrc,
+                                                      double* __restrict__ x, double* __restrict__ vec) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (c >= nchunks) return;
+  const int first = chunks[c];
+  const int4 it = items[first];
+  const MfFront f = fr[it.x];
+  const int rows = KIND == 1 ? f.m - f.k : f.k, r = it.y + lane, nseg = it.z & (kMfSeg - 1);
+  if (r >= rows) return;
+  double t = 0.0;
+  for (int q = 0; q < nseg; ++q) t += part[(int64_t)(first + q) * 64 + lane];
+  seg_epilogue<KIND>(f, r, t, idx, gptr, gsrc, x, vec);
 }
 
 // T-form: wavefront w of the workgroup takes outputs o0 + 16 w .. o0 + 16 w + 15 of the chunk;
@@ -578,6 +633,11 @@ void MultifrontalLU::release() {
   dev_free(d_pairs_);
   dev_free(d_info_);
   dev_free(d_growth_);
+  dev_free(d_gv_);
+  dev_free(d_part_);
+  dev_free(d_gvoff_);
+  dev_free(d_tile_);
+  dev_free(d_gr_);
   dev_free(d_r_);
   dev_free(d_ay_);
   dev_free(d_x0_);
@@ -611,6 +671,16 @@ void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
       if (!src2[i]->empty())
         MF_HIP(hipMemcpyAsync(*dst2[i], src2[i]->data(), sizeof(int2) * src2[i]->size(), hipMemcpyHostToDevice, st));
     }
+  }
+  {
+    int4* g = nullptr;
+    dev_alloc(g, s.gv.size());
+    if (!s.gv.empty()) MF_HIP(hipMemcpyAsync(g, s.gv.data(), sizeof(int4) * s.gv.size(), hipMemcpyHostToDevice, st));
+    d_gv_ = g;
+    dev_upload(d_gvoff_, s.gv_off, st);
+    dev_alloc(d_tile_, (size_t)s.gv_total);
+    dev_alloc(d_part_, s.gv.size() * 64);
+    dev_upload(d_gr_, s.gr, st);
   }
   dev_upload(d_asm_, s.asmap, st);
   dev_upload(d_unk_, s.unk, st);
@@ -720,6 +790,20 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
                          d_tiles_ + L.tile_beg, d_F_);
     for (int fi : L.big) factor_big(fi, st, rb);
   }
+  // the solve tiles of every item
+  for (const MfLevel& L : sym_.levels)
+    for (int kind = 0; kind < 4; ++kind) {
+      const int n = L.gv_end[kind] - L.gv_beg[kind];
+      if (n <= 0) continue;
+      const int4* items = d_gv_ + L.gv_beg[kind];
+      const int64_t* toff = d_gvoff_ + L.gv_beg[kind];
+      switch (kind) {
+        case 0: hipLaunchKernelGGL(mfs_pack_kernel<0>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
+        case 1: hipLaunchKernelGGL(mfs_pack_kernel<1>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
+        case 2: hipLaunchKernelGGL(mfs_pack_kernel<2>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
+        default: hipLaunchKernelGGL(mfs_pack_kernel<3>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
+      }
+    }
   MF_HIP(hipGetLastError());
   int status = 0;
   std::vector<int64_t> info(std::max(nf, 1), 0);
@@ -732,25 +816,41 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
   return bad < 0 ? 0 : 1 + bad;
 }
 
+void MultifrontalLU::launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x,
+                                hipStream_t st) {
+  const int n = L.gv_end[kind] - L.gv_beg[kind];
+  if (n <= 0) return;
+  const int4* items = d_gv_ + L.gv_beg[kind];
+  const int base = L.gv_beg[kind];
+  const int nc = L.gr_end[kind] - L.gr_beg[kind];
+  const dim3 gsum((unsigned)((nc + 3) / 4));
+  const int* chunks = d_gr_ + L.gr_beg[kind];
+  switch (kind) {
+#define MF_SEG(K)                                                                                                  \
+  case K:                                                                                                          \
+    hipLaunchKernelGGL(mfs_seg_kernel<K>, dim3(n), dim3(256), 0, st, d_fr_, items, base, d_gvoff_, d_tile_,       \
+                       d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, d_part_);                             \
+    if (nc > 0)                                                                                                    \
+      hipLaunchKernelGGL(mfs_sum_kernel<K>, gsum, dim3(256), 0, st, d_fr_, d_gv_, chunks, nc, d_part_, d_idx_,    \
+                         d_gptr_, d_gsrc_, x, d_vec_);                                                             \
+    break;
+    MF_SEG(0)
+    MF_SEG(1)
+    MF_SEG(2)
+    MF_SEG(3)
+#undef MF_SEG
+  }
+}
+
 void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStream_t st) {
   const int H = (int)sym_.levels.size();
   for (int h = 0; h < H; ++h) {
-    const MfLevel& L = sym_.levels[h];
-    if (L.ck_end > L.ck_beg)
-      hipLaunchKernelGGL(mfs_fwd1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_piv_, d_gptr_, d_gsrc_, d_inv_, b, req, d_vec_);
-    if (L.cb_end > L.cb_beg)
-      hipLaunchKernelGGL(mfs_fwd2_kernel, dim3(L.cb_end - L.cb_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_cb_ + L.cb_beg, d_gptr_, d_gsrc_, d_F_, d_vec_);
+    launch_seg(0, sym_.levels[h], b, req, x, st);
+    launch_seg(1, sym_.levels[h], b, req, x, st);
   }
   for (int h = H - 1; h >= 0; --h) {
-    const MfLevel& L = sym_.levels[h];
-    if (L.ck_end > L.ck_beg && L.maxb > 0)
-      hipLaunchKernelGGL(mfs_bwd1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxb, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_F_, x, d_vec_);
-    if (L.ck_end > L.ck_beg)
-      hipLaunchKernelGGL(mfs_bwd2_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_inv_, d_vec_, x);
+    launch_seg(2, sym_.levels[h], b, req, x, st);
+    launch_seg(3, sym_.levels[h], b, req, x, st);
   }
 }
 

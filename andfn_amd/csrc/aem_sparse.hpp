@@ -80,6 +80,11 @@ class MultifrontalLU {
   int2* d_ck_ = nullptr;
   int2* d_cb_ = nullptr;
   int2* d_ext_ = nullptr;
+  int4* d_gv_ = nullptr;        // solve items (MfSymbolic::gv)
+  int64_t* d_gvoff_ = nullptr;  // per solve item: first value of its tile
+  double* d_tile_ = nullptr;    // the solve tiles (repacked after every factorisation)
+  double* d_part_ = nullptr;    // per solve item: its 64 partial sums
+  int* d_gr_ = nullptr;         // first item of every solve chunk of more than one segment
   int64_t* d_asm_ = nullptr;
   int* d_unk_ = nullptr;     // every analysed unknown (its position in the solver's layout)
   int* d_status_ = nullptr;
@@ -101,6 +106,7 @@ class MultifrontalLU {
 
   void release();
   void factor_big(int f, hipStream_t st, rocblas_handle rb);
+  void launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x, hipStream_t st);
 };
 
 }  // namespace aem

@@ -489,6 +489,31 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
     }
     L.ck_end = (int)ck.size();
     L.cb_end = (int)cb.size();
+    for (int kind = 0; kind < 4; ++kind) {
+      L.gv_beg[kind] = (int)out.gv.size();
+      L.gr_beg[kind] = (int)out.gr.size();
+      for (int i : byh[h]) {
+        const MfFront& f = fr[i];
+        const int b = f.m - f.k;
+        int rows, cols;
+        mf_gemv_shape(kind, f.k, b, &rows, &cols);
+        if (rows == 0 || cols == 0) continue;
+        for (int r0 = 0; r0 < rows; r0 += 64) {
+          int s_lo, s_hi;
+          mf_gemv_segments(kind, f.k, b, r0, &s_lo, &s_hi);
+          const int first = (int)out.gv.size();
+          if (s_hi - s_lo > 1) out.gr.push_back(first);
+          for (int sg = s_lo; sg < s_hi; ++sg) {
+            out.gv.push_back(MfI4{i, r0, sg * kMfSeg + (s_hi - s_lo), first});
+            const int ncol = std::min(kMfSeg, cols - sg * kMfSeg);
+            out.gv_off.push_back(out.gv_total);
+            out.gv_total += (int64_t)kMfTileRows * ((ncol + 1) & ~1);
+          }
+        }
+      }
+      L.gv_end[kind] = (int)out.gv.size();
+      L.gr_end[kind] = (int)out.gr.size();
+    }
     if ((int64_t)std::max(L.maxk, L.maxb) * 8 > kMfMaxLds)
       throw std::runtime_error("multifrontal analysis: a front exceeds the solve kernels' LDS");
     int slots = 0;

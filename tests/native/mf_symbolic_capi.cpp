@@ -38,7 +38,8 @@ int mf_test_analyse(int F, int E, const int* etype, const int* f0, const int* f1
 const char* mf_test_error() { return g_err.c_str(); }
 
 // sizes: 0 fronts, 1 levels, 2 idx, 3 rel, 4 piv0, 5 gptr, 6 gsrc, 7 small_list, 8 unk, 9 tiles,
-// 10 ck, 11 cb, 12 ext, 13 asmap, 14 front_total, 15 inv_total, 16 nvec, 17 nnz_factors, 18 nunk
+// 10 ck, 11 cb, 12 ext, 13 asmap, 14 front_total, 15 inv_total, 16 nvec, 17 nnz_factors, 18 nunk, 19 gv,
+// 20 gv_total, 21 gr
 long long mf_test_size(int w) {
   const aem::MfSymbolic& s = g_sym;
   switch (w) {
@@ -61,6 +62,9 @@ long long mf_test_size(int w) {
     case 16: return s.nvec;
     case 17: return s.nnz_factors;
     case 18: return s.nunk;
+    case 19: return (long long)s.gv.size();
+    case 20: return s.gv_total;
+    case 21: return (long long)s.gr.size();
   }
   return -1;
 }
@@ -75,12 +79,15 @@ void mf_test_fronts(long long* out) {
   }
 }
 
-// levels as 16 ints each: small_beg[3], small_end[3], tile_beg, tile_end, ck_beg, ck_end, cb_beg, cb_end,
-// maxk, maxb, number of big fronts, number of extend slots
+// levels as 32 ints each: small_beg[3], small_end[3], tile_beg, tile_end, ck_beg, ck_end, cb_beg, cb_end,
+// maxk, maxb, number of big fronts, number of extend slots, gv_beg[4], gv_end[4], gr_beg[4], gr_end[4]
 void mf_test_levels(int* out) {
   for (size_t h = 0; h < g_sym.levels.size(); ++h) {
     const aem::MfLevel& L = g_sym.levels[h];
-    int* o = out + 16 * h;
+    int* o = out + 32 * h;
+    for (int c = 0; c < 4; ++c) {
+      o[16 + c] = L.gv_beg[c]; o[20 + c] = L.gv_end[c]; o[24 + c] = L.gr_beg[c]; o[28 + c] = L.gr_end[c];
+    }
     for (int c = 0; c < 3; ++c) { o[c] = L.small_beg[c]; o[3 + c] = L.small_end[c]; }
     o[6] = L.tile_beg; o[7] = L.tile_end; o[8] = L.ck_beg; o[9] = L.ck_end; o[10] = L.cb_beg; o[11] = L.cb_end;
     o[12] = L.maxk; o[13] = L.maxb; o[14] = (int)L.big.size(); o[15] = (int)L.ext_beg.size();
@@ -101,6 +108,12 @@ void mf_test_ints(int w, int* out) {
   else if (w == 10) std::memcpy(out, s.ck.data(), sizeof(int) * 2 * s.ck.size());
   else if (w == 11) std::memcpy(out, s.cb.data(), sizeof(int) * 2 * s.cb.size());
   else if (w == 12) std::memcpy(out, s.ext.data(), sizeof(int) * 2 * s.ext.size());
+  else if (w == 13) std::memcpy(out, s.gv.data(), sizeof(int) * 4 * s.gv.size());
+  else if (w == 14) std::memcpy(out, s.gr.data(), sizeof(int) * s.gr.size());
+}
+
+void mf_test_gvoff(long long* out) {
+  for (size_t i = 0; i < g_sym.gv_off.size(); ++i) out[i] = g_sym.gv_off[i];
 }
 
 void mf_test_asmap(long long* out) {

@@ -4,13 +4,15 @@ with g++ and checked on the CPU before any of its tables reaches a GPU kernel.
 * Bounds: every offset, index, chunk, tile and extend-add item the kernels of aem_sparse.hip
   index stays inside its buffer (fronts, inverses, index lists, solve vectors, gather lists,
   the parent's positions), every unknown is a pivot of exactly one front, children come before
-  their parents and sit on lower levels, every matrix entry has one place in one front, and the
-  solve kernels' LDS fits 64 KB.
+  their parents and sit on lower levels, every matrix entry has one place in one front, the
+  solve kernels' LDS fits 64 KB, and the segmented solve items cover each block's rows and the
+  columns of its triangle.
 * Semantics: the factorisation and both solves are replayed in numpy on the analysed tables, step
   by step as the kernels take them (scatter of the row-equilibrated matrix, extend-add through the
   parent positions, LU of each pivot block with partial pivoting over its own rows, L21, U12,
   Schur update, the two inverses; forward and backward solves through the gather lists and pivot
-  orders), and must reproduce SciPy's solutions of A x = b and A^T y = c to 1e-10 relative on the
+  orders, the forward and backward products item by item with per-chunk partial sums), and must
+  reproduce SciPy's solutions of A x = b and A^T y = c to 1e-10 relative on the
   discharge matrices of four networks (with wells, with constant-head lines, at 5,000 fractures).
 """
 
@@ -43,7 +45,7 @@ def lib(tmp_path_factory):
     L.mf_test_error.restype = ctypes.c_char_p
     L.mf_test_size.restype = ctypes.c_longlong
     L.mf_test_size.argtypes = [ctypes.c_int]
-    for n in ("mf_test_fronts", "mf_test_levels", "mf_test_asmap"):
+    for n in ("mf_test_fronts", "mf_test_levels", "mf_test_asmap", "mf_test_gvoff"):
         getattr(L, n).argtypes = [vp]
     L.mf_test_ints.argtypes = [ctypes.c_int, vp]
     L.mf_test_level_big.argtypes = [ctypes.c_int, vp]
@@ -90,19 +92,22 @@ def _analyse(L, net, delay=None):
     assert rc == 0, L.mf_test_error()
     sz = {k: L.mf_test_size(i) for i, k in enumerate(["fronts", "levels", "idx", "rel", "piv0", "gptr", "gsrc", "small",
                                                       "unk", "tiles", "ck", "cb", "ext", "asmap", "front_total",
-                                                      "inv_total", "nvec", "nnz_factors", "nunk"])}
+                                                      "inv_total", "nvec", "nnz_factors", "nunk", "gv",
+                                                      "gv_total", "gr"])}
     fr = np.zeros((sz["fronts"], 11), np.int64)
     L.mf_test_fronts(_p(fr))
-    lv = np.zeros((sz["levels"], 16), np.int32)
+    lv = np.zeros((sz["levels"], 32), np.int32)
     L.mf_test_levels(_p(lv))
     t = dict(sz=sz, fr=fr, lv=lv)
     for w, k, shape in ((2, "idx", None), (3, "rel", None), (4, "piv0", None), (5, "gptr", None), (6, "gsrc", None),
                         (7, "small", None), (8, "unk", None), (9, "tiles", 4), (10, "ck", 2), (11, "cb", 2),
-                        (12, "ext", 2)):
+                        (12, "ext", 2), (13, "gv", 4), (14, "gr", None)):
         n = sz[k]
         a = np.zeros(n * (shape or 1), np.int32)
         L.mf_test_ints(w, _p(a))
         t[k] = a.reshape(n, shape) if shape else a
+    t["gv_off"] = np.zeros(sz["gv"], np.int64)
+    L.mf_test_gvoff(_p(t["gv_off"]))
     t["asmap"] = np.zeros(sz["asmap"], np.int64)
     L.mf_test_asmap(_p(t["asmap"]))
     t["big"], t["ext_slots"] = [], []
@@ -165,7 +170,75 @@ def _check_bounds(t, net):
     assert np.all((par < 0) | (level[np.maximum(par, 0)] > level))
     nsmall = sum(hi - lo for row in t["lv"] for lo, hi in zip(row[0:3], row[3:6]))
     assert nsmall + sum(len(x) for x in t["big"]) == nf
+    _check_solve_items(t, level)
     return level
+
+
+SEG = 256  # columns per solve item
+
+
+def _kind_shape(kind, kk, bb):
+    return (bb if kind == 1 else kk), (bb if kind == 2 else kk)
+
+
+def _check_solve_items(t, level):
+    """Items of the segmented solve products: per level and kind, every front of the level with a
+    nonempty block has 64-row chunks of consecutive segments (column multiples of SEG, the chunk's
+    segment count in the low bits, the index of its first item), and a chunk's segments cover every
+    column its rows need: all of them for L21 and U12, columns 0..r for L11^-1, r..k-1 for U11^-1.
+    Every item owns a tile of 64 rows by its column count rounded up to even, the tiles packed
+    back to back in item order; the chunks of more than one segment are listed (their partials
+    are summed by a second launch) per level and kind."""
+    fr, gv, lv = t["fr"], t["gv"], t["lv"]
+    k, b = fr[:, 3], fr[:, 2] - fr[:, 3]
+    assert lv[0, 16] == 0 and lv[-1, 23] == len(gv)
+    pos = gpos = 0
+    tile_size = [0]  # size of the last tile seen
+    for h, row in enumerate(lv):
+        for kind in range(4):
+            lo, hi = row[16 + kind], row[20 + kind]
+            assert lo == pos and hi >= lo
+            pos = hi
+            it = gv[lo:hi]
+            multi = [q for q in range(lo, hi) if gv[q, 3] == q and gv[q, 2] & (SEG - 1) > 1]
+            assert list(t["gr"][row[24 + kind]: row[28 + kind]]) == multi
+            assert row[24 + kind] == gpos
+            gpos = row[28 + kind]
+            fs = np.unique(it[:, 0])
+            assert np.all(level[fs] == h)
+            expect = [i for i in np.nonzero(level == h)[0] if min(_kind_shape(kind, k[i], b[i])) > 0]
+            assert np.array_equal(fs, np.array(expect, dtype=fs.dtype).reshape(-1))
+            q = lo
+            while q < hi:
+                f, r0, z, first = gv[q]
+                nseg = z & (SEG - 1)
+                assert first == q and nseg >= 1 and q + nseg <= hi
+                rows, cols = _kind_shape(kind, k[f], b[f])
+                chunk = gv[q:q + nseg]
+                assert np.all(chunk[:, 0] == f) and np.all(chunk[:, 1] == r0) and np.all(chunk[:, 3] == q)
+                assert np.all(chunk[:, 2] & (SEG - 1) == nseg)
+                c0 = chunk[:, 2] & ~(SEG - 1)
+                assert np.all(np.diff(c0) == SEG) and np.all(c0 < cols) and 0 <= r0 < rows and r0 % 64 == 0
+                need_lo = r0 if kind == 3 else 0
+                need_hi = min(r0 + 64, rows) if kind == 0 else cols
+                assert c0[0] <= need_lo and min(c0[-1] + SEG, cols) >= need_hi
+                # no segment lies wholly outside the triangle its rows meet
+                assert c0[0] + SEG > need_lo and c0[-1] < need_hi
+                q += nseg
+            ncol = np.minimum(SEG, np.array([_kind_shape(kind, k[f], b[f])[1] for f in it[:, 0]], np.int64)
+                              - (it[:, 2] & ~(SEG - 1)))
+            size = 64 * ((ncol + 1) & ~1)
+            off = t["gv_off"][lo:hi]
+            assert np.array_equal(off[1:], (off + size)[:-1])
+            assert hi == lo or off[0] == (t["gv_off"][lo - 1] + tile_size[0] if lo else 0)
+            if hi > lo:
+                tile_size[0] = size[-1]
+            # chunks in row order, one per 64 rows of every front of the list
+            for i in expect:
+                rows, _ = _kind_shape(kind, k[i], b[i])
+                assert sorted(set(it[it[:, 0] == i][:, 1])) == list(range(0, rows, 64))
+    assert pos == len(gv) and gpos == len(t["gr"])
+    assert t["sz"]["gv_total"] == (t["gv_off"][-1] + tile_size[0] if len(gv) else 0)
 
 
 def _replay(t, net, level):
@@ -219,29 +292,59 @@ def _replay(t, net, level):
     def gathered(v, at):
         return v[t["gsrc"][t["gptr"][at]: t["gptr"][at + 1]]].sum()
 
+    def block(kind, i):
+        kk = k[i]
+        if kind == 0:
+            return np.tril(INV[inv[i]: inv[i] + kk * kk].reshape(kk, kk, order="F"))
+        if kind == 3:
+            return np.triu(INV[inv[i] + kk * kk: inv[i] + 2 * kk * kk].reshape(kk, kk, order="F"))
+        return front(i)[kk:, :kk] if kind == 1 else front(i)[:kk, kk:]
+
+    def products(kind, h, V, x, bvec):
+        """One launch of the segmented products: item partials, summed per chunk in segment order,
+        then the kind's epilogue (the tables drive everything, as on the device)."""
+        gv, row = t["gv"], t["lv"][h]
+        out = {}
+        for f, r0, z, first in gv[row[16 + kind]: row[20 + kind]]:
+            kk, bb = k[f], m[f] - k[f]
+            rows, cols = _kind_shape(kind, kk, bb)
+            c0 = z & ~(SEG - 1)
+            c1 = min(cols, c0 + SEG)
+            cs = np.arange(c0, c1)
+            if kind == 0:
+                pv = PIV[piv[f] + cs]
+                ixp = t["idx"][idx[f] + pv]
+                xin = np.array([bvec[g] * req[g] + gathered(V, gat[f] + j) for g, j in zip(ixp, pv)])
+            elif kind == 2:
+                xin = x[t["idx"][idx[f] + kk + cs]]
+            else:
+                xin = V[vec[f] + cs]
+            r1 = min(rows, r0 + 64)
+            out.setdefault((f, r0), []).append(block(kind, f)[r0:r1, c0:c1] @ xin)
+        for (f, r0), parts in out.items():
+            s_ = parts[0]
+            for p_ in parts[1:]:
+                s_ = s_ + p_
+            kk = k[f]
+            rr = np.arange(r0, r0 + len(s_))
+            if kind == 0:
+                V[vec[f] + rr] = s_
+            elif kind == 1:
+                V[vec[f] + kk + rr] = np.array([gathered(V, gat[f] + kk + r) for r in rr]) - s_
+            elif kind == 2:
+                V[vec[f] + rr] -= s_
+            else:
+                x[t["idx"][idx[f] + rr]] = s_
+
     def solve(bvec):
         V = np.zeros(sz["nvec"])
         x = np.zeros(N)
         for h in range(H):
-            for i in np.nonzero(level == h)[0]:
-                kk, bb = k[i], m[i] - k[i]
-                ix = t["idx"][idx[i]: idx[i] + m[i]]
-                pv = PIV[piv[i]: piv[i] + kk]
-                z = np.array([bvec[ix[j]] * req[ix[j]] + gathered(V, gat[i] + j) for j in pv])
-                Li = INV[inv[i]: inv[i] + kk * kk].reshape(kk, kk, order="F")
-                v = Li @ z
-                V[vec[i]: vec[i] + kk] = v
-                Fm = front(i)
-                for r in range(bb):
-                    V[vec[i] + kk + r] = gathered(V, gat[i] + kk + r) - Fm[kk + r, :kk] @ v
+            products(0, h, V, x, bvec)
+            products(1, h, V, x, bvec)
         for h in range(H - 1, -1, -1):
-            for i in np.nonzero(level == h)[0]:
-                kk = k[i]
-                ix = t["idx"][idx[i]: idx[i] + m[i]]
-                Fm = front(i)
-                w = V[vec[i]: vec[i] + kk] - Fm[:kk, kk:] @ x[ix[kk:]]
-                Ui = INV[inv[i] + kk * kk: inv[i] + 2 * kk * kk].reshape(kk, kk, order="F")
-                x[ix[:kk]] = Ui @ w
+            products(2, h, V, x, bvec)
+            products(3, h, V, x, bvec)
         return x
 
     def solve_t(cvec):
@@ -278,7 +381,8 @@ def test_symbolic_tables_and_replayed_solves(lib, name):
     t = _analyse(lib, net)
     level = _check_bounds(t, net)
     print(f"{name}: {net['N']} unknowns, {t['sz']['fronts']} fronts on {t['sz']['levels']} levels, largest front "
-          f"{int(t['fr'][:, 2].max())}, nnz(L+U) {t['sz']['nnz_factors']} (dense {net['N'] ** 2})")
+          f"{int(t['fr'][:, 2].max())}, nnz(L+U) {t['sz']['nnz_factors']} (dense {net['N'] ** 2}), solve tiles "
+          f"{t['sz']['gv_total']}")
     if name == "cfg2":
         return  # bounds only at this size: the replay loops are per row in Python
     solve, solve_t = _replay(t, net, level)
@@ -292,3 +396,26 @@ def test_symbolic_tables_and_replayed_solves(lib, name):
         y = solve_t(bvec)
         yr = spla.spsolve(A.T.tocsc(), bvec)
         assert np.max(np.abs(y - yr)) <= 1e-10 * np.max(np.abs(yr))
+
+
+@pytest.mark.parametrize("share", [0.5, 1.0])
+def test_lifted_pivots_and_multi_segment_solves(lib, share):
+    """Delayed pivots lifted towards the root (as the numeric delay rounds do) on the 200-fracture
+    network: with half or all of the unknowns lifted by up to 99 levels the root front holds more
+    than 256 pivots, so the solve products of its rows run over several column segments whose
+    partials are summed per chunk; the tables stay in bounds and the replayed solves still match
+    SciPy."""
+    net = _network("dfn200")
+    rng = np.random.default_rng(3)
+    delay = np.where(rng.random(net["N"]) < share, 99, 0).astype(np.int32)
+    t = _analyse(lib, net, delay)
+    level = _check_bounds(t, net)
+    nseg = t["gv"][:, 2] & (SEG - 1)
+    assert t["fr"][:, 3].max() > SEG and nseg.max() >= 2
+    solve, solve_t = _replay(t, net, level)
+    A = net["A"].tocsc()
+    bvec = rng.standard_normal(net["N"])
+    x, xr = solve(bvec), spla.spsolve(A, bvec)
+    assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
+    y, yr = solve_t(bvec), spla.spsolve(A.T.tocsc(), bvec)
+    assert np.max(np.abs(y - yr)) <= 1e-10 * np.max(np.abs(yr))
