@@ -365,141 +365,153 @@ __device__ __forceinline__ void seg_block(const MfFront& f, const double* __rest
   else { M = INV + f.inv + (int64_t)k * k; ld = k; }
 }
 
-// After a factorisation: copy each item's block into its tile.  Tile layout: the value of row
-// r0 + l and column c0 + 2p + e at tile[(p * 64 + l) * 2 + e]; rows beyond the block, the odd
-// column of an odd segment and entries outside a triangular block's triangle hold zeros.
-template <int KIND>
+// After a factorisation: copy each panel's rows of its block into the panel's tile (layout in
+// mf_symbolic.hpp); thread t = R p + l of the workgroup holds row r0 + l, columns 2 p and 2 p + 1
+// of every column block.
+template <int KIND, int R>
 __global__ __launch_bounds__(256) void mfs_pack_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
                                                        const int64_t* __restrict__ toff, const double* __restrict__ F,
                                                        const double* __restrict__ INV, double* __restrict__ T) {
+  constexpr int C = kMfBlockValues / R;
+  static_assert(kMfBlockValues == 512, "a column block is 256 threads x 2 values");
   const int4 it = items[blockIdx.x];
   const MfFront f = fr[it.x];
   const double* M;
   int64_t ld;
   int rows, cols;
   seg_block<KIND>(f, F, INV, M, ld, rows, cols);
-  const int r0 = it.y, c0 = it.z & ~(kMfSeg - 1);
-  const int ncol = min(kMfSeg, cols - c0), npair = (ncol + 1) >> 1;
-  double2* out = reinterpret_cast<double2*>(T + toff[blockIdx.x]);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = r0 + lane;
-  for (int p = w; p < npair; p += 4) {
+  const int l = threadIdx.x % R, p = threadIdx.x / R, r = it.y + l;
+  double2* out = reinterpret_cast<double2*>(T + toff[blockIdx.x]) + threadIdx.x;
+  for (int cb = 0; cb < it.w; ++cb) {
     double v[2];
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
-      const int c = c0 + 2 * p + e;
-      bool ok = r < rows && 2 * p + e < ncol;
+      const int c = it.z + cb * C + 2 * p + e;
+      bool ok = r < rows && c < cols;
       if (KIND == 0) ok = ok && c <= r;
       if (KIND == 3) ok = ok && c >= r;
       v[e] = ok ? M[r + (int64_t)c * ld] : 0.0;
     }
-    out[p * 64 + lane] = make_double2(v[0], v[1]);
+    out[cb * 256] = make_double2(v[0], v[1]);
   }
 }
 
-// The epilogue of a 64-row chunk's product t (row r of the kind's block):
+// N-form products for A x = b, one row panel per workgroup, every row computed in full, so no
+// partial leaves the workgroup:
 //   kind 0  vec_U  = L11^-1 P (R b + children)_U
 //   kind 1  vec_B  = (children)_B - L21 vec_U
 //   kind 2  vec_U -= U12 x_B
 //   kind 3  x_U    = U11^-1 vec_U   (scattered to the solver's layout)
-template <int KIND>
-__device__ __forceinline__ void seg_epilogue(const MfFront& f, int r, double t, const int* __restrict__ idx,
-                                             const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                             double* __restrict__ x, double* __restrict__ vec) {
-  if (KIND == 0) vec[f.vec + r] = t;
-  else if (KIND == 1) vec[f.vec + f.k + r] = gathered(gptr, gsrc, vec, f.gat + f.k + r) - t;
-  else if (KIND == 2) vec[f.vec + r] -= t;
-  else x[idx[f.idx + r]] = t;
-}
-
-// N-form products for A x = b, one item (front, first row r0, first column c0 plus the chunk's
-// segment count, first item of the chunk) per workgroup: the item's 64 x (at most kMfSeg) tile
-// times the kind's input over columns c0.. (lanes over the rows, the four wavefronts over
-// interleaved column pairs, sixteen 16-byte loads in flight per wavefront, four accumulators, a
-// fixed-order sum).  A chunk of one segment applies its epilogue here; the segments of a longer
-// chunk write their partials to part[item], summed in segment order by mfs_sum_kernel in the next
-// launch (the kernel boundary orders the two; a cross-workgroup handshake inside one launch would
-// need device-scope fences, which write back and invalidate the L2 of the XCD on every use).
-template <int KIND>
-__global__ __launch_bounds__(256) void mfs_seg_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
-                                                      int base, const int64_t* __restrict__ toff,
-                                                      const double* __restrict__ T, const int* __restrict__ idx,
-                                                      const int* __restrict__ piv, const int* __restrict__ gptr,
-                                                      const int* __restrict__ gsrc, const double* __restrict__ b,
-                                                      const double* __restrict__ req, double* __restrict__ x,
-                                                      double* __restrict__ vec, double* __restrict__ part) {
-  __shared__ double xs[kMfSeg];
-  __shared__ double ps[4][64];
-  const int4 it = items[blockIdx.x];
-  const MfFront f = fr[it.x];
-  const int k = f.k, nb = f.m - f.k;
+// The panel's record carries its front's fields, so one scalar load starts it; the first batch of
+// tile loads and the values the epilogue needs are issued before the input is staged, which keeps
+// the dependent memory round trips of a small panel few.  The kind's input over the panel's
+// columns is staged in LDS kMfStage columns at a time; thread t = 256 q + R p + l accumulates row
+// r0 + l over columns 2 p, 2 p + 1 of the column blocks q, q + NQ, ... (eight 16-byte loads in
+// flight, the workgroup streaming the panel's tile front to back), and the (256 / R) NQ partials
+// of a row are summed in a fixed order (quarters, then a pairwise tree over the column pairs).  NQ = 4 (1,024 threads)
+// for launches of few, long panels, so that the loads in flight still cover the memory latency of
+// the whole device.
+constexpr int kMfStage = 2048;
+template <int KIND, int NQ, int R>
+__global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1 ? 8 : 6))) void mfs_panel_kernel(
+    const MfPanel* __restrict__ panels, const double* __restrict__ T, const int* __restrict__ idx,
+    const int* __restrict__ piv, const int* __restrict__ gptr, const int* __restrict__ gsrc,
+    const double* __restrict__ b, const double* __restrict__ req, double* __restrict__ x, double* __restrict__ vec) {
+  constexpr int C = kMfBlockValues / R, NP = 256 / R;  // columns per block, column pairs per block
+  __shared__ double xs[kMfStage];
+  __shared__ double ps[NP * NQ][R + 1];
+  const MfPanel pn = panels[blockIdx.x];
+  const int k = pn.k, nb = pn.m - pn.k;
   const int rows = KIND == 1 ? nb : k, cols = KIND == 2 ? nb : k;
-  const int r0 = it.y, c0 = it.z & ~(kMfSeg - 1), nseg = it.z & (kMfSeg - 1);
-  const int ncol = min(kMfSeg, cols - c0), npair = (ncol + 1) >> 1;
-  for (int j = threadIdx.x; j < kMfSeg; j += 256) {
-    const int c = c0 + j;
-    double v = 0.0;  // beyond the segment: zero (the tile's padding column meets it)
-    if (j < ncol) {
-      if (KIND == 0) {
-        const int pj = piv[f.piv + c];
-        const int g = idx[f.idx + pj];
-        v = fma(b[g], req[g], gathered(gptr, gsrc, vec, f.gat + pj));
-      } else if (KIND == 2) {
-        v = x[idx[f.idx + k + c]];
-      } else {
-        v = vec[f.vec + c];
+  const int ncol = pn.nblk * C;
+  const int t = threadIdx.x, l = t % R, p = (t & 255) / R, q = t >> 8, r = pn.r0 + t;
+  // the panel's tile from a uniform base with 32-bit offsets (a tile is far below 4 GiB)
+  const char* P = reinterpret_cast<const char*>(T + pn.toff);
+  const unsigned tb = (unsigned)(t & 255) * 16u;
+  auto ld = [&](int cb) { return *reinterpret_cast<const double2*>(P + ((unsigned)cb * 4096u + tb)); };
+  double2 m[8];
+  {
+    const int nblk0 = min(kMfStage, ncol) / C;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int cb = q + u * NQ;
+      m[u] = cb < nblk0 ? ld(cb) : make_double2(0.0, 0.0);
+    }
+  }
+  double pre = 0.0;  // epilogue input of row r (threads t < R)
+  int xi = 0;
+  if (t < R && r < rows) {
+    if (KIND == 1) pre = gathered(gptr, gsrc, vec, pn.gat + k + r);
+    if (KIND == 2) pre = vec[pn.vec + r];
+    if (KIND == 3) xi = idx[pn.idx + r];
+  }
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  bool have = true;  // m holds the first batch of the first stage
+  for (int s0 = 0; s0 < ncol; s0 += kMfStage) {
+    const int ns = min(kMfStage, ncol - s0), nblk = ns / C;
+    __syncthreads();  // the previous stage is consumed
+    for (int j = t; j < kMfStage; j += 256 * NQ) {
+      const int c = pn.c_lo + s0 + j;
+      double v = 0.0;  // beyond the block or the stage: zero (the tile holds zeros there too)
+      if (j < ns && c < cols) {
+        if (KIND == 0) {
+          const int pj = piv[pn.piv + c];
+          const int g = idx[pn.idx + pj];
+          v = fma(b[g], req[g], gathered(gptr, gsrc, vec, pn.gat + pj));
+        } else if (KIND == 2) {
+          v = x[idx[pn.idx + k + c]];
+        } else {
+          v = vec[pn.vec + c];
+        }
+      }
+      xs[j] = v;
+    }
+    __syncthreads();
+    const int sb = s0 / C;
+    for (int cb = q; cb < nblk; cb += 8 * NQ) {
+      if (!have) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = cb + u * NQ;
+          m[u] = c < nblk ? ld(sb + c) : make_double2(0.0, 0.0);
+        }
+      }
+      have = false;
+#pragma unroll
+      for (int u = 0; u < 8; u += 2) {
+        // a block beyond the stage holds zeros in m and meets a finite staged value (the index
+        // wraps inside the stage buffer, every entry of which the staging wrote)
+        const int j0 = ((cb + u * NQ) * C + 2 * p) & (kMfStage - 1);
+        const int j1 = ((cb + (u + 1) * NQ) * C + 2 * p) & (kMfStage - 1);
+        a0 = fma(m[u].x, xs[j0], a0);
+        a1 = fma(m[u].y, xs[j0 + 1], a1);
+        a2 = fma(m[u + 1].x, xs[j1], a2);
+        a3 = fma(m[u + 1].y, xs[j1 + 1], a3);
       }
     }
-    xs[j] = v;
   }
+  ps[q * NP + p][l] = (a0 + a1) + (a2 + a3);
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, r = r0 + lane;
-  const double2* T2 = reinterpret_cast<const double2*>(T + toff[base + blockIdx.x]) + lane;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int p = w;
-  for (; p + 60 < npair; p += 64) {
-    double2 m[16];
-#pragma unroll
-    for (int u = 0; u < 16; ++u) m[u] = T2[(p + 4 * u) * 64];
-#pragma unroll
-    for (int u = 0; u < 16; u += 2) {
-      const int j = 2 * (p + 4 * u);
-      a0 = fma(m[u].x, xs[j], a0);
-      a1 = fma(m[u].y, xs[j + 1], a1);
-      a2 = fma(m[u + 1].x, xs[j + 8], a2);
-      a3 = fma(m[u + 1].y, xs[j + 9], a3);
+  if (NQ > 1) {
+    if (t < 256) {  // thread (p, l) folds the quarters of its own partial
+      double v = ps[p][l];
+      for (int i = 1; i < NQ; ++i) v += ps[i * NP + p][l];
+      ps[p][l] = v;
     }
+    __syncthreads();
   }
-  for (; p < npair; p += 4) {
-    const double2 m = T2[p * 64];
-    a0 = fma(m.x, xs[2 * p], a0);
-    a1 = fma(m.y, xs[2 * p + 1], a1);
+  // the column pairs of a row, as a pairwise tree in a fixed order
+#pragma unroll
+  for (int h = NP / 2; h >= 1; h >>= 1) {
+    if (t < h * R) ps[p][l] += ps[p + h][l];
+    __syncthreads();
   }
-  ps[w][lane] = (a0 + a1) + (a2 + a3);
-  __syncthreads();
-  if (w != 0) return;
-  const double t = (ps[0][lane] + ps[1][lane]) + (ps[2][lane] + ps[3][lane]);
-  if (nseg > 1) part[(int64_t)(base + blockIdx.x) * 64 + lane] = t;
-  else if (r < rows) seg_epilogue<KIND>(f, r, t, idx, gptr, gsrc, x, vec);
-}
-
-// the chunks of more than one segment: 64 threads per chunk sum its partials in segment order and
-// apply the epilogue
-template <int KIND>
-__global__ __launch_bounds__(256) void mfs_sum_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ items,
-                                                      const int* __restrict__ chunks, int nchunks,
-                                                      const double* __restrict__ part, const int* __restrict__ idx,
-                                                      const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                      double* __restrict__ x, double* __restrict__ vec) {
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (c >= nchunks) return;
-  const int first = chunks[c];
-  const int4 it = items[first];
-  const MfFront f = fr[it.x];
-  const int rows = KIND == 1 ? f.m - f.k : f.k, r = it.y + lane, nseg = it.z & (kMfSeg - 1);
-  if (r >= rows) return;
-  double t = 0.0;
-  for (int q = 0; q < nseg; ++q) t += part[(int64_t)(first + q) * 64 + lane];
-  seg_epilogue<KIND>(f, r, t, idx, gptr, gsrc, x, vec);
+  if (t >= R || r >= rows) return;
+  const double sum = ps[0][t];
+  if (KIND == 0) vec[pn.vec + r] = sum;
+  else if (KIND == 1) vec[pn.vec + k + r] = pre - sum;
+  else if (KIND == 2) vec[pn.vec + r] = pre - sum;
+  else x[xi] = sum;
 }
 
 // T-form: wavefront w of the workgroup takes outputs o0 + 16 w .. o0 + 16 w + 15 of the chunk;
@@ -634,10 +646,9 @@ void MultifrontalLU::release() {
   dev_free(d_info_);
   dev_free(d_growth_);
   dev_free(d_gv_);
-  dev_free(d_part_);
   dev_free(d_gvoff_);
+  dev_free(d_panel_);
   dev_free(d_tile_);
-  dev_free(d_gr_);
   dev_free(d_r_);
   dev_free(d_ay_);
   dev_free(d_x0_);
@@ -678,9 +689,13 @@ void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
     if (!s.gv.empty()) MF_HIP(hipMemcpyAsync(g, s.gv.data(), sizeof(int4) * s.gv.size(), hipMemcpyHostToDevice, st));
     d_gv_ = g;
     dev_upload(d_gvoff_, s.gv_off, st);
+    std::vector<MfPanel> pn(s.gv.size());
+    for (size_t i = 0; i < s.gv.size(); ++i) {
+      const MfFront& f = s.fronts[s.gv[i].x];
+      pn[i] = MfPanel{s.gv_off[i], s.gv[i].y, s.gv[i].z, s.gv[i].w, f.k, f.m, f.vec, f.gat, f.piv, f.idx, 0};
+    }
+    dev_upload(d_panel_, pn, st);
     dev_alloc(d_tile_, (size_t)s.gv_total);
-    dev_alloc(d_part_, s.gv.size() * 64);
-    dev_upload(d_gr_, s.gr, st);
   }
   dev_upload(d_asm_, s.asmap, st);
   dev_upload(d_unk_, s.unk, st);
@@ -790,19 +805,30 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
                          d_tiles_ + L.tile_beg, d_F_);
     for (int fi : L.big) factor_big(fi, st, rb);
   }
-  // the solve tiles of every item
+  // the solve tiles of every panel
   for (const MfLevel& L : sym_.levels)
     for (int kind = 0; kind < 4; ++kind) {
       const int n = L.gv_end[kind] - L.gv_beg[kind];
       if (n <= 0) continue;
       const int4* items = d_gv_ + L.gv_beg[kind];
       const int64_t* toff = d_gvoff_ + L.gv_beg[kind];
+      const int R = L.prow[kind];
+#define MF_PACK(K, RR) hipLaunchKernelGGL((mfs_pack_kernel<K, RR>), dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_)
+#define MF_PACK_K(K)  \
+  if (R == 16)         \
+    MF_PACK(K, 16);    \
+  else if (R == 8)     \
+    MF_PACK(K, 8);     \
+  else                 \
+    MF_PACK(K, 4)
       switch (kind) {
-        case 0: hipLaunchKernelGGL(mfs_pack_kernel<0>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
-        case 1: hipLaunchKernelGGL(mfs_pack_kernel<1>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
-        case 2: hipLaunchKernelGGL(mfs_pack_kernel<2>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
-        default: hipLaunchKernelGGL(mfs_pack_kernel<3>, dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_); break;
+        case 0: MF_PACK_K(0); break;
+        case 1: MF_PACK_K(1); break;
+        case 2: MF_PACK_K(2); break;
+        default: MF_PACK_K(3); break;
       }
+#undef MF_PACK_K
+#undef MF_PACK
     }
   MF_HIP(hipGetLastError());
   int status = 0;
@@ -816,29 +842,36 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
   return bad < 0 ? 0 : 1 + bad;
 }
 
+template <int K, int NQ, int R>
+static void launch_panels(int n, const MfPanel* pn, const double* T, const int* idx, const int* piv, const int* gptr,
+                          const int* gsrc, const double* b, const double* req, double* x, double* vec, hipStream_t st) {
+  hipLaunchKernelGGL((mfs_panel_kernel<K, NQ, R>), dim3(n), dim3(256 * NQ), 0, st, pn, T, idx, piv, gptr, gsrc, b, req, x,
+                     vec);
+}
+
+template <int K>
+static void launch_panels_k(int wide, int R, int n, const MfPanel* pn, const double* T, const int* idx, const int* piv,
+                            const int* gptr, const int* gsrc, const double* b, const double* req, double* x,
+                            double* vec, hipStream_t st) {
+#define MF_PANELS(NQ, RR) launch_panels<K, NQ, RR>(n, pn, T, idx, piv, gptr, gsrc, b, req, x, vec, st)
+  if (R == 16 && wide) MF_PANELS(4, 16);  // the analysis makes only 16-row panels wide
+  else if (R == 16) MF_PANELS(1, 16);
+  else if (R == 8) MF_PANELS(1, 8);
+  else MF_PANELS(1, 4);
+#undef MF_PANELS
+}
+
 void MultifrontalLU::launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x,
                                 hipStream_t st) {
   const int n = L.gv_end[kind] - L.gv_beg[kind];
   if (n <= 0) return;
-  const int4* items = d_gv_ + L.gv_beg[kind];
-  const int base = L.gv_beg[kind];
-  const int nc = L.gr_end[kind] - L.gr_beg[kind];
-  const dim3 gsum((unsigned)((nc + 3) / 4));
-  const int* chunks = d_gr_ + L.gr_beg[kind];
+  const MfPanel* pn = d_panel_ + L.gv_beg[kind];
+  const int w = L.pwide[kind], R = L.prow[kind];
   switch (kind) {
-#define MF_SEG(K)                                                                                                  \
-  case K:                                                                                                          \
-    hipLaunchKernelGGL(mfs_seg_kernel<K>, dim3(n), dim3(256), 0, st, d_fr_, items, base, d_gvoff_, d_tile_,       \
-                       d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, d_part_);                             \
-    if (nc > 0)                                                                                                    \
-      hipLaunchKernelGGL(mfs_sum_kernel<K>, gsum, dim3(256), 0, st, d_fr_, d_gv_, chunks, nc, d_part_, d_idx_,    \
-                         d_gptr_, d_gsrc_, x, d_vec_);                                                             \
-    break;
-    MF_SEG(0)
-    MF_SEG(1)
-    MF_SEG(2)
-    MF_SEG(3)
-#undef MF_SEG
+    case 0: launch_panels_k<0>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
+    case 1: launch_panels_k<1>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
+    case 2: launch_panels_k<2>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
+    default: launch_panels_k<3>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
   }
 }
 

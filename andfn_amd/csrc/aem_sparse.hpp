@@ -30,6 +30,15 @@
 
 namespace aem {
 
+// a solve panel with its front's fields (one record per workgroup of the solve kernels)
+struct MfPanel {
+  int64_t toff;               // first value of its tile
+  int r0, c_lo, nblk;         // first row, first column, column blocks
+  int k, m, vec, gat, piv, idx;  // of its front
+  int pad;
+};
+static_assert(sizeof(MfPanel) == 48, "panel record layout");
+
 class MultifrontalLU {
  public:
   MultifrontalLU() = default;
@@ -80,11 +89,10 @@ class MultifrontalLU {
   int2* d_ck_ = nullptr;
   int2* d_cb_ = nullptr;
   int2* d_ext_ = nullptr;
-  int4* d_gv_ = nullptr;        // solve items (MfSymbolic::gv)
-  int64_t* d_gvoff_ = nullptr;  // per solve item: first value of its tile
-  double* d_tile_ = nullptr;    // the solve tiles (repacked after every factorisation)
-  double* d_part_ = nullptr;    // per solve item: its 64 partial sums
-  int* d_gr_ = nullptr;         // first item of every solve chunk of more than one segment
+  int4* d_gv_ = nullptr;        // solve panels (MfSymbolic::gv)
+  int64_t* d_gvoff_ = nullptr;  // per solve panel: first value of its tile
+  double* d_tile_ = nullptr;    // the panel tiles (repacked after every factorisation)
+  MfPanel* d_panel_ = nullptr;  // per solve panel: its record
   int64_t* d_asm_ = nullptr;
   int* d_unk_ = nullptr;     // every analysed unknown (its position in the solver's layout)
   int* d_status_ = nullptr;

@@ -29,6 +29,8 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   out = MfSymbolic{};
   const int F = in.F, E = in.E;
   const int64_t N = in.N;
+  if (in.panel_rows != 0 && in.panel_rows != 4 && in.panel_rows != 8 && in.panel_rows != 16)
+    throw std::runtime_error("multifrontal analysis: panel rows must be 0, 4, 8 or 16");
   const int* rp = in.csr_ptr;
   const int* cl = in.csr_col;
   const int64_t nnz = N > 0 ? rp[N] : 0;
@@ -490,29 +492,46 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
     L.ck_end = (int)ck.size();
     L.cb_end = (int)cb.size();
     for (int kind = 0; kind < 4; ++kind) {
+      // panel rows: the tallest of 16, 8, 4 that still gives the launch kMfWidePanels panels (kind
+      // 0 keeps 16: each of its panels gathers its own inputs, so more panels cost more gathers)
+      auto panels = [&](int R) {
+        int64_t n = 0;
+        for (int i : byh[h]) {
+          int rows, cols;
+          mf_panel_shape(kind, fr[i].k, fr[i].m - fr[i].k, &rows, &cols);
+          if (cols > 0) n += (rows + R - 1) / R;
+        }
+        return n;
+      };
+      int R = 16;
+      if (kind != 0 && in.panel_rows) {
+        R = in.panel_rows;
+      } else if (kind != 0) {
+        while (R > 4 && panels(R) < kMfWidePanels) R /= 2;
+        if (panels(R) < kMfWidePanels / 4) R = 16;  // very few panels even at 4 rows: 16 rows, wide
+      }
+      L.prow[kind] = R;
       L.gv_beg[kind] = (int)out.gv.size();
-      L.gr_beg[kind] = (int)out.gr.size();
+      int64_t blocks = 0;
       for (int i : byh[h]) {
         const MfFront& f = fr[i];
         const int b = f.m - f.k;
         int rows, cols;
-        mf_gemv_shape(kind, f.k, b, &rows, &cols);
-        if (rows == 0 || cols == 0) continue;
-        for (int r0 = 0; r0 < rows; r0 += 64) {
-          int s_lo, s_hi;
-          mf_gemv_segments(kind, f.k, b, r0, &s_lo, &s_hi);
-          const int first = (int)out.gv.size();
-          if (s_hi - s_lo > 1) out.gr.push_back(first);
-          for (int sg = s_lo; sg < s_hi; ++sg) {
-            out.gv.push_back(MfI4{i, r0, sg * kMfSeg + (s_hi - s_lo), first});
-            const int ncol = std::min(kMfSeg, cols - sg * kMfSeg);
-            out.gv_off.push_back(out.gv_total);
-            out.gv_total += (int64_t)kMfTileRows * ((ncol + 1) & ~1);
-          }
+        mf_panel_shape(kind, f.k, b, &rows, &cols);
+        if (cols == 0) continue;
+        for (int r0 = 0; r0 < rows; r0 += R) {
+          int c_lo, nblk;
+          mf_panel_columns(kind, f.k, b, R, r0, &c_lo, &nblk);
+          out.gv.push_back(MfI4{i, r0, c_lo, nblk});
+          out.gv_off.push_back(out.gv_total);
+          out.gv_total += (int64_t)kMfBlockValues * nblk;
+          blocks += nblk;
         }
       }
       L.gv_end[kind] = (int)out.gv.size();
-      L.gr_end[kind] = (int)out.gr.size();
+      // few 16-row panels of many blocks each: 1,024 threads per panel
+      const int n = L.gv_end[kind] - L.gv_beg[kind];
+      L.pwide[kind] = R == 16 && n > 0 && n < kMfWidePanels && blocks >= 8 * (int64_t)n ? 1 : 0;
     }
     if ((int64_t)std::max(L.maxk, L.maxb) * 8 > kMfMaxLds)
       throw std::runtime_error("multifrontal analysis: a front exceeds the solve kernels' LDS");

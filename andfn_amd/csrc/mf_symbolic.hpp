@@ -52,6 +52,7 @@ struct MfInput {
   const int* csr_ptr = nullptr;    // [N + 1] pattern of A in that layout (structurally symmetric,
   const int* csr_col = nullptr;    //  columns sorted within a row)
   const int* delay = nullptr;      // [N] or null: levels to lift each unknown above its node (delayed pivots)
+  int panel_rows = 0;              // 0: solve panel heights chosen per level; 4, 8 or 16: that height for kinds 1-3
 };
 
 struct MfLevel {
@@ -62,32 +63,33 @@ struct MfLevel {
   int cb_beg, cb_end;                 // 64-row chunks over the boundary rows
   int maxk, maxb;                     // largest pivot count and boundary of its fronts
   std::vector<int> ext_beg, ext_end;  // extend-add items per child slot
-  int gv_beg[4], gv_end[4];           // solve products, per kind (mf_gemv_shape / mf_gemv_segments)
-  int gr_beg[4], gr_end[4];           // chunks of more than one segment, per kind (their partials to sum)
+  int gv_beg[4], gv_end[4];           // solve panels, per kind (mf_panel_shape / mf_panel_columns)
+  int prow[4], pwide[4];              // per kind: panel rows (16, 8 or 4); 1 if a panel takes 1,024 threads
 };
 
-// The products of the solve A x = b, per level and kind, are cut into items of 64 rows by
-// kMfSeg columns: kind 0 L11^-1 (k x k, lower), 1 L21 (b x k), 2 U12 (k x b), 3 U11^-1 (k x k,
-// upper).  A 64-row chunk's items are consecutive; a triangular kind has only the segments that
-// meet the triangle.  An item's first column is a multiple of kMfSeg and carries the chunk's
-// segment count in its low bits (at most kMfMaxLds / 8 / kMfSeg = 30 segments).  After each
-// factorisation every item's block is copied into a tile of its own (64 rows, the segment's columns
-// rounded up to even, column pairs interleaved per row, zeros outside the block and its triangle),
-// so that a solve streams each tile as one contiguous run of memory.
-constexpr int kMfSeg = 256;
-constexpr int kMfTileRows = 64;
-inline void mf_gemv_shape(int kind, int k, int b, int* rows, int* cols) {
+// The products of the solve A x = b are cut, per level and kind, into row panels of R rows (R =
+// 16, 8 or 4, chosen per level and kind so that a launch has enough panels to spread over the
+// device; launches of very few panels keep 16 rows and take 1,024 threads per panel): kind 0
+// L11^-1 (k x k, lower), 1 L21 (b x k), 2 U12 (k x b), 3 U11^-1 (k x k, upper).
+// A panel spans the column blocks (C = 512 / R columns each) its rows need (a triangular kind:
+// only those that meet the triangle), so one workgroup computes each of its rows in full.  After
+// each factorisation every panel is copied into a tile of its own (512 values per column block,
+// within a block the value of row r0 + l and column 2 p + e at (p R + l) 2 + e, zeros outside the
+// block and its triangle), so that a solve streams each panel as one contiguous run of memory.
+constexpr int kMfBlockValues = 512;
+constexpr int kMfWidePanels = 1024;  // launches of fewer panels than this (and long ones) go wide
+inline void mf_panel_shape(int kind, int k, int b, int* rows, int* cols) {
   *rows = kind == 1 ? b : k;
   *cols = kind == 2 ? b : k;
 }
-// first and one-past-last segment of the chunk starting at row r0
-inline void mf_gemv_segments(int kind, int k, int b, int r0, int* s_lo, int* s_hi) {
+// first column (a multiple of C = 512 / R) and number of column blocks of the panel at row r0
+inline void mf_panel_columns(int kind, int k, int b, int R, int r0, int* c_lo, int* nblk) {
   int rows, cols;
-  mf_gemv_shape(kind, k, b, &rows, &cols);
-  const int nseg = (cols + kMfSeg - 1) / kMfSeg;
-  const int last_row = r0 + 63 < rows - 1 ? r0 + 63 : rows - 1;
-  *s_lo = kind == 3 ? r0 / kMfSeg : 0;
-  *s_hi = kind == 0 ? (last_row / kMfSeg + 1 < nseg ? last_row / kMfSeg + 1 : nseg) : nseg;
+  mf_panel_shape(kind, k, b, &rows, &cols);
+  const int C = kMfBlockValues / R;
+  const int hi = kind == 0 ? (r0 + R < cols ? r0 + R : cols) : cols;
+  *c_lo = kind == 3 ? r0 / C * C : 0;
+  *nblk = (hi - *c_lo + C - 1) / C;
 }
 
 struct MfSymbolic {
@@ -101,9 +103,8 @@ struct MfSymbolic {
   std::vector<int> unk;               // every analysed unknown
   std::vector<MfI4> tiles;            // (front, first boundary row, first boundary column, 0)
   std::vector<MfI2> ck, cb, ext;      // (front, first row) chunks; (child, first column) items
-  std::vector<MfI4> gv;               // solve items (front, first row, first column + the chunk's segment count, its first item)
-  std::vector<int64_t> gv_off;        // per solve item: first value of its packed tile (kMfTileRows x even column count)
-  std::vector<int> gr;                // first item of every chunk of more than one segment
+  std::vector<MfI4> gv;               // solve panels (front, first row, first column, column blocks)
+  std::vector<int64_t> gv_off;        // per solve panel: first value of its tile (kMfBlockValues per column block)
   std::vector<int64_t> asmap;         // CSR entry -> value position in its front
   int64_t nunk = 0, front_total = 0, inv_total = 0, nnz_factors = 0, gv_total = 0;
   int nvec = 0, max_m = 0, big_k = 0;

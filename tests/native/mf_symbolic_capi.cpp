@@ -12,7 +12,8 @@ static std::string g_err;
 extern "C" {
 
 int mf_test_analyse(int F, int E, const int* etype, const int* f0, const int* f1, const int* de_pos, const int* fpos,
-                    const double* center, long long N, const int* ptr, const int* col, const int* delay) {
+                    const double* center, long long N, const int* ptr, const int* col, const int* delay,
+                    int panel_rows) {
   aem::MfInput in;
   in.F = F;
   in.E = E;
@@ -26,6 +27,7 @@ int mf_test_analyse(int F, int E, const int* etype, const int* f0, const int* f1
   in.csr_ptr = ptr;
   in.csr_col = col;
   in.delay = delay;
+  in.panel_rows = panel_rows;
   try {
     aem::mf_analyse(in, g_sym);
   } catch (const std::exception& e) {
@@ -39,7 +41,7 @@ const char* mf_test_error() { return g_err.c_str(); }
 
 // sizes: 0 fronts, 1 levels, 2 idx, 3 rel, 4 piv0, 5 gptr, 6 gsrc, 7 small_list, 8 unk, 9 tiles,
 // 10 ck, 11 cb, 12 ext, 13 asmap, 14 front_total, 15 inv_total, 16 nvec, 17 nnz_factors, 18 nunk, 19 gv,
-// 20 gv_total, 21 gr
+// 20 gv_total
 long long mf_test_size(int w) {
   const aem::MfSymbolic& s = g_sym;
   switch (w) {
@@ -64,7 +66,6 @@ long long mf_test_size(int w) {
     case 18: return s.nunk;
     case 19: return (long long)s.gv.size();
     case 20: return s.gv_total;
-    case 21: return (long long)s.gr.size();
   }
   return -1;
 }
@@ -80,13 +81,13 @@ void mf_test_fronts(long long* out) {
 }
 
 // levels as 32 ints each: small_beg[3], small_end[3], tile_beg, tile_end, ck_beg, ck_end, cb_beg, cb_end,
-// maxk, maxb, number of big fronts, number of extend slots, gv_beg[4], gv_end[4], gr_beg[4], gr_end[4]
+// maxk, maxb, number of big fronts, number of extend slots, gv_beg[4], gv_end[4], prow[4], pwide[4]
 void mf_test_levels(int* out) {
   for (size_t h = 0; h < g_sym.levels.size(); ++h) {
     const aem::MfLevel& L = g_sym.levels[h];
     int* o = out + 32 * h;
     for (int c = 0; c < 4; ++c) {
-      o[16 + c] = L.gv_beg[c]; o[20 + c] = L.gv_end[c]; o[24 + c] = L.gr_beg[c]; o[28 + c] = L.gr_end[c];
+      o[16 + c] = L.gv_beg[c]; o[20 + c] = L.gv_end[c]; o[24 + c] = L.prow[c]; o[28 + c] = L.pwide[c];
     }
     for (int c = 0; c < 3; ++c) { o[c] = L.small_beg[c]; o[3 + c] = L.small_end[c]; }
     o[6] = L.tile_beg; o[7] = L.tile_end; o[8] = L.ck_beg; o[9] = L.ck_end; o[10] = L.cb_beg; o[11] = L.cb_end;
@@ -109,7 +110,6 @@ void mf_test_ints(int w, int* out) {
   else if (w == 11) std::memcpy(out, s.cb.data(), sizeof(int) * 2 * s.cb.size());
   else if (w == 12) std::memcpy(out, s.ext.data(), sizeof(int) * 2 * s.ext.size());
   else if (w == 13) std::memcpy(out, s.gv.data(), sizeof(int) * 4 * s.gv.size());
-  else if (w == 14) std::memcpy(out, s.gr.data(), sizeof(int) * s.gr.size());
 }
 
 void mf_test_gvoff(long long* out) {

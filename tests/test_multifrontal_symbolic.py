@@ -5,13 +5,13 @@ with g++ and checked on the CPU before any of its tables reaches a GPU kernel.
   index stays inside its buffer (fronts, inverses, index lists, solve vectors, gather lists,
   the parent's positions), every unknown is a pivot of exactly one front, children come before
   their parents and sit on lower levels, every matrix entry has one place in one front, the
-  solve kernels' LDS fits 64 KB, and the segmented solve items cover each block's rows and the
-  columns of its triangle.
+  solve kernels' LDS fits 64 KB, and the solve panels cover each block's rows and the columns of
+  its triangle.
 * Semantics: the factorisation and both solves are replayed in numpy on the analysed tables, step
   by step as the kernels take them (scatter of the row-equilibrated matrix, extend-add through the
   parent positions, LU of each pivot block with partial pivoting over its own rows, L21, U12,
   Schur update, the two inverses; forward and backward solves through the gather lists and pivot
-  orders, the forward and backward products item by item with per-chunk partial sums), and must
+  orders, the forward and backward products panel by panel), and must
   reproduce SciPy's solutions of A x = b and A^T y = c to 1e-10 relative on the
   discharge matrices of four networks (with wells, with constant-head lines, at 5,000 fractures).
 """
@@ -41,7 +41,8 @@ def lib(tmp_path_factory):
     L = ctypes.CDLL(str(so))
     vp = ctypes.c_void_p
     L.mf_test_analyse.restype = ctypes.c_int
-    L.mf_test_analyse.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.c_longlong, vp, vp, vp]
+    L.mf_test_analyse.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp, vp, ctypes.c_longlong, vp, vp, vp,
+                                  ctypes.c_int]
     L.mf_test_error.restype = ctypes.c_char_p
     L.mf_test_size.restype = ctypes.c_longlong
     L.mf_test_size.argtypes = [ctypes.c_int]
@@ -85,15 +86,15 @@ def _network(name):
                 N=A.shape[0], ptr=P.indptr.astype(np.int32), col=P.indices.astype(np.int32), A=A, Av=Av)
 
 
-def _analyse(L, net, delay=None):
+def _analyse(L, net, delay=None, panel_rows=0):
     rc = L.mf_test_analyse(net["F"], net["E"], _p(net["ty"]), _p(net["f0"]), _p(net["f1"]), _p(net["de_pos"]),
                            _p(net["fpos"]), _p(net["center"]), net["N"], _p(net["ptr"]), _p(net["col"]),
-                           None if delay is None else _p(delay))
+                           None if delay is None else _p(delay), panel_rows)
     assert rc == 0, L.mf_test_error()
     sz = {k: L.mf_test_size(i) for i, k in enumerate(["fronts", "levels", "idx", "rel", "piv0", "gptr", "gsrc", "small",
                                                       "unk", "tiles", "ck", "cb", "ext", "asmap", "front_total",
                                                       "inv_total", "nvec", "nnz_factors", "nunk", "gv",
-                                                      "gv_total", "gr"])}
+                                                      "gv_total"])}
     fr = np.zeros((sz["fronts"], 11), np.int64)
     L.mf_test_fronts(_p(fr))
     lv = np.zeros((sz["levels"], 32), np.int32)
@@ -101,7 +102,7 @@ def _analyse(L, net, delay=None):
     t = dict(sz=sz, fr=fr, lv=lv)
     for w, k, shape in ((2, "idx", None), (3, "rel", None), (4, "piv0", None), (5, "gptr", None), (6, "gsrc", None),
                         (7, "small", None), (8, "unk", None), (9, "tiles", 4), (10, "ck", 2), (11, "cb", 2),
-                        (12, "ext", 2), (13, "gv", 4), (14, "gr", None)):
+                        (12, "ext", 2), (13, "gv", 4)):
         n = sz[k]
         a = np.zeros(n * (shape or 1), np.int32)
         L.mf_test_ints(w, _p(a))
@@ -174,7 +175,7 @@ def _check_bounds(t, net):
     return level
 
 
-SEG = 256  # columns per solve item
+BLOCK = 512  # values per column block of a solve panel (R rows x 512 / R columns)
 
 
 def _kind_shape(kind, kk, bb):
@@ -182,63 +183,40 @@ def _kind_shape(kind, kk, bb):
 
 
 def _check_solve_items(t, level):
-    """Items of the segmented solve products: per level and kind, every front of the level with a
-    nonempty block has 64-row chunks of consecutive segments (column multiples of SEG, the chunk's
-    segment count in the low bits, the index of its first item), and a chunk's segments cover every
-    column its rows need: all of them for L21 and U12, columns 0..r for L11^-1, r..k-1 for U11^-1.
-    Every item owns a tile of 64 rows by its column count rounded up to even, the tiles packed
-    back to back in item order; the chunks of more than one segment are listed (their partials
-    are summed by a second launch) per level and kind."""
+    """Row panels of the solve products: per level and kind, a panel height R of 16, 8 or 4 rows
+    (1,024 threads per panel only with 16 rows), and every front of the level with a nonempty block
+    has one panel per R rows, in row order; a panel's column blocks (C = 512 / R columns, the first
+    a multiple of C) cover every column its rows need and none wholly outside: all of them for L21
+    and U12, columns 0..r for L11^-1, r..k-1 for U11^-1.  Every panel owns a tile of 512 values per
+    column block, the tiles packed back to back in panel order."""
     fr, gv, lv = t["fr"], t["gv"], t["lv"]
     k, b = fr[:, 3], fr[:, 2] - fr[:, 3]
-    assert lv[0, 16] == 0 and lv[-1, 23] == len(gv)
-    pos = gpos = 0
-    tile_size = [0]  # size of the last tile seen
+    pos = 0
     for h, row in enumerate(lv):
         for kind in range(4):
             lo, hi = row[16 + kind], row[20 + kind]
+            R, wide = row[24 + kind], row[28 + kind]
+            assert R in (4, 8, 16) and wide in (0, 1) and (R == 16 or not wide)
+            C = BLOCK // R
             assert lo == pos and hi >= lo
             pos = hi
             it = gv[lo:hi]
-            multi = [q for q in range(lo, hi) if gv[q, 3] == q and gv[q, 2] & (SEG - 1) > 1]
-            assert list(t["gr"][row[24 + kind]: row[28 + kind]]) == multi
-            assert row[24 + kind] == gpos
-            gpos = row[28 + kind]
-            fs = np.unique(it[:, 0])
-            assert np.all(level[fs] == h)
             expect = [i for i in np.nonzero(level == h)[0] if min(_kind_shape(kind, k[i], b[i])) > 0]
-            assert np.array_equal(fs, np.array(expect, dtype=fs.dtype).reshape(-1))
-            q = lo
-            while q < hi:
-                f, r0, z, first = gv[q]
-                nseg = z & (SEG - 1)
-                assert first == q and nseg >= 1 and q + nseg <= hi
-                rows, cols = _kind_shape(kind, k[f], b[f])
-                chunk = gv[q:q + nseg]
-                assert np.all(chunk[:, 0] == f) and np.all(chunk[:, 1] == r0) and np.all(chunk[:, 3] == q)
-                assert np.all(chunk[:, 2] & (SEG - 1) == nseg)
-                c0 = chunk[:, 2] & ~(SEG - 1)
-                assert np.all(np.diff(c0) == SEG) and np.all(c0 < cols) and 0 <= r0 < rows and r0 % 64 == 0
-                need_lo = r0 if kind == 3 else 0
-                need_hi = min(r0 + 64, rows) if kind == 0 else cols
-                assert c0[0] <= need_lo and min(c0[-1] + SEG, cols) >= need_hi
-                # no segment lies wholly outside the triangle its rows meet
-                assert c0[0] + SEG > need_lo and c0[-1] < need_hi
-                q += nseg
-            ncol = np.minimum(SEG, np.array([_kind_shape(kind, k[f], b[f])[1] for f in it[:, 0]], np.int64)
-                              - (it[:, 2] & ~(SEG - 1)))
-            size = 64 * ((ncol + 1) & ~1)
-            off = t["gv_off"][lo:hi]
-            assert np.array_equal(off[1:], (off + size)[:-1])
-            assert hi == lo or off[0] == (t["gv_off"][lo - 1] + tile_size[0] if lo else 0)
-            if hi > lo:
-                tile_size[0] = size[-1]
-            # chunks in row order, one per 64 rows of every front of the list
+            assert np.array_equal(np.unique(it[:, 0]), np.array(expect, dtype=it.dtype).reshape(-1))
             for i in expect:
-                rows, _ = _kind_shape(kind, k[i], b[i])
-                assert sorted(set(it[it[:, 0] == i][:, 1])) == list(range(0, rows, 64))
-    assert pos == len(gv) and gpos == len(t["gr"])
-    assert t["sz"]["gv_total"] == (t["gv_off"][-1] + tile_size[0] if len(gv) else 0)
+                rows, cols = _kind_shape(kind, k[i], b[i])
+                mine = it[it[:, 0] == i]
+                assert np.array_equal(mine[:, 1], np.arange(0, rows, R))
+                for _, r0, c_lo, nblk in mine:
+                    need_lo = r0 if kind == 3 else 0
+                    need_hi = min(r0 + R, cols) if kind == 0 else cols
+                    assert c_lo % C == 0 and nblk >= 1
+                    assert c_lo <= need_lo < c_lo + C and need_hi <= c_lo + C * nblk < need_hi + C
+    assert pos == len(gv)
+    size = BLOCK * gv[:, 3].astype(np.int64)
+    off = t["gv_off"]
+    assert (len(gv) == 0 or off[0] == 0) and np.array_equal(off[1:], (off + size)[:-1])
+    assert t["sz"]["gv_total"] == int(size.sum())
 
 
 def _replay(t, net, level):
@@ -301,16 +279,14 @@ def _replay(t, net, level):
         return front(i)[kk:, :kk] if kind == 1 else front(i)[:kk, kk:]
 
     def products(kind, h, V, x, bvec):
-        """One launch of the segmented products: item partials, summed per chunk in segment order,
+        """One launch of the panel products: each panel's rows in full over its column blocks,
         then the kind's epilogue (the tables drive everything, as on the device)."""
         gv, row = t["gv"], t["lv"][h]
-        out = {}
-        for f, r0, z, first in gv[row[16 + kind]: row[20 + kind]]:
+        R = row[24 + kind]
+        for f, r0, c_lo, nblk in gv[row[16 + kind]: row[20 + kind]]:
             kk, bb = k[f], m[f] - k[f]
             rows, cols = _kind_shape(kind, kk, bb)
-            c0 = z & ~(SEG - 1)
-            c1 = min(cols, c0 + SEG)
-            cs = np.arange(c0, c1)
+            cs = np.arange(c_lo, min(cols, c_lo + BLOCK // R * nblk))
             if kind == 0:
                 pv = PIV[piv[f] + cs]
                 ixp = t["idx"][idx[f] + pv]
@@ -319,14 +295,8 @@ def _replay(t, net, level):
                 xin = x[t["idx"][idx[f] + kk + cs]]
             else:
                 xin = V[vec[f] + cs]
-            r1 = min(rows, r0 + 64)
-            out.setdefault((f, r0), []).append(block(kind, f)[r0:r1, c0:c1] @ xin)
-        for (f, r0), parts in out.items():
-            s_ = parts[0]
-            for p_ in parts[1:]:
-                s_ = s_ + p_
-            kk = k[f]
-            rr = np.arange(r0, r0 + len(s_))
+            rr = np.arange(r0, min(rows, r0 + R))
+            s_ = block(kind, f)[rr][:, cs] @ xin
             if kind == 0:
                 V[vec[f] + rr] = s_
             elif kind == 1:
@@ -402,16 +372,14 @@ def test_symbolic_tables_and_replayed_solves(lib, name):
 def test_lifted_pivots_and_multi_segment_solves(lib, share):
     """Delayed pivots lifted towards the root (as the numeric delay rounds do) on the 200-fracture
     network: with half or all of the unknowns lifted by up to 99 levels the root front holds more
-    than 256 pivots, so the solve products of its rows run over several column segments whose
-    partials are summed per chunk; the tables stay in bounds and the replayed solves still match
-    SciPy."""
+    than 256 pivots (panels of many column blocks, staged in several passes on the device when
+    beyond 2,048 columns); the tables stay in bounds and the replayed solves still match SciPy."""
     net = _network("dfn200")
     rng = np.random.default_rng(3)
     delay = np.where(rng.random(net["N"]) < share, 99, 0).astype(np.int32)
     t = _analyse(lib, net, delay)
     level = _check_bounds(t, net)
-    nseg = t["gv"][:, 2] & (SEG - 1)
-    assert t["fr"][:, 3].max() > SEG and nseg.max() >= 2
+    assert t["fr"][:, 3].max() > 256 and t["gv"][:, 3].max() >= 8
     solve, solve_t = _replay(t, net, level)
     A = net["A"].tocsc()
     bvec = rng.standard_normal(net["N"])
@@ -419,3 +387,26 @@ def test_lifted_pivots_and_multi_segment_solves(lib, share):
     assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
     y, yr = solve_t(bvec), spla.spsolve(A.T.tocsc(), bvec)
     assert np.max(np.abs(y - yr)) <= 1e-10 * np.max(np.abs(yr))
+
+
+@pytest.mark.parametrize("panel_rows", [4, 8])
+def test_short_solve_panels(lib, panel_rows):
+    """Solve panels of 4 and 8 rows (what large levels take on the device) on the 200-fracture
+    network: the tables stay in bounds and the replayed solves match SciPy."""
+    net = _network("dfn200")
+    t = _analyse(lib, net, panel_rows=panel_rows)
+    level = _check_bounds(t, net)
+    assert set(t["lv"][:, 25:28].ravel()) == {panel_rows} and set(t["lv"][:, 24]) == {16}
+    solve, solve_t = _replay(t, net, level)
+    rng = np.random.default_rng(5)
+    A = net["A"].tocsc()
+    bvec = rng.standard_normal(net["N"])
+    x, xr = solve(bvec), spla.spsolve(A, bvec)
+    assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
+
+
+def test_panel_rows_are_checked(lib):
+    net = _network("dfn40w")
+    rc = lib.mf_test_analyse(net["F"], net["E"], _p(net["ty"]), _p(net["f0"]), _p(net["f1"]), _p(net["de_pos"]),
+                             _p(net["fpos"]), _p(net["center"]), net["N"], _p(net["ptr"]), _p(net["col"]), None, 5)
+    assert rc == 1 and b"panel rows" in lib.mf_test_error()
