@@ -345,7 +345,7 @@ class Solver {
   int factor_ = ANDFN_MI_FACTOR_SPARSE;
   // multifrontal sparse LU (factor_ == ANDFN_MI_FACTOR_SPARSE): its analysis is built once per
   // handle from the CSR pattern and the fracture centres; a network the analysis cannot take
-  // (a front beyond the solve kernels' LDS) is factored densely instead
+  // (a pattern that is not the discharge matrix's) is factored densely instead
   aem::MultifrontalLU mf_;
   bool mf_failed_ = false;
   bool sparse_ = false;  // the current factors are the multifrontal ones
@@ -446,15 +446,23 @@ class Solver {
   void assemble_and_factor();
   double* discharge_solve(double* b, double* x);
   void solve_rhs();
-  hipGraph_t lu_graph_ = nullptr;
-  hipGraphExec_t lu_exec_ = nullptr;
-  bool lu_graph_tried_ = false;
+  void correct_rhs();
+  // a discharge solve captured as a HIP graph on its second call (the first runs uncaptured, so
+  // any lazy library set-up happens outside the capture)
+  struct CapturedSolve {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    bool tried = false;
+  };
+  CapturedSolve cap_solve_, cap_correct_;
+  template <class Fn>
+  void run_captured(CapturedSolve& c, Fn&& fn);
   void drop_graph() {
-    if (lu_exec_) (void)hipGraphExecDestroy(lu_exec_);
-    if (lu_graph_) (void)hipGraphDestroy(lu_graph_);
-    lu_exec_ = nullptr;
-    lu_graph_ = nullptr;
-    lu_graph_tried_ = false;
+    for (CapturedSolve* c : {&cap_solve_, &cap_correct_}) {
+      if (c->e) (void)hipGraphExecDestroy(c->e);
+      if (c->g) (void)hipGraphDestroy(c->g);
+      *c = CapturedSolve{};
+    }
   }
   double head_residual(const double* b, int* row);
   void head_residual_async(const double* b);
@@ -1483,38 +1491,51 @@ void Solver::switch_to_fp64_solves() {
 
 // d_x_ = A^-1 d_b_.  The triangular-solve sequence (one permutation gather and two gemv per
 // diagonal block) is launch-bound, so it is captured once into a HIP graph and replayed.
-void Solver::solve_rhs() {
-  if (lu_mode_ != ANDFN_MI_LU_TRSV) {
-    (void)discharge_solve(d_b_.p, d_x_.p);
-    return;
-  }
-  if (!lu_graph_tried_) {
-    lu_graph_tried_ = true;
-    (void)discharge_solve(d_b_.p, d_x_.p);  // uncaptured first call: any lazy library set-up happens here
+template <class Fn>
+void Solver::run_captured(CapturedSolve& c, Fn&& fn) {
+  if (!c.tried) {
+    c.tried = true;
+    fn();  // uncaptured first call: any lazy library set-up happens here
     HIP_CHECK(hipStreamSynchronize(st_));
     // ANDFN_MI_NO_GRAPH set: launch the solve's kernels directly (profiler counter collection
     // instruments every dispatch one by one)
     static const bool no_graph = std::getenv("ANDFN_MI_NO_GRAPH") != nullptr;
     if (!no_graph && hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess) {
-      (void)discharge_solve(d_b_.p, d_x_.p);
+      fn();
       hipGraph_t g = nullptr;
       if (hipStreamEndCapture(st_, &g) == hipSuccess && g) {
-        if (hipGraphInstantiate(&lu_exec_, g, nullptr, nullptr, 0) == hipSuccess) {
-          lu_graph_ = g;
+        if (hipGraphInstantiate(&c.e, g, nullptr, nullptr, 0) == hipSuccess) {
+          c.g = g;
         } else {
           (void)hipGraphDestroy(g);
-          lu_exec_ = nullptr;
+          c.e = nullptr;
         }
       }
       (void)hipGetLastError();
     }
-    return;  // the first call's result is already in d_x_
+    return;  // the first call's result is already in place
   }
-  if (lu_exec_) {
-    HIP_CHECK(hipGraphLaunch(lu_exec_, st_));
+  if (c.e) {
+    HIP_CHECK(hipGraphLaunch(c.e, st_));
   } else {
-    (void)discharge_solve(d_b_.p, d_x_.p);
+    fn();
   }
+}
+
+void Solver::solve_rhs() {
+  if (lu_mode_ != ANDFN_MI_LU_TRSV) {
+    (void)discharge_solve(d_b_.p, d_x_.p);
+    return;
+  }
+  run_captured(cap_solve_, [&] { (void)discharge_solve(d_b_.p, d_x_.p); });
+}
+
+// x = y + A^-1 (b - A y) with the sparse factors, A y of the current iterate y already in d_ay_
+// (the head residual's product): one solve, accurate to the condition number times the rounding
+// unit times |b - A y| rather than |b|
+void Solver::correct_rhs() {
+  run_captured(cap_correct_,
+               [&] { mf_.solve_correction(d_b_.p, d_ay_.p, d_y_.p, d_req_.p, d_x_.p, st_); });
 }
 
 // v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten), for the nblk
@@ -1862,10 +1883,13 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     ms_lu += d;
   };
 
-  // d_x_ = A^-1 d_b_, then y_old <- y, y <- x and q, constants <- x (one launch)
-  auto solve_and_accept = [&](int it) {
+  // d_x_ = A^-1 d_b_, then y_old <- y, y <- x and q, constants <- x (one launch).  With the
+  // sparse factors and A y of the current iterate at hand (the head residual's product), the
+  // solve is the correction x = y + A^-1 (b - A y); otherwise a refined solve of b.
+  auto solve_and_accept = [&](int it, bool have_ay) {
     HIP_CHECK(hipEventRecord(sol_ev_[it & 1][0], st_));
-    solve_rhs();
+    if (have_ay && sparse_ && lu_mode_ == ANDFN_MI_LU_TRSV) correct_rhs();
+    else solve_rhs();
     HIP_CHECK(hipEventRecord(sol_ev_[it & 1][1], st_));
     aem::launch_accept(nv, d_de_.p, nde, F_, sys(), d_x_.p, d_y_.p, d_yold_.p, n, st_);
   };
@@ -1890,7 +1914,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
       head_residual_async(d_b_.p);
       early32 = solve32_;
       if (early32) ++solves32_;
-      solve_and_accept(nit);
+      solve_and_accept(nit, true);
       early = true;
       HIP_CHECK(hipEventSynchronize(res_ev_));
       resid = h_scal_->residual;
@@ -1945,7 +1969,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     }
     if (!early) {
       if (solve32_) ++solves32_;
-      solve_and_accept(nit);
+      solve_and_accept(nit, false);
     }
     fl_rhs += rhs_flops;
     pr_rhs += rhs_pairs;
@@ -2472,6 +2496,10 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     if (!sparse_) make_transposed<double>(d_A_.p, a64_blocked_, d_AT_, d_linvT_, d_uinvT_);
     make_csr_transpose();
     adjT_valid_ = true;
+  }
+  if (sparse_) {
+    mf_.prepare_transposed(st_);  // (a no-op until the next factorisation)
+    HIP_CHECK(hipGetLastError());
   }
   d_x_.alloc(n);
   d_w_.alloc(n);

@@ -53,11 +53,6 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long x)
   return x;
 }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
 
 // ---- factorisation kernels --------------------------------------------------------------------
 
@@ -352,17 +347,25 @@ __device__ __forceinline__ double gathered(const int* __restrict__ gptr, const i
 }
 
 // The block of a solve item: kind 0 L11^-1 (lower), 1 L21, 2 U12, 3 U11^-1 (upper).
+// The block of a solve panel and its element (r, c) = M[r * rs + c * cs].  Kinds 0-3 are the
+// products of A x = b: 0 L11^-1 (lower), 1 L21, 2 U12, 3 U11^-1 (upper).  Kinds 4-7 those of
+// A^T y = c, with the shapes of kinds 0-3: 4 U11^-T (lower), 5 U12^T, 6 L21^T, 7 L11^-T (upper).
 template <int KIND>
 __device__ __forceinline__ void seg_block(const MfFront& f, const double* __restrict__ F,
-                                          const double* __restrict__ INV, const double*& M, int64_t& ld,
-                                          int& rows, int& cols) {
+                                          const double* __restrict__ INV, const double*& M, int64_t& rs,
+                                          int64_t& cs, int& rows, int& cols) {
+  constexpr int S = KIND & 3;                      // shape
+  constexpr int B = KIND < 4 ? KIND : 7 - KIND;    // stored block
   const int k = f.k, nb = f.m - f.k;
-  rows = KIND == 1 ? nb : k;
-  cols = KIND == 2 ? nb : k;
-  if (KIND == 0) { M = INV + f.inv; ld = k; }
-  else if (KIND == 1) { M = F + f.off + k; ld = f.m; }
-  else if (KIND == 2) { M = F + f.off + (int64_t)k * f.m; ld = f.m; }
+  rows = S == 1 ? nb : k;
+  cols = S == 2 ? nb : k;
+  int64_t ld;
+  if (B == 0) { M = INV + f.inv; ld = k; }
+  else if (B == 1) { M = F + f.off + k; ld = f.m; }
+  else if (B == 2) { M = F + f.off + (int64_t)k * f.m; ld = f.m; }
   else { M = INV + f.inv + (int64_t)k * k; ld = k; }
+  rs = KIND < 4 ? 1 : ld;
+  cs = KIND < 4 ? ld : 1;
 }
 
 // After a factorisation: copy each panel's rows of its block into the panel's tile (layout in
@@ -377,9 +380,9 @@ __global__ __launch_bounds__(256) void mfs_pack_kernel(const MfFront* __restrict
   const int4 it = items[blockIdx.x];
   const MfFront f = fr[it.x];
   const double* M;
-  int64_t ld;
+  int64_t rs, cs;
   int rows, cols;
-  seg_block<KIND>(f, F, INV, M, ld, rows, cols);
+  seg_block<KIND>(f, F, INV, M, rs, cs, rows, cols);
   const int l = threadIdx.x % R, p = threadIdx.x / R, r = it.y + l;
   double2* out = reinterpret_cast<double2*>(T + toff[blockIdx.x]) + threadIdx.x;
   for (int cb = 0; cb < it.w; ++cb) {
@@ -388,20 +391,25 @@ __global__ __launch_bounds__(256) void mfs_pack_kernel(const MfFront* __restrict
     for (int e = 0; e < 2; ++e) {
       const int c = it.z + cb * C + 2 * p + e;
       bool ok = r < rows && c < cols;
-      if (KIND == 0) ok = ok && c <= r;
-      if (KIND == 3) ok = ok && c >= r;
-      v[e] = ok ? M[r + (int64_t)c * ld] : 0.0;
+      if ((KIND & 3) == 0) ok = ok && c <= r;
+      if ((KIND & 3) == 3) ok = ok && c >= r;
+      v[e] = ok ? M[r * rs + c * cs] : 0.0;
     }
     out[cb * 256] = make_double2(v[0], v[1]);
   }
 }
 
-// N-form products for A x = b, one row panel per workgroup, every row computed in full, so no
-// partial leaves the workgroup:
+// Solve products, one row panel per workgroup, every row computed in full, so no partial leaves
+// the workgroup.  For A x = b (the row scales R and the pivot order P inside A = R^-1 P^T L U):
 //   kind 0  vec_U  = L11^-1 P (R b + children)_U
 //   kind 1  vec_B  = (children)_B - L21 vec_U
 //   kind 2  vec_U -= U12 x_B
 //   kind 3  x_U    = U11^-1 vec_U   (scattered to the solver's layout)
+// and for A^T y' = c (y = R y' after the last level; b is c and x is y' here):
+//   kind 4  vec_U  = U11^-T (c + children)_U
+//   kind 5  vec_B  = (children)_B - U12^T vec_U
+//   kind 6  vec_U -= L21^T y'_B
+//   kind 7  y'_{P U} = L11^-T vec_U
 // The panel's record carries its front's fields, so one scalar load starts it; the first batch of
 // tile loads and the values the epilogue needs are issued before the input is staged, which keeps
 // the dependent memory round trips of a small panel few.  The kind's input over the panel's
@@ -420,9 +428,10 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
   constexpr int C = kMfBlockValues / R, NP = 256 / R;  // columns per block, column pairs per block
   __shared__ double xs[kMfStage];
   __shared__ double ps[NP * NQ][R + 1];
+  constexpr int S = KIND & 3;  // shape (kinds 4-7: the transposed solve)
   const MfPanel pn = panels[blockIdx.x];
   const int k = pn.k, nb = pn.m - pn.k;
-  const int rows = KIND == 1 ? nb : k, cols = KIND == 2 ? nb : k;
+  const int rows = S == 1 ? nb : k, cols = S == 2 ? nb : k;
   const int ncol = pn.nblk * C;
   const int t = threadIdx.x, l = t % R, p = (t & 255) / R, q = t >> 8, r = pn.r0 + t;
   // the panel's tile from a uniform base with 32-bit offsets (a tile is far below 4 GiB)
@@ -441,9 +450,10 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
   double pre = 0.0;  // epilogue input of row r (threads t < R)
   int xi = 0;
   if (t < R && r < rows) {
-    if (KIND == 1) pre = gathered(gptr, gsrc, vec, pn.gat + k + r);
-    if (KIND == 2) pre = vec[pn.vec + r];
+    if (S == 1) pre = gathered(gptr, gsrc, vec, pn.gat + k + r);
+    if (S == 2) pre = vec[pn.vec + r];
     if (KIND == 3) xi = idx[pn.idx + r];
+    if (KIND == 7) xi = idx[pn.idx + piv[pn.piv + r]];
   }
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   bool have = true;  // m holds the first batch of the first stage
@@ -458,7 +468,9 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
           const int pj = piv[pn.piv + c];
           const int g = idx[pn.idx + pj];
           v = fma(b[g], req[g], gathered(gptr, gsrc, vec, pn.gat + pj));
-        } else if (KIND == 2) {
+        } else if (KIND == 4) {
+          v = b[idx[pn.idx + c]] + gathered(gptr, gsrc, vec, pn.gat + c);
+        } else if (S == 2) {
           v = x[idx[pn.idx + k + c]];
         } else {
           v = vec[pn.vec + c];
@@ -508,101 +520,10 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
   }
   if (t >= R || r >= rows) return;
   const double sum = ps[0][t];
-  if (KIND == 0) vec[pn.vec + r] = sum;
-  else if (KIND == 1) vec[pn.vec + k + r] = pre - sum;
-  else if (KIND == 2) vec[pn.vec + r] = pre - sum;
+  if (S == 0) vec[pn.vec + r] = sum;
+  else if (S == 1) vec[pn.vec + k + r] = pre - sum;
+  else if (S == 2) vec[pn.vec + r] = pre - sum;
   else x[xi] = sum;
-}
-
-// T-form: wavefront w of the workgroup takes outputs o0 + 16 w .. o0 + 16 w + 15 of the chunk;
-// output o = sum over rows q in [q0(o), q1(o)) of M[q + o ld] x[q]
-template <int TRI>  // 0: rows [0, nrows); 1: rows [0, o] (upper triangle); 2: rows [o, nrows) (lower)
-__device__ __forceinline__ double tform_col(const double* __restrict__ M, int64_t ld, int o, int nrows,
-                                            const double* __restrict__ x, int lane) {
-  const int q0 = TRI == 2 ? o : 0, q1 = TRI == 1 ? o + 1 : nrows;
-  const double* col = M + (int64_t)o * ld;
-  double s = 0.0;
-  for (int q = q0 + lane; q < q1; q += 64) s = fma(col[q], x[q], s);
-  return wave_sum_d(s);
-}
-
-// transposed 1: z = U11^-T (c + children)_U
-__global__ __launch_bounds__(256) void mft_1_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                    const int* __restrict__ idx, const int* __restrict__ gptr,
-                                                    const int* __restrict__ gsrc, const double* __restrict__ INV,
-                                                    const double* __restrict__ c, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k;
-  for (int i = threadIdx.x; i < k; i += 256) sh[i] = c[idx[f.idx + i]] + gathered(gptr, gsrc, vec, f.gat + i);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = 0; j < 16; ++j) {
-    const int o = cc.y + 16 * w + j;
-    if (o >= k) break;  // uniform per wavefront
-    const double s = tform_col<1>(INV + f.inv + (int64_t)k * k, k, o, k, sh, lane);
-    if (lane == 0) vec[f.vec + o] = s;
-  }
-}
-
-// transposed 2: u_B = (children)_B - U12^T z
-__global__ __launch_bounds__(256) void mft_2_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ cb,
-                                                    const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                    const double* __restrict__ F, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  const int2 cc = cb[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k, b = f.m - f.k;
-  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = 0; j < 16; ++j) {
-    const int o = cc.y + 16 * w + j;
-    if (o >= b) break;
-    const double s = tform_col<0>(F + f.off + (int64_t)k * f.m, f.m, o, k, sh, lane);
-    if (lane == 0) vec[f.vec + k + o] = gathered(gptr, gsrc, vec, f.gat + k + o) - s;
-  }
-}
-
-// transposed 3: t = z - L21^T y'_B (in place of z)
-__global__ __launch_bounds__(256) void mft_3_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                    const int* __restrict__ idx, const double* __restrict__ F,
-                                                    const double* __restrict__ ys, double* __restrict__ vec) {
-  extern __shared__ double sh[];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k, b = f.m - f.k;
-  if (b == 0) return;
-  for (int i = threadIdx.x; i < b; i += 256) sh[i] = ys[idx[f.idx + k + i]];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = 0; j < 16; ++j) {
-    const int o = cc.y + 16 * w + j;
-    if (o >= k) break;
-    const double s = tform_col<0>(F + f.off + k, f.m, o, b, sh, lane);
-    if (lane == 0) vec[f.vec + o] -= s;
-  }
-}
-
-// transposed 4: w = L11^-T t; y'[row piv[i]] = w_i
-__global__ __launch_bounds__(256) void mft_4_kernel(const MfFront* __restrict__ fr, const int2* __restrict__ ck,
-                                                    const int* __restrict__ idx, const int* __restrict__ piv,
-                                                    const double* __restrict__ INV, const double* __restrict__ vec,
-                                                    double* __restrict__ ys) {
-  extern __shared__ double sh[];
-  const int2 cc = ck[blockIdx.x];
-  const MfFront f = fr[cc.x];
-  const int k = f.k;
-  for (int i = threadIdx.x; i < k; i += 256) sh[i] = vec[f.vec + i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = 0; j < 16; ++j) {
-    const int o = cc.y + 16 * w + j;
-    if (o >= k) break;
-    const double s = tform_col<2>(INV + f.inv, k, o, k, sh, lane);
-    if (lane == 0) ys[idx[f.idx + piv[f.piv + o]]] = s;
-  }
 }
 
 __global__ void mft_scale_kernel(int n, const int* __restrict__ unk, const double* __restrict__ req,
@@ -627,8 +548,6 @@ void MultifrontalLU::release() {
   dev_free(d_gsrc_);
   dev_free(d_small_);
   dev_free(d_tiles_);
-  dev_free(d_ck_);
-  dev_free(d_cb_);
   dev_free(d_ext_);
   dev_free(d_asm_);
   dev_free(d_status_);
@@ -648,6 +567,8 @@ void MultifrontalLU::release() {
   dev_free(d_gv_);
   dev_free(d_gvoff_);
   dev_free(d_panel_);
+  dev_free(d_tile_t_);
+  tiles_t_ready_ = false;
   dev_free(d_tile_);
   dev_free(d_r_);
   dev_free(d_ay_);
@@ -675,13 +596,9 @@ void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
     dev_alloc(t, s.tiles.size());
     if (!s.tiles.empty()) MF_HIP(hipMemcpyAsync(t, s.tiles.data(), sizeof(int4) * s.tiles.size(), hipMemcpyHostToDevice, st));
     d_tiles_ = t;
-    const std::vector<MfI2>* src2[3] = {&s.ck, &s.cb, &s.ext};
-    int2** dst2[3] = {&d_ck_, &d_cb_, &d_ext_};
-    for (int i = 0; i < 3; ++i) {
-      dev_alloc(*dst2[i], src2[i]->size());
-      if (!src2[i]->empty())
-        MF_HIP(hipMemcpyAsync(*dst2[i], src2[i]->data(), sizeof(int2) * src2[i]->size(), hipMemcpyHostToDevice, st));
-    }
+    dev_alloc(d_ext_, s.ext.size());
+    if (!s.ext.empty())
+      MF_HIP(hipMemcpyAsync(d_ext_, s.ext.data(), sizeof(int2) * s.ext.size(), hipMemcpyHostToDevice, st));
   }
   {
     int4* g = nullptr;
@@ -805,31 +722,8 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
                          d_tiles_ + L.tile_beg, d_F_);
     for (int fi : L.big) factor_big(fi, st, rb);
   }
-  // the solve tiles of every panel
-  for (const MfLevel& L : sym_.levels)
-    for (int kind = 0; kind < 4; ++kind) {
-      const int n = L.gv_end[kind] - L.gv_beg[kind];
-      if (n <= 0) continue;
-      const int4* items = d_gv_ + L.gv_beg[kind];
-      const int64_t* toff = d_gvoff_ + L.gv_beg[kind];
-      const int R = L.prow[kind];
-#define MF_PACK(K, RR) hipLaunchKernelGGL((mfs_pack_kernel<K, RR>), dim3(n), dim3(256), 0, st, d_fr_, items, toff, d_F_, d_inv_, d_tile_)
-#define MF_PACK_K(K)  \
-  if (R == 16)         \
-    MF_PACK(K, 16);    \
-  else if (R == 8)     \
-    MF_PACK(K, 8);     \
-  else                 \
-    MF_PACK(K, 4)
-      switch (kind) {
-        case 0: MF_PACK_K(0); break;
-        case 1: MF_PACK_K(1); break;
-        case 2: MF_PACK_K(2); break;
-        default: MF_PACK_K(3); break;
-      }
-#undef MF_PACK_K
-#undef MF_PACK
-    }
+  pack_tiles(false, st);
+  tiles_t_ready_ = false;
   MF_HIP(hipGetLastError());
   int status = 0;
   std::vector<int64_t> info(std::max(nf, 1), 0);
@@ -840,6 +734,45 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
   for (int i = 0; i < nf && bad < 0; ++i)
     if (info[i] > 0) bad = i;
   return bad < 0 ? 0 : 1 + bad;
+}
+
+template <int K>
+static void launch_pack(int R, int n, const MfFront* fr, const int4* items, const int64_t* toff, const double* F,
+                        const double* INV, double* T, hipStream_t st) {
+  if (R == 16) hipLaunchKernelGGL((mfs_pack_kernel<K, 16>), dim3(n), dim3(256), 0, st, fr, items, toff, F, INV, T);
+  else if (R == 8) hipLaunchKernelGGL((mfs_pack_kernel<K, 8>), dim3(n), dim3(256), 0, st, fr, items, toff, F, INV, T);
+  else hipLaunchKernelGGL((mfs_pack_kernel<K, 4>), dim3(n), dim3(256), 0, st, fr, items, toff, F, INV, T);
+}
+
+void MultifrontalLU::pack_tiles(bool transposed, hipStream_t st) {
+  double* T = transposed ? d_tile_t_ : d_tile_;
+  for (const MfLevel& L : sym_.levels)
+    for (int kind = 0; kind < 4; ++kind) {
+      const int n = L.gv_end[kind] - L.gv_beg[kind];
+      if (n <= 0) continue;
+      const int4* items = d_gv_ + L.gv_beg[kind];
+      const int64_t* toff = d_gvoff_ + L.gv_beg[kind];
+      const int R = L.prow[kind];
+      switch (kind + (transposed ? 4 : 0)) {
+        case 0: launch_pack<0>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 1: launch_pack<1>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 2: launch_pack<2>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 3: launch_pack<3>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 4: launch_pack<4>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 5: launch_pack<5>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        case 6: launch_pack<6>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+        default: launch_pack<7>(R, n, d_fr_, items, toff, d_F_, d_inv_, T, st); break;
+      }
+    }
+  MF_HIP(hipGetLastError());
+}
+
+void MultifrontalLU::prepare_transposed(hipStream_t st) {
+  if (!analysed_) throw std::runtime_error("multifrontal transposed tiles before analysis");
+  if (tiles_t_ready_) return;
+  if (!d_tile_t_) dev_alloc(d_tile_t_, (size_t)sym_.gv_total);
+  pack_tiles(true, st);
+  tiles_t_ready_ = true;
 }
 
 template <int K, int NQ, int R>
@@ -863,16 +796,24 @@ static void launch_panels_k(int wide, int R, int n, const MfPanel* pn, const dou
 
 void MultifrontalLU::launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x,
                                 hipStream_t st) {
-  const int n = L.gv_end[kind] - L.gv_beg[kind];
+  const int s = kind & 3;
+  const int n = L.gv_end[s] - L.gv_beg[s];
   if (n <= 0) return;
-  const MfPanel* pn = d_panel_ + L.gv_beg[kind];
-  const int w = L.pwide[kind], R = L.prow[kind];
+  const MfPanel* pn = d_panel_ + L.gv_beg[s];
+  const int w = L.pwide[s], R = L.prow[s];
+  const double* T = kind < 4 ? d_tile_ : d_tile_t_;
+#define MF_K(K) launch_panels_k<K>(w, R, n, pn, T, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st)
   switch (kind) {
-    case 0: launch_panels_k<0>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
-    case 1: launch_panels_k<1>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
-    case 2: launch_panels_k<2>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
-    default: launch_panels_k<3>(w, R, n, pn, d_tile_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st); break;
+    case 0: MF_K(0); break;
+    case 1: MF_K(1); break;
+    case 2: MF_K(2); break;
+    case 3: MF_K(3); break;
+    case 4: MF_K(4); break;
+    case 5: MF_K(5); break;
+    case 6: MF_K(6); break;
+    default: MF_K(7); break;
   }
+#undef MF_K
 }
 
 void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStream_t st) {
@@ -888,24 +829,15 @@ void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStr
 }
 
 void MultifrontalLU::solve_t(const double* c, const double* req, double* y, hipStream_t st) {
+  if (!tiles_t_ready_) throw std::runtime_error("multifrontal transposed solve before prepare_transposed");
   const int H = (int)sym_.levels.size();
   for (int h = 0; h < H; ++h) {
-    const MfLevel& L = sym_.levels[h];
-    if (L.ck_end > L.ck_beg)
-      hipLaunchKernelGGL(mft_1_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_gptr_, d_gsrc_, d_inv_, c, d_vec_);
-    if (L.cb_end > L.cb_beg)
-      hipLaunchKernelGGL(mft_2_kernel, dim3(L.cb_end - L.cb_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_cb_ + L.cb_beg, d_gptr_, d_gsrc_, d_F_, d_vec_);
+    launch_seg(4, sym_.levels[h], c, nullptr, d_ys_, st);
+    launch_seg(5, sym_.levels[h], c, nullptr, d_ys_, st);
   }
   for (int h = H - 1; h >= 0; --h) {
-    const MfLevel& L = sym_.levels[h];
-    if (L.ck_end > L.ck_beg && L.maxb > 0)
-      hipLaunchKernelGGL(mft_3_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxb, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_F_, d_ys_, d_vec_);
-    if (L.ck_end > L.ck_beg)
-      hipLaunchKernelGGL(mft_4_kernel, dim3(L.ck_end - L.ck_beg), dim3(256), sizeof(double) * L.maxk, st, d_fr_,
-                         d_ck_ + L.ck_beg, d_idx_, d_piv_, d_inv_, d_vec_, d_ys_);
+    launch_seg(6, sym_.levels[h], c, nullptr, d_ys_, st);
+    launch_seg(7, sym_.levels[h], c, nullptr, d_ys_, st);
   }
   if (sym_.nunk > 0)
     hipLaunchKernelGGL(mft_scale_kernel, dim3((unsigned)((sym_.nunk + 255) / 256)), dim3(256), 0, st, (int)sym_.nunk, d_unk_,
@@ -938,6 +870,13 @@ bool MultifrontalLU::unstable_pivots(double limit, std::vector<int>& lift, bool&
   std::sort(lift.begin(), lift.end());
   lift.erase(std::unique(lift.begin(), lift.end()), lift.end());
   return !lift.empty();
+}
+
+void MultifrontalLU::solve_correction(const double* b, const double* ay, const double* y, const double* req, double* x,
+                                      hipStream_t st) {
+  launch_adj_refine_in<double>(b, ay, (int)N_, d_r_, st);
+  solve(d_r_, req, d_dx_, st);
+  hipLaunchKernelGGL(mf_axpy_kernel, dim3((unsigned)((N_ + 255) / 256)), dim3(256), 0, st, N_, y, d_dx_, x);
 }
 
 void MultifrontalLU::solve_refined(const double* b, const double* req, double* x, const int* csr_ptr,

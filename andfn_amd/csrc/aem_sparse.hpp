@@ -54,6 +54,8 @@ class MultifrontalLU {
   int factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st, rocblas_handle rb);
   // x = A^-1 b (b and x in the solver's layout; x is written on the analysed unknowns only)
   void solve(const double* b, const double* req, double* x, hipStream_t st);
+  // the transposed solve's tiles, once per factorisation (before solve_t; outside graph capture)
+  void prepare_transposed(hipStream_t st);
   // y = A^-T c (req == nullptr: y = (R A)^-T c, the row scales left to the caller)
   void solve_t(const double* c, const double* req, double* y, hipStream_t st);
   // x = A^-1 b with one step of iterative refinement against A's CSR values (residual in FP64):
@@ -61,6 +63,10 @@ class MultifrontalLU {
   // condition number times the rounding unit; one step brings the solve back to rounding
   void solve_refined(const double* b, const double* req, double* x, const int* csr_ptr, const int* csr_col,
                      const double* csr_val, hipStream_t st);
+  // x = y + A^-1 (b - A y) given ay = A y (all in the solver's layout): one solve, the correction
+  // of an iterate y, accurate to the condition number times the rounding unit times |b - A y|
+  void solve_correction(const double* b, const double* ay, const double* y, const double* req, double* x,
+                        hipStream_t st);
   // After factor: the unknowns of every pivot whose L21 column exceeds limit in magnitude (or that is
   // zero), its column and the row it was taken from, to be lifted to the parent front by a new
   // analysis (returns whether there are any).  root_zero: a zero pivot in a root front, which has no
@@ -86,13 +92,13 @@ class MultifrontalLU {
   int* d_gsrc_ = nullptr;
   int* d_small_ = nullptr;
   int4* d_tiles_ = nullptr;
-  int2* d_ck_ = nullptr;
-  int2* d_cb_ = nullptr;
   int2* d_ext_ = nullptr;
   int4* d_gv_ = nullptr;        // solve panels (MfSymbolic::gv)
   int64_t* d_gvoff_ = nullptr;  // per solve panel: first value of its tile
   double* d_tile_ = nullptr;    // the panel tiles (repacked after every factorisation)
   MfPanel* d_panel_ = nullptr;  // per solve panel: its record
+  double* d_tile_t_ = nullptr;  // the transposed solve's tiles (packed on demand)
+  bool tiles_t_ready_ = false;
   int64_t* d_asm_ = nullptr;
   int* d_unk_ = nullptr;     // every analysed unknown (its position in the solver's layout)
   int* d_status_ = nullptr;
@@ -115,6 +121,7 @@ class MultifrontalLU {
   void release();
   void factor_big(int f, hipStream_t st, rocblas_handle rb);
   void launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x, hipStream_t st);
+  void pack_tiles(bool transposed, hipStream_t st);
 };
 
 }  // namespace aem

@@ -212,7 +212,7 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   std::vector<int> bloc(N, -1), stamp(N, -1);
   int stampc = 0;
   std::vector<std::vector<int>> adj;
-  std::vector<int> matchc, matchr, vis, stk_r, stk_e;
+  std::vector<int> matchc, matchr, vis, stk_r, stk_e, la;
   for (int sidx = 0; sidx < (int)post.size(); ++sidx) {
     const int s = post[sidx];
     std::vector<int>& U = nodes[s].U;
@@ -232,25 +232,39 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
           }
         }
       }
-      // maximum matching of rows to columns: greedy, then augmenting paths (iterative search)
+      // maximum matching of rows to columns: depth-first augmenting paths with look-ahead (each
+      // row first takes a free column of its own if it has one; its look-ahead pointer only moves
+      // forward, since a matched column stays matched)
       matchc.assign(k, -1);
       matchr.assign(k, -1);
-      for (int i = 0; i < k; ++i)
-        for (int j : adj[i])
-          if (matchc[j] < 0) {
-            matchc[j] = i;
-            matchr[i] = j;
-            break;
-          }
+      la.assign(k, 0);
       vis.assign(k, -1);
+      auto free_col = [&](int r) {
+        for (int& e = la[r]; e < (int)adj[r].size(); ++e)
+          if (matchc[adj[r][e]] < 0) return adj[r][e];
+        return -1;
+      };
+      // augment along the stack: each row on it takes the column it went through, the top row col
+      auto augment = [&](int col) {
+        for (int d = (int)stk_r.size() - 1; d >= 0; --d) {
+          const int rr = stk_r[d];
+          const int prev = matchr[rr];
+          matchr[rr] = col;
+          matchc[col] = rr;
+          col = prev;
+        }
+      };
       for (int i0 = 0; i0 < k; ++i0) {
-        if (matchr[i0] >= 0) continue;
         // search from row i0: stk_r holds rows, stk_e the next edge index of each
         stk_r.assign(1, i0);
         stk_e.assign(1, 0);
         vis[i0] = i0;
-        bool found = false;
-        while (!stk_r.empty() && !found) {
+        int jf = free_col(i0);
+        if (jf >= 0) {
+          augment(jf);
+          continue;
+        }
+        while (!stk_r.empty()) {
           const int r = stk_r.back();
           int& e = stk_e.back();
           if (e >= (int)adj[r].size()) {
@@ -258,23 +272,15 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
             stk_e.pop_back();
             continue;
           }
-          const int j = adj[r][e++];
-          const int r2 = matchc[j];
-          if (r2 < 0) {
-            // augment along the stack: each row on it takes the column it went through
-            int col = j;
-            for (int d = (int)stk_r.size() - 1; d >= 0; --d) {
-              const int rr = stk_r[d];
-              const int prev = matchr[rr];
-              matchr[rr] = col;
-              matchc[col] = rr;
-              col = prev;
-            }
-            found = true;
-          } else if (vis[r2] != i0) {
-            vis[r2] = i0;
-            stk_r.push_back(r2);
-            stk_e.push_back(0);
+          const int r2 = matchc[adj[r][e++]];  // every column of a row on the stack is matched
+          if (vis[r2] == i0) continue;
+          vis[r2] = i0;
+          stk_r.push_back(r2);
+          stk_e.push_back(0);
+          jf = free_col(r2);
+          if (jf >= 0) {
+            augment(jf);
+            break;
           }
         }
       }
@@ -446,7 +452,7 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   }
   for (int64_t q = 0; q < nnz; ++q)
     if (asmap[q] < 0) throw std::runtime_error("multifrontal analysis: a matrix entry has no front");
-  // levels by height; per level the small-front lists, Schur tiles, solve chunks, extend items
+  // levels by height; per level the small-front lists, Schur tiles, solve panels, extend items
   const int H = nf ? *std::max_element(height.begin(), height.end()) + 1 : 0;
   std::vector<MfLevel>& lev = out.levels;
   lev.assign(H, MfLevel{});
@@ -454,8 +460,6 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   for (int i = 0; i < nf; ++i) byh[height[i]].push_back(i);
   std::vector<int>& small_list = out.small_list;
   std::vector<MfI4>& tiles = out.tiles;
-  std::vector<MfI2>& ck = out.ck;
-  std::vector<MfI2>& cb = out.cb;
   std::vector<MfI2>& ext = out.ext;
   for (int h = 0; h < H; ++h) {
     MfLevel& L = lev[h];
@@ -478,19 +482,6 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
         for (int j0 = 0; j0 < b; j0 += 64) tiles.push_back(MfI4{i, i0, j0, 0});
     }
     L.tile_end = (int)tiles.size();
-    L.ck_beg = (int)ck.size();
-    L.cb_beg = (int)cb.size();
-    L.maxk = 0;
-    L.maxb = 0;
-    for (int i : byh[h]) {
-      const MfFront& f = fr[i];
-      for (int r0 = 0; r0 < f.k; r0 += 64) ck.push_back(MfI2{i, r0});
-      for (int r0 = 0; r0 < f.m - f.k; r0 += 64) cb.push_back(MfI2{i, r0});
-      L.maxk = std::max(L.maxk, f.k);
-      L.maxb = std::max(L.maxb, f.m - f.k);
-    }
-    L.ck_end = (int)ck.size();
-    L.cb_end = (int)cb.size();
     for (int kind = 0; kind < 4; ++kind) {
       // panel rows: the tallest of 16, 8, 4 that still gives the launch kMfWidePanels panels (kind
       // 0 keeps 16: each of its panels gathers its own inputs, so more panels cost more gathers)
@@ -533,8 +524,6 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
       const int n = L.gv_end[kind] - L.gv_beg[kind];
       L.pwide[kind] = R == 16 && n > 0 && n < kMfWidePanels && blocks >= 8 * (int64_t)n ? 1 : 0;
     }
-    if ((int64_t)std::max(L.maxk, L.maxb) * 8 > kMfMaxLds)
-      throw std::runtime_error("multifrontal analysis: a front exceeds the solve kernels' LDS");
     int slots = 0;
     for (int i : byh[h]) slots = std::max(slots, (int)fkids[i].size());
     for (int s = 0; s < slots; ++s) {

@@ -17,7 +17,6 @@ namespace aem {
 
 constexpr int kMfSmallMax = 64;     // fronts with at most this many pivots: one workgroup in LDS
 constexpr int kMfLeafUnknowns = 48;  // a fracture set is split while its interior has more unknowns
-constexpr int kMfMaxLds = 61440;    // bytes of dynamic LDS of the solve kernels (a level's largest k or m - k; their static LDS is 2 KB)
 
 struct MfFront {
   int64_t off;  // first value of the front: m x m, column-major, leading dimension m
@@ -59,9 +58,6 @@ struct MfLevel {
   int small_beg[3], small_end[3];     // small-front lists per pivot class (<= 16, 32, 64)
   std::vector<int> big;               // fronts with more than kMfSmallMax pivots
   int tile_beg, tile_end;             // Schur tiles of its small fronts
-  int ck_beg, ck_end;                 // 64-row chunks over the pivot rows of its fronts
-  int cb_beg, cb_end;                 // 64-row chunks over the boundary rows
-  int maxk, maxb;                     // largest pivot count and boundary of its fronts
   std::vector<int> ext_beg, ext_end;  // extend-add items per child slot
   int gv_beg[4], gv_end[4];           // solve panels, per kind (mf_panel_shape / mf_panel_columns)
   int prow[4], pwide[4];              // per kind: panel rows (16, 8 or 4); 1 if a panel takes 1,024 threads
@@ -102,7 +98,7 @@ struct MfSymbolic {
   std::vector<int> small_list;        // fronts of the small classes, level by level
   std::vector<int> unk;               // every analysed unknown
   std::vector<MfI4> tiles;            // (front, first boundary row, first boundary column, 0)
-  std::vector<MfI2> ck, cb, ext;      // (front, first row) chunks; (child, first column) items
+  std::vector<MfI2> ext;              // extend-add items (child, first boundary column)
   std::vector<MfI4> gv;               // solve panels (front, first row, first column, column blocks)
   std::vector<int64_t> gv_off;        // per solve panel: first value of its tile (kMfBlockValues per column block)
   std::vector<int64_t> asmap;         // CSR entry -> value position in its front

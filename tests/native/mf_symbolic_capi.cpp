@@ -40,8 +40,8 @@ int mf_test_analyse(int F, int E, const int* etype, const int* f0, const int* f1
 const char* mf_test_error() { return g_err.c_str(); }
 
 // sizes: 0 fronts, 1 levels, 2 idx, 3 rel, 4 piv0, 5 gptr, 6 gsrc, 7 small_list, 8 unk, 9 tiles,
-// 10 ck, 11 cb, 12 ext, 13 asmap, 14 front_total, 15 inv_total, 16 nvec, 17 nnz_factors, 18 nunk, 19 gv,
-// 20 gv_total
+// 10 (unused), 11 (unused), 12 ext, 13 asmap, 14 front_total, 15 inv_total, 16 nvec, 17 nnz_factors,
+// 18 nunk, 19 gv, 20 gv_total
 long long mf_test_size(int w) {
   const aem::MfSymbolic& s = g_sym;
   switch (w) {
@@ -55,8 +55,6 @@ long long mf_test_size(int w) {
     case 7: return (long long)s.small_list.size();
     case 8: return (long long)s.unk.size();
     case 9: return (long long)s.tiles.size();
-    case 10: return (long long)s.ck.size();
-    case 11: return (long long)s.cb.size();
     case 12: return (long long)s.ext.size();
     case 13: return (long long)s.asmap.size();
     case 14: return s.front_total;
@@ -67,7 +65,7 @@ long long mf_test_size(int w) {
     case 19: return (long long)s.gv.size();
     case 20: return s.gv_total;
   }
-  return -1;
+  return 0;
 }
 
 // fronts as 11 int64 each: off, inv, m, k, idx, vec, piv, rel, parent, gat, 0
@@ -80,18 +78,17 @@ void mf_test_fronts(long long* out) {
   }
 }
 
-// levels as 32 ints each: small_beg[3], small_end[3], tile_beg, tile_end, ck_beg, ck_end, cb_beg, cb_end,
-// maxk, maxb, number of big fronts, number of extend slots, gv_beg[4], gv_end[4], prow[4], pwide[4]
+// levels as 26 ints each: small_beg[3], small_end[3], tile_beg, tile_end, number of big fronts,
+// number of extend slots, gv_beg[4], gv_end[4], prow[4], pwide[4]
 void mf_test_levels(int* out) {
   for (size_t h = 0; h < g_sym.levels.size(); ++h) {
     const aem::MfLevel& L = g_sym.levels[h];
-    int* o = out + 32 * h;
-    for (int c = 0; c < 4; ++c) {
-      o[16 + c] = L.gv_beg[c]; o[20 + c] = L.gv_end[c]; o[24 + c] = L.prow[c]; o[28 + c] = L.pwide[c];
-    }
+    int* o = out + 26 * h;
     for (int c = 0; c < 3; ++c) { o[c] = L.small_beg[c]; o[3 + c] = L.small_end[c]; }
-    o[6] = L.tile_beg; o[7] = L.tile_end; o[8] = L.ck_beg; o[9] = L.ck_end; o[10] = L.cb_beg; o[11] = L.cb_end;
-    o[12] = L.maxk; o[13] = L.maxb; o[14] = (int)L.big.size(); o[15] = (int)L.ext_beg.size();
+    o[6] = L.tile_beg; o[7] = L.tile_end; o[8] = (int)L.big.size(); o[9] = (int)L.ext_beg.size();
+    for (int c = 0; c < 4; ++c) {
+      o[10 + c] = L.gv_beg[c]; o[14 + c] = L.gv_end[c]; o[18 + c] = L.prow[c]; o[22 + c] = L.pwide[c];
+    }
   }
 }
 
@@ -106,8 +103,6 @@ void mf_test_ints(int w, int* out) {
   const std::vector<int>* v[] = {nullptr, nullptr, &s.idx, &s.rel, &s.piv0, &s.gptr, &s.gsrc, &s.small_list, &s.unk};
   if (w >= 2 && w <= 8) std::memcpy(out, v[w]->data(), sizeof(int) * v[w]->size());
   else if (w == 9) std::memcpy(out, s.tiles.data(), sizeof(int) * 4 * s.tiles.size());
-  else if (w == 10) std::memcpy(out, s.ck.data(), sizeof(int) * 2 * s.ck.size());
-  else if (w == 11) std::memcpy(out, s.cb.data(), sizeof(int) * 2 * s.cb.size());
   else if (w == 12) std::memcpy(out, s.ext.data(), sizeof(int) * 2 * s.ext.size());
   else if (w == 13) std::memcpy(out, s.gv.data(), sizeof(int) * 4 * s.gv.size());
 }

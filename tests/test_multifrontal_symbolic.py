@@ -4,14 +4,14 @@ with g++ and checked on the CPU before any of its tables reaches a GPU kernel.
 * Bounds: every offset, index, chunk, tile and extend-add item the kernels of aem_sparse.hip
   index stays inside its buffer (fronts, inverses, index lists, solve vectors, gather lists,
   the parent's positions), every unknown is a pivot of exactly one front, children come before
-  their parents and sit on lower levels, every matrix entry has one place in one front, the
-  solve kernels' LDS fits 64 KB, and the solve panels cover each block's rows and the columns of
-  its triangle.
+  their parents and sit on lower levels, every matrix entry has one place in one front, and the
+  solve panels cover each block's rows and the columns of its triangle.
 * Semantics: the factorisation and both solves are replayed in numpy on the analysed tables, step
   by step as the kernels take them (scatter of the row-equilibrated matrix, extend-add through the
   parent positions, LU of each pivot block with partial pivoting over its own rows, L21, U12,
   Schur update, the two inverses; forward and backward solves through the gather lists and pivot
-  orders, the forward and backward products panel by panel), and must
+  orders, the forward and backward products panel by panel, the transposed solve's through the
+  transposed blocks on the same panels), and must
   reproduce SciPy's solutions of A x = b and A^T y = c to 1e-10 relative on the
   discharge matrices of four networks (with wells, with constant-head lines, at 5,000 fractures).
 """
@@ -92,17 +92,16 @@ def _analyse(L, net, delay=None, panel_rows=0):
                            None if delay is None else _p(delay), panel_rows)
     assert rc == 0, L.mf_test_error()
     sz = {k: L.mf_test_size(i) for i, k in enumerate(["fronts", "levels", "idx", "rel", "piv0", "gptr", "gsrc", "small",
-                                                      "unk", "tiles", "ck", "cb", "ext", "asmap", "front_total",
+                                                      "unk", "tiles", "_10", "_11", "ext", "asmap", "front_total",
                                                       "inv_total", "nvec", "nnz_factors", "nunk", "gv",
                                                       "gv_total"])}
     fr = np.zeros((sz["fronts"], 11), np.int64)
     L.mf_test_fronts(_p(fr))
-    lv = np.zeros((sz["levels"], 32), np.int32)
+    lv = np.zeros((sz["levels"], 26), np.int32)
     L.mf_test_levels(_p(lv))
     t = dict(sz=sz, fr=fr, lv=lv)
     for w, k, shape in ((2, "idx", None), (3, "rel", None), (4, "piv0", None), (5, "gptr", None), (6, "gsrc", None),
-                        (7, "small", None), (8, "unk", None), (9, "tiles", 4), (10, "ck", 2), (11, "cb", 2),
-                        (12, "ext", 2), (13, "gv", 4)):
+                        (7, "small", None), (8, "unk", None), (9, "tiles", 4), (12, "ext", 2), (13, "gv", 4)):
         n = sz[k]
         a = np.zeros(n * (shape or 1), np.int32)
         L.mf_test_ints(w, _p(a))
@@ -113,10 +112,10 @@ def _analyse(L, net, delay=None, panel_rows=0):
     L.mf_test_asmap(_p(t["asmap"]))
     t["big"], t["ext_slots"] = [], []
     for h in range(sz["levels"]):
-        b = np.zeros(max(lv[h, 14], 1), np.int32)
+        b = np.zeros(max(lv[h, 8], 1), np.int32)
         L.mf_test_level_big(h, _p(b))
-        t["big"].append(b[: lv[h, 14]])
-        ns = lv[h, 15]
+        t["big"].append(b[: lv[h, 8]])
+        ns = lv[h, 9]
         eb, ee = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
         L.mf_test_level_ext(h, _p(eb), _p(ee))
         t["ext_slots"].append(list(zip(eb[:ns], ee[:ns])))
@@ -153,14 +152,13 @@ def _check_bounds(t, net):
     assert len(a) == len(net["col"]) and np.all((a >= 0) & (a < sz["front_total"])) and len(np.unique(a)) == len(a)
     ti = t["tiles"]
     assert np.all(k[ti[:, 0]] <= 64) and np.all(ti[:, 1] < b[ti[:, 0]]) and np.all(ti[:, 2] < b[ti[:, 0]])
-    assert np.all(t["ck"][:, 1] < k[t["ck"][:, 0]]) and np.all(t["cb"][:, 1] < b[t["cb"][:, 0]])
     ex = t["ext"]
     assert np.all(par[ex[:, 0]] >= 0) and np.all(ex[:, 1] < b[ex[:, 0]])
     level = np.full(nf, -1)
     for h, row in enumerate(t["lv"]):
-        fs = np.unique(t["ck"][row[8]:row[9], 0])
+        fs = np.concatenate([t["small"][lo:hi] for lo, hi in zip(row[0:3], row[3:6])] + [t["big"][h]])
+        assert np.all(level[fs] == -1)
         level[fs] = h
-        assert row[12] == k[fs].max() and row[13] == b[fs].max() and 8 * max(row[12], row[13]) <= 65536
         for c, (lo, hi) in enumerate(zip(row[0:3], row[3:6])):
             kk = k[t["small"][lo:hi]]
             assert np.all(kk <= 16 << c) and np.all(kk > (8 << c if c else 0))
@@ -194,8 +192,8 @@ def _check_solve_items(t, level):
     pos = 0
     for h, row in enumerate(lv):
         for kind in range(4):
-            lo, hi = row[16 + kind], row[20 + kind]
-            R, wide = row[24 + kind], row[28 + kind]
+            lo, hi = row[10 + kind], row[14 + kind]
+            R, wide = row[18 + kind], row[22 + kind]
             assert R in (4, 8, 16) and wide in (0, 1) and (R == 16 or not wide)
             C = BLOCK // R
             assert lo == pos and hi >= lo
@@ -282,8 +280,8 @@ def _replay(t, net, level):
         """One launch of the panel products: each panel's rows in full over its column blocks,
         then the kind's epilogue (the tables drive everything, as on the device)."""
         gv, row = t["gv"], t["lv"][h]
-        R = row[24 + kind]
-        for f, r0, c_lo, nblk in gv[row[16 + kind]: row[20 + kind]]:
+        R = row[18 + kind]
+        for f, r0, c_lo, nblk in gv[row[10 + kind]: row[14 + kind]]:
             kk, bb = k[f], m[f] - k[f]
             rows, cols = _kind_shape(kind, kk, bb)
             cs = np.arange(c_lo, min(cols, c_lo + BLOCK // R * nblk))
@@ -317,29 +315,48 @@ def _replay(t, net, level):
             products(3, h, V, x, bvec)
         return x
 
+    def block_t(kind, i):
+        """the block of transposed kind 4-7 (the shapes of kinds 0-3)"""
+        kk = k[i]
+        Li = INV[inv[i]: inv[i] + kk * kk].reshape(kk, kk, order="F")
+        Ui = INV[inv[i] + kk * kk: inv[i] + 2 * kk * kk].reshape(kk, kk, order="F")
+        return [np.tril(Ui.T), front(i)[:kk, kk:].T, front(i)[kk:, :kk].T, np.triu(Li.T)][kind - 4]
+
+    def products_t(kind, h, V, ys, cvec):
+        gv, row = t["gv"], t["lv"][h]
+        s0 = kind - 4
+        R = row[18 + s0]
+        for f, r0, c_lo, nblk in gv[row[10 + s0]: row[14 + s0]]:
+            kk, bb = k[f], m[f] - k[f]
+            rows, cols = _kind_shape(s0, kk, bb)
+            cs = np.arange(c_lo, min(cols, c_lo + BLOCK // R * nblk))
+            ix = t["idx"][idx[f]: idx[f] + m[f]]
+            if kind == 4:
+                xin = np.array([cvec[ix[c]] + gathered(V, gat[f] + c) for c in cs])
+            elif kind == 6:
+                xin = ys[ix[kk + cs]]
+            else:
+                xin = V[vec[f] + cs]
+            rr = np.arange(r0, min(rows, r0 + R))
+            s_ = block_t(kind, f)[rr][:, cs] @ xin
+            if kind == 4:
+                V[vec[f] + rr] = s_
+            elif kind == 5:
+                V[vec[f] + kk + rr] = np.array([gathered(V, gat[f] + kk + r) for r in rr]) - s_
+            elif kind == 6:
+                V[vec[f] + rr] -= s_
+            else:
+                ys[ix[PIV[piv[f] + rr]]] = s_
+
     def solve_t(cvec):
         V = np.zeros(sz["nvec"])
         ys = np.zeros(N)
         for h in range(H):
-            for i in np.nonzero(level == h)[0]:
-                kk, bb = k[i], m[i] - k[i]
-                ix = t["idx"][idx[i]: idx[i] + m[i]]
-                g = np.array([cvec[ix[j]] + gathered(V, gat[i] + j) for j in range(kk)])
-                Ui = INV[inv[i] + kk * kk: inv[i] + 2 * kk * kk].reshape(kk, kk, order="F")
-                z = Ui.T @ g
-                V[vec[i]: vec[i] + kk] = z
-                Fm = front(i)
-                for r in range(bb):
-                    V[vec[i] + kk + r] = gathered(V, gat[i] + kk + r) - Fm[:kk, kk + r] @ z
+            products_t(4, h, V, ys, cvec)
+            products_t(5, h, V, ys, cvec)
         for h in range(H - 1, -1, -1):
-            for i in np.nonzero(level == h)[0]:
-                kk = k[i]
-                ix = t["idx"][idx[i]: idx[i] + m[i]]
-                Fm = front(i)
-                tt = V[vec[i]: vec[i] + kk] - Fm[kk:, :kk].T @ ys[ix[kk:]]
-                Li = INV[inv[i]: inv[i] + kk * kk].reshape(kk, kk, order="F")
-                w = Li.T @ tt
-                ys[ix[PIV[piv[i]: piv[i] + kk]]] = w
+            products_t(6, h, V, ys, cvec)
+            products_t(7, h, V, ys, cvec)
         return req * ys
 
     return solve, solve_t
@@ -396,13 +413,15 @@ def test_short_solve_panels(lib, panel_rows):
     net = _network("dfn200")
     t = _analyse(lib, net, panel_rows=panel_rows)
     level = _check_bounds(t, net)
-    assert set(t["lv"][:, 25:28].ravel()) == {panel_rows} and set(t["lv"][:, 24]) == {16}
+    assert set(t["lv"][:, 19:22].ravel()) == {panel_rows} and set(t["lv"][:, 18]) == {16}
     solve, solve_t = _replay(t, net, level)
     rng = np.random.default_rng(5)
     A = net["A"].tocsc()
     bvec = rng.standard_normal(net["N"])
     x, xr = solve(bvec), spla.spsolve(A, bvec)
     assert np.max(np.abs(x - xr)) <= 1e-10 * np.max(np.abs(xr))
+    y, yr = solve_t(bvec), spla.spsolve(A.T.tocsc(), bvec)
+    assert np.max(np.abs(y - yr)) <= 1e-10 * np.max(np.abs(yr))
 
 
 def test_panel_rows_are_checked(lib):
