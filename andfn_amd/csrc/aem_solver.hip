@@ -1171,6 +1171,15 @@ void Solver::assemble_and_factor() {
       for (int64_t i = 0; i < N; ++i) id[i] = (int)i;
       d_perm_.upload(id, st_);
     }
+    if (precision_ == ANDFN_MI_MIXED && lu_mode_ == ANDFN_MI_LU_TRSV) {
+      // the correction solves x = y + A^-1 (b - A y) stream the factors rounded to FP32 (half the
+      // bytes); the FP64 residual b - A y keeps the iteration's fixed point the FP64 one.  They
+      // must contract well to pay off; otherwise the FP64 factors serve throughout.
+      mf_.prepare_f32(st_);
+      solve32_ = true;
+      refine_rate_ = refine_probe();
+      if (!(refine_rate_ <= 0.25)) solve32_ = false;
+    }
     HIP_CHECK(hipStreamSynchronize(st_));
     return;
   }
@@ -1463,9 +1472,13 @@ double Solver::refine_probe() {
     HIP_CHECK(hipStreamSynchronize(st_));
     r[k] = sc.residual;
     if (k == 3) break;
-    aem::launch_refine_rhs(b0.p, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
-    blocked_solve_all<float>(d_A32_.p, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
-    aem::launch_refine_update(y.p, d_v32_.p, n, x.p, st_);
+    if (sparse_) {
+      mf_.solve_correction(b0.p, d_ay_.p, y.p, d_req_.p, x.p, st_, true);
+    } else {
+      aem::launch_refine_rhs(b0.p, d_ay_.p, d_perm_.p, d_req_.p, n, d_x32_.p, st_);
+      blocked_solve_all<float>(d_A32_.p, d_linv32_.p, d_uinv32_.p, d_x32_.p, d_w32_.p, d_v32_.p);
+      aem::launch_refine_update(y.p, d_v32_.p, n, x.p, st_);
+    }
     HIP_CHECK(hipMemcpyAsync(y.p, x.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
   }
   HIP_CHECK(hipStreamSynchronize(st_));
@@ -1477,7 +1490,7 @@ double Solver::refine_probe() {
 // stopped contracting).  Builds the FP64 blocked operators on first use.
 void Solver::switch_to_fp64_solves() {
   if (!solve32_) return;
-  if (!a64_blocked_) {
+  if (!sparse_ && !a64_blocked_) {
     prepare_blocked_all<double>(d_A_.p, d_linv_, d_uinv_);
     d_w_.alloc(N_);
     d_v_.alloc(N_);
@@ -1535,7 +1548,7 @@ void Solver::solve_rhs() {
 // unit times |b - A y| rather than |b|
 void Solver::correct_rhs() {
   run_captured(cap_correct_,
-               [&] { mf_.solve_correction(d_b_.p, d_ay_.p, d_y_.p, d_req_.p, d_x_.p, st_); });
+               [&] { mf_.solve_correction(d_b_.p, d_ay_.p, d_y_.p, d_req_.p, d_x_.p, st_, solve32_); });
 }
 
 // v = U^-1 L^-1 x with the operators of prepare_blocked (x and w are overwritten), for the nblk

@@ -420,11 +420,23 @@ __global__ __launch_bounds__(256) void mfs_pack_kernel(const MfFront* __restrict
 // for launches of few, long panels, so that the loads in flight still cover the memory latency of
 // the whole device.
 constexpr int kMfStage = 2048;
-template <int KIND, int NQ, int R>
+template <class TT>
+struct TilePair;  // two tile values of one thread and column block
+template <>
+struct TilePair<double> {
+  using V = double2;
+};
+template <>
+struct TilePair<float> {
+  using V = float2;
+};
+
+template <int KIND, int NQ, int R, class TT>
 __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1 ? 8 : 6))) void mfs_panel_kernel(
-    const MfPanel* __restrict__ panels, const double* __restrict__ T, const int* __restrict__ idx,
+    const MfPanel* __restrict__ panels, const TT* __restrict__ T, const int* __restrict__ idx,
     const int* __restrict__ piv, const int* __restrict__ gptr, const int* __restrict__ gsrc,
     const double* __restrict__ b, const double* __restrict__ req, double* __restrict__ x, double* __restrict__ vec) {
+  using V2 = typename TilePair<TT>::V;
   constexpr int C = kMfBlockValues / R, NP = 256 / R;  // columns per block, column pairs per block
   __shared__ double xs[kMfStage];
   __shared__ double ps[NP * NQ][R + 1];
@@ -434,17 +446,19 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
   const int rows = S == 1 ? nb : k, cols = S == 2 ? nb : k;
   const int ncol = pn.nblk * C;
   const int t = threadIdx.x, l = t % R, p = (t & 255) / R, q = t >> 8, r = pn.r0 + t;
-  // the panel's tile from a uniform base with 32-bit offsets (a tile is far below 4 GiB)
+  // the panel's tile from a uniform base with 32-bit offsets (a tile is far below 4 GiB); FP32
+  // tiles (the mixed form's correction solves) hold the same values rounded, in the same layout
   const char* P = reinterpret_cast<const char*>(T + pn.toff);
-  const unsigned tb = (unsigned)(t & 255) * 16u;
-  auto ld = [&](int cb) { return *reinterpret_cast<const double2*>(P + ((unsigned)cb * 4096u + tb)); };
-  double2 m[8];
+  constexpr unsigned kPair = sizeof(V2), kBlockBytes = 256u * sizeof(V2);
+  const unsigned tb = (unsigned)(t & 255) * kPair;
+  auto ld = [&](int cb) { return *reinterpret_cast<const V2*>(P + ((unsigned)cb * kBlockBytes + tb)); };
+  V2 m[8];
   {
     const int nblk0 = min(kMfStage, ncol) / C;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int cb = q + u * NQ;
-      m[u] = cb < nblk0 ? ld(cb) : make_double2(0.0, 0.0);
+      m[u] = cb < nblk0 ? ld(cb) : V2{0, 0};
     }
   }
   double pre = 0.0;  // epilogue input of row r (threads t < R)
@@ -485,7 +499,7 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int c = cb + u * NQ;
-          m[u] = c < nblk ? ld(sb + c) : make_double2(0.0, 0.0);
+          m[u] = c < nblk ? ld(sb + c) : V2{0, 0};
         }
       }
       have = false;
@@ -495,10 +509,10 @@ __global__ __launch_bounds__(256 * NQ) __attribute__((amdgpu_waves_per_eu(NQ > 1
         // wraps inside the stage buffer, every entry of which the staging wrote)
         const int j0 = ((cb + u * NQ) * C + 2 * p) & (kMfStage - 1);
         const int j1 = ((cb + (u + 1) * NQ) * C + 2 * p) & (kMfStage - 1);
-        a0 = fma(m[u].x, xs[j0], a0);
-        a1 = fma(m[u].y, xs[j0 + 1], a1);
-        a2 = fma(m[u + 1].x, xs[j1], a2);
-        a3 = fma(m[u + 1].y, xs[j1 + 1], a3);
+        a0 = fma((double)m[u].x, xs[j0], a0);
+        a1 = fma((double)m[u].y, xs[j0 + 1], a1);
+        a2 = fma((double)m[u + 1].x, xs[j1], a2);
+        a3 = fma((double)m[u + 1].y, xs[j1 + 1], a3);
       }
     }
   }
@@ -569,6 +583,8 @@ void MultifrontalLU::release() {
   dev_free(d_panel_);
   dev_free(d_tile_t_);
   tiles_t_ready_ = false;
+  dev_free(d_tile32_);
+  tiles32_ready_ = false;
   dev_free(d_tile_);
   dev_free(d_r_);
   dev_free(d_ay_);
@@ -724,6 +740,7 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
   }
   pack_tiles(false, st);
   tiles_t_ready_ = false;
+  tiles32_ready_ = false;
   MF_HIP(hipGetLastError());
   int status = 0;
   std::vector<int64_t> info(std::max(nf, 1), 0);
@@ -775,18 +792,18 @@ void MultifrontalLU::prepare_transposed(hipStream_t st) {
   tiles_t_ready_ = true;
 }
 
-template <int K, int NQ, int R>
-static void launch_panels(int n, const MfPanel* pn, const double* T, const int* idx, const int* piv, const int* gptr,
+template <int K, int NQ, int R, class TT>
+static void launch_panels(int n, const MfPanel* pn, const TT* T, const int* idx, const int* piv, const int* gptr,
                           const int* gsrc, const double* b, const double* req, double* x, double* vec, hipStream_t st) {
-  hipLaunchKernelGGL((mfs_panel_kernel<K, NQ, R>), dim3(n), dim3(256 * NQ), 0, st, pn, T, idx, piv, gptr, gsrc, b, req, x,
-                     vec);
+  hipLaunchKernelGGL((mfs_panel_kernel<K, NQ, R, TT>), dim3(n), dim3(256 * NQ), 0, st, pn, T, idx, piv, gptr, gsrc, b,
+                     req, x, vec);
 }
 
-template <int K>
-static void launch_panels_k(int wide, int R, int n, const MfPanel* pn, const double* T, const int* idx, const int* piv,
+template <int K, class TT>
+static void launch_panels_k(int wide, int R, int n, const MfPanel* pn, const TT* T, const int* idx, const int* piv,
                             const int* gptr, const int* gsrc, const double* b, const double* req, double* x,
                             double* vec, hipStream_t st) {
-#define MF_PANELS(NQ, RR) launch_panels<K, NQ, RR>(n, pn, T, idx, piv, gptr, gsrc, b, req, x, vec, st)
+#define MF_PANELS(NQ, RR) launch_panels<K, NQ, RR, TT>(n, pn, T, idx, piv, gptr, gsrc, b, req, x, vec, st)
   if (R == 16 && wide) MF_PANELS(4, 16);  // the analysis makes only 16-row panels wide
   else if (R == 16) MF_PANELS(1, 16);
   else if (R == 8) MF_PANELS(1, 8);
@@ -795,14 +812,25 @@ static void launch_panels_k(int wide, int R, int n, const MfPanel* pn, const dou
 }
 
 void MultifrontalLU::launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x,
-                                hipStream_t st) {
+                                hipStream_t st, bool f32) {
   const int s = kind & 3;
   const int n = L.gv_end[s] - L.gv_beg[s];
   if (n <= 0) return;
   const MfPanel* pn = d_panel_ + L.gv_beg[s];
   const int w = L.pwide[s], R = L.prow[s];
   const double* T = kind < 4 ? d_tile_ : d_tile_t_;
-#define MF_K(K) launch_panels_k<K>(w, R, n, pn, T, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st)
+  if (f32) {  // the FP32 tiles of A x = b
+#define MF_K32(K) launch_panels_k<K, float>(w, R, n, pn, d_tile32_, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st)
+    switch (kind) {
+      case 0: MF_K32(0); break;
+      case 1: MF_K32(1); break;
+      case 2: MF_K32(2); break;
+      default: MF_K32(3); break;
+    }
+#undef MF_K32
+    return;
+  }
+#define MF_K(K) launch_panels_k<K, double>(w, R, n, pn, T, d_idx_, d_piv_, d_gptr_, d_gsrc_, b, req, x, d_vec_, st)
   switch (kind) {
     case 0: MF_K(0); break;
     case 1: MF_K(1); break;
@@ -816,16 +844,33 @@ void MultifrontalLU::launch_seg(int kind, const MfLevel& L, const double* b, con
 #undef MF_K
 }
 
-void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStream_t st) {
+void MultifrontalLU::solve(const double* b, const double* req, double* x, hipStream_t st, bool f32) {
+  if (f32 && !tiles32_ready_) throw std::runtime_error("multifrontal FP32 solve before prepare_f32");
   const int H = (int)sym_.levels.size();
   for (int h = 0; h < H; ++h) {
-    launch_seg(0, sym_.levels[h], b, req, x, st);
-    launch_seg(1, sym_.levels[h], b, req, x, st);
+    launch_seg(0, sym_.levels[h], b, req, x, st, f32);
+    launch_seg(1, sym_.levels[h], b, req, x, st, f32);
   }
   for (int h = H - 1; h >= 0; --h) {
-    launch_seg(2, sym_.levels[h], b, req, x, st);
-    launch_seg(3, sym_.levels[h], b, req, x, st);
+    launch_seg(2, sym_.levels[h], b, req, x, st, f32);
+    launch_seg(3, sym_.levels[h], b, req, x, st, f32);
   }
+}
+
+__global__ void mf_to_f32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = (float)a[i];
+}
+
+void MultifrontalLU::prepare_f32(hipStream_t st) {
+  if (!analysed_) throw std::runtime_error("multifrontal FP32 tiles before analysis");
+  if (tiles32_ready_) return;
+  if (!d_tile32_) dev_alloc(d_tile32_, (size_t)sym_.gv_total);
+  if (sym_.gv_total > 0)
+    hipLaunchKernelGGL(mf_to_f32_kernel, dim3((unsigned)((sym_.gv_total + 255) / 256)), dim3(256), 0, st,
+                       sym_.gv_total, d_tile_, d_tile32_);
+  MF_HIP(hipGetLastError());
+  tiles32_ready_ = true;
 }
 
 void MultifrontalLU::solve_t(const double* c, const double* req, double* y, hipStream_t st) {
@@ -873,9 +918,9 @@ bool MultifrontalLU::unstable_pivots(double limit, std::vector<int>& lift, bool&
 }
 
 void MultifrontalLU::solve_correction(const double* b, const double* ay, const double* y, const double* req, double* x,
-                                      hipStream_t st) {
+                                      hipStream_t st, bool f32) {
   launch_adj_refine_in<double>(b, ay, (int)N_, d_r_, st);
-  solve(d_r_, req, d_dx_, st);
+  solve(d_r_, req, d_dx_, st, f32);
   hipLaunchKernelGGL(mf_axpy_kernel, dim3((unsigned)((N_ + 255) / 256)), dim3(256), 0, st, N_, y, d_dx_, x);
 }
 

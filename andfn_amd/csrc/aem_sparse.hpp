@@ -52,8 +52,11 @@ class MultifrontalLU {
   // numeric factorisation of R A (row scales req[N]) from A's CSR values in the analysed pattern;
   // returns 0, or 1 + the first front with an exactly zero pivot
   int factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st, rocblas_handle rb);
-  // x = A^-1 b (b and x in the solver's layout; x is written on the analysed unknowns only)
-  void solve(const double* b, const double* req, double* x, hipStream_t st);
+  // x = A^-1 b (b and x in the solver's layout; x is written on the analysed unknowns only);
+  // f32: through the factors rounded to FP32 (prepare_f32), accumulated in FP64
+  void solve(const double* b, const double* req, double* x, hipStream_t st, bool f32 = false);
+  // the factors rounded to FP32 for solve(..., f32), once per factorisation (outside graph capture)
+  void prepare_f32(hipStream_t st);
   // the transposed solve's tiles, once per factorisation (before solve_t; outside graph capture)
   void prepare_transposed(hipStream_t st);
   // y = A^-T c (req == nullptr: y = (R A)^-T c, the row scales left to the caller)
@@ -66,7 +69,7 @@ class MultifrontalLU {
   // x = y + A^-1 (b - A y) given ay = A y (all in the solver's layout): one solve, the correction
   // of an iterate y, accurate to the condition number times the rounding unit times |b - A y|
   void solve_correction(const double* b, const double* ay, const double* y, const double* req, double* x,
-                        hipStream_t st);
+                        hipStream_t st, bool f32 = false);
   // After factor: the unknowns of every pivot whose L21 column exceeds limit in magnitude (or that is
   // zero), its column and the row it was taken from, to be lifted to the parent front by a new
   // analysis (returns whether there are any).  root_zero: a zero pivot in a root front, which has no
@@ -99,6 +102,8 @@ class MultifrontalLU {
   MfPanel* d_panel_ = nullptr;  // per solve panel: its record
   double* d_tile_t_ = nullptr;  // the transposed solve's tiles (packed on demand)
   bool tiles_t_ready_ = false;
+  float* d_tile32_ = nullptr;   // the tiles rounded to FP32 (the mixed form's correction solves)
+  bool tiles32_ready_ = false;
   int64_t* d_asm_ = nullptr;
   int* d_unk_ = nullptr;     // every analysed unknown (its position in the solver's layout)
   int* d_status_ = nullptr;
@@ -120,7 +125,8 @@ class MultifrontalLU {
 
   void release();
   void factor_big(int f, hipStream_t st, rocblas_handle rb);
-  void launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x, hipStream_t st);
+  void launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x, hipStream_t st,
+                  bool f32 = false);
   void pack_tiles(bool transposed, hipStream_t st);
 };
 
