@@ -89,6 +89,8 @@ class Info(ctypes.Structure):
         ("nnz_factors", ctypes.c_int64),
         ("factor_used", ctypes.c_int32),
         ("factor_rounds", ctypes.c_int32),
+        ("solve_passes", ctypes.c_int64),
+        ("solve_values", ctypes.c_int64),
     ]
 
     def as_dict(self):

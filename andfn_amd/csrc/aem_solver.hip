@@ -1829,7 +1829,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   double error_q = 1.0, error = 1.0, error_coef = 1.0;
   int argmax = 0;
   double ms_sweep = 0, ms_rhs = 0, ms_fit = 0, ms_lu = 0, fl_sweep = 0, fl_rhs = 0;
-  int64_t n_sweep = 0, n_rhs = 0, pr_sweep = 0, pr_rhs = 0;
+  int64_t n_sweep = 0, n_rhs = 0, pr_sweep = 0, pr_rhs = 0, solve_passes = 0;
   std::vector<int> grow_h(E_);
   std::vector<char> grown(E_);
   aem::IterScalars sc{};
@@ -1901,8 +1901,13 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   // solve is the correction x = y + A^-1 (b - A y); otherwise a refined solve of b.
   auto solve_and_accept = [&](int it, bool have_ay) {
     HIP_CHECK(hipEventRecord(sol_ev_[it & 1][0], st_));
-    if (have_ay && sparse_ && lu_mode_ == ANDFN_MI_LU_TRSV) correct_rhs();
-    else solve_rhs();
+    if (have_ay && sparse_ && lu_mode_ == ANDFN_MI_LU_TRSV) {
+      correct_rhs();
+      ++solve_passes;
+    } else {
+      solve_rhs();
+      solve_passes += sparse_ ? 2 : 1;
+    }
     HIP_CHECK(hipEventRecord(sol_ev_[it & 1][1], st_));
     aem::launch_accept(nv, d_de_.p, nde, F_, sys(), d_x_.p, d_y_.p, d_yold_.p, n, st_);
   };
@@ -2174,6 +2179,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
                                        ? ANDFN_MI_FACTOR_TOURNAMENT
                                        : ANDFN_MI_FACTOR_ROCSOLVER);
     info->factor_rounds = sparse_ ? mf_rounds_ : 0;
+    info->solve_passes = solve_passes;
+    info->solve_values = sparse_ ? mf_.symbolic().gv_total : nnz_factors_;
   }
   mixed_phase_ = false;
   have_state_ = true;

@@ -1,26 +1,31 @@
 """Benchmark: DFN solves per second on one MI355X.
 
-A step is one complete solve of the seeded 5,000-fracture network (500 m cube, radii on
-[10, 150] m, 20 coefficients / 40 collocation points per element): discharge-matrix assembly,
-LU factorisation, the fixed-point iterations with coefficient growth, and the final head
-residual, with the network already resident in device memory when the timed region starts.
-The solve stops as soon as the head residual (max |A y - b(c)| over the head equations) is at
-most 1e-6 m.  ``value`` is the FP64 form (``--precision``; the batched cfg5 workload defaults
-to its specified mixed form); ``forms`` adds the other form (the
-mixed-precision form: FP32 factors refined in FP64, FP32 far field and fits) with its error
-against the FP64 result; ``adjoint`` times the implicit adjoint (gradient of the x = 0 boundary
-discharge plus 100 observation heads with respect to every log10 transmissivity and both
-boundary heads) against one forward solve.
+The default workload is the 20,000-fracture network (1,000 m cube, radii on [10, 300] m, 20
+coefficients / 40 collocation points per element, 50 wells).  A step is one complete solve:
+discharge-matrix assembly, sparse LU factorisation, the fixed-point iterations with coefficient
+growth, stopping as soon as the head residual (max |A y - b(c)| over the head equations) is at
+most 1e-6 m.  ``value`` is the specified form of the workload (mixed precision for the
+20,000-fracture network and the batch: FP32 far field, fit and factors, FP64 residual, near
+field and correction; FP64 otherwise) on a handle whose network is resident in device memory
+(the numeric factorisation is redone every step; the symbolic analysis of the unchanged pattern
+is kept, as SuperLU's ``SamePattern`` reuse keeps it).  ``forms`` adds the other form with its
+error against the FP64 result; ``drop_in`` times the drop-in ``hpc_solve.solve`` call, which
+pays the record parse, upload, symbolic analysis and every pivot-delay round, and downloads the
+records; ``adjoint`` times the implicit adjoint against one forward solve.
 
-Prints one JSON line.  ``roofline`` describes the dominant kernel chain, the blocked triangular
-solves (HBM bound; one per iteration); ``roofline_sweep`` the collocation sweep kernel (FP64
-vector ALU bound); ``cpu_baseline`` times the CPU FP64 oracle (a restatement of AnDFN's Numba
-path with SciPy's SuperLU) on a bounded sample of the same solve on this host.
+Prints one JSON line.  ``roofline`` is the dominant kernel, the collocation sweep (FP64 vector
+ALU bound); ``roofline_solve`` the sparse triangular solves (HBM bound, bytes by the
+12 nnz(L+U) + 16 n convention per pass through the factors).  ``cpu_baseline`` times the CPU
+FP64 oracle (a restatement of AnDFN's Numba path with SciPy's SuperLU) on this host in a child
+process started before any GPU call and run after the GPU timing, on a bounded sample (element
+iterations, assembly, the SuperLU factorisation up to a wall-clock cap); a capped factorisation
+makes the CPU time a lower bound and the speed-up a lower bound.
 """
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -34,34 +39,38 @@ PEAK_FP64 = 78.6e12  # MI355X dense FP64 (vector and matrix), flop/s
 PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 HEAD_TOL = 1e-6
 ADJOINT_TOL = 1e-10
+MIXED_WORKLOADS = ("cfg3", "cfg5")  # workloads specified in the mixed-precision form
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cfg3")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
-                    help="form timed for value (default: fp64; mixed for cfg5, which is specified mixed)")
+                    help="form timed for value (default: mixed for cfg3 and cfg5, which are specified mixed)")
     ap.add_argument("--lu-mode", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="time only the --precision form")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--dropin-steps", type=int, default=3, help="drop-in hpc_solve.solve calls timed")
     ap.add_argument("--realisations", type=int, default=64, help="cfg5: number of realisations")
     ap.add_argument("--max-iterations", type=int, default=0, help="override MAX_ITERATIONS (short profiling runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
-    ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle iterations in the CPU sample")
+    ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle element iterations in the CPU sample")
+    ap.add_argument("--cpu-cap", type=float, default=90.0, help="wall-clock cap (s) of the CPU sample")
+    ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.precision is None:
-        args.precision = "mixed" if args.workload == "cfg5" else "fp64"
+        args.precision = "mixed" if args.workload in MIXED_WORKLOADS else "fp64"
     return args
 
 
 def traffic_from_profiles(name, workload):
-    """HBM bytes per launch (``sweep``) or per solve (``solve``) measured with rocprofv3 PMC
-    counters (profiles/<name>_<workload>_traffic.json), if present."""
+    """HBM bytes per launch measured with rocprofv3 PMC counters
+    (profiles/<name>_<workload>_traffic.json), if present."""
     p = ROOT / "profiles" / f"{name}_{workload}_traffic.json"
     if p.exists():
         try:
@@ -72,46 +81,138 @@ def traffic_from_profiles(name, workload):
     return None
 
 
-def cpu_baseline(fa, ea, K, consts, nit_gpu, n_iter):
-    """Oracle on this host: SuperLU factorisation + n_iter iterations, scaled to nit_gpu iterations."""
+# ---- CPU baseline: a child process that never touches the GPU -----------------------------------
+
+def cpu_child(args):
+    """Build the same network, report ready, wait for ``go``, then time the oracle's pieces one
+    by one (each printed as soon as it is known, so a capped run still reports the finished ones):
+    element iterations (the Numba path's per-iteration work without the triangular solve),
+    assembly of the discharge matrix, SuperLU factorisation, one SuperLU solve."""
+    from andfn_amd.generator import build_network, workload
     from oracle import oracle
 
-    fo, eo = fa.copy(), ea.copy()
-    O = oracle.OracleHandle(fo, eo, K)
+    spec = workload(args.workload)
+    d = build_network(spec)
+    d.consolidate_dfn()
+    fa, ea = d.fractures_struc_array_hpc.copy(), d.elements_struc_array_hpc.copy()
+    consts = d.constants
+    O = oracle.OracleHandle(fa, ea, d.discharge_int)
+    print("ready", flush=True)
+    line = sys.stdin.readline()
+    if not line.startswith("go"):
+        return
     lib = O.lib
-    t0 = time.perf_counter()
-    A = O.discharge_matrix()
-    import scipy.sparse.linalg as spla
-
-    t1 = time.perf_counter()
-    lu = spla.splu(A)
-    t2 = time.perf_counter()
     lib.oracle_init(O.h)
     y = np.zeros(O.n)
     lib.oracle_old_discharges(O.h, oracle._ptr(y))
-    t3 = time.perf_counter()
-    for nit in range(1, n_iter + 1):
+    O.set_discharges(y)
+    c = {k: consts[k].item() for k in consts.dtype.names}
+    for it in range(1, args.cpu_iterations + 1):
+        t0 = time.perf_counter()
         b = O.head_rhs()
-        y = lu.solve(b)
-        O.set_discharges(y)
-        lib.oracle_element_solver2(O.h, float(consts["MAX_ERROR"]), nit, float(consts["COEF_RATIO"]),
-                                   int(consts["MAX_NCOEF"]), int(consts["COEF_INCREASE"]), float(consts["DAMPING"]),
-                                   None)
-    t4 = time.perf_counter()
+        lib.oracle_element_solver2(O.h, float(c["MAX_ERROR"]), it, float(c["COEF_RATIO"]), int(c["MAX_NCOEF"]),
+                                   int(c["COEF_INCREASE"]), float(c["DAMPING"]), None)
+        print(f"iteration {time.perf_counter() - t0:.6f}", flush=True)
+    t0 = time.perf_counter()
+    A = O.discharge_matrix()
+    print(f"assembly {time.perf_counter() - t0:.6f}", flush=True)
+    import scipy.sparse.linalg as spla
+
+    print("splu_start", flush=True)
+    t0 = time.perf_counter()
+    lu = spla.splu(A)
+    print(f"splu {time.perf_counter() - t0:.6f}", flush=True)
+    t0 = time.perf_counter()
+    lu.solve(b)
+    print(f"lusolve {time.perf_counter() - t0:.6f}", flush=True)
     O.close()
-    t_iter = (t4 - t3) / n_iter
-    t_solve = (t1 - t0) + (t2 - t1) + nit_gpu * t_iter
+
+
+def start_cpu_child(args):
+    cmd = [sys.executable, "-u", str(ROOT / "bench.py"), "--cpu-child", "--workload", args.workload,
+           "--cpu-iterations", str(args.cpu_iterations)]
+    return subprocess.Popen(cmd, stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, bufsize=1, cwd=str(ROOT))
+
+
+def finish_cpu_child(proc, args, nit_gpu):
+    """Send go, collect the timings within the cap, stop the child, and assemble cpu_baseline."""
+    import selectors
+
+    t_start = time.perf_counter()
+    fd = proc.stdout.fileno()
+    sel = selectors.DefaultSelector()
+    sel.register(fd, selectors.EVENT_READ)
+    buf = b""
+
+    def lines_until(deadline):
+        """complete lines from the child's unbuffered pipe, until EOF or the deadline"""
+        nonlocal buf
+        while True:
+            while b"\n" in buf:
+                line, buf = buf.split(b"\n", 1)
+                yield line.decode()
+            left = deadline - time.perf_counter()
+            if left <= 0 or not sel.select(timeout=min(left, 5.0)):
+                if left <= 0:
+                    return
+                continue
+            chunk = os.read(fd, 4096)
+            if not chunk:
+                return
+            buf += chunk
+
+    ready = next(lines_until(time.perf_counter() + 600.0), "")
+    if ready.strip() != "ready":
+        proc.kill()
+        proc.wait()
+        return None
+    proc.stdin.write("go\n")
+    proc.stdin.flush()
+    t_go = time.perf_counter()
+    its, got, splu_started = [], {}, None
+    for line in lines_until(t_go + args.cpu_cap):
+        key, *val = line.split()
+        if key == "iteration":
+            its.append(float(val[0]))
+        elif key == "splu_start":
+            splu_started = time.perf_counter()
+        elif val:
+            got[key] = float(val[0])
+        if key == "lusolve":
+            break
+    capped = "lusolve" not in got
+    if proc.poll() is None:
+        proc.kill()  # the child started above (its own PID), capped or done
+    proc.wait()
+    if not its or "assembly" not in got:
+        return None
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t_iter = float(np.mean(its))
+    if "splu" in got:
+        t_splu, splu_note = got["splu"], f"SuperLU splu {got['splu']:.1f} s"
+    else:
+        t_splu = time.perf_counter() - splu_started if splu_started else 0.0
+        splu_note = f"SuperLU splu stopped at the {args.cpu_cap:g} s cap after {t_splu:.1f} s (unfinished)"
+    t_lus = got.get("lusolve", 0.0)
+    t_solve = got["assembly"] + t_splu + nit_gpu * (t_iter + t_lus)
+    lower = capped or "splu" not in got
     return {
         "value": 1.0 / t_solve,
         "unit": "solves/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"same network: assembly {t1 - t0:.2f}s + SuperLU splu {t2 - t1:.2f}s + {n_iter} oracle "
-                   f"iterations ({t_iter:.2f}s each, ncoef growing from the initial counts) scaled to the GPU "
-                   f"run's {nit_gpu} iterations; SuperLU single-threaded, element kernels on {threads} OpenMP threads"),
+        "bound": "upper (CPU time is a lower bound)" if lower else "measured pieces",
+        "seconds_per_solve": t_solve,
+        "sample": (f"same network, FP64 oracle on the host: {len(its)} element iterations ({t_iter:.2f} s each on "
+                   f"{threads} OpenMP threads: head RHS, element sweep and fit), discharge-matrix assembly "
+                   f"{got['assembly']:.2f} s, {splu_note}"
+                   + (f", one SuperLU solve {t_lus:.3f} s" if t_lus else ", SuperLU solves not reached (omitted)")
+                   + f"; the solve time counts assembly + factorisation + {nit_gpu} iterations (the GPU run's count)"),
+        "wall_s": time.perf_counter() - t_start,
     }
 
+
+# ---- GPU forms -----------------------------------------------------------------------------------
 
 def _obs_points(fa, n, seed=3):
     rng = np.random.default_rng(seed)
@@ -134,6 +235,70 @@ def _time_form(net, precision, lu_mode, warmup, steps, label):
     return infos, walls
 
 
+def _time_dropin(fa0, ea0, dfn, precision, steps):
+    """The drop-in entry point on the unsolved records, records in and out, a new handle per call."""
+    from andfn_amd.hpc_solve import last_solve_info, solve
+
+    walls, infos = [], []
+    for k in range(steps):
+        fa, ea = fa0.copy(), ea0.copy()
+        t0 = time.perf_counter()
+        solve(fa, ea, dfn.discharge_int, dfn.constants, precision=precision, head_tol=HEAD_TOL)
+        walls.append(time.perf_counter() - t0)
+        infos.append(last_solve_info())
+        print(f"# drop-in {precision} call {k}: {walls[-1]:.3f}s nit={infos[-1]['nit']} "
+              f"factor={infos[-1]['t_factor']:.3f}s rounds={infos[-1]['factor_rounds']}", file=sys.stderr, flush=True)
+    return walls, infos
+
+
+def _roofline_sweep(last):
+    n = max(int(last["n_sweep"]), 1)
+    ms = last["k_sweep_ms"] / n
+    fl = last["flops_sweep"] / n
+    achieved = fl / (ms * 1e-3) if ms > 0 else 0.0
+    return {
+        "bound": "mfma",
+        "kernel": "sweep_kernel<MODE_COLLOC> collocation sweep (FP64 vector ALU; MI355X FP64 vector peak = FP64 "
+                  "matrix peak; in the mixed form the far-field pairs run in FP32 and the FP64 peak stays the "
+                  "denominator)",
+        "achieved": achieved / 1e12,
+        "peak": PEAK_FP64 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP64,
+        "traffic": None,
+        "avg_launch_ms": ms,
+        "flops_per_launch": fl,
+        "launches": n,
+    }
+
+
+def _roofline_solve(last, f32):
+    passes = max(int(last["solve_passes"]), 1)
+    ms = last["k_lu_ms"] / passes  # per pass through the factors (correction solves: one each)
+    nnz, n = float(last["nnz_factors"]), float(last["n_unknowns"])
+    conv = 12.0 * nnz + 16.0 * n  # bytes per pass by the 12 nnz(L+U) + 16 n convention
+    streamed = (4.0 if f32 else 8.0) * float(last["solve_values"]) + 16.0 * n
+    bw = conv / (ms * 1e-3) if ms > 0 else 0.0
+    return {
+        "bound": "hbm",
+        "kernel": "multifrontal triangular solves (mfs_panel_kernel, four launches per level of the elimination tree, "
+                  "one HIP graph per solve)",
+        "achieved": bw / 1e9,
+        "peak": PEAK_HBM / 1e9,
+        "unit": "GB/s",
+        "frac": bw / PEAK_HBM,
+        "traffic": None,
+        "avg_pass_ms": ms,
+        "bytes_per_pass": conv,
+        "bytes_streamed_per_pass": streamed,
+        "streamed_GBps": streamed / (ms * 1e-3) / 1e9 if ms > 0 else 0.0,
+        "passes": passes,
+        "note": ("bytes_per_pass follows the 12 nnz(L+U) + 16 n convention (values plus 32-bit indices); the "
+                 "panel tiles hold dense blocks without indices, so a pass streams bytes_streamed_per_pass"
+                 + (" (tiles in FP32)" if f32 else "")),
+    }
+
+
 def bench_batch(args):
     """Batched realisations: 64 seeded DFNs (fracture counts uniform on [500, 4000], the 200-disk law
     at constant density, 10 to 20 coefficients) as one network; the mixed form to head residual
@@ -153,10 +318,12 @@ def bench_batch(args):
     for r, d in enumerate(dfns):
         dq[off[r][1] + boundary_groups(d)["x0"]] = 1.0
     res = {}
+    lasts = {}
     for prec in (args.precision, "fp64" if args.precision == "mixed" else "mixed"):
         if prec != args.precision and args.no_compare:
             continue
         infos, walls = _time_form(net, prec, 0, args.warmup, args.steps, prec)
+        lasts[prec] = infos[-1]
         aw = []
         for k in range(1 + args.steps):
             t1 = time.perf_counter()
@@ -170,6 +337,7 @@ def bench_batch(args):
         res[prec]["adjoint_forward_ratio"] = res[prec]["adjoint_ms"] / res[prec]["ms_per_batch"]
     net.close()
     main_r = res[args.precision]
+    last = lasts[args.precision]
     out = {
         "metric": "DFN realisation solves/sec at max head residual 1e-6 (batched, one network)",
         "value": count / (main_r["ms_per_batch"] / 1e3),
@@ -187,7 +355,8 @@ def bench_batch(args):
                                f"at constant density, 10-20 coefficients",
                    "fractures": int(len(fa)), "elements": int(len(ea))},
         "forms": res,
-        "roofline": None,
+        "roofline": _roofline_sweep(last),
+        "roofline_solve": _roofline_solve(last, args.precision == "mixed" and last["solves_fp32"] > 0),
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -195,16 +364,36 @@ def bench_batch(args):
 
 def main():
     args = parse()
-    if args.workload == "cfg5":
-        if args.gpus != 1:
-            print("bench.py runs on one GPU only", file=sys.stderr)
-            sys.exit(2)
-        bench_batch(args)
+    if args.cpu_child:
+        cpu_child(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "cfg5":
+        bench_batch(args)
+        return
+    # the CPU baseline's child starts before anything touches the GPU and waits for the GPU timing
+    child = None if args.no_cpu_baseline else start_cpu_child(args)
+    try:
+        out = gpu_part(args)
+        if child is not None:
+            out["cpu_baseline"] = finish_cpu_child(child, args, int(out["nit"]))
+            child = None
+            cb = out["cpu_baseline"]
+            if cb:
+                out["speedup_vs_cpu"] = {"value": out["value"] / cb["value"],
+                                         "bound": "lower" if cb["bound"].startswith("upper") else "measured",
+                                         "drop_in": out["drop_in"]["solves_per_s"] / cb["value"]
+                                         if out.get("drop_in") else None}
+        print(json.dumps(out))
+    finally:
+        if child is not None and child.poll() is None:
+            child.kill()
+
+
+def gpu_part(args):
     from andfn_amd import _lib
     from andfn_amd.autograd import boundary_groups
     from andfn_amd.generator import build_network, workload
@@ -225,16 +414,17 @@ def main():
         del src, dst
     spec = workload(args.workload, constants={"MAX_ITERATIONS": args.max_iterations}) if args.max_iterations \
         else workload(args.workload)
+    t0 = time.perf_counter()
     dfn = build_network(spec)
     dfn.consolidate_dfn()
+    t_build = time.perf_counter() - t0
     fa, ea = dfn.fractures_struc_array_hpc, dfn.elements_struc_array_hpc
-    fa0, ea0 = fa.copy(), ea.copy()
+    fa0, ea0 = fa.copy(), ea.copy()  # the unsolved records (the handle's downloads overwrite fa, ea)
     net = DeviceNetwork(fa, ea, dfn.discharge_int, dfn.constants)
-    print(f"# network: {len(fa)} fractures, {len(ea)} elements; warmup {args.warmup}, steps {args.steps}",
-          file=sys.stderr, flush=True)
+    print(f"# network: {len(fa)} fractures, {len(ea)} elements (built in {t_build:.1f}s); warmup {args.warmup}, "
+          f"steps {args.steps}", file=sys.stderr, flush=True)
     other = "mixed" if args.precision == "fp64" else "fp64"
-    forms = {}
-    states = {}
+    forms, states, lasts = {}, {}, {}
     obs_frac, obs_z = _obs_points(fa, 100)
     for prec in ([args.precision] + ([] if args.no_compare else [other])):
         infos, walls = _time_form(net, prec, args.lu_mode, args.warmup, args.steps, prec)
@@ -242,15 +432,18 @@ def main():
         net.download()
         states[prec] = (ea["coef"].copy(), q, c, net.state_heads(obs_frac, obs_z))
         last = infos[-1]
+        lasts[prec] = last
         forms[prec] = {"solves_per_s": args.steps / sum(walls), "ms_per_solve": 1e3 * sum(walls) / args.steps,
                        "nit": int(last["nit"]), "nit_fp32": int(last["nit_fp32"]),
+                       "solves_fp32": int(last["solves_fp32"]), "refine_rate": last["refine_rate"],
                        "head_residual": last["head_residual"], "t_factor_s": last["t_factor"],
-                       "t_iterate_s": last["t_iterate"], "info": last, "walls": walls}
+                       "t_iterate_s": last["t_iterate"], "k_sweep_ms": last["k_sweep_ms"],
+                       "k_rhs_ms": last["k_rhs_ms"], "k_fit_ms": last["k_fit_ms"], "k_lu_ms": last["k_lu_ms"],
+                       "walls": walls}
     main_form = forms[args.precision]
-    last = main_form.pop("info")
+    last = lasts[args.precision]
     walls = main_form.pop("walls")
     if other in forms:
-        forms[other].pop("info")
         forms[other].pop("walls")
         a, b = states["fp64"], states["mixed"]
 
@@ -260,8 +453,9 @@ def main():
         forms["mixed"]["error_vs_fp64"] = {"coef_rel": rel(b[0], a[0]), "q_rel": rel(b[1], a[1]),
                                            "constant_rel": rel(b[2], a[2]),
                                            "heads_abs_m": float(np.max(np.abs(b[3] - a[3]))),
-                                           "note": "both forms stopped at head residual <= 1e-6 m; free-running "
-                                                   "coefficient growth; heads at 100 observation points"}
+                                           "note": "both forms stopped at head residual <= 1e-6 m; heads at 100 "
+                                                   "observation points"}
+        forms["mixed_over_fp64"] = forms["mixed"]["solves_per_s"] / forms["fp64"]["solves_per_s"]
     adjoint = None
     if not args.no_adjoint:
         # gradient of Q(x=0) + sum of 100 observation heads w.r.t. every log10 T and the two boundary heads
@@ -271,14 +465,14 @@ def main():
         dq = np.zeros(len(ea))
         dq[groups["x0"]] = 1.0
         aw = []
-        for k in range(1 + args.steps):
+        for k in range(1 + min(args.steps, 3)):
             t0 = time.perf_counter()
             gt, gh, ai = net.adjoint(dq=dq, obs_frac=obs_frac, obs_z=obs_z, dhead=np.ones(len(obs_frac)),
                                      tol=ADJOINT_TOL)
             aw.append(time.perf_counter() - t0)
             print(f"# adjoint {k}: {aw[-1]:.3f}s iterations={ai['nit']} residual={ai['delta']:.2e}",
                   file=sys.stderr, flush=True)
-        t_adj = sum(aw[1:]) / args.steps
+        t_adj = sum(aw[1:]) / (len(aw) - 1)
         adjoint = {"objective": "Q through x=0 + sum of heads at 100 observation points",
                    "gradient": f"d/d log10 T for all {len(fa)} fractures and d/d head of the 2 boundary planes",
                    "method": "implicit fixed-point adjoint, GMRES on the transposed iteration (FP64)",
@@ -286,20 +480,17 @@ def main():
                    "ms": 1e3 * t_adj, "first_call_ms": 1e3 * aw[0],
                    "adjoint_forward_ratio": (t_adj / t_fwd) if t_fwd else None}
     net.close()
+    drop_in = None
+    if args.dropin_steps > 0:
+        dw, dinfo = _time_dropin(fa0, ea0, dfn, args.precision, args.dropin_steps)
+        drop_in = {"solves_per_s": len(dw) / sum(dw), "ms_per_solve": 1e3 * sum(dw) / len(dw), "calls": len(dw),
+                   "t_setup_s": dinfo[-1]["t_setup"], "t_factor_s": dinfo[-1]["t_factor"],
+                   "t_iterate_s": dinfo[-1]["t_iterate"], "factor_rounds": int(dinfo[-1]["factor_rounds"]),
+                   "note": "andfn_amd.hpc_solve.solve per call: records in, new handle (upload, symbolic analysis, "
+                           "every pivot-delay round, factorisation), iterations, records out"}
     total = sum(walls)
-    sweep_ms = last["k_sweep_ms"] / max(last["n_sweep"], 1)
-    flops_launch = last["flops_sweep"] / max(last["n_sweep"], 1)
-    achieved = flops_launch / (sweep_ms * 1e-3) if sweep_ms > 0 else 0.0
-    # the dominant kernel chain: the blocked triangular solves (one HIP graph per solve), timed
-    # by HIP events around every graph launch.  One connected part, so one dense block of n
-    # unknowns: the chain reads the strictly triangular panels of L and U and the triangles of
-    # the diagonal-block inverses, n^2 + n values.
-    n_solve = max(int(last["n_rhs"]), 1)
-    solve_ms = last["k_lu_ms"] / n_solve
-    n_unk = int(last["n_unknowns"])
-    solve_bytes = (8.0 if args.precision == "fp64" else 4.0) * (n_unk * n_unk + n_unk)
-    solve_bw = solve_bytes / (solve_ms * 1e-3) if solve_ms > 0 else 0.0
-    out = {
+    f32 = args.precision == "mixed" and last["solves_fp32"] > 0
+    return {
         "metric": "DFN solves/sec at max head residual 1e-6 (FP64 vs mixed GPU); adjoint/forward ratio",
         "value": args.steps / total,
         "unit": "solves/s",
@@ -315,15 +506,15 @@ def main():
         "config": {
             "workload": f"{args.workload}: {spec.n_fractures} disks, {spec.cube:g} m cube, radii [{spec.rmin:g}, "
                         f"{spec.rmax:g}] m, pdf r^-{spec.alpha:g}, ncoef {spec.ncoef}, nint {spec.nint}, "
-                        f"K {spec.discharge_int}",
+                        f"K {spec.discharge_int}" + (f", {spec.n_wells} wells" if spec.n_wells else ""),
             "fractures_kept": int(len(fa)),
             "elements": int(len(ea)),
             "unknowns": int(last["n_unknowns"]),
             "precision": args.precision,
             "stop": f"head residual <= {HEAD_TOL:g} m (max |A y - b(c)| over the head equations), "
                     f"MAX_ITERATIONS {int(dfn.constants['MAX_ITERATIONS'])}",
-            "filters": f"rim clearance {spec.rim_clearance:g} R, near-coincident pairs < {spec.min_separation:g} L "
-                       f"at < {spec.min_angle_deg:g} deg",
+            "timed": "DeviceNetwork.run on a resident handle: assembly, numeric factorisation and the iterations "
+                     "every step (drop_in times the full drop-in call)",
         },
         "nit": int(last["nit"]),
         "head_residual": last["head_residual"],
@@ -331,41 +522,14 @@ def main():
         "converged": bool(last["converged"]),
         "t_factor_s": last["t_factor"],
         "t_iterate_s": last["t_iterate"],
+        "nnz_factors": int(last["nnz_factors"]),
         "forms": forms,
+        "drop_in": drop_in,
         "adjoint": adjoint,
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "blocked triangular solves (gemv_cm_kernel + gemv_diag_kernel chain, one HIP graph per "
-                      "solve)",
-            "share_of_step": (last["k_lu_ms"] / (1e3 * total / args.steps)) if total > 0 else None,
-            "achieved": solve_bw / 1e9,
-            "peak": PEAK_HBM / 1e9,
-            "unit": "GB/s",
-            "frac": solve_bw / PEAK_HBM,
-            "traffic": traffic_from_profiles("solve", args.workload) if args.precision == "fp64" else None,
-            "avg_launch_ms": solve_ms,
-            "bytes_per_launch": solve_bytes,
-            "launches": n_solve,
-            "unit_of_launch": "one triangular-solve chain (graph launch)",
-        },
-        "roofline_sweep": {
-            "bound": "mfma",
-            "kernel": "sweep_kernel<MODE_COLLOC> (FP64 vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
-            "achieved": achieved / 1e12,
-            "peak": PEAK_FP64 / 1e12,
-            "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64,
-            "traffic": traffic_from_profiles("sweep", args.workload),
-            "avg_launch_ms": sweep_ms,
-            "flops_per_launch": flops_launch,
-            "launches": int(last["n_sweep"]),
-        },
+        "roofline": _roofline_sweep(last),
+        "roofline_solve": _roofline_solve(last, f32),
         "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(fa0, ea0, dfn.discharge_int, dfn.constants, int(last["nit"]),
-                                           args.cpu_iterations)
-    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -130,6 +130,10 @@ typedef struct andfn_mi_info {
   int32_t factor_used;    /* ANDFN_MI_FACTOR_* that factored the last run's matrix */
   int32_t factor_rounds;  /* multifrontal LU: factorisations of the last run (more than one when unstable
                            * pivots were delayed to parent fronts and the tree analysed again) */
+  int64_t solve_passes;   /* passes through the stored factors by the iterations' discharge solves (a
+                           * refined solve makes two, a correction solve x = y + A^-1 (b - A y) one) */
+  int64_t solve_values;   /* values one pass streams (multifrontal: the packed panel tiles, FP32 in the
+                           * mixed form's correction solves; dense: the n^2 entries of the blocks) */
 } andfn_mi_info;
 
 /*
