@@ -1133,7 +1133,7 @@ void Solver::assemble_and_factor() {
       int bad = 0;
       std::vector<int> lift;
       try {
-        bad = mf_.factor(d_csr_val_.p, d_csr_ptr_.p, d_req_.p, st_, rb_);
+        bad = mf_.factor(d_csr_val_.p, d_csr_ptr_.p, d_req_.p, st_);
         ++mf_rounds_;
         mf_.unstable_pivots(kGrowthLimit, lift, root_zero, st_);
       } catch (const std::exception& e) {

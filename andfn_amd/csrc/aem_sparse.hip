@@ -551,7 +551,26 @@ __global__ void mft_scale_kernel(int n, const int* __restrict__ unk, const doubl
 
 }  // namespace
 
-MultifrontalLU::~MultifrontalLU() { release(); }
+MultifrontalLU::~MultifrontalLU() {
+  release();
+  for (Lane& l : lanes_) {
+    if (l.rb) (void)rocblas_destroy_handle(l.rb);
+    if (l.done) (void)hipEventDestroy(l.done);
+    if (l.s) (void)hipStreamDestroy(l.s);
+  }
+  if (fork_) (void)hipEventDestroy(fork_);
+}
+
+void MultifrontalLU::ensure_lanes() {
+  if (lanes_[0].s) return;
+  MF_HIP(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+  for (Lane& l : lanes_) {
+    MF_HIP(hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking));  // default priority
+    MF_HIP(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+    MF_RB(rocblas_create_handle(&l.rb));
+    MF_RB(rocblas_set_stream(l.rb, l.s));
+  }
+}
 
 void MultifrontalLU::release() {
   dev_free(d_fr_);
@@ -570,12 +589,14 @@ void MultifrontalLU::release() {
   dev_free(d_vec_);
   dev_free(d_ys_);
   dev_free(d_unk_);
-  for (int i = 0; i < 2; ++i) {
-    dev_free(d_cand_[i]);
-    dev_free(d_cnt_[i]);
+  for (Lane& l : lanes_) {
+    for (int i = 0; i < 2; ++i) {
+      dev_free(l.cand[i]);
+      dev_free(l.cnt[i]);
+    }
+    dev_free(l.lu11);
+    dev_free(l.pairs);
   }
-  dev_free(d_lu11_);
-  dev_free(d_pairs_);
   dev_free(d_info_);
   dev_free(d_growth_);
   dev_free(d_gv_);
@@ -595,6 +616,7 @@ void MultifrontalLU::release() {
 
 void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
   release();
+  ensure_lanes();
   mf_analyse(in, sym_);
   N_ = in.N;
   nrows_csr_ = (int)in.N;
@@ -639,12 +661,14 @@ void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
   dev_alloc(d_ys_, (size_t)N_);
   MF_HIP(hipMemsetAsync(d_ys_, 0, sizeof(double) * std::max<int64_t>(N_, 1), st));
   const int64_t groups = (s.big_k + kLuGroupRows - 1) / kLuGroupRows;
-  for (int i = 0; i < 2; ++i) {
-    dev_alloc(d_cand_[i], (size_t)std::max<int64_t>(groups, 1) * kLuPanel);
-    dev_alloc(d_cnt_[i], (size_t)std::max<int64_t>(groups, 1));
+  for (Lane& l : lanes_) {
+    for (int i = 0; i < 2; ++i) {
+      dev_alloc(l.cand[i], (size_t)std::max<int64_t>(groups, 1) * kLuPanel);
+      dev_alloc(l.cnt[i], (size_t)std::max<int64_t>(groups, 1));
+    }
+    dev_alloc(l.lu11, (size_t)kLuPanel * kLuPanel);
+    dev_alloc(l.pairs, (size_t)(kLuOuter / kLuPanel) * kLuPairStride);
   }
-  dev_alloc(d_lu11_, (size_t)kLuPanel * kLuPanel);
-  dev_alloc(d_pairs_, (size_t)(kLuOuter / kLuPanel) * kLuPairStride);
   dev_alloc(d_info_, std::max<size_t>(s.fronts.size(), 1));
   dev_alloc(d_growth_, s.piv0.size());
   dev_alloc(d_r_, (size_t)N_);
@@ -657,32 +681,34 @@ void MultifrontalLU::analyse(const MfInput& in, hipStream_t st) {
   analysed_ = true;
 }
 
-void MultifrontalLU::factor_big(int fi, hipStream_t st, rocblas_handle rb) {
+void MultifrontalLU::factor_big(int fi, const Lane& lane) {
   const MfFront& f = sym_.fronts[fi];
   const int64_t m = f.m, k = f.k;
   double* A = d_F_ + f.off;
   int* perm = d_piv_ + f.piv;
   int64_t* info = d_info_ + fi;
   const double one = 1.0, mone = -1.0;
-  int* cand[2] = {d_cand_[0], d_cand_[1]};
-  int* cnt[2] = {d_cnt_[0], d_cnt_[1]};
+  hipStream_t st = lane.s;
+  rocblas_handle rb = lane.rb;
+  int* cand[2] = {lane.cand[0], lane.cand[1]};
+  int* cnt[2] = {lane.cnt[0], lane.cnt[1]};
   // the first k steps of the front's LU, pivot search over its first k rows only
   for (int64_t J0 = 0; J0 < k; J0 += kLuOuter) {
     const int64_t J1 = std::min<int64_t>(k, J0 + kLuOuter);
     int nl = 0;
     for (int64_t j0 = J0; j0 < J1; j0 += kLuPanel, ++nl) {
       const int bw = (int)std::min<int64_t>(kLuPanel, J1 - j0);
-      int* pairs = d_pairs_ + (int64_t)nl * kLuPairStride;
-      const int* win = launch_tslu(A, m, k, j0, bw, cand, cnt, d_lu11_, pairs, perm, info, st);
-      launch_calu_apply(A, m, k, J0, J1, j0, bw, pairs, win, d_lu11_, st);
-      launch_calu_l21(A, m, m, j0, bw, d_lu11_, st);
+      int* pairs = lane.pairs + (int64_t)nl * kLuPairStride;
+      const int* win = launch_tslu(A, m, k, j0, bw, cand, cnt, lane.lu11, pairs, perm, info, st);
+      launch_calu_apply(A, m, k, J0, J1, j0, bw, pairs, win, lane.lu11, st);
+      launch_calu_l21(A, m, m, j0, bw, lane.lu11, st);
       const int64_t rows = m - j0 - bw, cols = J1 - j0 - bw;
       if (rows > 0 && cols > 0)
         MF_RB(rocblas_dgemm_64(rb, rocblas_operation_none, rocblas_operation_none, rows, cols, bw, &mone,
                                A + (j0 + bw) + j0 * m, m, A + j0 + (j0 + bw) * m, m, &one,
                                A + (j0 + bw) + (j0 + bw) * m, m));
     }
-    launch_calu_swap(A, m, m, J0, J1, d_pairs_, nl, st);
+    launch_calu_swap(A, m, m, J0, J1, lane.pairs, nl, st);
     const int64_t rest = m - J1;
     if (rest > 0) {
       MF_RB(rocblas_dtrsm_64(rb, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit,
@@ -703,8 +729,7 @@ void MultifrontalLU::factor_big(int fi, hipStream_t st, rocblas_handle rb) {
                      d_growth_ + f.piv);
 }
 
-int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st,
-                           rocblas_handle rb) {
+int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st) {
   if (!analysed_) throw std::runtime_error("multifrontal factor before analysis");
   const int nf = (int)sym_.fronts.size();
   MF_HIP(hipMemsetAsync(d_F_, 0, sizeof(double) * std::max<int64_t>(sym_.front_total, 1), st));
@@ -736,7 +761,33 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
     if (L.tile_end > L.tile_beg)
       hipLaunchKernelGGL(mf_schur_kernel, dim3(L.tile_end - L.tile_beg), dim3(256), 0, st, d_fr_,
                          d_tiles_ + L.tile_beg, d_F_);
-    for (int fi : L.big) factor_big(fi, st, rb);
+    if (!L.big.empty()) {
+      // the level's large fronts are independent: spread over the lanes (largest first, each to
+      // the least loaded lane), forked from and joined back into st
+      MF_HIP(hipEventRecord(fork_, st));
+      std::vector<int> order(L.big);
+      auto work = [&](int i) {
+        const double kk = sym_.fronts[i].k, bb = sym_.fronts[i].m - sym_.fronts[i].k;
+        return kk * kk * kk / 1.5 + 2.0 * kk * kk * bb + 2.0 * kk * bb * bb;
+      };
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+      double load[kMfLanes] = {0.0};
+      bool used[kMfLanes] = {false};
+      for (int fi : order) {
+        const int li = (int)(std::min_element(load, load + kMfLanes) - load);
+        if (!used[li]) {
+          MF_HIP(hipStreamWaitEvent(lanes_[li].s, fork_, 0));
+          used[li] = true;
+        }
+        factor_big(fi, lanes_[li]);
+        load[li] += work(fi) + 1e7;  // (a launch-latency floor per front)
+      }
+      for (int li = 0; li < kMfLanes; ++li)
+        if (used[li]) {
+          MF_HIP(hipEventRecord(lanes_[li].done, lanes_[li].s));
+          MF_HIP(hipStreamWaitEvent(st, lanes_[li].done, 0));
+        }
+    }
   }
   pack_tiles(false, st);
   tiles_t_ready_ = false;

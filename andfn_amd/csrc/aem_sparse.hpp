@@ -51,7 +51,7 @@ class MultifrontalLU {
   void analyse(const MfInput& in, hipStream_t st);
   // numeric factorisation of R A (row scales req[N]) from A's CSR values in the analysed pattern;
   // returns 0, or 1 + the first front with an exactly zero pivot
-  int factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st, rocblas_handle rb);
+  int factor(const double* csr_val, const int* csr_ptr, const double* req, hipStream_t st);
   // x = A^-1 b (b and x in the solver's layout; x is written on the analysed unknowns only);
   // f32: through the factors rounded to FP32 (prepare_f32), accumulated in FP64
   void solve(const double* b, const double* req, double* x, hipStream_t st, bool f32 = false);
@@ -111,11 +111,21 @@ class MultifrontalLU {
   double* d_inv_ = nullptr;  // L11^-1, U11^-1 of every front
   double* d_vec_ = nullptr;  // per-front solve vectors
   double* d_ys_ = nullptr;   // transposed solve: unscaled result in the solver's layout
-  // tournament work buffers for the large fronts
-  int* d_cand_[2] = {nullptr, nullptr};
-  int* d_cnt_[2] = {nullptr, nullptr};
-  double* d_lu11_ = nullptr;
-  int* d_pairs_ = nullptr;
+  // The large fronts of a level are factored concurrently on kMfLanes streams of default priority,
+  // each with its own rocBLAS handle (its own workspace) and tournament work buffers.
+  static constexpr int kMfLanes = 4;
+  struct Lane {
+    hipStream_t s = nullptr;
+    rocblas_handle rb = nullptr;
+    hipEvent_t done = nullptr;
+    int* cand[2] = {nullptr, nullptr};
+    int* cnt[2] = {nullptr, nullptr};
+    double* lu11 = nullptr;
+    int* pairs = nullptr;
+  };
+  Lane lanes_[kMfLanes];
+  hipEvent_t fork_ = nullptr;
+  void ensure_lanes();
   int64_t* d_info_ = nullptr;
   unsigned long long* d_growth_ = nullptr;  // per pivot: bits of max |L21 column| (inf: zero pivot)
   double* d_r_ = nullptr;  // refinement: residual, A x0, first solve, correction
@@ -124,7 +134,7 @@ class MultifrontalLU {
   double* d_dx_ = nullptr;
 
   void release();
-  void factor_big(int f, hipStream_t st, rocblas_handle rb);
+  void factor_big(int f, const Lane& lane);
   void launch_seg(int kind, const MfLevel& L, const double* b, const double* req, double* x, hipStream_t st,
                   bool f32 = false);
   void pack_tiles(bool transposed, hipStream_t st);
