@@ -699,8 +699,7 @@ constexpr int kRhsMaxQ = 8;  // discharge points per lane (K <= 512)
 
 __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView tg, int K,
                                                        const int64_t* __restrict__ wptr, const int* __restrict__ mcap,
-                                                       double2* __restrict__ W, int2* __restrict__ widx,
-                                                       int* __restrict__ side_nan) {
+                                                       double2* __restrict__ W, int* __restrict__ side_nan) {
   const int d = blockIdx.x;
   const int lane = threadIdx.x;
   const int e = tg.side_elem[d];
@@ -764,10 +763,7 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);
-      if (lane == 0) {
-        W[o + j] = c2(sg * ar, sg * ai);
-        widx[o + j] = make_int2(src, j);
-      }
+      if (lane == 0) W[o + j] = c2(sg * ar, sg * ai);
     }
     o += m;
   }
@@ -775,26 +771,30 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
   if (lane == 0) side_nan[d] = inside;
 }
 
+// om_d: the entries of side d are the columns of its fracture's table col[coff[d] ...] (source,
+// coefficient index), the same for every side of that fracture, so the table and the gathered
+// coefficients stay in cache while W streams from HBM once per iteration
+template <class WT>
 __device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const int64_t* __restrict__ wptr,
-                                                const double2* __restrict__ W, const int2* __restrict__ widx,
-                                                const int* __restrict__ side_nan) {
-  // four entries per lane in flight (index, count and coefficient loads are a dependent chain)
+                                                const WT* __restrict__ W, const int2* __restrict__ col,
+                                                const int64_t* __restrict__ coff, const int* __restrict__ side_nan) {
+  // four entries per lane in flight (column, count and coefficient loads are a dependent chain)
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int64_t t1 = wptr[d + 1];
-  for (int64_t t0 = wptr[d] + threadIdx.x; t0 < t1; t0 += 256) {
+  const int64_t w0 = wptr[d], t1 = wptr[d + 1], c0 = coff[d] - w0;
+  for (int64_t t0 = w0 + threadIdx.x; t0 < t1; t0 += 256) {
     int2 q[4];
-    double2 w[4];
+    WT w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t t = t0 + 64 * u;
-      q[u] = t < t1 ? widx[t] : make_int2(0, 0x7fffffff);
-      w[u] = t < t1 ? W[t] : c2(0.0, 0.0);
+      q[u] = t < t1 ? col[c0 + t] : make_int2(0, 0x7fffffff);
+      w[u] = t < t1 ? W[t] : WT{0, 0};
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (q[u].y < net.ncoef[q[u].x]) {
         const double2 c = net.coef[(size_t)q[u].x * net.M + q[u].y];
-        acc[u] = fma(c.x, w[u].x, fma(-c.y, w[u].y, acc[u]));
+        acc[u] = fma(c.x, (double)w[u].x, fma(-c.y, (double)w[u].y, acc[u]));
       }
     }
   }
@@ -802,16 +802,17 @@ __device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const
   return side_nan[d] ? __builtin_nan("") : a;
 }
 
+template <class WT>
 __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
-                                                      const int64_t* __restrict__ wptr,
-                                                      const double2* __restrict__ W, const int2* __restrict__ widx,
+                                                      const int64_t* __restrict__ wptr, const WT* __restrict__ W,
+                                                      const int2* __restrict__ col, const int64_t* __restrict__ coff,
                                                       const int* __restrict__ side_nan, const int* __restrict__ de_pos,
                                                       double* __restrict__ b) {
   const int j = blockIdx.x;
   if (j >= nde) return;
   const RhsItem it = items[j];
-  const double om0 = rhs_side_mean(net, it.s0, wptr, W, widx, side_nan);
-  const double om1 = it.s1 >= 0 ? rhs_side_mean(net, it.s1, wptr, W, widx, side_nan) : 0.0;
+  const double om0 = rhs_side_mean(net, it.s0, wptr, W, col, coff, side_nan);
+  const double om1 = it.s1 >= 0 ? rhs_side_mean(net, it.s1, wptr, W, col, coff, side_nan) : 0.0;
   if (threadIdx.x == 0) {
     const int e = it.elem;
     if (net.etype[e] == 0) b[de_pos[e]] = om1 / net.t[net.efrac1[e]] - om0 / net.t[net.efrac0[e]];
@@ -819,17 +820,32 @@ __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem
   }
 }
 
+__global__ void w_to_f32_kernel(int64_t n, const double2* __restrict__ W, float2* __restrict__ o) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = make_float2((float)W[i].x, (float)W[i].y);
+}
+
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int2* widx, int* side_nan, hipStream_t st) {
+                      const int* mcap, double2* W, int* side_nan, hipStream_t st) {
   if (nsides <= 0) return;
-  hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, widx, side_nan);
+  hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, side_nan);
+}
+
+void launch_rhs_to_f32(int64_t n, const double2* W, float2* W32, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(w_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, W, W32);
 }
 
 void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
-                     const int2* widx, const int* side_nan, const int* de_pos, double* b, hipStream_t st) {
+                     const float2* W32, const int2* col, const int64_t* coff, const int* side_nan, const int* de_pos,
+                     double* b, hipStream_t st) {
   if (nde <= 0) return;
-  hipLaunchKernelGGL(rhs_eval_kernel, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, widx, side_nan, de_pos,
-                     b);
+  if (W32)
+    hipLaunchKernelGGL(rhs_eval_kernel<float2>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W32, col, coff,
+                       side_nan, de_pos, b);
+  else
+    hipLaunchKernelGGL(rhs_eval_kernel<double2>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, col, coff,
+                       side_nan, de_pos, b);
 }
 
 // --------------------------------------------------------------------------------------

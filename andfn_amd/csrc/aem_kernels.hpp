@@ -221,9 +221,14 @@ void launch_cross(const NetView& net, const FitView& fv, const int* circles, int
 // discharge side d, with their (source, power) in widx) built once, applied per iteration
 constexpr int kRhsMaxK = 512;
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int2* widx, int* side_nan, hipStream_t st);
+                      const int* mcap, double2* W, int* side_nan, hipStream_t st);
+// the basis rounded to FP32 (the mixed form's FP32 phase)
+void launch_rhs_to_f32(int64_t n, const double2* W, float2* W32, hipStream_t st);
+// b from the basis (W32 non-null: its FP32 copy); col / coff: per fracture the (source, index)
+// of every column and per side the offset of its fracture's columns
 void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
-                     const int2* widx, const int* side_nan, const int* de_pos, double* b, hipStream_t st);
+                     const float2* W32, const int2* col, const int64_t* coff, const int* side_nan, const int* de_pos,
+                     double* b, hipStream_t st);
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
                        const int* de_pos, double* b, hipStream_t st);
 void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,

@@ -376,7 +376,10 @@ class Solver {
   DBuf<RhsItem> d_rhs_items_;
   // right-hand side as a linear map of the coefficients (rhs_basis_kernel), built once per handle
   DBuf<double2> d_rbW_;
-  DBuf<int2> d_rbIdx_;
+  DBuf<float2> d_rbW32_;    // the basis rounded to FP32 (the mixed form's FP32 phase), made on demand
+  bool rb32_ok_ = false;
+  DBuf<int2> d_rbCol_;      // per fracture: (source, coefficient index) of every basis column
+  DBuf<int64_t> d_rbCoff_;  // per side: first column of its fracture's table
   DBuf<int64_t> d_rbPtr_;
   DBuf<int> d_rbNan_, d_rbMcap_;
   std::vector<int> rb_mcap_;
@@ -1706,26 +1709,40 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
       mcap[e] = ncoef_[e] < max_coef ? (int)std::min<int64_t>(L_.M, max_coef + coef_increase - 1) : ncoef_[e];
   if (rb_ok_ && mcap == rb_mcap_) return true;
   const int ns = (int)rhs_.side_elem.size();
-  std::vector<int64_t> ptr(ns + 1, 0);
+  std::vector<int64_t> ptr(ns + 1, 0), coff(std::max(ns, 1), 0);
+  std::vector<int64_t> fcol(F_, -1);  // first column of each fracture's table (-1: not built yet)
+  std::vector<int2> col;
   for (int d = 0; d < ns; ++d) {
     const int e = rhs_.side_elem[d];
     const int f = rhs_.side_which[d] ? efrac1_[e] : efrac0_[e];
+    if (fcol[f] < 0) {
+      // the fracture's columns in the order rhs_basis_kernel writes a side's entries
+      fcol[f] = (int64_t)col.size();
+      for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
+        if (slots_[k].kind != aem::KIND_WELL)
+          for (int j = 0; j < mcap[slots_[k].elem]; ++j) col.push_back(make_int2(slots_[k].elem, j));
+    }
+    coff[d] = fcol[f];
+    const int64_t fend = (int64_t)col.size();
     int64_t cnt = 0;
     for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
       if (slots_[k].kind != aem::KIND_WELL) cnt += mcap[slots_[k].elem];
+    if (fcol[f] + cnt > fend) throw std::logic_error("basis column table shorter than a side's entries");
     ptr[d + 1] = ptr[d] + cnt;
   }
   const int64_t entries = ptr[ns];
-  if (entries * (int64_t)(sizeof(double2) + sizeof(int2)) > (int64_t)16 << 30) {
+  if (entries * (int64_t)sizeof(double2) > (int64_t)16 << 30) {
     rb_ok_ = false;
     return false;
   }
   d_rbPtr_.upload(ptr, st_);
   d_rbMcap_.upload(mcap, st_);
+  d_rbCol_.upload(col, st_);
+  d_rbCoff_.upload(coff, st_);
   d_rbW_.alloc((size_t)std::max<int64_t>(entries, 1));
-  d_rbIdx_.alloc((size_t)std::max<int64_t>(entries, 1));
   d_rbNan_.alloc((size_t)std::max(ns, 1));
-  aem::launch_rhs_basis(net(), rhs_.view(), ns, K_, d_rbPtr_.p, d_rbMcap_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, st_);
+  rb32_ok_ = false;
+  aem::launch_rhs_basis(net(), rhs_.view(), ns, K_, d_rbPtr_.p, d_rbMcap_.p, d_rbW_.p, d_rbNan_.p, st_);
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipStreamSynchronize(st_));
   rb_mcap_ = mcap;
@@ -1843,10 +1860,16 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     rhs_flops = 4.0 * (double)rb_entries_;  // two multiply-adds per basis entry
     rhs_pairs = rb_entries_;
   }
+  if (use_basis && mixed_phase_ && !rb32_ok_) {
+    d_rbW32_.alloc((size_t)std::max<int64_t>(rb_entries_, 1));
+    aem::launch_rhs_to_f32(rb_entries_, d_rbW_.p, d_rbW32_.p, st_);
+    rb32_ok_ = true;
+  }
   auto rhs_head = [&](bool mixed) {
     if (use_basis) {
-      aem::launch_rhs_eval(nv, d_rhs_items_.p, nde, d_rbPtr_.p, d_rbW_.p, d_rbIdx_.p, d_rbNan_.p, d_de_pos_.p, d_b_.p,
-                           st_);
+      // the FP32 phase of the mixed form applies the basis rounded to FP32
+      aem::launch_rhs_eval(nv, d_rhs_items_.p, nde, d_rbPtr_.p, d_rbW_.p, mixed ? d_rbW32_.p : nullptr, d_rbCol_.p,
+                           d_rbCoff_.p, d_rbNan_.p, d_de_pos_.p, d_b_.p, st_);
     } else {
       sweep(aem::MODE_RHS, rhs_, d_valr_.p, mixed);
       aem::launch_rhs_reduce(nv, d_rhs_items_.p, nde, d_valr_.p, rhs_.d_pt_off.p, K_, d_de_pos_.p, d_b_.p, st_);

@@ -15,7 +15,8 @@ records; ``adjoint`` times the implicit adjoint against one forward solve.
 
 Prints one JSON line.  ``roofline`` is the dominant kernel, the collocation sweep (FP64 vector
 ALU bound); ``roofline_solve`` the sparse triangular solves (HBM bound, bytes by the
-12 nnz(L+U) + 16 n convention per pass through the factors).  ``cpu_baseline`` times the CPU
+12 nnz(L+U) + 16 n convention per pass through the factors); both from the FP64 form's run,
+whose kernels are the same throughout.  ``cpu_baseline`` times the CPU
 FP64 oracle (a restatement of AnDFN's Numba path with SciPy's SuperLU) on this host in a child
 process started before any GPU call and run after the GPU timing, on a bounded sample (element
 iterations, assembly, the SuperLU factorisation up to a wall-clock cap); a capped factorisation
@@ -69,8 +70,8 @@ def parse():
 
 
 def traffic_from_profiles(name, workload):
-    """HBM bytes per launch measured with rocprofv3 PMC counters
-    (profiles/<name>_<workload>_traffic.json), if present."""
+    """HBM bytes per launch (a sweep launch; a pass through the factors) measured with rocprofv3
+    PMC counters (profiles/<name>_<workload>_traffic.json), if present."""
     p = ROOT / "profiles" / f"{name}_{workload}_traffic.json"
     if p.exists():
         try:
@@ -251,28 +252,29 @@ def _time_dropin(fa0, ea0, dfn, precision, steps):
     return walls, infos
 
 
-def _roofline_sweep(last):
+def _roofline_sweep(last, fp64=True, workload=None):
     n = max(int(last["n_sweep"]), 1)
     ms = last["k_sweep_ms"] / n
     fl = last["flops_sweep"] / n
     achieved = fl / (ms * 1e-3) if ms > 0 else 0.0
     return {
         "bound": "mfma",
-        "kernel": "sweep_kernel<MODE_COLLOC> collocation sweep (FP64 vector ALU; MI355X FP64 vector peak = FP64 "
-                  "matrix peak; in the mixed form the far-field pairs run in FP32 and the FP64 peak stays the "
-                  "denominator)",
+        "kernel": ("sweep_kernel<MODE_COLLOC, FP64> collocation sweep of the FP64 form" if fp64 else
+                   "sweep_kernel<MODE_COLLOC> collocation sweep of the mixed form (FP32 far field in its FP32 "
+                   "phase)") + " (FP64 vector ALU; MI355X FP64 vector peak = FP64 matrix peak)",
+        "form": "fp64" if fp64 else "mixed",
         "achieved": achieved / 1e12,
         "peak": PEAK_FP64 / 1e12,
         "unit": "TFLOP/s",
         "frac": achieved / PEAK_FP64,
-        "traffic": None,
+        "traffic": traffic_from_profiles("sweep", workload) if (fp64 and workload) else None,
         "avg_launch_ms": ms,
         "flops_per_launch": fl,
         "launches": n,
     }
 
 
-def _roofline_solve(last, f32):
+def _roofline_solve(last, f32, workload=None):
     passes = max(int(last["solve_passes"]), 1)
     ms = last["k_lu_ms"] / passes  # per pass through the factors (correction solves: one each)
     nnz, n = float(last["nnz_factors"]), float(last["n_unknowns"])
@@ -283,11 +285,12 @@ def _roofline_solve(last, f32):
         "bound": "hbm",
         "kernel": "multifrontal triangular solves (mfs_panel_kernel, four launches per level of the elimination tree, "
                   "one HIP graph per solve)",
+        "form": "mixed (FP32 tiles)" if f32 else "fp64",
         "achieved": bw / 1e9,
         "peak": PEAK_HBM / 1e9,
         "unit": "GB/s",
         "frac": bw / PEAK_HBM,
-        "traffic": None,
+        "traffic": traffic_from_profiles("solve", workload) if (not f32 and workload) else None,
         "avg_pass_ms": ms,
         "bytes_per_pass": conv,
         "bytes_streamed_per_pass": streamed,
@@ -355,8 +358,10 @@ def bench_batch(args):
                                f"at constant density, 10-20 coefficients",
                    "fractures": int(len(fa)), "elements": int(len(ea))},
         "forms": res,
-        "roofline": _roofline_sweep(last),
-        "roofline_solve": _roofline_solve(last, args.precision == "mixed" and last["solves_fp32"] > 0),
+        "roofline": _roofline_sweep(lasts["fp64"] if "fp64" in lasts else last, "fp64" in lasts),
+        "roofline_solve": _roofline_solve(lasts["fp64"] if "fp64" in lasts else last,
+                                          False if "fp64" in lasts else (args.precision == "mixed"
+                                                                         and last["solves_fp32"] > 0)),
         "cpu_baseline": None,
     }
     print(json.dumps(out))
@@ -526,8 +531,11 @@ def gpu_part(args):
         "forms": forms,
         "drop_in": drop_in,
         "adjoint": adjoint,
-        "roofline": _roofline_sweep(last),
-        "roofline_solve": _roofline_solve(last, f32),
+        # the FP64 form's kernels (one sweep kernel throughout; the mixed form switches between
+        # its FP32-far-field and FP64 sweeps and between FP32 and FP64 tiles)
+        "roofline": _roofline_sweep(lasts["fp64"] if "fp64" in lasts else last, "fp64" in lasts, args.workload),
+        "roofline_solve": _roofline_solve(lasts["fp64"] if "fp64" in lasts else last,
+                                          False if "fp64" in lasts else f32, args.workload),
         "cpu_baseline": None,
     }
 
