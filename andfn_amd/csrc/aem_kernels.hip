@@ -113,6 +113,7 @@ template <int MODE, bool MIX, int P>
 __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
   const int t = blockIdx.x;
   const Tile tile = tg.tiles[t];
+  if (tile.p0 >= tile.pend) return;  // padding of the XCD layout (uniform per workgroup)
   const int lane = threadIdx.x;
   const int f = tile.frac;
 

@@ -172,8 +172,11 @@ struct TargetSet {
   std::vector<int> dft_sides;
   DBuf<aem::DftTile> d_dft_tiles;
   DBuf<int> d_dft_sides;
+  // launch grids: the device copies of the tile lists are laid out for the XCDs (xcd_order) and
+  // padded with empty tiles, so they are at least as long as the host lists
+  int ndev[2] = {0, 0};
   int64_t npts() const { return pt_off.empty() ? 0 : pt_off.back(); }
-  int ntiles(int ppl) const { return (int)(ppl == 2 ? tiles2.size() : tiles.size()); }
+  int ntiles(int ppl) const { return ppl == 2 ? ndev[1] : ndev[0]; }
   aem::TargetView view(int ppl = 1) const {
     aem::TargetView v;
     v.tiles = ppl == 2 ? d_tiles2.p : d_tiles.p;
@@ -444,6 +447,7 @@ class Solver {
   void restore();
   void build_targets(TargetSet& ts, const std::vector<char>* only, bool rhs);
   void upload_targets(TargetSet& ts);
+  std::vector<Tile> xcd_order(const std::vector<Tile>& tiles, const TargetSet& ts) const;
   void build_items(const TargetSet& ts, std::vector<FitItem>& items, const std::vector<char>* only);
   double set_flops(const TargetSet& ts, bool with_q, int64_t* pairs) const;
   void assemble_and_factor();
@@ -853,12 +857,49 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
   upload_targets(ts);
 }
 
+// Device order of a tile list.  Workgroups go to the 8 XCDs of the MI355X round-robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  Every fracture's tiles are given to
+// one XCD (fractures by decreasing work, each to the least loaded XCD), each XCD's tiles keep the
+// heaviest-first order, and the device list interleaves the 8 lists so that workgroup b takes
+// tile b / 8 of XCD b % 8's list: the tiles that read one fracture's sources share an L2.  Lists
+// are padded with empty tiles (p0 == pend), which every tile kernel skips.
+std::vector<Tile> Solver::xcd_order(const std::vector<Tile>& tiles, const TargetSet& ts) const {
+  constexpr int kXcd = 8;
+  std::vector<double> fw(F_, 0.0);
+  for (const Tile& t : tiles) fw[t.frac] += (double)std::min(t.pend - t.p0, 128) * (mem_ptr_[t.frac + 1] - mem_ptr_[t.frac]);
+  std::vector<int> fr;
+  for (int f = 0; f < F_; ++f)
+    if (fw[f] > 0.0) fr.push_back(f);
+  std::stable_sort(fr.begin(), fr.end(), [&](int a, int b) { return fw[a] > fw[b]; });
+  std::vector<int> xcd(F_, 0);
+  double load[kXcd] = {0.0};
+  for (int f : fr) {
+    const int x = (int)(std::min_element(load, load + kXcd) - load);
+    xcd[f] = x;
+    load[x] += fw[f];
+  }
+  std::vector<std::vector<Tile>> lists(kXcd);
+  for (const Tile& t : tiles) lists[xcd[t.frac]].push_back(t);  // (keeps the heaviest-first order)
+  size_t len = 0;
+  for (const auto& l : lists) len = std::max(len, l.size());
+  std::vector<Tile> out(len * kXcd, Tile{0, 0, 0, 0, 0});
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t i = 0; i < lists[x].size(); ++i) out[i * kXcd + x] = lists[x][i];
+  (void)ts;
+  return out;
+}
+
 void Solver::upload_targets(TargetSet& ts) {
   ts.d_side_elem.upload(ts.side_elem, st_);
   ts.d_side_which.upload(ts.side_which, st_);
   ts.d_pt_off.upload(ts.pt_off, st_);
-  ts.d_tiles.upload(ts.tiles, st_);
-  ts.d_tiles2.upload(ts.tiles2, st_);
+  {
+    const std::vector<Tile> t1 = xcd_order(ts.tiles, ts), t2 = xcd_order(ts.tiles2, ts);
+    ts.d_tiles.upload(t1, st_);
+    ts.d_tiles2.upload(t2, st_);
+    ts.ndev[0] = (int)t1.size();
+    ts.ndev[1] = (int)t2.size();
+  }
   ts.d_dft_tiles.upload(ts.dft_tiles, st_);
   ts.d_dft_sides.upload(ts.dft_sides, st_);
 }
@@ -1668,14 +1709,14 @@ double Solver::bc_residual() {
   DBuf<double> hv, hw;
   hv.alloc(std::max<int64_t>(full_.npts(), 1));
   hw.alloc(std::max<int64_t>(rhs_.npts(), 1));
-  aem::launch_sweep(aem::MODE_COLLOC_HEADS, nv, full_.view(), (int)full_.tiles.size(), K_, hv.p, st_);
+  aem::launch_sweep(aem::MODE_COLLOC_HEADS, nv, full_.view(), full_.ntiles(1), K_, hv.p, st_);
   std::vector<RhsItem> witems;
   for (const RhsItem& it : rhs_items_)
     if (etype_[it.elem] == 2) witems.push_back(it);
   DBuf<RhsItem> dw;
   dw.upload(witems, st_);
   if (!witems.empty())
-    aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, hw.p, st_);
+    aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), rhs_.ntiles(1), K_, hw.p, st_);
   // collocation items of the current counts
   std::vector<FitItem> items;
   build_items(full_, items, nullptr);
@@ -2350,7 +2391,7 @@ void Solver::adj_tables(const TargetSet& ts, bool colloc, AdjTables& t) {
   const int64_t np = ts.npts();
   t.pz.alloc(std::max<int64_t>(np, 1));
   t.pelem.alloc(std::max<int64_t>(np, 1));
-  aem::launch_points(net(), ts.view(), (int)ts.tiles.size(), K_, colloc ? 1 : 0, t.pz.p, t.pelem.p, st_);
+  aem::launch_points(net(), ts.view(), ts.ntiles(1), K_, colloc ? 1 : 0, t.pz.p, t.pelem.p, st_);
   std::vector<int> b(F_), e(F_);
   for (int f = 0; f < F_; ++f) {
     b[f] = ts.pt_off[ts.fsb[f]];
@@ -2790,7 +2831,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   sweep(aem::MODE_COLLOC, full_, d_val_.p, false);
   aem::launch_adj_tgrad(nv, d_titems_.p, (int)titems.size(), d_lam_.p, d_val_.p, full_.d_pt_off.p, d_ts_.p, st_);
   d_hw_.alloc(std::max<int64_t>(rhs_.npts(), 1));
-  aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), (int)rhs_.tiles.size(), K_, d_hw_.p, st_);
+  aem::launch_sweep(aem::MODE_DISCH_HEADS, nv, rhs_.view(), rhs_.ntiles(1), K_, d_hw_.p, st_);
   HIP_CHECK(hipGetLastError());
   std::vector<double> ts(titems.size()), hw(rhs_.npts()), mu(n), qv(E_), cv(F_);
   if (!titems.empty())
