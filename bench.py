@@ -41,6 +41,7 @@ PEAK_HBM = 8.0e12  # MI355X HBM3E, bytes/s
 HEAD_TOL = 1e-6
 ADJOINT_TOL = 1e-10
 MIXED_WORKLOADS = ("cfg3", "cfg5")  # workloads specified in the mixed-precision form
+BATCH_CONSTANTS = dict(MAX_ERROR=1e-9, MAX_NCOEF=10, MAX_ITERATIONS=500)  # the batched realisations
 
 
 def parse():
@@ -92,12 +93,16 @@ def cpu_child(args):
     from andfn_amd.generator import build_network, workload
     from oracle import oracle
 
-    spec = workload(args.workload)
-    d = build_network(spec)
-    d.consolidate_dfn()
-    fa, ea = d.fractures_struc_array_hpc.copy(), d.elements_struc_array_hpc.copy()
-    consts = d.constants
-    O = oracle.OracleHandle(fa, ea, d.discharge_int)
+    if args.workload == "cfg5":
+        from andfn_amd.batch import build_batch, realisation_specs
+
+        _, fa, ea, _, K, consts = build_batch(realisation_specs(args.realisations), BATCH_CONSTANTS)
+    else:
+        d = build_network(workload(args.workload))
+        d.consolidate_dfn()
+        fa, ea, K, consts = d.fractures_struc_array_hpc, d.elements_struc_array_hpc, d.discharge_int, d.constants
+    fa, ea = fa.copy(), ea.copy()
+    O = oracle.OracleHandle(fa, ea, K)
     print("ready", flush=True)
     line = sys.stdin.readline()
     if not line.startswith("go"):
@@ -131,7 +136,7 @@ def cpu_child(args):
 
 def start_cpu_child(args):
     cmd = [sys.executable, "-u", str(ROOT / "bench.py"), "--cpu-child", "--workload", args.workload,
-           "--cpu-iterations", str(args.cpu_iterations)]
+           "--cpu-iterations", str(args.cpu_iterations), "--realisations", str(args.realisations)]
     return subprocess.Popen(cmd, stdin=subprocess.PIPE, stdout=subprocess.PIPE, text=True, bufsize=1, cwd=str(ROOT))
 
 
@@ -313,7 +318,7 @@ def bench_batch(args):
     count = args.realisations
     t0 = time.perf_counter()
     specs = realisation_specs(count)
-    dfns, fa, ea, off, K, consts = build_batch(specs, dict(MAX_ERROR=1e-9, MAX_NCOEF=10, MAX_ITERATIONS=500))
+    dfns, fa, ea, off, K, consts = build_batch(specs, BATCH_CONSTANTS)
     print(f"# batch: {count} realisations, {len(fa)} fractures, {len(ea)} elements, built in "
           f"{time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
     net = DeviceNetwork(fa, ea, K, consts)
@@ -328,7 +333,7 @@ def bench_batch(args):
         infos, walls = _time_form(net, prec, 0, args.warmup, args.steps, prec)
         lasts[prec] = infos[-1]
         aw = []
-        for k in range(1 + args.steps):
+        for k in range(1 + min(args.steps, 3)):
             t1 = time.perf_counter()
             gt, gh, ai = net.adjoint(dq=dq, tol=ADJOINT_TOL)
             aw.append(time.perf_counter() - t1)
@@ -336,7 +341,7 @@ def bench_batch(args):
         res[prec] = {"ms_per_batch": 1e3 * sum(walls) / args.steps, "nit": int(infos[-1]["nit"]),
                      "head_residual": infos[-1]["head_residual"], "solves_fp32": int(infos[-1]["solves_fp32"]),
                      "nit_fp32": int(infos[-1]["nit_fp32"]), "t_factor_s": infos[-1]["t_factor"],
-                     "adjoint_ms": 1e3 * sum(aw[1:]) / args.steps, "adjoint_iterations": int(ai["nit"])}
+                     "adjoint_ms": 1e3 * sum(aw[1:]) / (len(aw) - 1), "adjoint_iterations": int(ai["nit"])}
         res[prec]["adjoint_forward_ratio"] = res[prec]["adjoint_ms"] / res[prec]["ms_per_batch"]
     net.close()
     main_r = res[args.precision]
@@ -358,13 +363,14 @@ def bench_batch(args):
                                f"at constant density, 10-20 coefficients",
                    "fractures": int(len(fa)), "elements": int(len(ea))},
         "forms": res,
-        "roofline": _roofline_sweep(lasts["fp64"] if "fp64" in lasts else last, "fp64" in lasts),
+        "nit": int(last["nit"]),
+        "roofline": _roofline_sweep(lasts["fp64"] if "fp64" in lasts else last, "fp64" in lasts, "cfg5"),
         "roofline_solve": _roofline_solve(lasts["fp64"] if "fp64" in lasts else last,
                                           False if "fp64" in lasts else (args.precision == "mixed"
-                                                                         and last["solves_fp32"] > 0)),
+                                                                         and last["solves_fp32"] > 0), "cfg5"),
         "cpu_baseline": None,
     }
-    print(json.dumps(out))
+    return out
 
 
 def main():
@@ -376,17 +382,17 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU runs are not part of this build",
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == "cfg5":
-        bench_batch(args)
-        return
     # the CPU baseline's child starts before anything touches the GPU and waits for the GPU timing
     child = None if args.no_cpu_baseline else start_cpu_child(args)
     try:
-        out = gpu_part(args)
+        out = bench_batch(args) if args.workload == "cfg5" else gpu_part(args)
         if child is not None:
             out["cpu_baseline"] = finish_cpu_child(child, args, int(out["nit"]))
             child = None
             cb = out["cpu_baseline"]
+            if cb and args.workload == "cfg5":  # one solve of the batch network = all realisations
+                cb["value"] *= args.realisations
+                cb["unit"] = "realisations/s"
             if cb:
                 out["speedup_vs_cpu"] = {"value": out["value"] / cb["value"],
                                          "bound": "lower" if cb["bound"].startswith("upper") else "measured",
