@@ -118,6 +118,7 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
   const int f = tile.frac;
 
   constexpr bool colloc_pts = (MODE == MODE_COLLOC || MODE == MODE_COLLOC_HEADS);
+  constexpr bool explicit_pts = (MODE == MODE_POINTS || MODE == MODE_POINTS_OMEGA);
   constexpr bool use_q = (MODE != MODE_RHS);
   constexpr bool heads_out = (MODE == MODE_POINTS || MODE == MODE_COLLOC_HEADS || MODE == MODE_DISCH_HEADS);
   double2 z[P];
@@ -132,8 +133,9 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
     excl[k] = -1;
     want_imag[k] = 0;
     if (valid[k]) {
-      if (MODE == MODE_POINTS) {
+      if (explicit_pts) {
         z[k] = tg.pt_z[p];
+        if (MODE == MODE_POINTS_OMEGA) want_imag[k] = 1;  // the stream function too
       } else {
         const int sd = find_side(tg.pt_off, tile.sbeg, tile.send, p);
         const int e = tg.side_elem[sd];
@@ -295,9 +297,12 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
     }
     if (valid[k]) {
       const int p = tile.p0 + lane + 64 * k;
-      if (MODE == MODE_COLLOC)
+      if (MODE == MODE_COLLOC) {
         val[p] = want_imag[k] ? oim[k] : ore[k] + net.constant[f];
-      else if (heads_out)
+      } else if (MODE == MODE_POINTS_OMEGA) {
+        val[2 * (int64_t)p] = ore[k] + net.constant[f];
+        val[2 * (int64_t)p + 1] = oim[k];
+      } else if (heads_out)
         val[p] = (ore[k] + net.constant[f]) / net.t[f];
       else
         val[p] = ore[k];
@@ -1114,6 +1119,8 @@ static void launch_sweep_p(int mode, const NetView& net, const TargetView& tg, i
     hipLaunchKernelGGL((sweep_kernel<MODE_COLLOC_HEADS, false, P>), g, b, 0, st, net, tg, K, val);
   else if (mode == MODE_DISCH_HEADS)
     hipLaunchKernelGGL((sweep_kernel<MODE_DISCH_HEADS, false, P>), g, b, 0, st, net, tg, K, val);
+  else if (mode == MODE_POINTS_OMEGA)
+    hipLaunchKernelGGL((sweep_kernel<MODE_POINTS_OMEGA, false, P>), g, b, 0, st, net, tg, K, val);
   else
     hipLaunchKernelGGL((sweep_kernel<MODE_POINTS, false, P>), g, b, 0, st, net, tg, K, val);
 }

@@ -14,7 +14,8 @@ enum {
   MODE_RHS = 1,           // discharge points, series only (q = C = 0), Re
   MODE_POINTS = 2,        // explicit points, full potential, head = Re Omega / t
   MODE_COLLOC_HEADS = 3,  // collocation points, full potential, head
-  MODE_DISCH_HEADS = 4    // discharge points, full potential, head
+  MODE_DISCH_HEADS = 4,   // discharge points, full potential, head
+  MODE_POINTS_OMEGA = 5   // explicit points, full complex potential Omega (C_f included), two values per point
 };
 enum { KIND_LINE = 0, KIND_BOUND = 1, KIND_WELL = 2, KIND_IMPC = 3, KIND_IMPL = 4 };
 

@@ -230,7 +230,7 @@ class Solver {
 
   int solve(const andfn_mi_options* opt, andfn_mi_info* info);
   void write_back();
-  void heads(int64_t npts, const int64_t* frac, const double* z, double* out);
+  void heads(int64_t npts, const int64_t* frac, const double* z, double* out, bool omega = false);
   // heads() with every device buffer in the caller-provided scratch region (no allocation)
   void heads_noalloc(int64_t npts, const int64_t* frac, const double* z, double* out, unsigned char* scratch_dev);
   int device() const { return dev_; }
@@ -2293,8 +2293,9 @@ void Solver::write_back() {
   for (int f = 0; f < F_; ++f) wr<double>(fr_ + (size_t)f * L_.fsz, L_.f_const, constant_[f]);
 }
 
-void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* out) {
-  // group the points by fracture into tiles, evaluate Re Omega / t (hpc_fracture.py:204-250)
+void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* out, bool omega) {
+  // group the points by fracture into tiles, evaluate Re Omega / t (hpc_fracture.py:204-250), or
+  // the complex potential Omega itself (omega: two values per point, calc_omega hpc_fracture.py:51-92)
   std::vector<int64_t> order(npts);
   for (int64_t i = 0; i < npts; ++i) {
     if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "point %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
@@ -2317,16 +2318,18 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
   DBuf<double> dv;
   dz.upload(zs, st_);
   dt.upload(tiles, st_);
-  dv.alloc(std::max<int64_t>(npts, 1));
+  const int w = omega ? 2 : 1;
+  dv.alloc(std::max<int64_t>(w * npts, 1));
   aem::TargetView tv{};
   tv.tiles = dt.p;
   tv.pt_z = dz.p;
-  aem::launch_sweep(aem::MODE_POINTS, net(), tv, (int)tiles.size(), 1, dv.p, st_);
+  aem::launch_sweep(omega ? aem::MODE_POINTS_OMEGA : aem::MODE_POINTS, net(), tv, (int)tiles.size(), 1, dv.p, st_);
   HIP_CHECK(hipGetLastError());
-  std::vector<double> v(npts);
-  HIP_CHECK(hipMemcpyAsync(v.data(), dv.p, sizeof(double) * npts, hipMemcpyDeviceToHost, st_));
+  std::vector<double> v(w * npts);
+  HIP_CHECK(hipMemcpyAsync(v.data(), dv.p, sizeof(double) * w * npts, hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
-  for (int64_t k = 0; k < npts; ++k) out[order[k]] = v[k];
+  for (int64_t k = 0; k < npts; ++k)
+    for (int c = 0; c < w; ++c) out[w * order[k] + c] = v[w * k + c];
 }
 
 // ---- heads at points without allocation (andfn_mi_state_heads_noalloc) ----------------------
@@ -3023,6 +3026,26 @@ int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_eleme
     if (!fractures || !elements || (n_points > 0 && (!frac || !z || !heads))) fail(ANDFN_MI_EINVAL, "null pointer");
     Solver s((void*)fractures, n_fractures, max_elements, (void*)elements, n_elements, max_ncoef, 1, nullptr, device);
     if (n_points > 0) s.heads(n_points, frac, z, heads);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_omega(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
+                   int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
+                   double* omega, int32_t device) {
+  return guarded([&]() {
+    if (!fractures || !elements || (n_points > 0 && (!frac || !z || !omega))) fail(ANDFN_MI_EINVAL, "null pointer");
+    Solver s((void*)fractures, n_fractures, max_elements, (void*)elements, n_elements, max_ncoef, 1, nullptr, device);
+    if (n_points > 0) s.heads(n_points, frac, z, omega, true);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_omega(andfn_mi_handle* h, int64_t n_points, const int64_t* frac, const double* z, double* omega) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (n_points > 0 && (!frac || !z || !omega)) fail(ANDFN_MI_EINVAL, "null pointer");
+    if (n_points > 0) h->s->heads(n_points, frac, z, omega, true);
     return ANDFN_MI_OK;
   });
 }

@@ -130,6 +130,21 @@ def get_heads(fracture_struc_array, element_struc_array, frac, z, device=0):
     return out
 
 
+def get_omega(fracture_struc_array, element_struc_array, frac, z, device=0):
+    """Complex potential Omega_f(z) = C_f + the fracture's elements at points z (local complex
+    coordinates) on fractures ``frac`` (``andfn/hpc/hpc_fracture.py:51-92``, calc_omega)."""
+    _lib.require_records(fracture_struc_array, element_struc_array)
+    lib = _lib.load()
+    me, m = _lib.capacities(fracture_struc_array, element_struc_array)
+    frac = np.ascontiguousarray(frac, dtype=np.int64)
+    zz = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros(len(zz), dtype=np.complex128)
+    _lib.check(lib.andfn_mi_omega(_lib.ptr(fracture_struc_array), len(fracture_struc_array), me,
+                                  _lib.ptr(element_struc_array), len(element_struc_array), m, len(zz), _lib.ptr(frac),
+                                  _lib.ptr(zz), _lib.ptr(out), int(device)))
+    return out
+
+
 def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0, factor="sparse"):
     if factor not in _lib.FACTORS:
         raise ValueError(f"factor must be one of {sorted(_lib.FACTORS)}")
@@ -195,6 +210,14 @@ class DeviceNetwork:
         zz = np.ascontiguousarray(z, dtype=np.complex128)
         out = np.zeros(len(zz))
         _lib.check(self.lib.andfn_mi_state_heads(self.h, len(zz), _lib.ptr(frac), _lib.ptr(zz), _lib.ptr(out)))
+        return out
+
+    def state_omega(self, frac, z):
+        """Complex potential at points (fracture index, local complex coordinate) of the current state."""
+        frac = np.ascontiguousarray(frac, dtype=np.int64)
+        zz = np.ascontiguousarray(z, dtype=np.complex128)
+        out = np.zeros(len(zz), dtype=np.complex128)
+        _lib.check(self.lib.andfn_mi_state_omega(self.h, len(zz), _lib.ptr(frac), _lib.ptr(zz), _lib.ptr(out)))
         return out
 
     def adjoint(self, dq=None, dconstant=None, obs_frac=None, obs_z=None, dhead=None, tol=1e-10,

@@ -175,6 +175,17 @@ int andfn_mi_heads(const void* fractures, int64_t n_fractures, int64_t max_eleme
                    double* heads, int32_t device);
 
 /*
+ * Complex potential at points of the solved network: Omega_f(z) = C_f + the sum of every element of
+ * fracture f (andfn/hpc/hpc_fracture.py:51-92 calc_omega; the flow nets of calc_flow_net 178-201 and
+ * get_flow_nets 254-299 evaluate it on a sunflower pattern per fracture).  z and omega are
+ * interleaved complex (re, im); frac gives the fracture index of every point.  Re Omega is the
+ * discharge potential Phi (head times t), Im Omega the stream function Psi.
+ */
+int andfn_mi_omega(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
+                   int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
+                   double* omega, int32_t device);
+
+/*
  * Implicit adjoint of the converged fixed point.  No counterpart in the reference (AnDFN has no
  * gradients); it is the gradient companion of andfn_mi_run.  For the objective
  *
@@ -240,6 +251,10 @@ int andfn_mi_set_parameters(andfn_mi_handle* handle, const double* t, const doub
 /* Heads at points of the handle's current state (after a run), as andfn_mi_heads. */
 int andfn_mi_state_heads(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
                          double* heads);
+
+/* Complex potential at points of the handle's current state (after a run), as andfn_mi_omega. */
+int andfn_mi_state_omega(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
+                         double* omega);
 
 /* andfn_mi_state_heads without allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses

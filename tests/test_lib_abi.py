@@ -55,5 +55,6 @@ def test_no_device_or_bad_input_is_an_error_not_a_crash():
     assert rc == _lib.EINVAL
     for rc in (lib.andfn_mi_adjoint(None, None, None, None, None, None), lib.andfn_mi_set_parameters(None, None, None),
                lib.andfn_mi_state_heads(None, 0, None, None, None), lib.andfn_mi_state_discharges(None, None, None),
-               lib.andfn_mi_state_heads_noalloc(None, 0, None, None, None)):
+               lib.andfn_mi_state_heads_noalloc(None, 0, None, None, None),
+               lib.andfn_mi_state_omega(None, 0, None, None, None)):
         assert rc == _lib.EINVAL
