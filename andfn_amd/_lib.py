@@ -31,6 +31,9 @@ LU_INVERSE = 2
 
 FACTORS = {"sparse": 0, "rocsolver": 1, "tournament": 2}
 
+W_JIT = 0  # hpc_fracture.calc_w
+W_OBJECT = 1  # Fracture.calc_w (streamline tracking)
+
 
 class Options(ctypes.Structure):
     _fields_ = [
@@ -145,6 +148,9 @@ SIGNATURES = {
     "andfn_mi_omega": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
                                       ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_void_p, ctypes.c_int32]),
+    "andfn_mi_w": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                  ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]),
     "andfn_mi_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p,
                                        ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int32,
                                        ctypes.POINTER(ctypes.c_void_p)]),
@@ -158,6 +164,12 @@ SIGNATURES = {
                                             ctypes.c_void_p]),
     "andfn_mi_state_omega": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                             ctypes.c_void_p]),
+    "andfn_mi_state_w": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int64, ctypes.c_int32, ctypes.c_void_p]),
+    "andfn_mi_state_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                            ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
     "andfn_mi_state_heads_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]),
     "andfn_mi_state_discharges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),

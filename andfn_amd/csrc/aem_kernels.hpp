@@ -108,6 +108,24 @@ struct TargetView {
   const double2* pt_z;     // explicit points (MODE_POINTS) or null
 };
 
+// one launch of the streamline tracer: n lines, each in its own fracture
+struct TraceArgs {
+  int n;
+  const int* frac;           // [n] fracture of each line
+  const double2* z0;         // [n] point the tracking continues from
+  const double2* z_start;    // [n] first point recorded for the segment
+  const int64_t* remaining;  // [n] point budget: the segment ends once it holds more than this + 1
+  const double* fradius;     // [F] fracture radii
+  const double* aperture;    // [F] apertures (speed = |w| / aperture)
+  double ds;                 // step as a fraction of the fracture radius
+  int backward;              // 1: against the flow
+  int64_t capacity;          // entries per line in pts / vel (>= 3)
+  double2* pts;              // [n][capacity]
+  double* vel;               // [n][capacity]
+  int64_t* npts;             // [n] points of the segment, -1 when the first step is NaN (no segment)
+  int* exit_elem;            // [n] discharge element the last step crossed, -1 if none
+};
+
 struct FitItem {
   int elem;
   int s0, s1;  // side indices in the target set (s1 = -1 unless intersection)
@@ -147,6 +165,11 @@ struct IterScalars {
 // points_per_lane 1 or 2: the target view's tiles must hold 64 x points_per_lane points each
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
                   bool mixed = false, int points_per_lane = 1);
+// w = -dOmega/dz at the explicit points of a target view (two values per point); jit != 0 gives
+// the jitted form (hpc_fracture.calc_w), 0 the object model's (Fracture.calc_w)
+void launch_w_points(const NetView& net, const TargetView& tg, int ntiles, int exclude, int jit, double* val,
+                     hipStream_t st);
+void launch_trace(const NetView& net, const TraceArgs& ta, hipStream_t st);
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 // unwrap (circle-type sides, in place in val), DFT sums of every side into dft (side s: X_j at
 // s 2M + j, Y_j at s 2M + M + j), then the element solves

@@ -230,7 +230,12 @@ class Solver {
 
   int solve(const andfn_mi_options* opt, andfn_mi_info* info);
   void write_back();
-  void heads(int64_t npts, const int64_t* frac, const double* z, double* out, bool omega = false);
+  // values at points grouped by fracture: what = 0 heads, 1 Omega (two values per point), 2 w in
+  // the jitted form, 3 w in the object model's form (two values per point)
+  void heads(int64_t npts, const int64_t* frac, const double* z, double* out, int what = 0, int64_t exclude = -1);
+  void trace(int64_t n, const int64_t* frac, const double* z0, const double* zs, const int64_t* remaining,
+             const double* aperture, double ds, int backward, int64_t capacity, double* pts, double* vel,
+             int64_t* npts, int64_t* exit_elem);
   // heads() with every device buffer in the caller-provided scratch region (no allocation)
   void heads_noalloc(int64_t npts, const int64_t* frac, const double* z, double* out, unsigned char* scratch_dev);
   int device() const { return dev_; }
@@ -357,6 +362,7 @@ class Solver {
   int mf_rounds_ = 0;                    // factorisations of the last run (1 + analyses redone)
   void mf_analyse_now();
   std::vector<double> fcenter_;          // [F][3] fracture centres (fracture record offset 24)
+  std::vector<double> fradius_;          // [F] fracture radii (fracture record offset 16)
   std::vector<int> h_csr_ptr_, h_csr_col_;  // host copy of the CSR pattern
   void ensure_csr_pattern();
   // blocked LU with tournament pivoting (one connected part): candidate sets, the panel's
@@ -567,6 +573,7 @@ void Solver::parse() {
   constant_.resize(F_);
   frac_id_.resize(F_);
   fcenter_.resize(3 * (size_t)F_);
+  fradius_.resize(F_);
   mem_ptr_.assign(F_ + 1, 0);
   slots_.clear();
   for (int f = 0; f < F_; ++f) {
@@ -574,6 +581,7 @@ void Solver::parse() {
     frac_id_[f] = rd<int64_t>(p, 0);
     t_[f] = rd<double>(p, 8);
     for (int a = 0; a < 3; ++a) fcenter_[3 * (size_t)f + a] = rd<double>(p, 24 + 8 * a);
+    fradius_[f] = rd<double>(p, 16);
     constant_[f] = rd<double>(p, L_.f_const);
     int64_t nel = rd<int64_t>(p, L_.f_nel);
     if (nel < 0 || nel > L_.ME) fail(ANDFN_MI_ECAPACITY, "fracture %d: %lld elements exceed capacity %lld", f, (long long)nel, (long long)L_.ME);
@@ -2293,9 +2301,10 @@ void Solver::write_back() {
   for (int f = 0; f < F_; ++f) wr<double>(fr_ + (size_t)f * L_.fsz, L_.f_const, constant_[f]);
 }
 
-void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* out, bool omega) {
-  // group the points by fracture into tiles, evaluate Re Omega / t (hpc_fracture.py:204-250), or
-  // the complex potential Omega itself (omega: two values per point, calc_omega hpc_fracture.py:51-92)
+void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* out, int what, int64_t exclude) {
+  // group the points by fracture into tiles, evaluate Re Omega / t (hpc_fracture.py:204-250), the
+  // complex potential Omega itself (calc_omega hpc_fracture.py:51-92) or the discharge vector w
+  // (hpc_fracture.py:139-174 / fracture.py:453-477)
   std::vector<int64_t> order(npts);
   for (int64_t i = 0; i < npts; ++i) {
     if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "point %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
@@ -2318,18 +2327,106 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
   DBuf<double> dv;
   dz.upload(zs, st_);
   dt.upload(tiles, st_);
-  const int w = omega ? 2 : 1;
+  const int w = what == 0 ? 1 : 2;
   dv.alloc(std::max<int64_t>(w * npts, 1));
   aem::TargetView tv{};
   tv.tiles = dt.p;
   tv.pt_z = dz.p;
-  aem::launch_sweep(omega ? aem::MODE_POINTS_OMEGA : aem::MODE_POINTS, net(), tv, (int)tiles.size(), 1, dv.p, st_);
+  if (what >= 2) {
+    const int ex = (exclude >= 0 && exclude < E_) ? (int)exclude : -1;
+    aem::launch_w_points(net(), tv, (int)tiles.size(), ex, what == 2 ? 1 : 0, dv.p, st_);
+  } else {
+    aem::launch_sweep(what == 1 ? aem::MODE_POINTS_OMEGA : aem::MODE_POINTS, net(), tv, (int)tiles.size(), 1, dv.p,
+                      st_);
+  }
   HIP_CHECK(hipGetLastError());
   std::vector<double> v(w * npts);
   HIP_CHECK(hipMemcpyAsync(v.data(), dv.p, sizeof(double) * w * npts, hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
   for (int64_t k = 0; k < npts; ++k)
     for (int c = 0; c < w; ++c) out[w * order[k] + c] = v[w * k + c];
+}
+
+void Solver::trace(int64_t n, const int64_t* frac, const double* z0, const double* zs, const int64_t* remaining,
+                   const double* aperture, double ds, int backward, int64_t capacity, double* pts, double* vel,
+                   int64_t* npts, int64_t* exit_elem) {
+  // one segment of every line (the inner loop of DFN.streamline_tracking, dfn.py:2663-2705) in its
+  // current fracture; lines are launched grouped by fracture so a wavefront mostly walks one
+  // fracture's elements
+  if (n > (int64_t)INT32_MAX / 2) fail(ANDFN_MI_EINVAL, "%lld lines exceed the limit of one launch", (long long)n);
+  if (capacity < 3) fail(ANDFN_MI_EINVAL, "capacity %lld: a segment holds at least 3 points", (long long)capacity);
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "line %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
+    if (std::max<int64_t>(remaining[i], 1) > capacity - 2)
+      fail(ANDFN_MI_EINVAL, "line %lld: point budget %lld needs a capacity of %lld (have %lld)", (long long)i,
+           (long long)remaining[i], (long long)std::max<int64_t>(remaining[i] + 2, 3), (long long)capacity);
+    order[i] = i;
+  }
+  for (int f = 0; f < F_; ++f)
+    if (!(aperture[f] > 0.0) || !std::isfinite(aperture[f]))
+      fail(ANDFN_MI_EINVAL, "fracture %d: aperture %g is not a positive number", f, aperture[f]);
+  if (n == 0) return;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return frac[a] < frac[b]; });
+  std::vector<int> hf(n);
+  std::vector<double2> hz0(n), hzs(n);
+  std::vector<int64_t> hrem(n);
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = order[k];
+    hf[k] = (int)frac[i];
+    hz0[k] = make_double2(z0[2 * i], z0[2 * i + 1]);
+    hzs[k] = make_double2(zs[2 * i], zs[2 * i + 1]);
+    hrem[k] = remaining[i];
+  }
+  DBuf<int> d_f, d_exit;
+  DBuf<double2> d_z0, d_zs, d_pts;
+  DBuf<int64_t> d_rem, d_np;
+  DBuf<double> d_rad, d_ap, d_vel;
+  d_f.upload(hf, st_);
+  d_z0.upload(hz0, st_);
+  d_zs.upload(hzs, st_);
+  d_rem.upload(hrem, st_);
+  d_rad.upload(fradius_, st_);
+  d_ap.upload(std::vector<double>(aperture, aperture + F_), st_);
+  d_pts.alloc((size_t)n * capacity);
+  d_vel.alloc((size_t)n * capacity);
+  d_np.alloc(n);
+  d_exit.alloc(n);
+  aem::TraceArgs ta{};
+  ta.n = (int)n;
+  ta.frac = d_f.p;
+  ta.z0 = d_z0.p;
+  ta.z_start = d_zs.p;
+  ta.remaining = d_rem.p;
+  ta.fradius = d_rad.p;
+  ta.aperture = d_ap.p;
+  ta.ds = ds;
+  ta.backward = backward ? 1 : 0;
+  ta.capacity = capacity;
+  ta.pts = d_pts.p;
+  ta.vel = d_vel.p;
+  ta.npts = d_np.p;
+  ta.exit_elem = d_exit.p;
+  aem::launch_trace(net(), ta, st_);
+  HIP_CHECK(hipGetLastError());
+  std::vector<int64_t> hn(n);
+  std::vector<int> he(n);
+  HIP_CHECK(hipMemcpyAsync(hn.data(), d_np.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipMemcpyAsync(he.data(), d_exit.p, sizeof(int) * n, hipMemcpyDeviceToHost, st_));
+  HIP_CHECK(hipStreamSynchronize(st_));
+  // rows back in the caller's order, only the points each segment holds
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = order[k];
+    npts[i] = hn[k];
+    exit_elem[i] = he[k];
+    if (hn[k] > 0) {
+      HIP_CHECK(hipMemcpyAsync(pts + 2 * i * capacity, d_pts.p + k * capacity, sizeof(double2) * hn[k],
+                               hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipMemcpyAsync(vel + i * capacity, d_vel.p + k * capacity, sizeof(double) * hn[k],
+                               hipMemcpyDeviceToHost, st_));
+    }
+  }
+  HIP_CHECK(hipStreamSynchronize(st_));
 }
 
 // ---- heads at points without allocation (andfn_mi_state_heads_noalloc) ----------------------
@@ -3036,7 +3133,7 @@ int andfn_mi_omega(const void* fractures, int64_t n_fractures, int64_t max_eleme
   return guarded([&]() {
     if (!fractures || !elements || (n_points > 0 && (!frac || !z || !omega))) fail(ANDFN_MI_EINVAL, "null pointer");
     Solver s((void*)fractures, n_fractures, max_elements, (void*)elements, n_elements, max_ncoef, 1, nullptr, device);
-    if (n_points > 0) s.heads(n_points, frac, z, omega, true);
+    if (n_points > 0) s.heads(n_points, frac, z, omega, 1);
     return ANDFN_MI_OK;
   });
 }
@@ -3045,7 +3142,46 @@ int andfn_mi_state_omega(andfn_mi_handle* h, int64_t n_points, const int64_t* fr
   return guarded([&]() {
     if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
     if (n_points > 0 && (!frac || !z || !omega)) fail(ANDFN_MI_EINVAL, "null pointer");
-    if (n_points > 0) h->s->heads(n_points, frac, z, omega, true);
+    if (n_points > 0) h->s->heads(n_points, frac, z, omega, 1);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_w(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
+               int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
+               int64_t exclude, int32_t form, double* w, int32_t device) {
+  return guarded([&]() {
+    if (!fractures || !elements || (n_points > 0 && (!frac || !z || !w))) fail(ANDFN_MI_EINVAL, "null pointer");
+    if (form != ANDFN_MI_W_JIT && form != ANDFN_MI_W_OBJECT) fail(ANDFN_MI_EINVAL, "unknown form %d", (int)form);
+    Solver s((void*)fractures, n_fractures, max_elements, (void*)elements, n_elements, max_ncoef, 1, nullptr, device);
+    if (n_points > 0) s.heads(n_points, frac, z, w, form == ANDFN_MI_W_JIT ? 2 : 3, exclude);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_w(andfn_mi_handle* h, int64_t n_points, const int64_t* frac, const double* z, int64_t exclude,
+                     int32_t form, double* w) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (n_points > 0 && (!frac || !z || !w)) fail(ANDFN_MI_EINVAL, "null pointer");
+    if (form != ANDFN_MI_W_JIT && form != ANDFN_MI_W_OBJECT) fail(ANDFN_MI_EINVAL, "unknown form %d", (int)form);
+    if (n_points > 0) h->s->heads(n_points, frac, z, w, form == ANDFN_MI_W_JIT ? 2 : 3, exclude);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_trace(andfn_mi_handle* h, int64_t n_lines, const int64_t* frac, const double* z0,
+                         const double* z_start, const int64_t* remaining, const double* aperture, double ds,
+                         int32_t backward, int64_t capacity, double* points, double* speed, int64_t* n_points,
+                         int64_t* exit_element) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (!aperture) fail(ANDFN_MI_EINVAL, "null pointer");
+    if (n_lines > 0 && (!frac || !z0 || !z_start || !remaining || !points || !speed || !n_points || !exit_element))
+      fail(ANDFN_MI_EINVAL, "null pointer");
+    if (n_lines < 0) fail(ANDFN_MI_EINVAL, "negative line count");
+    h->s->trace(n_lines, frac, z0, z_start, remaining, aperture, ds, backward, capacity, points, speed, n_points,
+                exit_element);
     return ANDFN_MI_OK;
   });
 }

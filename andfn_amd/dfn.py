@@ -180,6 +180,8 @@ class DFN:
         self.fractures_struc_array_hpc = None
         self.elements_struc_array = None
         self.last_solve_info = None
+        self._flow_net = None
+        self._flow_key = None
         self.set_kwargs(**kwargs)
 
     # -- configuration ---------------------------------------------------------------------
@@ -399,6 +401,7 @@ class DFN:
             fa[i]["constant"] = f.constant
         self.elements_struc_array_hpc = ea
         self.fractures_struc_array_hpc = fa
+        self._flow_net = None
 
     def unconsolidate_dfn(self, hpc=True):
         ea = self.elements_struc_array_hpc
@@ -431,6 +434,7 @@ class DFN:
         from .hpc_solve import last_solve_info
 
         self.last_solve_info = last_solve_info()
+        self._flow_net = None
         if unconsolidate:
             self.unconsolidate_dfn(hpc=True)
 
@@ -439,3 +443,74 @@ class DFN:
         ea = self.elements_struc_array_hpc
         sel = (ea["_type"] == WELL) | (ea["_type"] == CONST_HEAD_LINE)
         return float(np.sum(np.abs(ea["q"][sel])) / 2)
+
+    # -- flow on the solved network ----------------------------------------------------------
+    def _device_network(self):
+        """The solved records resident on the GPU (created on first use after a solve)."""
+        from .hpc_solve import DeviceNetwork
+
+        fa, ea = self.fractures_struc_array_hpc, self.elements_struc_array_hpc
+        if fa is None or ea is None:
+            raise RuntimeError("no solved network: call solve() first")
+        if self._flow_net is None:
+            self._flow_net = DeviceNetwork(fa, ea, self.discharge_int, self.constants)
+        return self._flow_net
+
+    def _frac_index(self, frac):
+        return frac._id if isinstance(frac, Fracture) else int(frac)
+
+    def calc_w(self, z, frac, exclude=None):
+        """Discharge vector w at points z of fracture ``frac`` (``Fracture.calc_w``,
+        ``andfn/fracture.py:453-477``), every element but ``exclude``."""
+        zz = np.atleast_1d(np.asarray(z, dtype=np.complex128))
+        f = np.full(len(zz), self._frac_index(frac), dtype=np.int64)
+        ex = -1 if exclude is None else (exclude._id if isinstance(exclude, Element) else int(exclude))
+        w = self._device_network().state_w(f, zz, exclude=ex, form="object")
+        return w[0] if np.isscalar(z) else w
+
+    def _apertures(self):
+        ap = [f.aperture for f in self.fractures]
+        if any(a is None for a in ap):
+            raise ValueError("streamline speeds need an aperture on every fracture (|w| / aperture)")
+        return np.array(ap, dtype=np.float64)
+
+    def streamline_tracking(self, z0, frac, elevation, ds, max_length, backward):
+        """Track one streamline from z0 on ``frac`` (``andfn/dfn.py:2638-2725``).
+
+        Returns (streamline, streamline_frac, velocity, element) as the reference does: lists of
+        per-fracture point and speed lists, the fractures visited, and the element the line left
+        through (``False`` if none)."""
+        s, f, v, e = self.track_streamlines(np.array([z0], dtype=np.complex128), frac, ds=ds, max_length=max_length,
+                                            elevation=np.array([float(elevation)]), backward=backward)
+        return s[0], f[0], v[0], e[0]
+
+    def track_streamlines(self, z0, frac, ds=1e-2, max_length=1000, elevation=0.0, remove_false=True, backward=False):
+        """Every streamline of ``plot_streamline_tracking`` (``andfn/dfn.py:2558-2636``: each start
+        point with each elevation, start-point major), traced together on the GPU; returns
+        (streamlines, streamlines_frac, velocities, elements) like that function, without the plot.
+        ``remove_false`` only concerns the plot there and is accepted for the same signature."""
+        from . import streamlines as sl
+
+        z0 = np.atleast_1d(np.asarray(z0, dtype=np.complex128))
+        elev = np.atleast_1d(np.asarray(elevation, dtype=np.float64))
+        f = self._frac_index(frac)
+        zz = np.repeat(z0, len(elev))
+        ee = np.tile(elev, len(z0))
+        if self.elements is None:
+            self.get_elements()
+        segs, segf, segv, el = sl.track(self._device_network(), self.fractures_struc_array_hpc,
+                                        self.elements_struc_array_hpc, zz, f, ee, ds, max_length, backward,
+                                        self._apertures())
+        streamlines = [[p.tolist() for p in s] for s in segs]
+        fracs = [[self.fractures[i] for i in fs] for fs in segf]
+        vels = [[v.tolist() for v in vs] for vs in segv]
+        elements = [self.elements[e] if e >= 0 else False for e in el]
+        return streamlines, fracs, vels, elements
+
+    @staticmethod
+    def get_travel_time_and_length(streamline, velocity):
+        """Travel time and length of a streamline (``andfn/dfn.py:2836-2883``)."""
+        from .streamlines import get_travel_time_and_length
+
+        return get_travel_time_and_length(streamline, velocity)
+

@@ -145,6 +145,31 @@ def get_omega(fracture_struc_array, element_struc_array, frac, z, device=0):
     return out
 
 
+def _w_form(form):
+    forms = {"jit": _lib.W_JIT, "object": _lib.W_OBJECT}
+    if form not in forms:
+        raise ValueError(f"form must be one of {sorted(forms)}")
+    return forms[form]
+
+
+def get_w(fracture_struc_array, element_struc_array, frac, z, exclude=-1, form="jit", device=0):
+    """Complex discharge vector w = -dOmega/dz at points z (local complex coordinates) on fractures
+    ``frac``, every element but ``exclude`` summed.  ``form="jit"`` is ``hpc_fracture.calc_w``
+    (``andfn/hpc/hpc_fracture.py:139-174``: no impermeable objects, the bounding circle as
+    ``taylor_series_d1`` alone); ``form="object"`` is ``Fracture.calc_w`` (``andfn/fracture.py:453-477``),
+    the form the streamline tracking uses."""
+    _lib.require_records(fracture_struc_array, element_struc_array)
+    lib = _lib.load()
+    me, m = _lib.capacities(fracture_struc_array, element_struc_array)
+    frac = np.ascontiguousarray(frac, dtype=np.int64)
+    zz = np.ascontiguousarray(z, dtype=np.complex128)
+    out = np.zeros(len(zz), dtype=np.complex128)
+    _lib.check(lib.andfn_mi_w(_lib.ptr(fracture_struc_array), len(fracture_struc_array), me,
+                              _lib.ptr(element_struc_array), len(element_struc_array), m, len(zz), _lib.ptr(frac),
+                              _lib.ptr(zz), int(exclude), _w_form(form), _lib.ptr(out), int(device)))
+    return out
+
+
 def _options(precision="fp64", device=0, verbose=False, lu_mode=0, head_tol=0.0, factor="sparse"):
     if factor not in _lib.FACTORS:
         raise ValueError(f"factor must be one of {sorted(_lib.FACTORS)}")
@@ -219,6 +244,38 @@ class DeviceNetwork:
         out = np.zeros(len(zz), dtype=np.complex128)
         _lib.check(self.lib.andfn_mi_state_omega(self.h, len(zz), _lib.ptr(frac), _lib.ptr(zz), _lib.ptr(out)))
         return out
+
+    def state_w(self, frac, z, exclude=-1, form="object"):
+        """Discharge vector w at points of the current state (``get_w``'s forms)."""
+        frac = np.ascontiguousarray(frac, dtype=np.int64)
+        zz = np.ascontiguousarray(z, dtype=np.complex128)
+        out = np.zeros(len(zz), dtype=np.complex128)
+        _lib.check(self.lib.andfn_mi_state_w(self.h, len(zz), _lib.ptr(frac), _lib.ptr(zz), int(exclude),
+                                             _w_form(form), _lib.ptr(out)))
+        return out
+
+    def trace_segments(self, frac, z0, z_start, remaining, aperture, ds, backward=False):
+        """One streamline segment per line in its current fracture (``andfn_mi_state_trace``).
+
+        Returns (points [n, capacity] complex, speeds [n, capacity], n_points [n], exit_element [n]);
+        row i holds n_points[i] valid entries (-1: the first step was NaN, no segment)."""
+        frac = np.ascontiguousarray(frac, dtype=np.int64)
+        z0 = np.ascontiguousarray(z0, dtype=np.complex128)
+        zs = np.ascontiguousarray(z_start, dtype=np.complex128)
+        rem = np.ascontiguousarray(remaining, dtype=np.int64)
+        ap = np.ascontiguousarray(aperture, dtype=np.float64)
+        n = len(frac)
+        if not (len(z0) == len(zs) == len(rem) == n) or ap.shape != (len(self.fa),):
+            raise ValueError("one start, budget and fracture per line; one aperture per fracture")
+        cap = int(max(3, int(np.max(np.maximum(rem, 1))) + 2)) if n else 3
+        pts = np.zeros((n, cap), dtype=np.complex128)
+        vel = np.zeros((n, cap), dtype=np.float64)
+        npts = np.zeros(n, dtype=np.int64)
+        ex = np.zeros(n, dtype=np.int64)
+        _lib.check(self.lib.andfn_mi_state_trace(self.h, n, _lib.ptr(frac), _lib.ptr(z0), _lib.ptr(zs), _lib.ptr(rem),
+                                                 _lib.ptr(ap), float(ds), 1 if backward else 0, cap, _lib.ptr(pts),
+                                                 _lib.ptr(vel), _lib.ptr(npts), _lib.ptr(ex)))
+        return pts, vel, npts, ex
 
     def adjoint(self, dq=None, dconstant=None, obs_frac=None, obs_z=None, dhead=None, tol=1e-10,
                 max_iterations=5000, damping=0.75, verbose=False, method="gmres", accept_unconverged=False):

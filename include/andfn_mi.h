@@ -185,6 +185,25 @@ int andfn_mi_omega(const void* fractures, int64_t n_fractures, int64_t max_eleme
                    int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
                    double* omega, int32_t device);
 
+/* Forms of the discharge vector (andfn_mi_w, andfn_mi_state_w). */
+#define ANDFN_MI_W_JIT 0     /* andfn/hpc/hpc_fracture.py:139-174 calc_w: intersections, bounding circle, wells and
+                                constant-head lines; the bounding circle as taylor_series_d1(z / r) alone
+                                (hpc_bounding_circle.py:97-115), wells without an inside test (hpc_well.py:89-108) */
+#define ANDFN_MI_W_OBJECT 1  /* andfn/fracture.py:453-477 Fracture.calc_w, the form streamline tracking uses: every
+                                element type, the bounding circle as -taylor_series_d1(z / r) / r (bounding.py:91-108),
+                                NaN inside wells and impermeable circles (well.py:129-153,
+                                impermeable_object.py:131-224) */
+
+/*
+ * Complex discharge vector w = -dOmega/dz at points of the solved network (Q_x - i Q_y), every element
+ * of fracture f but element `exclude` (-1: none) summed, in the given form.  z and w are interleaved
+ * complex; frac gives the fracture index of every point.  Replaces the per-point Python loops of
+ * hpc_fracture.calc_w and Fracture.calc_w.
+ */
+int andfn_mi_w(const void* fractures, int64_t n_fractures, int64_t max_elements, const void* elements,
+               int64_t n_elements, int64_t max_ncoef, int64_t n_points, const int64_t* frac, const double* z,
+               int64_t exclude, int32_t form, double* w, int32_t device);
+
 /*
  * Implicit adjoint of the converged fixed point.  No counterpart in the reference (AnDFN has no
  * gradients); it is the gradient companion of andfn_mi_run.  For the objective
@@ -255,6 +274,31 @@ int andfn_mi_state_heads(andfn_mi_handle* handle, int64_t n_points, const int64_
 /* Complex potential at points of the handle's current state (after a run), as andfn_mi_omega. */
 int andfn_mi_state_omega(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
                          double* omega);
+
+/* Discharge vector at points of the handle's current state, as andfn_mi_w. */
+int andfn_mi_state_w(andfn_mi_handle* handle, int64_t n_points, const int64_t* frac, const double* z,
+                     int64_t exclude, int32_t form, double* w);
+
+/*
+ * One segment of every streamline on the handle's current state: the inner loop of
+ * DFN.streamline_tracking (andfn/dfn.py:2663-2705) in each line's current fracture, i.e.
+ * predictor-corrector steps of ds times the fracture radius along w (runge_kutta, dfn.py:2779-2834;
+ * backward != 0 against the flow) until a step crosses a discharge element of the fracture
+ * (check_streamline_exit, dfn.py:2727-2742: intersections, constant-head lines, wells, in the
+ * fracture's element order), w turns NaN or the segment holds more than remaining[i] + 1 points.
+ * Row i of points (capacity interleaved complex values) receives z_start[i], every step's end and
+ * the exit point; speed the matching |w| / aperture[frac] (fracture.py:479-497; the first speed is
+ * taken at z0[i]).  n_points[i] is the segment's length, or -1 when the first step from z0 is NaN
+ * (the reference then leaves its loop without a segment); exit_element[i] the element crossed last,
+ * -1 if none.  Requires capacity >= max(remaining[i] + 2, 3) for every line; aperture has one
+ * positive value per fracture.  Moving into the neighbouring fracture at an intersection
+ * (get_exit_intersection, dfn.py:2744-2777) is left to the caller, which starts the next segment
+ * from there.
+ */
+int andfn_mi_state_trace(andfn_mi_handle* handle, int64_t n_lines, const int64_t* frac, const double* z0,
+                         const double* z_start, const int64_t* remaining, const double* aperture, double ds,
+                         int32_t backward, int64_t capacity, double* points, double* speed, int64_t* n_points,
+                         int64_t* exit_element);
 
 /* andfn_mi_state_heads without allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses

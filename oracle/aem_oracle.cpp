@@ -11,6 +11,8 @@
 //   * Horner asymptotic / Taylor series with a division per term   hpc_math_functions.py:16-40, 103-128
 //   * well log term                                                hpc_math_functions.py:193-213
 //   * superposition per fracture                                   hpc_fracture.py:51-92
+//   * discharge vector w, series derivatives                       hpc_fracture.py:139-174,
+//     hpc_math_functions.py:75-100, 163-190, and the object model's calc_w (fracture.py:453-477)
 //   * Cauchy fits, powers by repeated multiplication               hpc_math_functions.py:243-297, 450-608
 //   * branch-cut bookkeeping                                       hpc_math_functions.py:300-447
 //   * error measures                                               hpc_math_functions.py:611-670
@@ -218,6 +220,95 @@ inline cd calc_omega(const Oracle& O, i64 f, cd z, i64 exclude) {
     if (id != exclude) omega += element_omega(O.E(id), z, F.id());
   }
   return omega;
+}
+
+// ---- complex discharge vector w ---------------------------------------------------------
+// d/dchi of the asymptotic series: -sum_j j c_j chi^(-j-1), a division per term
+// (hpc_math_functions.py:75-100)
+inline cd asym_expansion_d1(cd chi, const cd* coef, i64 n, double sgn) {
+  cd res(0.0, 0.0);
+  for (i64 k = 0; k < n; ++k) {
+    res -= sgn * coef[n - k - 1] * (double)(n - k - 1);
+    res /= chi;
+  }
+  return res;
+}
+
+// d/dchi of the Taylor series: sum_j j c_j chi^(j-1) (hpc_math_functions.py:163-190)
+inline cd taylor_series_d1(cd chi, const cd* coef, i64 n) {
+  cd res(0.0, 0.0);
+  if (n < 2) return res;  // a constant series (the reference would read coef[1] past the slice)
+  for (i64 k = 0; k < n - 2; ++k) {
+    res += coef[n - k - 1] * (double)(n - k - 1);
+    res *= chi;
+  }
+  res += coef[1];
+  return res;
+}
+
+// dchi/dz of the line map times the line terms: 2 chi^2 / (chi^2 - 1) * 2 / (ep1 - ep0)
+inline cd line_jacobian(cd chi, const cd* ep) { return 2.0 * chi * chi / (chi * chi - 1.0) * 2.0 / (ep[1] - ep[0]); }
+
+// w of one element seen from fracture frac_id.  jit: the jitted functions of the hpc modules,
+// summed by hpc_fracture.calc_w (hpc_fracture.py:139-174): intersections, bounding circle, wells
+// and constant-head lines only, the bounding circle as taylor_series_d1 alone
+// (hpc_bounding_circle.py:97-115), wells without the inside test (hpc_well.py:89-108).
+// !jit: the object model's calc_w (fracture.py:453-477), which streamline tracking uses
+// (dfn.py:2676-2700): every element type, the bounding circle as -taylor_series_d1 / r
+// (bounding.py:91-108), wells and impermeable circles NaN inside (well.py:129-153,
+// impermeable_object.py:131-156), impermeable lines (impermeable_object.py:205-224).
+inline cd element_w(const El& e, cd z, i64 frac_id, bool jit) {
+  switch (e.type()) {
+    case 0: {  // hpc_intersection.py:179-222, intersection.py:158-195
+      if (frac_id == e.frac0()) {
+        cd chi = map_z_line_to_chi(z, e.ep0());
+        cd w = -asym_expansion_d1(chi, e.coef(), e.ncoef(), 1.0) - e.q() / (2 * PI * chi);
+        return w * line_jacobian(chi, e.ep0());
+      }
+      cd chi = map_z_line_to_chi(z, e.ep1());
+      cd w = -asym_expansion_d1(chi, e.coef(), e.ncoef(), -1.0) + e.q() / (2 * PI * chi);
+      return w * line_jacobian(chi, e.ep1());
+    }
+    case 1: {
+      cd chi = z / e.radius();
+      if (jit) return taylor_series_d1(chi, e.coef(), e.ncoef());
+      return -taylor_series_d1(chi, e.coef(), e.ncoef()) / e.radius();
+    }
+    case 2: {  // hpc_well.py:106-107, well.py:144-152
+      cd chi = (z - e.center()) / e.radius();
+      if (!jit && std::abs(chi) < 1.0 - 1e-10) return cd(NaN, NaN);
+      return -e.q() / (2 * PI * chi) / e.radius();
+    }
+    case 3: {  // hpc_const_head_line.py:135-164, const_head.py:166-185
+      cd chi = map_z_line_to_chi(z, e.ep0());
+      cd w = -asym_expansion_d1(chi, e.coef(), e.ncoef(), 1.0) - e.q() / (2 * PI * chi);
+      return w * line_jacobian(chi, e.ep0());
+    }
+    case 4: {
+      if (jit) return cd(0.0, 0.0);
+      cd chi = (z - e.center()) / e.radius();
+      if (std::abs(chi) < 1.0 - 1e-10) return cd(NaN, NaN);
+      return -asym_expansion_d1(chi, e.coef(), e.ncoef(), 1.0) / e.radius();
+    }
+    case 5: {
+      if (jit) return cd(0.0, 0.0);
+      cd chi = map_z_line_to_chi(z, e.ep0());
+      return -asym_expansion_d1(chi, e.coef(), e.ncoef(), 1.0) * line_jacobian(chi, e.ep0());
+    }
+  }
+  return cd(0.0, 0.0);
+}
+
+inline cd calc_w(const Oracle& O, i64 f, cd z, i64 exclude, bool jit) {
+  Fr F = O.F(f);
+  cd w(0.0, 0.0);
+  i64 n = F.nel();
+  const i64* ids = F.elements();
+  for (i64 k = 0; k < n; ++k) {
+    i64 id = ids[k];
+    if (id != exclude) w += element_w(O.E(id), z, F.id(), jit);
+  }
+  return w;
 }
 
 // ---- collocation angles and fits ------------------------------------------------------
@@ -796,6 +887,13 @@ void oracle_omega_at(void* h, i64 n, const i64* frac, const cd* z, i64 exclude, 
   for (i64 i = 0; i < n; ++i) out[i] = calc_omega(O, frac[i], z[i], exclude);
 }
 
+// w of fracture f at points z; jit != 0: hpc_fracture.calc_w, else Fracture.calc_w
+void oracle_w_at(void* h, i64 n, const i64* frac, const cd* z, i64 exclude, i64 jit, cd* out) {
+  Oracle& O = *(Oracle*)h;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (i64 i = 0; i < n; ++i) out[i] = calc_w(O, frac[i], z[i], exclude, jit != 0);
+}
+
 // ---- direct access to the building blocks, for closed-form checks ----------------------
 void oracle_t_map_z_line_to_chi(i64 n, const cd* z, const cd* ep, cd* out) {
   for (i64 i = 0; i < n; ++i) out[i] = map_z_line_to_chi(z[i], ep);
@@ -808,6 +906,12 @@ void oracle_t_asym(i64 n, const cd* chi, const cd* coef, i64 m, cd* out) {
 }
 void oracle_t_taylor(i64 n, const cd* chi, const cd* coef, i64 m, cd* out) {
   for (i64 i = 0; i < n; ++i) out[i] = taylor_series(chi[i], coef, m);
+}
+void oracle_t_asym_d1(i64 n, const cd* chi, const cd* coef, i64 m, cd* out) {
+  for (i64 i = 0; i < n; ++i) out[i] = asym_expansion_d1(chi[i], coef, m, 1.0);
+}
+void oracle_t_taylor_d1(i64 n, const cd* chi, const cd* coef, i64 m, cd* out) {
+  for (i64 i = 0; i < n; ++i) out[i] = taylor_series_d1(chi[i], coef, m);
 }
 // the fit step of cauchy_integral_real (kind 0) / cauchy_integral_domega (kind 1) on given values
 void oracle_t_fit(i64 n, i64 m, i64 type, const double* vals, i64 kind, cd* out) {

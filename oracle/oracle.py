@@ -49,6 +49,7 @@ def load():
     lib.oracle_ncoef.argtypes = [vp, vp]
     lib.oracle_errors.argtypes = [vp, vp]
     lib.oracle_omega_at.argtypes = [vp, i64, vp, vp, i64, vp]
+    lib.oracle_w_at.argtypes = [vp, i64, vp, vp, i64, i64, vp]
     _LIB = lib
     return lib
 
@@ -104,6 +105,15 @@ class OracleHandle:
         z = np.ascontiguousarray(z, dtype=np.complex128)
         out = np.zeros(len(z), dtype=np.complex128)
         self.lib.oracle_omega_at(self.h, len(z), _ptr(frac), _ptr(z), int(exclude), _ptr(out))
+        return out
+
+    def w_at(self, frac, z, exclude=-1, jit=False):
+        """Complex discharge vector w at points: ``jit`` follows hpc_fracture.calc_w
+        (hpc_fracture.py:139-174), otherwise the object model's Fracture.calc_w (fracture.py:453-477)."""
+        frac = np.ascontiguousarray(frac, dtype=np.int64)
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        out = np.zeros(len(z), dtype=np.complex128)
+        self.lib.oracle_w_at(self.h, len(z), _ptr(frac), _ptr(z), int(exclude), int(bool(jit)), _ptr(out))
         return out
 
 
