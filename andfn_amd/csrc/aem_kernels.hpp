@@ -108,22 +108,57 @@ struct TargetView {
   const double2* pt_z;     // explicit points (MODE_POINTS) or null
 };
 
-// one launch of the streamline tracer: n lines, each in its own fracture
+// One launch of the streamline tracer over lines act[0..n): every line advances from its saved
+// state through as many segments as its output rows hold, then saves its state again.
 struct TraceArgs {
   int n;
-  const int* frac;           // [n] fracture of each line
-  const double2* z0;         // [n] point the tracking continues from
-  const double2* z_start;    // [n] first point recorded for the segment
-  const int64_t* remaining;  // [n] point budget: the segment ends once it holds more than this + 1
-  const double* fradius;     // [F] fracture radii
-  const double* aperture;    // [F] apertures (speed = |w| / aperture)
-  double ds;                 // step as a fraction of the fracture radius
-  int backward;              // 1: against the flow
-  int64_t capacity;          // entries per line in pts / vel (>= 3)
-  double2* pts;              // [n][capacity]
-  double* vel;               // [n][capacity]
-  int64_t* npts;             // [n] points of the segment, -1 when the first step is NaN (no segment)
-  int* exit_elem;            // [n] discharge element the last step crossed, -1 if none
+  const int* act;          // [n] lines of this launch
+  // per line (indexed by line): state read and written by every launch
+  int* frac;               // current fracture
+  double2* z0;             // point the tracking continues from
+  double2* z_start;        // first point recorded for the next segment
+  double* elev;            // elevation (the split at intersection exits)
+  int64_t* length;         // points recorded so far
+  int* nseg;               // segments recorded so far
+  int* done;               // 1 once the line has ended
+  int* exit_elem;          // discharge element crossed last, -1 if none
+  // network
+  const double* fradius;   // [F] fracture radii
+  const double* aperture;  // [F] apertures (speed = |w| / aperture)
+  const double* frame;     // [F][12] centre, x vector, y vector, normal of each fracture
+  double ds;               // step as a fraction of the fracture radius
+  int backward;            // 1: against the flow
+  int max_segments;        // a line ends after this many segments
+  int64_t max_length;      // the point budget of DFN.streamline_tracking
+  // this launch's output rows (indexed by position in act)
+  int64_t capacity;        // points per row (>= max_length + 2)
+  int seg_capacity;        // segments per row (>= 1)
+  double2* pts;            // [n][capacity]
+  double* vel;             // [n][capacity]
+  int* seg_frac;           // [n][seg_capacity]
+  int* seg_len;            // [n][seg_capacity]
+  int64_t* out_pts;        // [n] points written this launch
+  int* out_segs;           // [n] segments written this launch
+};
+
+// packs the rows of a trace launch: row k's out_pts[k] points to poff[k], its out_segs[k]
+// segments to soff[k]
+struct TracePack {
+  int n;
+  int64_t capacity;
+  int seg_capacity;
+  const double2* pts;
+  const double* vel;
+  const int* seg_frac;
+  const int* seg_len;
+  const int64_t* out_pts;
+  const int* out_segs;
+  const int64_t* poff;
+  const int64_t* soff;
+  double2* ppts;
+  double* pvel;
+  int* pseg_frac;
+  int* pseg_len;
 };
 
 struct FitItem {
@@ -170,6 +205,7 @@ void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles
 void launch_w_points(const NetView& net, const TargetView& tg, int ntiles, int exclude, int jit, double* val,
                      hipStream_t st);
 void launch_trace(const NetView& net, const TraceArgs& ta, hipStream_t st);
+void launch_trace_pack(const TracePack& tp, hipStream_t st);
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 // unwrap (circle-type sides, in place in val), DFT sums of every side into dft (side s: X_j at
 // s 2M + j, Y_j at s 2M + M + j), then the element solves

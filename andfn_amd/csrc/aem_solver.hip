@@ -233,9 +233,11 @@ class Solver {
   // values at points grouped by fracture: what = 0 heads, 1 Omega (two values per point), 2 w in
   // the jitted form, 3 w in the object model's form (two values per point)
   void heads(int64_t npts, const int64_t* frac, const double* z, double* out, int what = 0, int64_t exclude = -1);
-  void trace(int64_t n, const int64_t* frac, const double* z0, const double* zs, const int64_t* remaining,
-             const double* aperture, double ds, int backward, int64_t capacity, double* pts, double* vel,
-             int64_t* npts, int64_t* exit_elem);
+  // streamline tracking of n lines; the results stay in the solver until fetched
+  void streamlines(int64_t n, const int64_t* frac, const double* z0, const double* elevation, const double* aperture,
+                   double ds, int backward, int64_t max_length, int64_t* n_points, int64_t* n_segments,
+                   int64_t* exit_element);
+  void streamlines_fetch(double* points, double* speed, int64_t* segment_fracture, int64_t* segment_points);
   // heads() with every device buffer in the caller-provided scratch region (no allocation)
   void heads_noalloc(int64_t npts, const int64_t* frac, const double* z, double* out, unsigned char* scratch_dev);
   int device() const { return dev_; }
@@ -363,6 +365,11 @@ class Solver {
   void mf_analyse_now();
   std::vector<double> fcenter_;          // [F][3] fracture centres (fracture record offset 24)
   std::vector<double> fradius_;          // [F] fracture radii (fracture record offset 16)
+  std::vector<double> fframe_;           // [F][12] centre, x vector, y vector, normal (offsets 24-120)
+  // streamline results of the last streamlines() call, per line in input order
+  std::vector<std::vector<double2>> sl_pts_;
+  std::vector<std::vector<double>> sl_vel_;
+  std::vector<std::vector<int>> sl_sfrac_, sl_slen_;
   std::vector<int> h_csr_ptr_, h_csr_col_;  // host copy of the CSR pattern
   void ensure_csr_pattern();
   // blocked LU with tournament pivoting (one connected part): candidate sets, the panel's
@@ -574,6 +581,7 @@ void Solver::parse() {
   frac_id_.resize(F_);
   fcenter_.resize(3 * (size_t)F_);
   fradius_.resize(F_);
+  fframe_.resize(12 * (size_t)F_);
   mem_ptr_.assign(F_ + 1, 0);
   slots_.clear();
   for (int f = 0; f < F_; ++f) {
@@ -582,6 +590,13 @@ void Solver::parse() {
     t_[f] = rd<double>(p, 8);
     for (int a = 0; a < 3; ++a) fcenter_[3 * (size_t)f + a] = rd<double>(p, 24 + 8 * a);
     fradius_[f] = rd<double>(p, 16);
+    // centre 24, normal 48, x vector 72, y vector 96 -> frame order centre, x, y, normal
+    for (int a = 0; a < 3; ++a) {
+      fframe_[12 * (size_t)f + a] = rd<double>(p, 24 + 8 * a);
+      fframe_[12 * (size_t)f + 3 + a] = rd<double>(p, 72 + 8 * a);
+      fframe_[12 * (size_t)f + 6 + a] = rd<double>(p, 96 + 8 * a);
+      fframe_[12 * (size_t)f + 9 + a] = rd<double>(p, 48 + 8 * a);
+    }
     constant_[f] = rd<double>(p, L_.f_const);
     int64_t nel = rd<int64_t>(p, L_.f_nel);
     if (nel < 0 || nel > L_.ME) fail(ANDFN_MI_ECAPACITY, "fracture %d: %lld elements exceed capacity %lld", f, (long long)nel, (long long)L_.ME);
@@ -2347,86 +2362,199 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
     for (int c = 0; c < w; ++c) out[w * order[k] + c] = v[w * k + c];
 }
 
-void Solver::trace(int64_t n, const int64_t* frac, const double* z0, const double* zs, const int64_t* remaining,
-                   const double* aperture, double ds, int backward, int64_t capacity, double* pts, double* vel,
-                   int64_t* npts, int64_t* exit_elem) {
-  // one segment of every line (the inner loop of DFN.streamline_tracking, dfn.py:2663-2705) in its
-  // current fracture; lines are launched grouped by fracture so a wavefront mostly walks one
-  // fracture's elements
-  if (n > (int64_t)INT32_MAX / 2) fail(ANDFN_MI_EINVAL, "%lld lines exceed the limit of one launch", (long long)n);
-  if (capacity < 3) fail(ANDFN_MI_EINVAL, "capacity %lld: a segment holds at least 3 points", (long long)capacity);
-  std::vector<int64_t> order(n);
-  for (int64_t i = 0; i < n; ++i) {
+void Solver::streamlines(int64_t n, const int64_t* frac, const double* z0, const double* elevation,
+                         const double* aperture, double ds, int backward, int64_t max_length, int64_t* n_points,
+                         int64_t* n_segments, int64_t* exit_element) {
+  // DFN.streamline_tracking (dfn.py:2638-2725) for every line at once.  Each launch advances every
+  // unfinished line of a chunk as far as its output row allows (kSegRow segments, the point budget
+  // plus three points per segment); the rows are packed on the device and appended to the lines'
+  // host results, and lines that ran out of row continue in the next launch.
+  constexpr int kSegRow = 64;
+  constexpr int64_t kChunkBytes = int64_t(1) << 30;  // output rows of one launch
+  if (n < 0 || n > (int64_t)INT32_MAX / 2) fail(ANDFN_MI_EINVAL, "%lld lines: outside [0, %d]", (long long)n, INT32_MAX / 2);
+  if (max_length < 0 || max_length > (int64_t)1 << 24)
+    fail(ANDFN_MI_EINVAL, "max_length %lld outside [0, 2^24]", (long long)max_length);
+  if (!(ds > 0.0) || !std::isfinite(ds)) fail(ANDFN_MI_EINVAL, "step %g is not a positive number", ds);
+  for (int64_t i = 0; i < n; ++i)
     if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "line %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
-    if (std::max<int64_t>(remaining[i], 1) > capacity - 2)
-      fail(ANDFN_MI_EINVAL, "line %lld: point budget %lld needs a capacity of %lld (have %lld)", (long long)i,
-           (long long)remaining[i], (long long)std::max<int64_t>(remaining[i] + 2, 3), (long long)capacity);
-    order[i] = i;
-  }
   for (int f = 0; f < F_; ++f)
     if (!(aperture[f] > 0.0) || !std::isfinite(aperture[f]))
       fail(ANDFN_MI_EINVAL, "fracture %d: aperture %g is not a positive number", f, aperture[f]);
+  sl_pts_.assign(n, {});
+  sl_vel_.assign(n, {});
+  sl_sfrac_.assign(n, {});
+  sl_slen_.assign(n, {});
   if (n == 0) return;
+  // a line ends after max(max_length, 1) + 1 segments (the reference has no bound: a line that
+  // crossed an intersection at every first step would never end)
+  const int max_seg = (int)std::min<int64_t>(std::max<int64_t>(max_length, 1) + 1, INT32_MAX);
+  const int64_t cap = std::max<int64_t>(max_length + 2, 3) + 3 * kSegRow;
+  const int64_t row_bytes = cap * (int64_t)(sizeof(double2) + sizeof(double)) + kSegRow * 8 + 16;
+  const int64_t chunk = std::max<int64_t>(64, std::min<int64_t>(n, kChunkBytes / row_bytes));
+  // lines grouped by fracture: a wavefront starts in one fracture's elements
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return frac[a] < frac[b]; });
-  std::vector<int> hf(n);
-  std::vector<double2> hz0(n), hzs(n);
-  std::vector<int64_t> hrem(n);
+  std::vector<int> hf(n), hseg(n, 0), hdone(n, 0), hex(n, -1);
+  std::vector<double2> hz(n);
+  std::vector<double> hel(n);
+  std::vector<int64_t> hlen(n, 0);
   for (int64_t k = 0; k < n; ++k) {
     const int64_t i = order[k];
     hf[k] = (int)frac[i];
-    hz0[k] = make_double2(z0[2 * i], z0[2 * i + 1]);
-    hzs[k] = make_double2(zs[2 * i], zs[2 * i + 1]);
-    hrem[k] = remaining[i];
+    hz[k] = make_double2(z0[2 * i], z0[2 * i + 1]);
+    hel[k] = elevation[i];
   }
-  DBuf<int> d_f, d_exit;
-  DBuf<double2> d_z0, d_zs, d_pts;
-  DBuf<int64_t> d_rem, d_np;
-  DBuf<double> d_rad, d_ap, d_vel;
+  DBuf<int> d_f, d_seg, d_done, d_ex, d_act, d_sfrac, d_slen, d_osegs, d_psf, d_psl;
+  DBuf<double2> d_z0, d_zs, d_pts, d_ppts;
+  DBuf<double> d_el, d_vel, d_pvel, d_rad, d_ap, d_frame;
+  DBuf<int64_t> d_len, d_opts, d_poff, d_soff;
   d_f.upload(hf, st_);
-  d_z0.upload(hz0, st_);
-  d_zs.upload(hzs, st_);
-  d_rem.upload(hrem, st_);
+  d_z0.upload(hz, st_);
+  d_zs.upload(hz, st_);
+  d_el.upload(hel, st_);
+  d_len.upload(hlen, st_);
+  d_seg.upload(hseg, st_);
+  d_done.upload(hdone, st_);
+  d_ex.upload(hex, st_);
   d_rad.upload(fradius_, st_);
   d_ap.upload(std::vector<double>(aperture, aperture + F_), st_);
-  d_pts.alloc((size_t)n * capacity);
-  d_vel.alloc((size_t)n * capacity);
-  d_np.alloc(n);
-  d_exit.alloc(n);
-  aem::TraceArgs ta{};
-  ta.n = (int)n;
-  ta.frac = d_f.p;
-  ta.z0 = d_z0.p;
-  ta.z_start = d_zs.p;
-  ta.remaining = d_rem.p;
-  ta.fradius = d_rad.p;
-  ta.aperture = d_ap.p;
-  ta.ds = ds;
-  ta.backward = backward ? 1 : 0;
-  ta.capacity = capacity;
-  ta.pts = d_pts.p;
-  ta.vel = d_vel.p;
-  ta.npts = d_np.p;
-  ta.exit_elem = d_exit.p;
-  aem::launch_trace(net(), ta, st_);
-  HIP_CHECK(hipGetLastError());
-  std::vector<int64_t> hn(n);
-  std::vector<int> he(n);
-  HIP_CHECK(hipMemcpyAsync(hn.data(), d_np.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st_));
-  HIP_CHECK(hipMemcpyAsync(he.data(), d_exit.p, sizeof(int) * n, hipMemcpyDeviceToHost, st_));
-  HIP_CHECK(hipStreamSynchronize(st_));
-  // rows back in the caller's order, only the points each segment holds
-  for (int64_t k = 0; k < n; ++k) {
-    const int64_t i = order[k];
-    npts[i] = hn[k];
-    exit_elem[i] = he[k];
-    if (hn[k] > 0) {
-      HIP_CHECK(hipMemcpyAsync(pts + 2 * i * capacity, d_pts.p + k * capacity, sizeof(double2) * hn[k],
-                               hipMemcpyDeviceToHost, st_));
-      HIP_CHECK(hipMemcpyAsync(vel + i * capacity, d_vel.p + k * capacity, sizeof(double) * hn[k],
-                               hipMemcpyDeviceToHost, st_));
+  d_frame.upload(fframe_, st_);
+  d_pts.alloc((size_t)chunk * cap);
+  d_vel.alloc((size_t)chunk * cap);
+  d_sfrac.alloc((size_t)chunk * kSegRow);
+  d_slen.alloc((size_t)chunk * kSegRow);
+  d_opts.alloc(chunk);
+  d_osegs.alloc(chunk);
+  d_poff.alloc(chunk);
+  d_soff.alloc(chunk);
+  std::vector<int64_t> h_opts(chunk), h_poff(chunk), h_soff(chunk);
+  std::vector<int> h_osegs(chunk), act;
+  std::vector<double2> h_pts;
+  std::vector<double> h_vel;
+  std::vector<int> h_sf, h_sl;
+  for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+    const int64_t c1 = std::min(n, c0 + chunk);
+    act.clear();
+    for (int64_t k = c0; k < c1; ++k) act.push_back((int)k);
+    while (!act.empty()) {
+      const int na = (int)act.size();
+      d_act.upload(act, st_);
+      aem::TraceArgs ta{};
+      ta.n = na;
+      ta.act = d_act.p;
+      ta.frac = d_f.p;
+      ta.z0 = d_z0.p;
+      ta.z_start = d_zs.p;
+      ta.elev = d_el.p;
+      ta.length = d_len.p;
+      ta.nseg = d_seg.p;
+      ta.done = d_done.p;
+      ta.exit_elem = d_ex.p;
+      ta.fradius = d_rad.p;
+      ta.aperture = d_ap.p;
+      ta.frame = d_frame.p;
+      ta.ds = ds;
+      ta.backward = backward ? 1 : 0;
+      ta.max_segments = max_seg;
+      ta.max_length = max_length;
+      ta.capacity = cap;
+      ta.seg_capacity = kSegRow;
+      ta.pts = d_pts.p;
+      ta.vel = d_vel.p;
+      ta.seg_frac = d_sfrac.p;
+      ta.seg_len = d_slen.p;
+      ta.out_pts = d_opts.p;
+      ta.out_segs = d_osegs.p;
+      aem::launch_trace(net(), ta, st_);
+      HIP_CHECK(hipGetLastError());
+      HIP_CHECK(hipMemcpyAsync(h_opts.data(), d_opts.p, sizeof(int64_t) * na, hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipMemcpyAsync(h_osegs.data(), d_osegs.p, sizeof(int) * na, hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipStreamSynchronize(st_));
+      int64_t tp = 0, tsg = 0;
+      for (int k = 0; k < na; ++k) {
+        h_poff[k] = tp;
+        h_soff[k] = tsg;
+        tp += h_opts[k];
+        tsg += h_osegs[k];
+      }
+      if (tp > 0) {
+        d_ppts.alloc(tp);
+        d_pvel.alloc(tp);
+        d_psf.alloc(std::max<int64_t>(tsg, 1));
+        d_psl.alloc(std::max<int64_t>(tsg, 1));
+        HIP_CHECK(hipMemcpyAsync(d_poff.p, h_poff.data(), sizeof(int64_t) * na, hipMemcpyHostToDevice, st_));
+        HIP_CHECK(hipMemcpyAsync(d_soff.p, h_soff.data(), sizeof(int64_t) * na, hipMemcpyHostToDevice, st_));
+        aem::TracePack pk{};
+        pk.n = na;
+        pk.capacity = cap;
+        pk.seg_capacity = kSegRow;
+        pk.pts = d_pts.p;
+        pk.vel = d_vel.p;
+        pk.seg_frac = d_sfrac.p;
+        pk.seg_len = d_slen.p;
+        pk.out_pts = d_opts.p;
+        pk.out_segs = d_osegs.p;
+        pk.poff = d_poff.p;
+        pk.soff = d_soff.p;
+        pk.ppts = d_ppts.p;
+        pk.pvel = d_pvel.p;
+        pk.pseg_frac = d_psf.p;
+        pk.pseg_len = d_psl.p;
+        aem::launch_trace_pack(pk, st_);
+        HIP_CHECK(hipGetLastError());
+        h_pts.resize(tp);
+        h_vel.resize(tp);
+        h_sf.resize(tsg);
+        h_sl.resize(tsg);
+        HIP_CHECK(hipMemcpyAsync(h_pts.data(), d_ppts.p, sizeof(double2) * tp, hipMemcpyDeviceToHost, st_));
+        HIP_CHECK(hipMemcpyAsync(h_vel.data(), d_pvel.p, sizeof(double) * tp, hipMemcpyDeviceToHost, st_));
+        if (tsg > 0) {
+          HIP_CHECK(hipMemcpyAsync(h_sf.data(), d_psf.p, sizeof(int) * tsg, hipMemcpyDeviceToHost, st_));
+          HIP_CHECK(hipMemcpyAsync(h_sl.data(), d_psl.p, sizeof(int) * tsg, hipMemcpyDeviceToHost, st_));
+        }
+      }
+      HIP_CHECK(hipMemcpyAsync(hdone.data(), d_done.p, sizeof(int) * n, hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipStreamSynchronize(st_));
+      std::vector<int> next;
+      for (int k = 0; k < na; ++k) {
+        const int64_t i = order[act[k]];
+        sl_pts_[i].insert(sl_pts_[i].end(), h_pts.begin() + h_poff[k], h_pts.begin() + h_poff[k] + h_opts[k]);
+        sl_vel_[i].insert(sl_vel_[i].end(), h_vel.begin() + h_poff[k], h_vel.begin() + h_poff[k] + h_opts[k]);
+        sl_sfrac_[i].insert(sl_sfrac_[i].end(), h_sf.begin() + h_soff[k], h_sf.begin() + h_soff[k] + h_osegs[k]);
+        sl_slen_[i].insert(sl_slen_[i].end(), h_sl.begin() + h_soff[k], h_sl.begin() + h_soff[k] + h_osegs[k]);
+        if (!hdone[act[k]]) {
+          // a fresh row always takes the next segment, so a line that wrote nothing is stuck
+          if (h_opts[k] == 0 && h_osegs[k] == 0) fail(ANDFN_MI_EHIP, "streamline %lld made no progress", (long long)i);
+          next.push_back(act[k]);
+        }
+      }
+      act.swap(next);
     }
   }
+  HIP_CHECK(hipMemcpyAsync(hex.data(), d_ex.p, sizeof(int) * n, hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t i = order[k];
+    n_points[i] = (int64_t)sl_pts_[i].size();
+    n_segments[i] = (int64_t)sl_sfrac_[i].size();
+    exit_element[i] = hex[k];
+  }
+}
+
+void Solver::streamlines_fetch(double* points, double* speed, int64_t* segment_fracture, int64_t* segment_points) {
+  int64_t p = 0, q = 0;
+  for (size_t i = 0; i < sl_pts_.size(); ++i) {
+    for (size_t j = 0; j < sl_pts_[i].size(); ++j, ++p) {
+      points[2 * p] = sl_pts_[i][j].x;
+      points[2 * p + 1] = sl_pts_[i][j].y;
+      speed[p] = sl_vel_[i][j];
+    }
+    for (size_t j = 0; j < sl_sfrac_[i].size(); ++j, ++q) {
+      segment_fracture[q] = sl_sfrac_[i][j];
+      segment_points[q] = sl_slen_[i][j];
+    }
+  }
 }
 
 // ---- heads at points without allocation (andfn_mi_state_heads_noalloc) ----------------------
@@ -3170,18 +3298,26 @@ int andfn_mi_state_w(andfn_mi_handle* h, int64_t n_points, const int64_t* frac, 
   });
 }
 
-int andfn_mi_state_trace(andfn_mi_handle* h, int64_t n_lines, const int64_t* frac, const double* z0,
-                         const double* z_start, const int64_t* remaining, const double* aperture, double ds,
-                         int32_t backward, int64_t capacity, double* points, double* speed, int64_t* n_points,
-                         int64_t* exit_element) {
+int andfn_mi_state_streamlines(andfn_mi_handle* h, int64_t n_lines, const int64_t* frac, const double* z0,
+                               const double* elevation, const double* aperture, double ds, int32_t backward,
+                               int64_t max_length, int64_t* n_points, int64_t* n_segments, int64_t* exit_element) {
   return guarded([&]() {
     if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
     if (!aperture) fail(ANDFN_MI_EINVAL, "null pointer");
-    if (n_lines > 0 && (!frac || !z0 || !z_start || !remaining || !points || !speed || !n_points || !exit_element))
+    if (n_lines > 0 && (!frac || !z0 || !elevation || !n_points || !n_segments || !exit_element))
       fail(ANDFN_MI_EINVAL, "null pointer");
-    if (n_lines < 0) fail(ANDFN_MI_EINVAL, "negative line count");
-    h->s->trace(n_lines, frac, z0, z_start, remaining, aperture, ds, backward, capacity, points, speed, n_points,
-                exit_element);
+    h->s->streamlines(n_lines, frac, z0, elevation, aperture, ds, backward, max_length, n_points, n_segments,
+                      exit_element);
+    return ANDFN_MI_OK;
+  });
+}
+
+int andfn_mi_state_streamlines_fetch(andfn_mi_handle* h, double* points, double* speed, int64_t* segment_fracture,
+                                     int64_t* segment_points) {
+  return guarded([&]() {
+    if (!h || !h->s) fail(ANDFN_MI_EINVAL, "null handle");
+    if (!points || !speed || !segment_fracture || !segment_points) fail(ANDFN_MI_EINVAL, "null pointer");
+    h->s->streamlines_fetch(points, speed, segment_fracture, segment_points);
     return ANDFN_MI_OK;
   });
 }

@@ -254,28 +254,32 @@ class DeviceNetwork:
                                              _w_form(form), _lib.ptr(out)))
         return out
 
-    def trace_segments(self, frac, z0, z_start, remaining, aperture, ds, backward=False):
-        """One streamline segment per line in its current fracture (``andfn_mi_state_trace``).
+    def streamlines(self, frac, z0, elevation, aperture, ds, max_length, backward=False):
+        """Streamline tracking of many lines on the current state (``andfn_mi_state_streamlines``).
 
-        Returns (points [n, capacity] complex, speeds [n, capacity], n_points [n], exit_element [n]);
-        row i holds n_points[i] valid entries (-1: the first step was NaN, no segment)."""
+        Returns (points, speeds, n_points [n], segment_fracture, segment_points, n_segments [n],
+        exit_element [n]): points and speeds of every line back to back, the segments' fractures and
+        lengths back to back, exit_element -1 where a line left through no element."""
         frac = np.ascontiguousarray(frac, dtype=np.int64)
         z0 = np.ascontiguousarray(z0, dtype=np.complex128)
-        zs = np.ascontiguousarray(z_start, dtype=np.complex128)
-        rem = np.ascontiguousarray(remaining, dtype=np.int64)
+        el = np.ascontiguousarray(elevation, dtype=np.float64)
         ap = np.ascontiguousarray(aperture, dtype=np.float64)
         n = len(frac)
-        if not (len(z0) == len(zs) == len(rem) == n) or ap.shape != (len(self.fa),):
-            raise ValueError("one start, budget and fracture per line; one aperture per fracture")
-        cap = int(max(3, int(np.max(np.maximum(rem, 1))) + 2)) if n else 3
-        pts = np.zeros((n, cap), dtype=np.complex128)
-        vel = np.zeros((n, cap), dtype=np.float64)
+        if len(z0) != n or len(el) != n or ap.shape != (len(self.fa),):
+            raise ValueError("one start and elevation per line; one aperture per fracture")
         npts = np.zeros(n, dtype=np.int64)
+        nseg = np.zeros(n, dtype=np.int64)
         ex = np.zeros(n, dtype=np.int64)
-        _lib.check(self.lib.andfn_mi_state_trace(self.h, n, _lib.ptr(frac), _lib.ptr(z0), _lib.ptr(zs), _lib.ptr(rem),
-                                                 _lib.ptr(ap), float(ds), 1 if backward else 0, cap, _lib.ptr(pts),
-                                                 _lib.ptr(vel), _lib.ptr(npts), _lib.ptr(ex)))
-        return pts, vel, npts, ex
+        _lib.check(self.lib.andfn_mi_state_streamlines(self.h, n, _lib.ptr(frac), _lib.ptr(z0), _lib.ptr(el),
+                                                       _lib.ptr(ap), float(ds), 1 if backward else 0, int(max_length),
+                                                       _lib.ptr(npts), _lib.ptr(nseg), _lib.ptr(ex)))
+        pts = np.zeros(int(npts.sum()), dtype=np.complex128)
+        vel = np.zeros(len(pts), dtype=np.float64)
+        sf = np.zeros(int(nseg.sum()), dtype=np.int64)
+        sp = np.zeros(len(sf), dtype=np.int64)
+        _lib.check(self.lib.andfn_mi_state_streamlines_fetch(self.h, _lib.ptr(pts), _lib.ptr(vel), _lib.ptr(sf),
+                                                             _lib.ptr(sp)))
+        return pts, vel, npts, sf, sp, nseg, ex
 
     def adjoint(self, dq=None, dconstant=None, obs_frac=None, obs_z=None, dhead=None, tol=1e-10,
                 max_iterations=5000, damping=0.75, verbose=False, method="gmres", accept_unconverged=False):

@@ -280,25 +280,33 @@ int andfn_mi_state_w(andfn_mi_handle* handle, int64_t n_points, const int64_t* f
                      int64_t exclude, int32_t form, double* w);
 
 /*
- * One segment of every streamline on the handle's current state: the inner loop of
- * DFN.streamline_tracking (andfn/dfn.py:2663-2705) in each line's current fracture, i.e.
- * predictor-corrector steps of ds times the fracture radius along w (runge_kutta, dfn.py:2779-2834;
- * backward != 0 against the flow) until a step crosses a discharge element of the fracture
- * (check_streamline_exit, dfn.py:2727-2742: intersections, constant-head lines, wells, in the
- * fracture's element order), w turns NaN or the segment holds more than remaining[i] + 1 points.
- * Row i of points (capacity interleaved complex values) receives z_start[i], every step's end and
- * the exit point; speed the matching |w| / aperture[frac] (fracture.py:479-497; the first speed is
- * taken at z0[i]).  n_points[i] is the segment's length, or -1 when the first step from z0 is NaN
- * (the reference then leaves its loop without a segment); exit_element[i] the element crossed last,
- * -1 if none.  Requires capacity >= max(remaining[i] + 2, 3) for every line; aperture has one
- * positive value per fracture.  Moving into the neighbouring fracture at an intersection
- * (get_exit_intersection, dfn.py:2744-2777) is left to the caller, which starts the next segment
- * from there.
+ * Streamline tracking on the handle's current state: DFN.streamline_tracking (andfn/dfn.py:2638-2725)
+ * for n_lines lines at once, line i from z0[i] (interleaved complex, local coordinates) in fracture
+ * frac[i] with elevation[i].  In a fracture, predictor-corrector steps of ds times the fracture radius
+ * along the object model's w (runge_kutta, dfn.py:2779-2834; backward != 0 against the flow) until a
+ * step crosses a discharge element of the fracture (check_streamline_exit, dfn.py:2727-2742:
+ * intersections, constant-head lines, wells, in the fracture's element order), w turns NaN or the
+ * segment holds more than max_length - (points recorded so far) + 1 points; through an intersection
+ * the line continues in the neighbouring fracture, on the side its elevation selects
+ * (get_exit_intersection, dfn.py:2744-2777), otherwise it ends.  Speeds are |w| / aperture[fracture]
+ * (fracture.py:479-497), one positive aperture per fracture.  A line also ends after
+ * max(max_length, 1) + 1 segments (the reference's loop has no such bound).
+ * Returns per line the number of points, of segments (one per fracture visited) and the element it
+ * left through last (-1: none, the reference's False).  The points themselves stay in the handle
+ * until andfn_mi_state_streamlines_fetch (the next call replaces them).
  */
-int andfn_mi_state_trace(andfn_mi_handle* handle, int64_t n_lines, const int64_t* frac, const double* z0,
-                         const double* z_start, const int64_t* remaining, const double* aperture, double ds,
-                         int32_t backward, int64_t capacity, double* points, double* speed, int64_t* n_points,
-                         int64_t* exit_element);
+int andfn_mi_state_streamlines(andfn_mi_handle* handle, int64_t n_lines, const int64_t* frac, const double* z0,
+                               const double* elevation, const double* aperture, double ds, int32_t backward,
+                               int64_t max_length, int64_t* n_points, int64_t* n_segments, int64_t* exit_element);
+
+/*
+ * The points of the last andfn_mi_state_streamlines call, line after line: points (interleaved
+ * complex, sum of n_points values) and speed (one per point); per segment, line after line, its
+ * fracture and its number of points (sum of n_segments values).  A segment's first point is where it
+ * starts recording (the reference's z_start), its last the exit point.
+ */
+int andfn_mi_state_streamlines_fetch(andfn_mi_handle* handle, double* points, double* speed,
+                                     int64_t* segment_fracture, int64_t* segment_points);
 
 /* andfn_mi_state_heads without allocation.  No counterpart in the reference: it is for callers that
  * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses

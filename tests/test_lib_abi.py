@@ -58,6 +58,7 @@ def test_no_device_or_bad_input_is_an_error_not_a_crash():
                lib.andfn_mi_state_heads_noalloc(None, 0, None, None, None),
                lib.andfn_mi_state_omega(None, 0, None, None, None),
                lib.andfn_mi_state_w(None, 0, None, None, -1, 1, None),
-               lib.andfn_mi_state_trace(None, 0, None, None, None, None, None, 0.01, 0, 3, None, None, None, None),
+               lib.andfn_mi_state_streamlines(None, 0, None, None, None, None, 0.01, 0, 10, None, None, None),
+               lib.andfn_mi_state_streamlines_fetch(None, None, None, None, None),
                lib.andfn_mi_w(None, 1, 500, None, 1, 200, 0, None, None, -1, 1, None, 0)):
         assert rc == _lib.EINVAL
