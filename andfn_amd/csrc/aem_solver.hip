@@ -2319,28 +2319,43 @@ void Solver::write_back() {
 void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* out, int what, int64_t exclude) {
   // group the points by fracture into tiles, evaluate Re Omega / t (hpc_fracture.py:204-250), the
   // complex potential Omega itself (calc_omega hpc_fracture.py:51-92) or the discharge vector w
-  // (hpc_fracture.py:139-174 / fracture.py:453-477)
-  std::vector<int64_t> order(npts);
+  // (hpc_fracture.py:139-174 / fracture.py:453-477).  Points already grouped by fracture (a flow-net
+  // pattern, fracture after fracture) go to the device and back as they are; others are grouped by
+  // a counting sort that keeps their order within a fracture.
+  if (npts > (int64_t)INT32_MAX) fail(ANDFN_MI_EINVAL, "%lld points exceed the limit of one call", (long long)npts);
+  bool grouped = true;
   for (int64_t i = 0; i < npts; ++i) {
     if (frac[i] < 0 || frac[i] >= F_) fail(ANDFN_MI_EINVAL, "point %lld: fracture %lld out of range", (long long)i, (long long)frac[i]);
-    order[i] = i;
+    if (i > 0 && frac[i] < frac[i - 1]) grouped = false;
   }
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return frac[a] < frac[b]; });
-  std::vector<double2> zs(npts);
+  std::vector<int64_t> order;
+  if (!grouped) {
+    std::vector<int64_t> pos(F_ + 1, 0);
+    for (int64_t i = 0; i < npts; ++i) ++pos[frac[i] + 1];
+    for (int f = 0; f < F_; ++f) pos[f + 1] += pos[f];
+    order.resize(npts);
+    for (int64_t i = 0; i < npts; ++i) order[pos[frac[i]]++] = i;
+  }
+  auto at = [&](int64_t k) { return grouped ? k : order[k]; };
   std::vector<Tile> tiles;
   for (int64_t k = 0; k < npts;) {
-    int64_t f = frac[order[k]];
-    int64_t b = k;
-    while (k < npts && frac[order[k]] == f) {
-      zs[k] = make_double2(z[2 * order[k]], z[2 * order[k] + 1]);
-      ++k;
-    }
+    const int64_t f = frac[at(k)];
+    const int64_t b = k;
+    while (k < npts && frac[at(k)] == f) ++k;
     for (int64_t p0 = b; p0 < k; p0 += 64) tiles.push_back(Tile{(int)f, 0, 0, (int)p0, (int)k});
   }
   DBuf<double2> dz;
   DBuf<Tile> dt;
   DBuf<double> dv;
-  dz.upload(zs, st_);
+  dz.alloc(std::max<int64_t>(npts, 1));
+  if (grouped) {
+    if (npts > 0) HIP_CHECK(hipMemcpyAsync(dz.p, z, sizeof(double2) * npts, hipMemcpyHostToDevice, st_));
+  } else {
+    std::vector<double2> zs(npts);
+    for (int64_t k = 0; k < npts; ++k) zs[k] = make_double2(z[2 * order[k]], z[2 * order[k] + 1]);
+    HIP_CHECK(hipMemcpyAsync(dz.p, zs.data(), sizeof(double2) * npts, hipMemcpyHostToDevice, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));  // zs leaves scope
+  }
   dt.upload(tiles, st_);
   const int w = what == 0 ? 1 : 2;
   dv.alloc(std::max<int64_t>(w * npts, 1));
@@ -2355,6 +2370,11 @@ void Solver::heads(int64_t npts, const int64_t* frac, const double* z, double* o
                       st_);
   }
   HIP_CHECK(hipGetLastError());
+  if (grouped) {
+    if (npts > 0) HIP_CHECK(hipMemcpyAsync(out, dv.p, sizeof(double) * w * npts, hipMemcpyDeviceToHost, st_));
+    HIP_CHECK(hipStreamSynchronize(st_));
+    return;
+  }
   std::vector<double> v(w * npts);
   HIP_CHECK(hipMemcpyAsync(v.data(), dv.p, sizeof(double) * w * npts, hipMemcpyDeviceToHost, st_));
   HIP_CHECK(hipStreamSynchronize(st_));

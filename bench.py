@@ -11,7 +11,9 @@ field and correction; FP64 otherwise) on a handle whose network is resident in d
 is kept, as SuperLU's ``SamePattern`` reuse keeps it).  ``forms`` adds the other form with its
 error against the FP64 result; ``drop_in`` times the drop-in ``hpc_solve.solve`` call, which
 pays the record parse, upload, symbolic analysis and every pivot-delay round, and downloads the
-records; ``adjoint`` times the implicit adjoint against one forward solve.
+records; ``adjoint`` times the implicit adjoint against one forward solve; ``flow`` times the
+post-processing of the solved network on the device (streamline tracking of 4,096 lines and the
+complex potential on every fracture's flow-net pattern) beside the oracle on bounded samples.
 
 Prints one JSON line.  ``roofline`` is the dominant kernel, the collocation sweep (FP64 vector
 ALU bound); ``roofline_solve`` the sparse triangular solves (HBM bound, bytes by the
@@ -63,6 +65,8 @@ def parse():
                     help="first run one device-to-device copy of this many MiB (HBM counter calibration)")
     ap.add_argument("--cpu-iterations", type=int, default=2, help="oracle element iterations in the CPU sample")
     ap.add_argument("--cpu-cap", type=float, default=90.0, help="wall-clock cap (s) of the CPU sample")
+    ap.add_argument("--flow-lines", type=int, default=4096,
+                    help="streamlines traced on the solved network in the flow block (0: no flow block)")
     ap.add_argument("--cpu-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.precision is None:
@@ -307,6 +311,80 @@ def _roofline_solve(last, f32, workload=None):
     }
 
 
+FLOW_DS, FLOW_MAX_LENGTH, FLOW_APERTURE, FLOW_NET_POINTS = 0.01, 500, 1e-4, (900, 100)
+
+
+def _flow_starts(fa, n, seed=5):
+    rng = np.random.default_rng(seed)
+    fr = rng.choice(len(fa), n)
+    z0 = 0.5 * fa["radius"][fr] * np.sqrt(rng.uniform(size=n)) * np.exp(2j * np.pi * rng.uniform(size=n))
+    return fr, z0
+
+
+def _flow_bench(net, fa, n_lines):
+    """Post-processing of the solved network: streamlines (DFN.streamline_tracking for n_lines start
+    points) and the complex potential on every fracture's flow-net pattern (get_flow_nets)."""
+    from andfn_amd.hpc_fracture import sunflower_spiral
+
+    fr, z0 = _flow_starts(fa, n_lines)
+    ap = np.full(len(fa), FLOW_APERTURE)
+    el = np.full(n_lines, 0.5)
+    net.streamlines(fr[:64], z0[:64], el[:64], ap, FLOW_DS, FLOW_MAX_LENGTH)  # warm-up
+    t0 = time.perf_counter()
+    pts, vel, npts, sf, sp, nseg, ex = net.streamlines(fr, z0, el, ap, FLOW_DS, FLOW_MAX_LENGTH)
+    t_sl = time.perf_counter() - t0
+    zu = sunflower_spiral(*FLOW_NET_POINTS)
+    ffr = np.repeat(np.arange(len(fa)), len(zu))
+    zz = (zu[None, :] * fa["radius"][:, None]).ravel()
+    net.state_omega(ffr[:4096], zz[:4096])  # warm-up
+    t0 = time.perf_counter()
+    om = net.state_omega(ffr, zz)
+    t_om = time.perf_counter() - t0
+    return {
+        "streamlines": {"lines": int(n_lines), "points": int(npts.sum()), "segments": int(nseg.sum()),
+                        "s": t_sl, "points_per_s": float(npts.sum() / t_sl),
+                        "setup": f"start at 0.5 r in random fractures, elevation 0.5, ds {FLOW_DS} r, max_length "
+                                 f"{FLOW_MAX_LENGTH}, aperture {FLOW_APERTURE}; andfn_mi_state_streamlines + fetch"},
+        "flow_nets": {"points": int(len(zz)), "s": t_om, "points_per_s": float(len(zz) / t_om),
+                      "nan_points": int(np.isnan(om).sum()),
+                      "setup": f"Omega at sunflower_spiral{FLOW_NET_POINTS} x radius on every fracture "
+                               "(andfn_mi_state_omega, points in and values out of host memory)"},
+        "cpu_baseline": None,
+    }
+
+
+def _flow_cpu(fa, ea, K, n_lines, cap_s=10.0):
+    """The oracle on the same work, bounded samples: lines traced one after another for cap_s
+    seconds (its C++ w behind a Python restatement of the tracking), Omega at the first 200,000
+    flow-net points (C++, OpenMP)."""
+    from andfn_amd.hpc_fracture import sunflower_spiral
+    from oracle import oracle as orc
+    from oracle import streamlines as osl
+
+    O = orc.OracleHandle(fa, ea, K)
+    try:
+        fr, z0 = _flow_starts(fa, n_lines)
+        T = osl.Tracker(O, fa, ea, np.full(len(fa), FLOW_APERTURE))
+        t0 = time.perf_counter()
+        pts = lines = 0
+        while lines < n_lines and time.perf_counter() - t0 < cap_s:
+            s, _, _, _ = T.track(complex(z0[lines]), int(fr[lines]), 0.5, FLOW_DS, FLOW_MAX_LENGTH, False)
+            pts += sum(len(x) for x in s)
+            lines += 1
+        t_sl = time.perf_counter() - t0
+        zu = sunflower_spiral(*FLOW_NET_POINTS)
+        ffr = np.repeat(np.arange(len(fa)), len(zu))[:200000]
+        zz = (zu[None, :] * fa["radius"][:, None]).ravel()[:200000]
+        t0 = time.perf_counter()
+        O.omega_at(ffr, zz)
+        t_om = time.perf_counter() - t0
+    finally:
+        O.close()
+    return {"streamlines_points_per_s": pts / t_sl, "streamline_sample": f"{lines} lines, {pts} points",
+            "flow_net_points_per_s": len(zz) / t_om, "flow_net_sample": f"{len(zz)} points",
+            "cores_streamlines": 1, "cores_flow_nets": os.cpu_count(), "kind": "port"}
+
+
 def bench_batch(args):
     """Batched realisations: 64 seeded DFNs (fracture counts uniform on [500, 4000], the 200-disk law
     at constant density, 10 to 20 coefficients) as one network; the mixed form to head residual
@@ -393,11 +471,19 @@ def main():
             if cb and args.workload == "cfg5":  # one solve of the batch network = all realisations
                 cb["value"] *= args.realisations
                 cb["unit"] = "realisations/s"
+            recs = out.pop("_flow_records", None)
+            if out.get("flow") is not None and recs is not None:
+                fl = out["flow"]
+                fl["cpu_baseline"] = _flow_cpu(*recs, fl["streamlines"]["lines"])
+                fl["speedup_vs_cpu"] = {
+                    "streamlines": fl["streamlines"]["points_per_s"] / fl["cpu_baseline"]["streamlines_points_per_s"],
+                    "flow_nets": fl["flow_nets"]["points_per_s"] / fl["cpu_baseline"]["flow_net_points_per_s"]}
             if cb:
                 out["speedup_vs_cpu"] = {"value": out["value"] / cb["value"],
                                          "bound": "lower" if cb["bound"].startswith("upper") else "measured",
                                          "drop_in": out["drop_in"]["solves_per_s"] / cb["value"]
                                          if out.get("drop_in") else None}
+        out.pop("_flow_records", None)
         print(json.dumps(out))
     finally:
         if child is not None and child.poll() is None:
@@ -490,6 +576,7 @@ def gpu_part(args):
                    "tol": ADJOINT_TOL, "iterations": int(ai["nit"]), "residual": ai["delta"],
                    "ms": 1e3 * t_adj, "first_call_ms": 1e3 * aw[0],
                    "adjoint_forward_ratio": (t_adj / t_fwd) if t_fwd else None}
+    flow = _flow_bench(net, fa, args.flow_lines) if args.flow_lines > 0 else None
     net.close()
     drop_in = None
     if args.dropin_steps > 0:
@@ -537,6 +624,8 @@ def gpu_part(args):
         "forms": forms,
         "drop_in": drop_in,
         "adjoint": adjoint,
+        "flow": flow,
+        "_flow_records": (fa, ea, dfn.discharge_int) if flow is not None else None,  # for the CPU leg
         # the FP64 form's kernels (one sweep kernel throughout; the mixed form switches between
         # its FP32-far-field and FP64 sweeps and between FP32 and FP64 tiles)
         "roofline": _roofline_sweep(lasts["fp64"] if "fp64" in lasts else last, "fp64" in lasts, args.workload),
