@@ -216,24 +216,32 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
         }
         continue;
       }
-      double2 chi[P], w[P], h[P];
-      double r2[P];
+      // the log term first, so chi and |chi|^2 are dead during the series
+      const double qq = (sl.kind == KIND_LINE && use_q) ? sl.sign * net.q[src] * kInv2Pi : 0.0;
+      double2 w[P], h[P];
+      double lg[P], at[P];
 #pragma unroll
       for (int k = 0; k < P; ++k) {
-        chi[k] = joukowsky_exterior(Z[k]);
-        r2[k] = fma(chi[k].x, chi[k].x, chi[k].y * chi[k].y);
-        const double inv = rcp_pos(r2[k]);
-        w[k] = c2(chi[k].x * inv, -chi[k].y * inv);
+        const double2 chi = joukowsky_exterior(Z[k]);
+        const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
+        lg[k] = 0.0;
+        at[k] = 0.0;
+        if (qq != 0.0) {
+          lg[k] = 0.5 * log_pos(r2);
+          if (any_imag) at[k] = atan2(chi.y, chi.x);
+        }
+        const double inv = rcp_pos(r2);
+        w[k] = c2(chi.x * inv, -chi.y * inv);
       }
-      horner_p<P>(cf, m, w, h);
-      const double qq = (sl.kind == KIND_LINE && use_q) ? sl.sign * net.q[src] * kInv2Pi : 0.0;
+      if (sl.kind == KIND_LINE && net.real_lines) horner_real_p<P>(cf, m, w, h);
+      else horner_p<P>(cf, m, w, h);
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         tre[k] = sl.sign * h[k].x;
         tim[k] = sl.sign * h[k].y;
         if (qq != 0.0) {
-          tre[k] = fma(qq, 0.5 * log_pos(r2[k]), tre[k]);
-          if (any_imag) tim[k] = fma(qq, atan2(chi[k].y, chi[k].x), tim[k]);
+          tre[k] = fma(qq, lg[k], tre[k]);
+          if (any_imag) tim[k] = fma(qq, at[k], tim[k]);
         }
       }
     } else if (sl.kind == KIND_BOUND) {
@@ -705,7 +713,7 @@ constexpr int kRhsMaxQ = 8;  // discharge points per lane (K <= 512)
 
 __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView tg, int K,
                                                        const int64_t* __restrict__ wptr, const int* __restrict__ mcap,
-                                                       double2* __restrict__ W, int* __restrict__ side_nan) {
+                                                       double* __restrict__ W, int* __restrict__ side_nan) {
   const int d = blockIdx.x;
   const int lane = threadIdx.x;
   const int e = tg.side_elem[d];
@@ -758,6 +766,8 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
     }
     const double sg = (sl.kind == KIND_LINE || sl.kind == KIND_IMPL) ? sl.sign * invK : invK;
     const int m = mcap[src];
+    // Re(c W) = Re c Re W - Im c Im W; real coefficients need the first weight only
+    const bool real_c = sl.kind == KIND_LINE && net.real_lines;
     for (int j = 0; j < m; ++j) {
       double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -769,39 +779,47 @@ __global__ __launch_bounds__(64) void rhs_basis_kernel(NetView net, TargetView t
       }
       ar = wave_sum(ar);
       ai = wave_sum(ai);
-      if (lane == 0) W[o + j] = c2(sg * ar, sg * ai);
+      if (lane == 0) {
+        if (real_c) {
+          W[o + j] = sg * ar;
+        } else {
+          W[o + 2 * j] = sg * ar;
+          W[o + 2 * j + 1] = -(sg * ai);
+        }
+      }
     }
-    o += m;
+    o += real_c ? m : 2 * m;
   }
   inside = __any(inside) ? 1 : 0;
   if (lane == 0) side_nan[d] = inside;
 }
 
-// om_d: the entries of side d are the columns of its fracture's table col[coff[d] ...] (source,
-// coefficient index), the same for every side of that fracture, so the table and the gathered
-// coefficients stay in cache while W streams from HBM once per iteration
+// om_d: the entries of side d are the columns of its fracture's table col[coff[d] ...], the same
+// for every side of that fracture, so the table and the gathered coefficients stay in cache while
+// W streams from HBM once per iteration
 template <class WT>
 __device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const int64_t* __restrict__ wptr,
-                                                const WT* __restrict__ W, const int2* __restrict__ col,
+                                                const WT* __restrict__ W, const unsigned* __restrict__ col,
                                                 const int64_t* __restrict__ coff, const int* __restrict__ side_nan) {
   // four entries per lane in flight (column, count and coefficient loads are a dependent chain)
+  const double* __restrict__ cre = reinterpret_cast<const double*>(net.coef);
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   const int64_t w0 = wptr[d], t1 = wptr[d + 1], c0 = coff[d] - w0;
   for (int64_t t0 = w0 + threadIdx.x; t0 < t1; t0 += 256) {
-    int2 q[4];
+    unsigned q[4];
     WT w[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int64_t t = t0 + 64 * u;
-      q[u] = t < t1 ? col[c0 + t] : make_int2(0, 0x7fffffff);
-      w[u] = t < t1 ? W[t] : WT{0, 0};
+      q[u] = t < t1 ? col[c0 + t] : 0xffffffffu;
+      w[u] = t < t1 ? W[t] : WT(0);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      if (q[u].y < net.ncoef[q[u].x]) {
-        const double2 c = net.coef[(size_t)q[u].x * net.M + q[u].y];
-        acc[u] = fma(c.x, (double)w[u].x, fma(-c.y, (double)w[u].y, acc[u]));
-      }
+      const int src = (int)(q[u] >> kRhsColShift);
+      const int jj = (int)(q[u] & ((1u << kRhsColShift) - 1));
+      if (q[u] != 0xffffffffu && (jj >> 1) < net.ncoef[src])
+        acc[u] = fma(cre[2 * ((size_t)src * net.M + (jj >> 1)) + (jj & 1)], (double)w[u], acc[u]);
     }
   }
   double a = wave_sum((acc[0] + acc[1]) + (acc[2] + acc[3]));
@@ -811,7 +829,7 @@ __device__ __forceinline__ double rhs_side_mean(const NetView& net, int d, const
 template <class WT>
 __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem* __restrict__ items, int nde,
                                                       const int64_t* __restrict__ wptr, const WT* __restrict__ W,
-                                                      const int2* __restrict__ col, const int64_t* __restrict__ coff,
+                                                      const unsigned* __restrict__ col, const int64_t* __restrict__ coff,
                                                       const int* __restrict__ side_nan, const int* __restrict__ de_pos,
                                                       double* __restrict__ b) {
   const int j = blockIdx.x;
@@ -826,31 +844,31 @@ __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem
   }
 }
 
-__global__ void w_to_f32_kernel(int64_t n, const double2* __restrict__ W, float2* __restrict__ o) {
+__global__ void w_to_f32_kernel(int64_t n, const double* __restrict__ W, float* __restrict__ o) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) o[i] = make_float2((float)W[i].x, (float)W[i].y);
+  if (i < n) o[i] = (float)W[i];
 }
 
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int* side_nan, hipStream_t st) {
+                      const int* mcap, double* W, int* side_nan, hipStream_t st) {
   if (nsides <= 0) return;
   hipLaunchKernelGGL(rhs_basis_kernel, dim3(nsides), dim3(64), 0, st, net, tg, K, wptr, mcap, W, side_nan);
 }
 
-void launch_rhs_to_f32(int64_t n, const double2* W, float2* W32, hipStream_t st) {
+void launch_rhs_to_f32(int64_t n, const double* W, float* W32, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(w_to_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, W, W32);
 }
 
-void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
-                     const float2* W32, const int2* col, const int64_t* coff, const int* side_nan, const int* de_pos,
-                     double* b, hipStream_t st) {
+void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double* W,
+                     const float* W32, const unsigned* col, const int64_t* coff, const int* side_nan,
+                     const int* de_pos, double* b, hipStream_t st) {
   if (nde <= 0) return;
   if (W32)
-    hipLaunchKernelGGL(rhs_eval_kernel<float2>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W32, col, coff,
+    hipLaunchKernelGGL(rhs_eval_kernel<float>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W32, col, coff,
                        side_nan, de_pos, b);
   else
-    hipLaunchKernelGGL(rhs_eval_kernel<double2>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, col, coff,
+    hipLaunchKernelGGL(rhs_eval_kernel<double>, dim3(nde), dim3(64), 0, st, net, items, nde, wptr, W, col, coff,
                        side_nan, de_pos, b);
 }
 

@@ -53,6 +53,7 @@ struct NetView {
   double2* wcoef;         // [E][M]
   const float2* coef32;   // [E][M] single-precision copy (mixed-precision far field)
   double far2;            // mixed precision: a line source is evaluated in FP32 where |Z|^2 >= far2 on every lane
+  int real_lines;         // 1: every KIND_LINE source (intersection, constant-head line) has real coefficients
   const double* t;
   const double* constant;
   double* constant_w;
@@ -278,17 +279,21 @@ void launch_row_scale(double* A, int64_t npad, int nblk, double* rs, unsigned lo
 void launch_scale(const double* b, const double* rs, int n, double* x, hipStream_t st);
 void launch_cross(const NetView& net, const FitView& fv, const int* circles, int nc, hipStream_t st);
 // right-hand side as a linear map of the coefficients: basis W (entries [wptr[d], wptr[d + 1]) of
-// discharge side d, with their (source, power) in widx) built once, applied per iteration
+// discharge side d) built once, applied per iteration.  An entry is the real weight of one real
+// number of the coefficient state: Re c_j of a source whose coefficients are real (one entry
+// per power), Re c_j and Im c_j of the other sources (two entries per power).  A column is
+// packed as (source << kRhsColShift) | (2 j + part), part 1 for Im c_j.
 constexpr int kRhsMaxK = 512;
+constexpr int kRhsColShift = 9;  // 2 j + part < 2 x 256 (coefficient capacity <= kMaxPts / 2)
 void launch_rhs_basis(const NetView& net, const TargetView& tg, int nsides, int K, const int64_t* wptr,
-                      const int* mcap, double2* W, int* side_nan, hipStream_t st);
+                      const int* mcap, double* W, int* side_nan, hipStream_t st);
 // the basis rounded to FP32 (the mixed form's FP32 phase)
-void launch_rhs_to_f32(int64_t n, const double2* W, float2* W32, hipStream_t st);
-// b from the basis (W32 non-null: its FP32 copy); col / coff: per fracture the (source, index)
-// of every column and per side the offset of its fracture's columns
-void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double2* W,
-                     const float2* W32, const int2* col, const int64_t* coff, const int* side_nan, const int* de_pos,
-                     double* b, hipStream_t st);
+void launch_rhs_to_f32(int64_t n, const double* W, float* W32, hipStream_t st);
+// b from the basis (W32 non-null: its FP32 copy); col / coff: per fracture the packed column of
+// every entry and per side the offset of its fracture's columns
+void launch_rhs_eval(const NetView& net, const RhsItem* items, int nde, const int64_t* wptr, const double* W,
+                     const float* W32, const unsigned* col, const int64_t* coff, const int* side_nan,
+                     const int* de_pos, double* b, hipStream_t st);
 void launch_rhs_reduce(const NetView& net, const RhsItem* items, int nde, const double* val, const int* pt_off, int K,
                        const int* de_pos, double* b, hipStream_t st);
 void launch_assemble(const NetView& net, const TargetView& tg, int nsides, const SysLayout& sys, int K, int nf,

@@ -150,6 +150,72 @@ __device__ __forceinline__ void horner_p(const double2* __restrict__ c, int m, c
   }
 }
 
+// r = sum_{j<m} c_j w^j for coefficients with zero imaginary parts (intersections and
+// constant-head lines: their fits keep Re only), P points per lane.  The polynomial has real
+// coefficients, so it is reduced modulo the real quadratic (x - w)(x - conj w) = x^2 - s x + t,
+// s = 2 Re w, t = |w|^2, whose roots are w and conj w (Knuth, TAOCP vol. 2, 4.6.4):
+//   b_j = c_j + s b_{j+1} - t b_{j+2}  (j = m-1 .. 0, b_m = b_{m+1} = 0),  p(w) = b_0 - b_1 conj(w).
+// Two real multiply-adds per coefficient instead of four, and half the coefficient bytes.
+template <int P>
+__device__ __forceinline__ void horner_real_p(const double2* __restrict__ c, int m, const double2 (&w)[P],
+                                              double2 (&r)[P]) {
+  const double* __restrict__ cr = reinterpret_cast<const double*>(c);  // Re c_j at cr[2 j]
+  double s[P], t[P], b1[P], b2[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    s[k] = 2.0 * w[k].x;
+    t[k] = fma(w[k].x, w[k].x, w[k].y * w[k].y);
+    b1[k] = cr[2 * (m - 1)];
+    b2[k] = 0.0;
+  }
+  int j = m - 2;
+  if (j >= 3) {
+    // the next four coefficients are requested before the current four are used
+    double n0 = cr[2 * j], n1 = cr[2 * (j - 1)], n2 = cr[2 * (j - 2)], n3 = cr[2 * (j - 3)];
+    for (; j >= 7; j -= 4) {
+      const double c0 = n0, c1 = n1, c2_ = n2, c3 = n3;
+      n0 = cr[2 * (j - 4)];
+      n1 = cr[2 * (j - 5)];
+      n2 = cr[2 * (j - 6)];
+      n3 = cr[2 * (j - 7)];
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        double b = fma(s[k], b1[k], fma(-t[k], b2[k], c0));
+        b2[k] = fma(s[k], b, fma(-t[k], b1[k], c1));
+        b1[k] = fma(s[k], b2[k], fma(-t[k], b, c2_));
+        b2[k] = fma(s[k], b1[k], fma(-t[k], b2[k], c3));
+        // (b1, b2) now hold (b_{j-2}, b_{j-3}) in swapped roles: restore the order
+        const double u = b1[k];
+        b1[k] = b2[k];
+        b2[k] = u;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      double b = fma(s[k], b1[k], fma(-t[k], b2[k], n0));
+      b2[k] = fma(s[k], b, fma(-t[k], b1[k], n1));
+      b1[k] = fma(s[k], b2[k], fma(-t[k], b, n2));
+      b2[k] = fma(s[k], b1[k], fma(-t[k], b2[k], n3));
+      const double u = b1[k];
+      b1[k] = b2[k];
+      b2[k] = u;
+    }
+    j -= 4;
+  }
+  for (; j >= 0; --j) {
+    const double cj = cr[2 * j];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const double b = fma(s[k], b1[k], fma(-t[k], b2[k], cj));
+      b2[k] = b1[k];
+      b1[k] = b;
+    }
+  }
+  // Re w = s / 2 exactly, so w.x need not stay live through the recurrence
+#pragma unroll
+  for (int k = 0; k < P; ++k) r[k] = c2(fma(-b2[k], 0.5 * s[k], b1[k]), b2[k] * w[k].y);
+}
+
 // sum_{j<m} c_j w^j by Horner; c is wave-uniform (scalar loads).  Coefficients are fetched
 // eight at a time ahead of their multiply-adds so the scalar-load latency is paid once per chunk.
 __device__ __forceinline__ double2 horner(const double2* __restrict__ c, int m, double2 w) {

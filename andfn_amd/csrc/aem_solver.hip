@@ -344,6 +344,9 @@ class Solver {
   void switch_to_fp64_solves();
   double refine_probe();
   double far2_ = 16.0;
+  // every intersection and constant-head line starts with real coefficients (their fits and the
+  // damping keep the imaginary parts at zero), so their series are evaluated as real polynomials
+  bool real_lines_ = false;
   int precision_ = ANDFN_MI_FP64;
   // initial state of the mutable arrays, restored at the start of every run
   DBuf<double2> s_coef_;
@@ -391,10 +394,10 @@ class Solver {
   }
   DBuf<RhsItem> d_rhs_items_;
   // right-hand side as a linear map of the coefficients (rhs_basis_kernel), built once per handle
-  DBuf<double2> d_rbW_;
-  DBuf<float2> d_rbW32_;    // the basis rounded to FP32 (the mixed form's FP32 phase), made on demand
+  DBuf<double> d_rbW_;
+  DBuf<float> d_rbW32_;     // the basis rounded to FP32 (the mixed form's FP32 phase), made on demand
   bool rb32_ok_ = false;
-  DBuf<int2> d_rbCol_;      // per fracture: (source, coefficient index) of every basis column
+  DBuf<unsigned> d_rbCol_;  // per fracture: packed (source, coefficient index, part) of every basis column
   DBuf<int64_t> d_rbCoff_;  // per side: first column of its fracture's table
   DBuf<int64_t> d_rbPtr_;
   DBuf<int> d_rbNan_, d_rbMcap_;
@@ -436,6 +439,7 @@ class Solver {
     v.slots = d_slots_.p;
     v.coef32 = d_coef32_.p;
     v.far2 = far2_;
+    v.real_lines = real_lines_ ? 1 : 0;
     return v;
   }
   aem::FitView fit() const {
@@ -576,6 +580,16 @@ void Solver::parse() {
     error_old2_[i] = rd<double>(p, L_.e_err + 16);
     error_coef_[i] = rd<double>(p, L_.e_err + 24);
   }
+  // ANDFN_MI_COMPLEX_LINES set: evaluate every line series with complex coefficients (tests
+  // compare the two)
+  real_lines_ = std::getenv("ANDFN_MI_COMPLEX_LINES") == nullptr;
+  for (int i = 0; i < E_ && real_lines_; ++i)
+    if (etype_[i] == 0 || etype_[i] == 3)
+      for (int64_t j = 0; j < L_.M; ++j)
+        if (coef_[(size_t)i * L_.M + j].y != 0.0) {
+          real_lines_ = false;
+          break;
+        }
   t_.resize(F_);
   constant_.resize(F_);
   frac_id_.resize(F_);
@@ -1772,10 +1786,15 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
     if (etype_[e] != 2)
       mcap[e] = ncoef_[e] < max_coef ? (int)std::min<int64_t>(L_.M, max_coef + coef_increase - 1) : ncoef_[e];
   if (rb_ok_ && mcap == rb_mcap_) return true;
+  // a column packs (source, 2 j + part) into 32 bits
+  if ((int64_t)E_ > ((int64_t)1 << (32 - aem::kRhsColShift)) - 1 || 2 * L_.M > ((int64_t)1 << aem::kRhsColShift))
+    return false;
   const int ns = (int)rhs_.side_elem.size();
   std::vector<int64_t> ptr(ns + 1, 0), coff(std::max(ns, 1), 0);
   std::vector<int64_t> fcol(F_, -1);  // first column of each fracture's table (-1: not built yet)
-  std::vector<int2> col;
+  std::vector<unsigned> col;
+  // entries per power of a source: one for real coefficients (Re c_j), two otherwise
+  auto parts = [&](const Slot& sl) { return sl.kind == aem::KIND_LINE && real_lines_ ? 1 : 2; };
   for (int d = 0; d < ns; ++d) {
     const int e = rhs_.side_elem[d];
     const int f = rhs_.side_which[d] ? efrac1_[e] : efrac0_[e];
@@ -1783,19 +1802,23 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
       // the fracture's columns in the order rhs_basis_kernel writes a side's entries
       fcol[f] = (int64_t)col.size();
       for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-        if (slots_[k].kind != aem::KIND_WELL)
-          for (int j = 0; j < mcap[slots_[k].elem]; ++j) col.push_back(make_int2(slots_[k].elem, j));
+        if (slots_[k].kind != aem::KIND_WELL) {
+          const unsigned src = (unsigned)slots_[k].elem << aem::kRhsColShift;
+          const int np = parts(slots_[k]);
+          for (int j = 0; j < mcap[slots_[k].elem]; ++j)
+            for (int part = 0; part < np; ++part) col.push_back(src | (unsigned)(2 * j + part));
+        }
     }
     coff[d] = fcol[f];
     const int64_t fend = (int64_t)col.size();
     int64_t cnt = 0;
     for (int k = mem_ptr_[f]; k < mem_ptr_[f + 1]; ++k)
-      if (slots_[k].kind != aem::KIND_WELL) cnt += mcap[slots_[k].elem];
+      if (slots_[k].kind != aem::KIND_WELL) cnt += (int64_t)parts(slots_[k]) * mcap[slots_[k].elem];
     if (fcol[f] + cnt > fend) throw std::logic_error("basis column table shorter than a side's entries");
     ptr[d + 1] = ptr[d] + cnt;
   }
   const int64_t entries = ptr[ns];
-  if (entries * (int64_t)sizeof(double2) > (int64_t)16 << 30) {
+  if (entries * (int64_t)sizeof(double) > (int64_t)16 << 30) {
     rb_ok_ = false;
     return false;
   }
@@ -1921,7 +1944,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   double rhs_flops = set_flops(rhs_, false, &pr_tmp);
   int64_t rhs_pairs = pr_tmp;
   if (use_basis) {
-    rhs_flops = 4.0 * (double)rb_entries_;  // two multiply-adds per basis entry
+    rhs_flops = 2.0 * (double)rb_entries_;  // one multiply-add per basis entry
     rhs_pairs = rb_entries_;
   }
   if (use_basis && mixed_phase_ && !rb32_ok_) {
