@@ -110,7 +110,7 @@ __device__ __forceinline__ double2 discharge_point(const NetView& net, int e, in
 // applied to all P points (P independent arithmetic chains); each point's value is summed over
 // the sources in the same order for every P, so the result does not depend on P.
 template <int MODE, bool MIX, int P>
-__global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
   const int t = blockIdx.x;
   const Tile tile = tg.tiles[t];
   if (tile.p0 >= tile.pend) return;  // padding of the XCD layout (uniform per workgroup)
@@ -199,7 +199,8 @@ __global__ __launch_bounds__(64) void sweep_kernel(NetView net, TargetView tg, i
           const float invf = 1.0f / r2f[k];
           wf[k] = make_float2(chif[k].x * invf, -chif[k].y * invf);
         }
-        horner_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
+        if (sl.kind == KIND_LINE && net.real_lines) horner_real_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
+        else horner_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
         const float sg = (float)sl.sign;
         const float qq = (sl.kind == KIND_LINE && use_q) ? (float)(sl.sign * net.q[src] * kInv2Pi) : 0.0f;
 #pragma unroll

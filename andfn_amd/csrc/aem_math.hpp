@@ -79,6 +79,21 @@ __device__ __forceinline__ double log_pos(double x) {
   return fma(de, ln2_hi, fma(2.0, s, t));
 }
 
+// sqrt(x) for x > 0 (normal) as sqrt_pos, with h = 1 / (2 sqrt x) from the same iteration
+__device__ __forceinline__ double sqrt_half_rcp_pos(double x, double& h) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y;
+  h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
 // principal square root (branch cut on the negative real axis)
 __device__ __forceinline__ double2 csqrt_principal(double x, double y) {
   const double r2 = fma(x, x, y * y);
@@ -90,14 +105,21 @@ __device__ __forceinline__ double2 csqrt_principal(double x, double y) {
   return c2(fabs(y) * h, copysign(t, y));
 }
 
-// exterior Joukowsky inverse of Z (|chi| >= 1)
+// exterior Joukowsky inverse of Z (|chi| >= 1): chi = Z + s, s^2 = u = Z^2 - 1, the root with
+// Re(conj(Z) s) >= 0.  The principal root of u is (t, uy / (2 t)) for ux >= 0 and
+// (|uy| / (2 t), sign(uy) t) otherwise, t = sqrt((|u| + |ux|) / 2); 1/(2 t) comes out of the
+// square-root iteration itself.
 __device__ __forceinline__ double2 joukowsky_exterior(double2 Z) {
-  double ux = fma(Z.x - 1.0, Z.x + 1.0, -Z.y * Z.y);
-  double uy = 2.0 * Z.x * Z.y;
-  double2 s = csqrt_principal(ux, uy);
-  if (fma(Z.x, s.x, Z.y * s.y) < 0.0) {
-    s.x = -s.x;
-    s.y = -s.y;
+  const double ux = fma(-Z.y, Z.y, fma(Z.x, Z.x, -1.0));
+  const double uy = 2.0 * Z.x * Z.y;
+  const double r2 = fma(ux, ux, uy * uy);
+  double2 s = c2(0.0, 0.0);
+  if (r2 != 0.0) {
+    double h;
+    const double t = sqrt_half_rcp_pos(0.5 * (sqrt_pos(r2) + fabs(ux)), h);
+    const double v = uy * h;
+    s = ux >= 0.0 ? c2(t, v) : c2(fabs(v), copysign(t, uy));
+    if (fma(Z.x, s.x, Z.y * s.y) < 0.0) s = c2(-s.x, -s.y);
   }
   return c2(Z.x + s.x, Z.y + s.y);
 }
@@ -274,6 +296,32 @@ __device__ __forceinline__ void horner_f_p(const float2* __restrict__ c, int m, 
 #pragma unroll
     for (int k = 0; k < P; ++k) r[k] = cmaddf(r[k], w[k], cj);
   }
+}
+
+// FP32 form of horner_real_p (real coefficients: the second-order recurrence)
+template <int P>
+__device__ __forceinline__ void horner_real_f_p(const float2* __restrict__ c, int m, const float2 (&w)[P],
+                                                float2 (&r)[P]) {
+  const float* __restrict__ cr = reinterpret_cast<const float*>(c);  // Re c_j at cr[2 j]
+  float s[P], t[P], b1[P], b2[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    s[k] = 2.0f * w[k].x;
+    t[k] = fmaf(w[k].x, w[k].x, w[k].y * w[k].y);
+    b1[k] = cr[2 * (m - 1)];
+    b2[k] = 0.0f;
+  }
+  for (int j = m - 2; j >= 0; --j) {
+    const float cj = cr[2 * j];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const float b = fmaf(s[k], b1[k], fmaf(-t[k], b2[k], cj));
+      b2[k] = b1[k];
+      b1[k] = b;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < P; ++k) r[k] = make_float2(fmaf(-b2[k], 0.5f * s[k], b1[k]), b2[k] * w[k].y);
 }
 
 // FP32 Horner, coefficients fetched eight at a time (wave-uniform scalar loads)
