@@ -94,6 +94,8 @@ class Info(ctypes.Structure):
         ("factor_rounds", ctypes.c_int32),
         ("solve_passes", ctypes.c_int64),
         ("solve_values", ctypes.c_int64),
+        ("runahead_iterations", ctypes.c_int64),
+        ("graph_iterations", ctypes.c_int64),
     ]
 
     def as_dict(self):

@@ -114,6 +114,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sw
   const int t = blockIdx.x;
   const Tile tile = tg.tiles[t];
   if (tile.p0 >= tile.pend) return;  // padding of the XCD layout (uniform per workgroup)
+  if (stopped(net)) return;
   const int lane = threadIdx.x;
   const int f = tile.frac;
 
@@ -351,6 +352,7 @@ void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, 
 __global__ __launch_bounds__(64) void unwrap_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
                                                     double* __restrict__ val, const int* __restrict__ pt_off) {
   __shared__ double sdpsi[kMaxPts];
+  if (stopped(net)) return;
   const FitItem it = items[blockIdx.x];
   const int e = it.elem;
   const int ty = net.etype[e];
@@ -450,6 +452,7 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
 
 __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
                                                  const double* __restrict__ dft, double damping, int blend) {
+  if (stopped(net)) return;
   const FitItem it = items[blockIdx.x];
   const int e = it.elem;
   const int lane = threadIdx.x;
@@ -523,7 +526,7 @@ __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const 
 // second damping step for every element (hpc_solve.py:561-575); also the only update of wells
 __global__ void blend_kernel(NetView net, int ne, double damping) {
   const int e = blockIdx.x;
-  if (e >= ne) return;
+  if (e >= ne || stopped(net)) return;
   const int m = net.ncoef[e];
   double2* cf = net.coef + (size_t)e * net.M;
   const double2* wc = net.wcoef + (size_t)e * net.M;
@@ -536,7 +539,7 @@ __global__ void blend_kernel(NetView net, int ne, double damping) {
 // wells: error = 0 and the first damping step (their work coefficients stay zero)
 __global__ void well_kernel(NetView net, FitView fv, const int* __restrict__ wells, int nw, double damping) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nw) return;
+  if (k >= nw || stopped(net)) return;
   const int e = wells[k];
   fv.error[e] = 0.0;
   const int m = net.ncoef[e];
@@ -676,7 +679,7 @@ __global__ __launch_bounds__(64) void rhs_reduce_kernel(NetView net, const RhsIt
                                                         int K, const int* __restrict__ de_pos,
                                                         double* __restrict__ b) {
   const int j = blockIdx.x;
-  if (j >= nde) return;
+  if (j >= nde || stopped(net)) return;
   const RhsItem it = items[j];
   const int lane = threadIdx.x;
   double s0 = 0.0, s1 = 0.0;
@@ -834,7 +837,7 @@ __global__ __launch_bounds__(64) void rhs_eval_kernel(NetView net, const RhsItem
                                                       const int* __restrict__ side_nan, const int* __restrict__ de_pos,
                                                       double* __restrict__ b) {
   const int j = blockIdx.x;
-  if (j >= nde) return;
+  if (j >= nde || stopped(net)) return;
   const RhsItem it = items[j];
   const double om0 = rhs_side_mean(net, it.s0, wptr, W, col, coff, side_nan);
   const double om1 = it.s1 >= 0 ? rhs_side_mean(net, it.s1, wptr, W, col, coff, side_nan) : 0.0;
@@ -993,6 +996,7 @@ __global__ void scatter_kernel(NetView net, const int* __restrict__ de, int nde,
 __global__ void accept_kernel(NetView net, const int* __restrict__ de, int nde, int nf, SysLayout sys,
                               const double* __restrict__ x, double* __restrict__ y, double* __restrict__ yold,
                               int64_t n) {
+  if (stopped(net)) return;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) {
     yold[k] = y[k];
@@ -1050,6 +1054,7 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(FitView fv, int ne,
                                                              IterScalars* __restrict__ out) {
   __shared__ double se[kRedThreads], sc[kRedThreads], sd[kRedThreads], so[kRedThreads];
   __shared__ int si[kRedThreads];
+  if (stopped(fv)) return;  // the same for every workgroup of the launch
   const int tid = threadIdx.x, g = blockIdx.x, G = gridDim.x;
   double em = -__builtin_inf(), cm = -__builtin_inf(), dm = 0.0, om = 0.0;
   int ie = 0x7fffffff, dummy = 0;
@@ -1154,7 +1159,7 @@ void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles
 // single-precision copy of every element's coefficients (mixed-precision far field)
 __global__ void coef_to_f32_kernel(NetView net, int ne, float2* __restrict__ out) {
   const int e = blockIdx.x;
-  if (e >= ne) return;
+  if (e >= ne || stopped(net)) return;
   const int m = net.ncoef[e];
   const double2* c = net.coef + (size_t)e * net.M;
   float2* o = out + (size_t)e * net.M;
@@ -1561,6 +1566,54 @@ void launch_scatter(const NetView& net, const int* de, int nde, int nf, const Sy
   int tot = nde + nf;
   if (tot <= 0) return;
   hipLaunchKernelGGL(scatter_kernel, dim3((tot + 255) / 256), dim3(256), 0, st, net, de, nde, nf, sys, y);
+}
+
+// run-ahead iteration: the residual of iteration it is recorded in pinned host memory; with
+// may_stop (the FP64 phase) a residual <= tol stops the run, and every later state kernel of the
+// stream returns at once
+__global__ void iter_decide_kernel(const IterScalars* __restrict__ sc, IterCtl* __restrict__ ctl, int may_stop,
+                                   double tol, IterHostLog log) {
+  if (threadIdx.x != 0 || ctl->stop) return;
+  const int it = ctl->it + 1;
+  ctl->it = it;
+  const double r = sc->residual;
+  if (it < log.cap) {
+    log.resid[it] = r;
+    log.arg[it] = sc->arg_residual;
+  }
+  if (may_stop && r <= tol) {
+    ctl->stop = 1;
+    *log.stop_it = it;
+  }
+  if (log.stamps && it < log.cap) log.stamps[6 * (size_t)it + 1] = __builtin_amdgcn_s_memrealtime();
+  __threadfence_system();
+}
+
+__global__ void iter_stamp_kernel(const IterCtl* __restrict__ ctl, unsigned long long* __restrict__ stamps, int cap,
+                                  int slot, int next) {
+  if (threadIdx.x != 0 || ctl->stop) return;
+  const int it = ctl->it + next;
+  if (it < cap) stamps[6 * (size_t)it + slot] = __builtin_amdgcn_s_memrealtime();
+}
+
+void launch_iter_stamp(const IterCtl* ctl, unsigned long long* stamps, int cap, int slot, int next, hipStream_t st) {
+  hipLaunchKernelGGL(iter_stamp_kernel, dim3(1), dim3(64), 0, st, ctl, stamps, cap, slot, next);
+}
+
+__global__ void iter_reset_kernel(IterCtl* __restrict__ ctl, int it) {
+  if (threadIdx.x == 0) {
+    ctl->stop = 0;
+    ctl->it = it;
+  }
+}
+
+void launch_iter_decide(const IterScalars* sc, IterCtl* ctl, int may_stop, double tol, IterHostLog log,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(iter_decide_kernel, dim3(1), dim3(64), 0, st, sc, ctl, may_stop, tol, log);
+}
+
+void launch_iter_reset(IterCtl* ctl, int it, hipStream_t st) {
+  hipLaunchKernelGGL(iter_reset_kernel, dim3(1), dim3(64), 0, st, ctl, it);
 }
 
 void launch_reduce(const FitView& fv, int ne, const double* y, const double* yold, int n, void* scratch,

@@ -54,6 +54,7 @@ struct NetView {
   const float2* coef32;   // [E][M] single-precision copy (mixed-precision far field)
   double far2;            // mixed precision: a line source is evaluated in FP32 where |Z|^2 >= far2 on every lane
   int real_lines;         // 1: every KIND_LINE source (intersection, constant-head line) has real coefficients
+  const int* stop;        // device flag of the run-ahead iteration (null outside it): set, the state kernels return
   const double* t;
   const double* constant;
   double* constant_w;
@@ -73,6 +74,30 @@ struct FitView {
   int2* cross;             // (edge position, crossing element)
   int* cross_sign;
   int* status;             // bit 0: a crossing list overflowed
+  const int* stop;         // as NetView::stop
+};
+
+// a kernel of the run-ahead iteration returns at once when its stop flag is set (the head
+// residual met the tolerance earlier in the stream), so iterations enqueued past the stop leave
+// the state unchanged
+template <class V>
+__device__ __forceinline__ bool stopped(const V& v) {
+  return v.stop != nullptr && *(volatile const int*)v.stop != 0;
+}
+
+// control block of the run-ahead iteration (device memory)
+struct IterCtl {
+  int stop;  // 1 once an iteration's head residual met the tolerance
+  int it;    // iterations whose residual has been decided
+};
+// per iteration, written by the device into pinned host memory: residual and its row, and the
+// iteration that stopped the run (0: none yet)
+struct IterHostLog {
+  double* resid;
+  int* arg;
+  int* stop_it;
+  int cap;
+  unsigned long long* stamps;  // device memory, [cap][6] (launch_iter_stamp)
 };
 
 // Unknowns of the discharge system: nblk independent blocks (connected components of the
@@ -197,6 +222,15 @@ struct IterScalars {
   int argmax;
   int arg_residual;
 };
+
+// after the head residual of an iteration: record it, stop when may_stop and it is <= tol
+void launch_iter_decide(const IterScalars* sc, IterCtl* ctl, int may_stop, double tol, IterHostLog log,
+                        hipStream_t st);
+void launch_iter_reset(IterCtl* ctl, int it, hipStream_t st);
+// device wall clock (constant rate) of the iteration's phase boundaries: stamps[6 it + slot],
+// slot 0 iteration start, 1 head residual decided (written by iter_decide_kernel), 2 solve
+// accepted, 3 sweep done, 4 fits and wells done; next = 1 before the decision of the iteration
+void launch_iter_stamp(const IterCtl* ctl, unsigned long long* stamps, int cap, int slot, int next, hipStream_t st);
 
 // points_per_lane 1 or 2: the target view's tiles must hold 64 x points_per_lane points each
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,

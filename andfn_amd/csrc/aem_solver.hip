@@ -221,6 +221,12 @@ class Solver {
         if (e) (void)hipEventDestroy(e);
     if (res_ev_) (void)hipEventDestroy(res_ev_);
     if (h_scal_) (void)hipHostFree(h_scal_);
+    drop_iter_graphs();
+    for (auto& e : it_end_ev_)
+      if (e) (void)hipEventDestroy(e);
+    if (h_it_resid_) (void)hipHostFree(h_it_resid_);
+    if (h_it_arg_) (void)hipHostFree(h_it_arg_);
+    if (h_it_stop_) (void)hipHostFree(h_it_stop_);
     for (auto& e : lu_ev_)
       if (e) (void)hipEventDestroy(e);
     if (st3_) (void)hipStreamDestroy(st3_);
@@ -440,6 +446,7 @@ class Solver {
     v.coef32 = d_coef32_.p;
     v.far2 = far2_;
     v.real_lines = real_lines_ ? 1 : 0;
+    v.stop = nullptr;
     return v;
   }
   aem::FitView fit() const {
@@ -455,6 +462,7 @@ class Solver {
     v.cross = d_cross_.p;
     v.cross_sign = d_cross_sign_.p;
     v.status = d_status_.p;
+    v.stop = nullptr;
     return v;
   }
 
@@ -482,6 +490,7 @@ class Solver {
   template <class Fn>
   void run_captured(CapturedSolve& c, Fn&& fn);
   void drop_graph() {
+    drop_iter_graphs();
     for (CapturedSolve* c : {&cap_solve_, &cap_correct_}) {
       if (c->e) (void)hipGraphExecDestroy(c->e);
       if (c->g) (void)hipGraphDestroy(c->g);
@@ -491,6 +500,49 @@ class Solver {
   double head_residual(const double* b, int* row);
   void head_residual_async(const double* b);
   double bc_residual();
+
+  // ---- run-ahead iterations: with the head tolerance as the stop rule and no coefficient growth,
+  // the iterations after the first are enqueued one ahead of the host's read of the head
+  // residual; the stop is decided on the device (iter_decide_kernel), and the state kernels of an
+  // iteration enqueued past the stop return at once.  Each iteration is one HIP graph launch,
+  // captured per (FP32 phase, FP32-factor solves, iteration mod 3) on its second use; three
+  // copies, so the events of the iteration whose times the host reads are not re-recorded by the
+  // one enqueued ahead.
+  DBuf<aem::IterCtl> d_ctl_;
+  double* h_it_resid_ = nullptr;  // pinned, written by the device: residual of iteration k at [k]
+  int* h_it_arg_ = nullptr;
+  int* h_it_stop_ = nullptr;      // iteration whose residual stopped the run (0: none)
+  aem::IterHostLog it_log_dev_{};  // the same three arrays as the device addresses them
+  int it_cap_ = 0;
+  struct IterGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    int uses = 0;
+    bool failed = false;
+  };
+  IterGraph it_graphs_[2][2][3];  // [FP32 phase][FP32-factor solves][iteration mod 3]
+  // per iteration mod 3: recorded after the iteration's launch, outside any graph (the host waits
+  // on it; phase times come from the device's wall-clock stamps, launch_iter_stamp)
+  hipEvent_t it_end_ev_[3] = {};
+  DBuf<unsigned long long> d_stamps_;
+  double wall_khz_ = 0.0;
+  int64_t it_graph_launches_ = 0;  // iterations run as graph launches in the last run
+  void drop_iter_graphs() {
+    for (auto& a : it_graphs_)
+      for (auto& b : a)
+        for (IterGraph& g : b) {
+          if (g.e) (void)hipGraphExecDestroy(g.e);
+          if (g.g) (void)hipGraphDestroy(g.g);
+          g = IterGraph{};
+        }
+  }
+  void ensure_iter_log(int cap);
+  void enqueue_iteration(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                         double head_tol, double damping);
+  void launch_iteration(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                        double head_tol, double damping);
+  void launch_iteration_body(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                             double head_tol, double damping);
 
   // tile shape of the collocation and right-hand-side sweeps (points per lane, 1 or 2), picked on
   // the first sweep of each kind by timing both shapes on the same inputs (same results)
@@ -1571,6 +1623,7 @@ double Solver::refine_probe() {
 // stopped contracting).  Builds the FP64 blocked operators on first use.
 void Solver::switch_to_fp64_solves() {
   if (!solve32_) return;
+  HIP_CHECK(hipStreamSynchronize(st_));  // no graph of this handle is in flight when they are dropped
   if (!sparse_ && !a64_blocked_) {
     prepare_blocked_all<double>(d_A_.p, d_linv_, d_uinv_);
     d_w_.alloc(N_);
@@ -1838,6 +1891,115 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
   return true;
 }
 
+void Solver::ensure_iter_log(int cap) {
+  if (cap <= it_cap_) return;
+  if (h_it_resid_) HIP_CHECK(hipHostFree(h_it_resid_));
+  if (h_it_arg_) HIP_CHECK(hipHostFree(h_it_arg_));
+  h_it_resid_ = nullptr;
+  h_it_arg_ = nullptr;
+  HIP_CHECK(hipHostMalloc((void**)&h_it_resid_, sizeof(double) * cap, hipHostMallocCoherent | hipHostMallocMapped));
+  HIP_CHECK(hipHostMalloc((void**)&h_it_arg_, sizeof(int) * cap, hipHostMallocCoherent | hipHostMallocMapped));
+  if (!h_it_stop_)
+    HIP_CHECK(hipHostMalloc((void**)&h_it_stop_, sizeof(int), hipHostMallocCoherent | hipHostMallocMapped));
+  it_cap_ = cap;
+  void *dr = nullptr, *da = nullptr, *ds = nullptr;
+  HIP_CHECK(hipHostGetDevicePointer(&dr, h_it_resid_, 0));
+  HIP_CHECK(hipHostGetDevicePointer(&da, h_it_arg_, 0));
+  HIP_CHECK(hipHostGetDevicePointer(&ds, h_it_stop_, 0));
+  it_log_dev_ = aem::IterHostLog{(double*)dr, (int*)da, (int*)ds, cap, nullptr};
+  drop_iter_graphs();  // captured graphs hold the previous addresses
+  if (!it_end_ev_[0]) {
+    for (auto& e : it_end_ev_) HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    int khz = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev_));
+    wall_khz_ = khz > 0 ? (double)khz : 100000.0;
+  }
+  d_ctl_.alloc(1);
+  d_stamps_.alloc(6 * (size_t)cap);
+  it_log_dev_.stamps = d_stamps_.p;
+}
+
+// One iteration after the first (hpc_solve.py:152-239 with the head-residual stop): right-hand side
+// of the current coefficients, head residual of the current discharges and the device's stop
+// decision, the correction solve and its acceptance, then the sweep, fits, damping and the
+// iteration scalars.  The same launches in the same order as the host loop below.
+void Solver::enqueue_iteration(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                               double head_tol, double damping) {
+  const int n = (int)N_;
+  const int nde = (int)de_.size();
+  (void)par;
+  unsigned long long* stamps = d_stamps_.p;
+  aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 0, 1, st_);
+  if (phase32) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
+  aem::launch_rhs_eval(nv, d_rhs_items_.p, nde, d_rbPtr_.p, d_rbW_.p, phase32 ? d_rbW32_.p : nullptr, d_rbCol_.p,
+                       d_rbCoff_.p, d_rbNan_.p, d_de_pos_.p, d_b_.p, st_);
+  aem::launch_spmv(n, d_csr_ptr_.p, d_csr_col_.p, d_csr_val_.p, d_y_.p, d_ay_.p, st_);
+  aem::launch_residual(d_ay_.p, d_b_.p, d_rowscale_.p, n, d_red_.p, d_scal_.p, st_);
+  aem::launch_iter_decide(d_scal_.p, d_ctl_.p, phase32 ? 0 : 1, head_tol, it_log_dev_, st_);
+  mf_.solve_correction(d_b_.p, d_ay_.p, d_y_.p, d_req_.p, d_x_.p, st_, s32);
+  aem::launch_accept(nv, d_de_.p, nde, F_, sys(), d_x_.p, d_y_.p, d_yold_.p, n, st_);
+  aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 2, 0, st_);
+  const int ppl = ppl_[0] ? ppl_[0] : 1;
+  aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(ppl), full_.ntiles(ppl), K_, d_val_.p, st_, phase32, ppl);
+  aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 3, 0, st_);
+  aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_citems_.p, ncitems_, d_val_.p, full_.d_pt_off.p,
+                  full_.d_dft_tiles.p, (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, 1, st_,
+                  phase32);
+  aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_);
+  aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 4, 0, st_);
+  aem::launch_blend(nv, E_, damping, st_);
+  aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_red_.p, d_scal_.p, st_);
+}
+
+// enqueue_iteration as a graph launch: the first use of a configuration runs its launches
+// directly (any lazy set-up happens outside a capture), the second captures and instantiates
+// them; a configuration whose capture fails keeps running directly
+void Solver::launch_iteration(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                              double head_tol, double damping) {
+  launch_iteration_body(nv, fv, phase32, s32, par, head_tol, damping);
+  HIP_CHECK(hipEventRecord(it_end_ev_[par], st_));
+}
+
+void Solver::launch_iteration_body(const aem::NetView& nv, const aem::FitView& fv, bool phase32, bool s32, int par,
+                                   double head_tol, double damping) {
+  const bool no_graph = std::getenv("ANDFN_MI_NO_GRAPH") != nullptr;
+  IterGraph& g = it_graphs_[phase32][s32][par];
+  if (g.e) {
+    HIP_CHECK(hipGraphLaunch(g.e, st_));
+    ++it_graph_launches_;
+    return;
+  }
+  if (no_graph || g.failed || g.uses++ == 0) {
+    enqueue_iteration(nv, fv, phase32, s32, par, head_tol, damping);
+    return;
+  }
+  bool ok = hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+  if (ok) {
+    try {
+      enqueue_iteration(nv, fv, phase32, s32, par, head_tol, damping);
+    } catch (...) {
+      ok = false;  // nothing captured ran; the launches are made directly below
+    }
+    hipGraph_t gr = nullptr;
+    ok = hipStreamEndCapture(st_, &gr) == hipSuccess && gr && ok;
+    if (ok && hipGraphInstantiate(&g.e, gr, nullptr, nullptr, 0) == hipSuccess) {
+      g.g = gr;
+    } else {
+      if (gr) (void)hipGraphDestroy(gr);
+      g.e = nullptr;
+      ok = false;
+    }
+  }
+  (void)hipGetLastError();
+  if (ok) {
+    HIP_CHECK(hipGraphLaunch(g.e, st_));
+    ++it_graph_launches_;
+  } else {
+    g.failed = true;
+    enqueue_iteration(nv, fv, phase32, s32, par, head_tol, damping);
+  }
+}
+
 int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
@@ -2026,7 +2188,136 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     HIP_CHECK(hipMemcpyAsync(d_y_.p, d_yold_.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st_));
     aem::launch_scatter(nv, d_de_.p, nde, F_, sys(), d_y_.p, st_);
   };
+  // run-ahead iterations (see enqueue_iteration) after the first: head tolerance as the stop rule,
+  // no per-iteration log, schedule or record, the basis right-hand side, the multifrontal
+  // factors, and no element that can still grow
+  bool runahead = lazy && log_n == 0 && !sched && !record && use_basis && sparse_ && lu_mode_ == ANDFN_MI_LU_TRSV &&
+                  std::getenv("ANDFN_MI_NO_RUNAHEAD") == nullptr && max_iterations >= 2;
+  for (int e = 0; e < E_ && runahead; ++e)
+    if (etype_[e] != 2 && ncoef_[e] < max_coef) runahead = false;
+  it_graph_launches_ = 0;
+  int64_t runahead_iterations = 0;
   while (cnt_error < 2 && nit < max_iterations) {
+    if (runahead && nit >= 1) {
+      // iterations 2 .. : the host keeps one iteration enqueued ahead of the one whose head
+      // residual it waits for; the device stops the run (iter_decide_kernel)
+      ensure_iter_log(max_iterations + 2);
+      *h_it_stop_ = 0;
+      aem::launch_iter_reset(d_ctl_.p, nit, st_);
+      aem::NetView nvg = nv;
+      aem::FitView fvg = fv;
+      nvg.stop = &d_ctl_.p->stop;
+      fvg.stop = nvg.stop;
+      std::vector<char> ph32(max_iterations + 2, 0), sv32(max_iterations + 2, 0);
+      int launched = nit, done = nit;
+      auto launch_next = [&]() {
+        const int k = ++launched;
+        ph32[k] = mixed_phase_;
+        sv32[k] = solve32_;
+        launch_iteration(nvg, fvg, mixed_phase_, solve32_, k % 3, head_tol, damping);
+      };
+      launch_next();
+      int stopped_at = 0;
+      while (done < launched) {
+        const int k = done + 1;
+        if (launched == k && k < max_iterations) launch_next();
+        HIP_CHECK(hipEventSynchronize(it_end_ev_[k % 3]));
+        HIP_CHECK(hipGetLastError());
+        if (times_pending) {
+          add_times(times_pending);  // iteration 1 (the host loop's events)
+          times_pending = 0;
+        }
+        const int st_it = *(volatile int*)h_it_stop_;
+        if (st_it > 0) {
+          // the device stopped at iteration st_it <= k; later iterations left the state alone
+          stopped_at = st_it;
+          resid = h_it_resid_[st_it];
+          resid_row = h_it_arg_[st_it];
+          break;
+        }
+        resid = h_it_resid_[k];
+        resid_row = h_it_arg_[k];
+        done = k;
+        if (solve32_) {
+          // as in the host loop: a residual 4x above the best so far (or not finite) hands the
+          // solves to the FP64 factors; the iteration enqueued ahead still used the FP32 ones
+          if (!(resid == resid) || !std::isfinite(resid) || (k > 3 && resid > 4.0 * best_resid)) {
+            if (verbose)
+              fprintf(stderr, "mixed form: head residual %.3e at iteration %d (best %.3e); FP64 factors from here\n",
+                      resid, k, best_resid);
+            switch_to_fp64_solves();
+            mixed_phase_ = false;
+          }
+          if (resid == resid) best_resid = std::min(best_resid, resid);
+        }
+        if (mixed_phase_) {
+          rhist.push_back(resid);
+          const size_t h = rhist.size();
+          bool stalled = false;
+          if (h >= 20) {
+            const double last = *std::min_element(rhist.end() - 10, rhist.end());
+            const double prev = *std::min_element(rhist.end() - 20, rhist.end() - 10);
+            stalled = last > 0.7 * prev;
+          }
+          if (resid <= 30.0 * head_tol || stalled) mixed_phase_ = false;
+        }
+      }
+      HIP_CHECK(hipStreamSynchronize(st_));
+      HIP_CHECK(hipGetLastError());
+      if (times_pending) {
+        add_times(times_pending);
+        times_pending = 0;
+      }
+      nit = stopped_at ? stopped_at - 1 : done;
+      {
+        // work of iterations 2 .. nit (the same right-hand side, solve and sweep every iteration)
+        int64_t pr = 0;
+        const double fs = set_flops(full_, true, &pr);
+        for (int k = 2; k <= nit; ++k) {
+          fl_rhs += rhs_flops;
+          pr_rhs += rhs_pairs;
+          ++n_rhs;
+          ++solve_passes;
+          fl_sweep += fs;
+          pr_sweep += pr;
+          ++n_sweep;
+          if (sv32[k]) ++solves32_;
+        }
+      }
+      if (nit >= 2) {
+        // phase times of iterations 2 .. nit from the device's wall-clock stamps
+        std::vector<unsigned long long> stv(6 * (size_t)(nit + 1));
+        HIP_CHECK(hipMemcpy(stv.data(), d_stamps_.p, sizeof(unsigned long long) * stv.size(), hipMemcpyDeviceToHost));
+        const double ms = 1.0 / wall_khz_;
+        for (int k = 2; k <= nit; ++k) {
+          const unsigned long long* t = stv.data() + 6 * (size_t)k;
+          ms_rhs += (double)(t[1] - t[0]) * ms;
+          ms_lu += (double)(t[2] - t[1]) * ms;
+          ms_sweep += (double)(t[3] - t[2]) * ms;
+          ms_fit += (double)(t[4] - t[3]) * ms;
+        }
+      }
+      nit_fp32 = precision_ == ANDFN_MI_MIXED ? 1 : 0;  // iteration 1 runs in the FP32 phase
+      for (int k = 2; k <= nit; ++k) nit_fp32 += ph32[k] ? 1 : 0;
+      runahead_iterations = nit - 1;
+      HIP_CHECK(hipMemcpyAsync(&sc, d_scal_.p, sizeof(sc), hipMemcpyDeviceToHost, st_));
+      HIP_CHECK(hipStreamSynchronize(st_));
+      error = sc.error;
+      argmax = sc.argmax;
+      error_coef = sc.error_coef;
+      error_q = sc.dq_max / (sc.yold_max + 1e-16);
+      if (!stopped_at && nit == max_iterations) {
+        sweep_full(true);
+        HIP_CHECK(hipStreamSynchronize(st_));
+        float a = 0, b = 0;
+        HIP_CHECK(hipEventElapsedTime(&a, ev_[0], ev_[1]));
+        HIP_CHECK(hipEventElapsedTime(&b, ev_[1], ev_[2]));
+        ms_sweep += a;
+        ms_fit += b;
+      }
+      if (!stopped_at) resid = -1.0;  // not stopped on the head residual: recomputed below
+      break;
+    }
     ++nit;
     // ---- discharge solve (hpc_solve.py:155-174, 594-663)
     if (mixed_phase_) aem::launch_coef_to_f32(nv, E_, d_coef32_.p, st_);
@@ -2291,6 +2582,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->factor_rounds = sparse_ ? mf_rounds_ : 0;
     info->solve_passes = solve_passes;
     info->solve_values = sparse_ ? mf_.symbolic().gv_total : nnz_factors_;
+    info->runahead_iterations = runahead_iterations;
+    info->graph_iterations = it_graph_launches_;
   }
   mixed_phase_ = false;
   have_state_ = true;

@@ -134,6 +134,9 @@ typedef struct andfn_mi_info {
                            * refined solve makes two, a correction solve x = y + A^-1 (b - A y) one) */
   int64_t solve_values;   /* values one pass streams (multifrontal: the packed panel tiles, FP32 in the
                            * mixed form's correction solves; dense: the n^2 entries of the blocks) */
+  int64_t runahead_iterations;  /* iterations enqueued one ahead of the host's read of the head residual,
+                                 * stopped on the device (head tolerance, no coefficient growth) */
+  int64_t graph_iterations;     /* of those, iterations run as one HIP graph launch each */
 } andfn_mi_info;
 
 /*
