@@ -96,6 +96,8 @@ class Info(ctypes.Structure):
         ("solve_values", ctypes.c_int64),
         ("runahead_iterations", ctypes.c_int64),
         ("graph_iterations", ctypes.c_int64),
+        ("sweep_persistent", ctypes.c_int32),
+        ("rhs_persistent", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -110,11 +110,10 @@ __device__ __forceinline__ double2 discharge_point(const NetView& net, int e, in
 // applied to all P points (P independent arithmetic chains); each point's value is summed over
 // the sources in the same order for every P, so the result does not depend on P.
 template <int MODE, bool MIX, int P>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sweep_kernel(NetView net, TargetView tg, int K, double* __restrict__ val) {
-  const int t = blockIdx.x;
+__device__ __forceinline__ void sweep_tile(const NetView net, const TargetView tg, int K, double* __restrict__ val,
+                                           int t) {
   const Tile tile = tg.tiles[t];
   if (tile.p0 >= tile.pend) return;  // padding of the XCD layout (uniform per workgroup)
-  if (stopped(net)) return;
   const int lane = threadIdx.x;
   const int f = tile.frac;
 
@@ -316,6 +315,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sw
         val[p] = (ore[k] + net.constant[f]) / net.t[f];
       else
         val[p] = ore[k];
+    }
+  }
+}
+
+template <int MODE, bool MIX, int P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void sweep_kernel(NetView net, TargetView tg, int K,
+                                                                                           double* __restrict__ val) {
+  if (stopped(net)) return;
+  sweep_tile<MODE, MIX, P>(net, tg, K, val, blockIdx.x);
+}
+
+// XCD (accelerator complex die) of the executing wavefront: hardware register XCC_ID, bits 3:0
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 7; }
+
+// Persistent form of the sweep: a grid sized to the resident wavefronts, each pulling tiles from
+// its own XCD's list first (the host's XCD layout puts every fracture's tiles in one list, tile x
+// + 8 t of list x, heaviest first), then from the other lists, so a fracture's coefficients stay
+// in one L2 and the uneven fracture sizes balance across the dies.  queue[0..7] are the lists'
+// heads, queue[8] counts finished workgroups; the last one resets all nine for the next launch.
+template <int MODE, bool MIX, int P>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void sweep_persistent_kernel(
+    NetView net, TargetView tg, int K, double* __restrict__ val, int per_xcd, unsigned* __restrict__ queue) {
+  if (stopped(net)) return;  // uniform: no workgroup touches the queue
+  const int home = xcc_id();
+  for (int d = 0; d < 8; ++d) {
+    const int x = (home + d) & 7;
+    for (;;) {
+      unsigned t = 0;
+      if (threadIdx.x == 0) t = atomicAdd(&queue[x], 1u);
+      t = (unsigned)__builtin_amdgcn_readfirstlane((int)t);
+      if (t >= (unsigned)per_xcd) break;
+      sweep_tile<MODE, MIX, P>(net, tg, K, val, x + 8 * (int)t);
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&queue[8], 1u) == gridDim.x - 1) {
+      for (int x = 0; x < 9; ++x) queue[x] = 0u;
+      __threadfence();
     }
   }
 }
@@ -1147,6 +1185,46 @@ static void launch_sweep_p(int mode, const NetView& net, const TargetView& tg, i
     hipLaunchKernelGGL((sweep_kernel<MODE_POINTS_OMEGA, false, P>), g, b, 0, st, net, tg, K, val);
   else
     hipLaunchKernelGGL((sweep_kernel<MODE_POINTS, false, P>), g, b, 0, st, net, tg, K, val);
+}
+
+template <int P>
+static void launch_sweep_persistent_p(int mode, const NetView& net, const TargetView& tg, int ntiles, int K,
+                                      double* val, hipStream_t st, bool mixed, int nwg, unsigned* queue) {
+  const dim3 g(nwg), b(64);
+  const int per = ntiles / 8;
+  if (mode == MODE_COLLOC && mixed)
+    hipLaunchKernelGGL((sweep_persistent_kernel<MODE_COLLOC, true, P>), g, b, 0, st, net, tg, K, val, per, queue);
+  else if (mode == MODE_COLLOC)
+    hipLaunchKernelGGL((sweep_persistent_kernel<MODE_COLLOC, false, P>), g, b, 0, st, net, tg, K, val, per, queue);
+  else if (mode == MODE_RHS && mixed)
+    hipLaunchKernelGGL((sweep_persistent_kernel<MODE_RHS, true, P>), g, b, 0, st, net, tg, K, val, per, queue);
+  else
+    hipLaunchKernelGGL((sweep_persistent_kernel<MODE_RHS, false, P>), g, b, 0, st, net, tg, K, val, per, queue);
+}
+
+template <int MODE, bool MIX, int P>
+static int resident_groups(int cus) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, sweep_persistent_kernel<MODE, MIX, P>, 64, 0) != hipSuccess)
+    nb = 0;
+  return nb * cus;
+}
+
+int sweep_persistent_groups(int mode, bool mixed, int points_per_lane, int cus) {
+  if (points_per_lane == 2) {
+    if (mode == MODE_COLLOC) return mixed ? resident_groups<MODE_COLLOC, true, 2>(cus) : resident_groups<MODE_COLLOC, false, 2>(cus);
+    return mixed ? resident_groups<MODE_RHS, true, 2>(cus) : resident_groups<MODE_RHS, false, 2>(cus);
+  }
+  if (mode == MODE_COLLOC) return mixed ? resident_groups<MODE_COLLOC, true, 1>(cus) : resident_groups<MODE_COLLOC, false, 1>(cus);
+  return mixed ? resident_groups<MODE_RHS, true, 1>(cus) : resident_groups<MODE_RHS, false, 1>(cus);
+}
+
+void launch_sweep_persistent(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val,
+                             hipStream_t st, bool mixed, int points_per_lane, int nwg, unsigned* queue) {
+  if (ntiles <= 0 || nwg <= 0) return;
+  if (ntiles % 8 != 0) return;  // the XCD layout always pads to whole rows of eight
+  if (points_per_lane == 2) launch_sweep_persistent_p<2>(mode, net, tg, ntiles, K, val, st, mixed, nwg, queue);
+  else launch_sweep_persistent_p<1>(mode, net, tg, ntiles, K, val, st, mixed, nwg, queue);
 }
 
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,

@@ -235,6 +235,12 @@ void launch_iter_stamp(const IterCtl* ctl, unsigned long long* stamps, int cap, 
 // points_per_lane 1 or 2: the target view's tiles must hold 64 x points_per_lane points each
 void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val, hipStream_t st,
                   bool mixed = false, int points_per_lane = 1);
+// the collocation or right-hand-side sweep as one persistent launch of nwg workgroups over the
+// XCD tile layout (ntiles a multiple of 8); queue: 9 zeroed counters, left zeroed by the launch
+void launch_sweep_persistent(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val,
+                             hipStream_t st, bool mixed, int points_per_lane, int nwg, unsigned* queue);
+// workgroups of the persistent sweep that fit on the device at once (occupancy x compute units)
+int sweep_persistent_groups(int mode, bool mixed, int points_per_lane, int cus);
 // w = -dOmega/dz at the explicit points of a target view (two values per point); jit != 0 gives
 // the jitted form (hpc_fracture.calc_w), 0 the object model's (Fracture.calc_w)
 void launch_w_points(const NetView& net, const TargetView& tg, int ntiles, int exclude, int jit, double* val,

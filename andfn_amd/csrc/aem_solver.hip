@@ -547,6 +547,13 @@ class Solver {
   // tile shape of the collocation and right-hand-side sweeps (points per lane, 1 or 2), picked on
   // the first sweep of each kind by timing both shapes on the same inputs (same results)
   int ppl_[2] = {0, 0};
+  // and whether the sweep runs as one persistent launch over the XCD tile lists (the autotuner's
+  // other axis); the work queue's nine counters stay zero between launches
+  int persist_[2] = {0, 0};
+  DBuf<unsigned> d_queue_;
+  int persist_groups_[2][2][2] = {};  // [right-hand-side mode][mixed][two points per lane]: resident workgroups
+  void launch_sweep_shape(int mode, const aem::NetView& nv, const TargetSet& ts, double* val, bool mixed, int ppl,
+                          int persist);
   void sweep(int mode, const TargetSet& ts, double* val, bool mixed);
 
   // ---- implicit adjoint
@@ -1749,27 +1756,60 @@ double* Solver::discharge_solve(double* b, double* x) {
 
 // Collocation / right-hand-side sweep with the tuned tile shape.  The first call of each kind runs
 // both shapes twice on the same inputs (identical values) and keeps the faster.
+void Solver::launch_sweep_shape(int mode, const aem::NetView& nv, const TargetSet& ts, double* val, bool mixed,
+                                int ppl, int persist) {
+  if (persist) {
+    if (!d_queue_.p) {
+      // first use (the autotuner's trials, never inside a graph capture): the queue and the
+      // resident workgroup counts of every variant
+      d_queue_.zero(9, st_);
+      int cus = 0;
+      HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_));
+      for (int r = 0; r < 2; ++r)
+        for (int m = 0; m < 2; ++m)
+          for (int p = 0; p < 2; ++p)
+            persist_groups_[r][m][p] = aem::sweep_persistent_groups(r ? aem::MODE_RHS : aem::MODE_COLLOC, m, p + 1, cus);
+    }
+    const int nwg = persist_groups_[mode == aem::MODE_RHS][mixed ? 1 : 0][ppl == 2];
+    if (nwg > 0 && ts.ntiles(ppl) % 8 == 0) {
+      aem::launch_sweep_persistent(mode, nv, ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl, nwg, d_queue_.p);
+      return;
+    }
+  }
+  aem::launch_sweep(mode, nv, ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl);
+}
+
+// Collocation / right-hand-side sweep with the tuned tile shape.  The first call of each kind runs
+// every shape twice on the same inputs (identical values): one or two points per lane, one
+// workgroup per tile or one persistent launch; the fastest is kept, unless it beats the plain
+// one-point grid by less than 3%.
 void Solver::sweep(int mode, const TargetSet& ts, double* val, bool mixed) {
   const int slot = mode == aem::MODE_RHS ? 1 : 0;
+  const aem::NetView nv = net();
   if (ppl_[slot] == 0 && ts.ntiles(1) > 0) {
-    // two rounds in alternating order; the best time of each shape decides
-    float best[2] = {1e30f, 1e30f};
-    for (int r = 0; r < 4; ++r) {
-      const int k = (r == 1 || r == 2) ? 1 : 0;  // shapes 1, 2, 2, 1
-      const int ppl = k + 1;
+    // shape k: points per lane 1 + (k & 1), persistent k >> 1; rounds 0 1 2 3 3 2 1 0
+    static const bool no_persist = std::getenv("ANDFN_MI_NO_PERSISTENT") != nullptr;
+    const int nshape = no_persist ? 2 : 4;
+    float best[4] = {1e30f, 1e30f, 1e30f, 1e30f};
+    for (int r = 0; r < 2 * nshape; ++r) {
+      const int k = r < nshape ? r : 2 * nshape - 1 - r;
       float ms = 0.f;
       HIP_CHECK(hipEventRecord(ev_[6], st_));
-      aem::launch_sweep(mode, net(), ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl);
+      launch_sweep_shape(mode, nv, ts, val, mixed, 1 + (k & 1), k >> 1);
       HIP_CHECK(hipEventRecord(ev_[7], st_));
       HIP_CHECK(hipEventSynchronize(ev_[7]));
       HIP_CHECK(hipEventElapsedTime(&ms, ev_[6], ev_[7]));
       best[k] = std::min(best[k], ms);
     }
-    ppl_[slot] = best[1] < 0.97f * best[0] ? 2 : 1;  // the two-point shape must win clearly
+    int pick = 0;
+    for (int k = 1; k < nshape; ++k)
+      if (best[k] < 0.97f * best[0] && best[k] < best[pick]) pick = k;
+    ppl_[slot] = 1 + (pick & 1);
+    persist_[slot] = pick >> 1;
     return;
   }
   const int ppl = ppl_[slot] ? ppl_[slot] : 1;
-  aem::launch_sweep(mode, net(), ts.view(ppl), ts.ntiles(ppl), K_, val, st_, mixed, ppl);
+  launch_sweep_shape(mode, nv, ts, val, mixed, ppl, persist_[slot]);
 }
 
 // max_j |(A0 y)_j - b_j| * row_scale_j over the head equations for the current y
@@ -1939,8 +1979,7 @@ void Solver::enqueue_iteration(const aem::NetView& nv, const aem::FitView& fv, b
   mf_.solve_correction(d_b_.p, d_ay_.p, d_y_.p, d_req_.p, d_x_.p, st_, s32);
   aem::launch_accept(nv, d_de_.p, nde, F_, sys(), d_x_.p, d_y_.p, d_yold_.p, n, st_);
   aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 2, 0, st_);
-  const int ppl = ppl_[0] ? ppl_[0] : 1;
-  aem::launch_sweep(aem::MODE_COLLOC, nv, full_.view(ppl), full_.ntiles(ppl), K_, d_val_.p, st_, phase32, ppl);
+  launch_sweep_shape(aem::MODE_COLLOC, nv, full_, d_val_.p, phase32, ppl_[0] ? ppl_[0] : 1, persist_[0]);
   aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 3, 0, st_);
   aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_citems_.p, ncitems_, d_val_.p, full_.d_pt_off.p,
                   full_.d_dft_tiles.p, (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, 1, st_,
@@ -2573,6 +2612,8 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
     info->refine_rate = precision_ == ANDFN_MI_MIXED ? refine_rate_ : 0.0;
     info->sweep_points_per_lane = ppl_[0];
     info->rhs_points_per_lane = ppl_[1];
+    info->sweep_persistent = persist_[0];
+    info->rhs_persistent = persist_[1];
     info->nnz_discharge = nnz_A_;
     info->nnz_factors = nnz_factors_;
     info->factor_used = sparse_ ? ANDFN_MI_FACTOR_SPARSE

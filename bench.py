@@ -231,6 +231,14 @@ def _obs_points(fa, n, seed=3):
     return of, oz
 
 
+def _launch_info(info):
+    """How the iterations ran: the sweep's tile shape (autotuned) and the run-ahead graph launches."""
+    return {"sweep_points_per_lane": int(info["sweep_points_per_lane"]),
+            "sweep_persistent": int(info["sweep_persistent"]),
+            "runahead_iterations": int(info["runahead_iterations"]),
+            "graph_iterations": int(info["graph_iterations"])}
+
+
 def _time_form(net, precision, lu_mode, warmup, steps, label):
     for _ in range(warmup):
         net.run(precision, lu_mode, HEAD_TOL)
@@ -419,7 +427,9 @@ def bench_batch(args):
         res[prec] = {"ms_per_batch": 1e3 * sum(walls) / args.steps, "nit": int(infos[-1]["nit"]),
                      "head_residual": infos[-1]["head_residual"], "solves_fp32": int(infos[-1]["solves_fp32"]),
                      "nit_fp32": int(infos[-1]["nit_fp32"]), "t_factor_s": infos[-1]["t_factor"],
-                     "adjoint_ms": 1e3 * sum(aw[1:]) / (len(aw) - 1), "adjoint_iterations": int(ai["nit"])}
+                     "adjoint_ms": 1e3 * sum(aw[1:]) / (len(aw) - 1), "adjoint_iterations": int(ai["nit"]),
+                     "k_sweep_ms": infos[-1]["k_sweep_ms"], "k_rhs_ms": infos[-1]["k_rhs_ms"],
+                     "k_fit_ms": infos[-1]["k_fit_ms"], "k_lu_ms": infos[-1]["k_lu_ms"], **_launch_info(infos[-1])}
         res[prec]["adjoint_forward_ratio"] = res[prec]["adjoint_ms"] / res[prec]["ms_per_batch"]
     net.close()
     main_r = res[args.precision]
@@ -536,7 +546,7 @@ def gpu_part(args):
                        "head_residual": last["head_residual"], "t_factor_s": last["t_factor"],
                        "t_iterate_s": last["t_iterate"], "k_sweep_ms": last["k_sweep_ms"],
                        "k_rhs_ms": last["k_rhs_ms"], "k_fit_ms": last["k_fit_ms"], "k_lu_ms": last["k_lu_ms"],
-                       "walls": walls}
+                       **_launch_info(last), "walls": walls}
     main_form = forms[args.precision]
     last = lasts[args.precision]
     walls = main_form.pop("walls")

@@ -137,6 +137,9 @@ typedef struct andfn_mi_info {
   int64_t runahead_iterations;  /* iterations enqueued one ahead of the host's read of the head residual,
                                  * stopped on the device (head tolerance, no coefficient growth) */
   int64_t graph_iterations;     /* of those, iterations run as one HIP graph launch each */
+  int32_t sweep_persistent;     /* 1: the autotuner chose one persistent launch over the XCD tile lists for
+                                 * the collocation sweep (0: one workgroup per tile) */
+  int32_t rhs_persistent;       /* and for the right-hand-side sweep */
 } andfn_mi_info;
 
 /*

@@ -42,7 +42,8 @@ def test_runahead_iterates_equal_the_host_loop(monkeypatch, max_it, tol):
     fg, eg, ig = runs[None]
     print({k: ig[k] for k in ("nit", "head_residual", "runahead_iterations", "graph_iterations")})
     assert ig["runahead_iterations"] == ig["nit"] - 1
-    assert ig["graph_iterations"] > 0.9 * ig["runahead_iterations"]
+    # the first use of each of the three per-iteration graphs runs its launches directly
+    assert ig["graph_iterations"] >= ig["runahead_iterations"] - 3
     assert runs["ANDFN_MI_NO_GRAPH"][2]["graph_iterations"] == 0
     assert runs["ANDFN_MI_NO_RUNAHEAD"][2]["runahead_iterations"] == 0
     for env in ("ANDFN_MI_NO_GRAPH", "ANDFN_MI_NO_RUNAHEAD"):
