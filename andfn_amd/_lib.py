@@ -141,6 +141,18 @@ class AdjointInfo(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Disks(ctypes.Structure):
+    """andfn_mi_disks: disks as structure of arrays ([n][3] centre, normal, x and y vectors; [n] radius)."""
+    _fields_ = [
+        ("n", ctypes.c_int64),
+        ("center", ctypes.c_void_p),
+        ("normal", ctypes.c_void_p),
+        ("x_vector", ctypes.c_void_p),
+        ("y_vector", ctypes.c_void_p),
+        ("radius", ctypes.c_void_p),
+    ]
+
+
 # every exported symbol and its signature
 SIGNATURES = {
     "andfn_mi_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64,
@@ -178,6 +190,14 @@ SIGNATURES = {
     "andfn_mi_state_heads_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p]),
     "andfn_mi_state_discharges": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "andfn_mi_candidate_pairs": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(Disks), ctypes.c_double, ctypes.c_int64,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
+    "andfn_mi_disk_chords": (ctypes.c_int, [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(Disks), ctypes.POINTER(Disks), ctypes.c_int32,
+                                            ctypes.c_double, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_void_p, ctypes.c_void_p]),
+    "andfn_mi_components": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_void_p]),
     "andfn_mi_device_count": (ctypes.c_int, []),
     "andfn_mi_last_error": (ctypes.c_char_p, []),
 }

@@ -239,6 +239,27 @@ void launch_sweep(int mode, const NetView& net, const TargetView& tg, int ntiles
 // XCD tile layout (ntiles a multiple of 8); queue: 9 zeroed counters, left zeroed by the launch
 void launch_sweep_persistent(int mode, const NetView& net, const TargetView& tg, int ntiles, int K, double* val,
                              hipStream_t st, bool mixed, int points_per_lane, int nwg, unsigned* queue);
+// ---- DFN construction (aem_geometry.hip)
+struct DiskArrays {
+  const double* center;  // [n][3]
+  const double* normal;  // [n][3] unit normal
+  const double* xvec;    // [n][3] in-plane basis
+  const double* yvec;    // [n][3]
+  const double* radius;  // [n]
+};
+struct ChordOptions {
+  int shorten_a, shorten_b;  // shrink the chord in disk a's / b's plane about its midpoint
+  int clip;                  // 0 off, 1 inside rho r of both disks, 2 inside rho r of disk b
+  double se_factor, rho;
+};
+void geo_chords(int64_t np, const int64_t* ia, const int64_t* ib, const DiskArrays& A, const DiskArrays& B,
+                const ChordOptions& opt, double* e0, double* e1, int8_t* ok, hipStream_t st);
+void geo_pair_count(int64_t n, const double* c, const double* r, int64_t* count, hipStream_t st);
+void geo_pair_write(int64_t n, const double* c, const double* r, const int64_t* offset, int64_t* pi, int64_t* pk,
+                    hipStream_t st);
+// the message andfn_mi_last_error() returns on this thread
+void set_last_error(const char* msg);
+
 // workgroups of the persistent sweep that fit on the device at once (occupancy x compute units)
 int sweep_persistent_groups(int mode, bool mixed, int points_per_lane, int cus);
 // w = -dOmega/dz at the explicit points of a target view (two values per point); jit != 0 gives

@@ -3568,6 +3568,10 @@ unsigned char* scratch_on_device(int dev) {
 }
 }  // namespace
 
+namespace aem {
+void set_last_error(const char* msg) { g_last_error = msg ? msg : ""; }
+}  // namespace aem
+
 extern "C" {
 
 int andfn_mi_create(void* fractures, int64_t n_fractures, int64_t max_elements, void* elements, int64_t n_elements,

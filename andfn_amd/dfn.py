@@ -124,7 +124,8 @@ class ImpermeableLine(Element):
 
 
 class Fracture:
-    def __init__(self, label, t, radius, center, normal, aperture=None, ncoef=5, nint=10, elements=None):
+    def __init__(self, label, t, radius, center, normal, aperture=None, ncoef=5, nint=10, elements=None,
+                 basis=None):
         self.label = label
         self._id = 0
         self.t = t
@@ -132,7 +133,11 @@ class Fracture:
         self.radius = radius
         self.center = np.asarray(center, dtype=np.float64)
         self._raw_normal = np.asarray(normal, dtype=np.float64)
-        self.normal, self.x_vector, self.y_vector = geo.fracture_basis(np.asarray(normal, dtype=np.float64))
+        if basis is None:
+            basis = geo.fracture_basis(np.asarray(normal, dtype=np.float64))
+        # (normal, x_vector, y_vector); a builder of many fractures passes rows of one batched
+        # fracture_basis, which gives the same values as the per-fracture call
+        self.normal, self.x_vector, self.y_vector = basis
         self.elements = []
         self.constant = 0.0
         if elements is None:
@@ -234,7 +239,7 @@ class DFN:
         r = np.array([f.radius for f in fracs], dtype=np.float64)
         return c, n, x, y, r
 
-    def get_fracture_intersections(self, ncoef=5, nint=10, se_factor=None, rim_clearance=0.0):
+    def get_fracture_intersections(self, ncoef=5, nint=10, se_factor=None, rim_clearance=0.0, device=False):
         """Add an Intersection for every intersecting pair i < k, in (i, k) order.
 
         Same pairs and creation order as ``andfn/dfn.py:1437-1455``; the O(N^2) pair loop is
@@ -243,11 +248,27 @@ class DFN:
         AnDFN; default off) skips an intersection whose shortened endpoints come closer than
         rim_clearance * R to the rim of either fracture: the segment is clipped to the part
         inside (1 - rim_clearance) R of both fractures, and dropped only when nothing remains.
+        ``device`` runs the pair search and the chords on the GPU (``construction.py``; the same
+        pairs and chords).
         """
         if se_factor is None:
             se_factor = float(self.constants["SE_FACTOR"])
         fr = self.fractures
         c, n, x, y, r = self._frac_arrays(fr)
+        if device:
+            from . import construction
+
+            ds = construction.DiskSet(c, n, x, y, r)
+            pairs = construction.candidate_pairs(ds, 1e-8)
+            if len(pairs):
+                i, k = pairs[:, 0], pairs[:, 1]
+                e0, e1, ok = construction.chords(i, k, ds, ds, 3, se_factor, 1 if rim_clearance > 0 else 0,
+                                                 1.0 - rim_clearance)
+                for p in np.nonzero(ok)[0]:
+                    a, b = fr[i[p]], fr[k[p]]
+                    Intersection(f"{a.label}_{b.label}", e0[p], e1[p], a, b, ncoef, nint)
+            self.get_elements()
+            return
         pairs = geo.candidate_pairs(c, r + 1e-8)
         if len(pairs):
             i, k = pairs[:, 0], pairs[:, 1]
@@ -263,7 +284,7 @@ class DFN:
         self.get_elements()
 
     def set_constant_head_boundary(self, center, normal, radius, head, label="Constant Head Boundary",
-                                   ncoef=5, nint=10, se_factor=None, tolerance=-1, rim_clearance=0.0):
+                                   ncoef=5, nint=10, se_factor=None, tolerance=-1, rim_clearance=0.0, device=False):
         """Constant-head lines where a boundary disk cuts the fractures (``andfn/geometry_functions.py:878-926``).
 
         ``rim_clearance`` > 0 (not in AnDFN; default off) clips a line to the part inside
@@ -282,6 +303,22 @@ class DFN:
         if len(idx) == 0:
             return
         P = len(idx)
+        if device:
+            # the boundary disk against every nearby fracture on the GPU: shortened and clipped in
+            # the fracture's plane (the constant-head line), unshortened in the boundary's
+            from . import construction
+
+            bd = construction.DiskSet(bc, bn, bx, by, [float(radius)])
+            fd = construction.DiskSet(c, n, x, y, r)
+            e_full = construction.chords(np.zeros(P, dtype=np.int64), idx, bd, fd, 0)[1]
+            _, ends_d, ok = construction.chords(np.zeros(P, dtype=np.int64), idx, bd, fd, 2, se_factor,
+                                                2 if rim_clearance > 0 else 0, 1.0 - rim_clearance)
+            for p in np.nonzero(ok)[0]:
+                f = fr[idx[p]]
+                if abs(e_full[p][0] - e_full[p][1]) < tolerance * f.radius:
+                    continue
+                ConstantHeadLine(f"{label}_{f.label}", ends_d[p], head, f, ncoef, nint)
+            return
         rep = lambda v: np.repeat(v[None, :], P, axis=0)
         e0, e1, ok = geo.fracture_intersection(rep(bc), rep(bn), rep(bx), rep(by), np.full(P, float(radius)),
                                                c[idx], n[idx], x[idx], y[idx], r[idx])
@@ -313,7 +350,7 @@ class DFN:
                     boundary[i] = True
         return np.array(a, dtype=np.int64), np.array(b, dtype=np.int64), boundary
 
-    def keep_largest_connected_component(self):
+    def keep_largest_connected_component(self, device=False):
         """Delete every fracture outside the largest connected component that touches a boundary.
 
         A component without a constant-head line or well leaves its fracture constants
@@ -321,13 +358,19 @@ class DFN:
         element are candidates.  AnDFN's ``check_connectivity`` (``andfn/dfn.py:1674-1695``)
         removes fractures not connected to any boundary; this keeps one component of those.
         """
-        from scipy.sparse import coo_matrix
-        from scipy.sparse.csgraph import connected_components
-
         nf = len(self.fractures)
         a, b, boundary = self.fracture_graph()
-        g = coo_matrix((np.ones(len(a)), (a, b)), shape=(nf, nf))
-        ncomp, lab = connected_components(g, directed=False)
+        if device:
+            from . import construction
+
+            lab = construction.components(nf, a, b)  # the library's union-find (host C++)
+            ncomp = int(lab.max()) + 1 if nf else 0
+        else:
+            from scipy.sparse import coo_matrix
+            from scipy.sparse.csgraph import connected_components
+
+            g = coo_matrix((np.ones(len(a)), (a, b)), shape=(nf, nf))
+            ncomp, lab = connected_components(g, directed=False)
         has_bnd = np.zeros(ncomp, dtype=bool)
         has_bnd[lab[boundary]] = True
         sizes = np.bincount(lab, minlength=ncomp)

@@ -183,25 +183,33 @@ def drop_near_coincident(dfn, rel_dist, min_angle_deg):
     return removed
 
 
-def build_network(spec):
-    """Build and return the DFN for ``spec`` (elements collected, not yet consolidated)."""
+def build_network(spec, device=False):
+    """Build and return the DFN for ``spec`` (elements collected, not yet consolidated).
+
+    ``device`` runs the pair search, the intersection and constant-head chords on the GPU and the
+    connectivity in the library's C++ (the same network).
+    """
+    from .geometry import fracture_basis
+
     centers, normals, radii, t = draw_fractures(spec)
     dfn = DFN(f"synthetic-{spec.n_fractures}-{spec.seed}", discharge_int=spec.discharge_int, **spec.constants)
-    fr = [Fracture(f"{i}", t[i], radii[i], centers[i], normals[i], ncoef=spec.ncoef, nint=spec.nint)
+    nb, xb, yb = fracture_basis(normals)
+    fr = [Fracture(f"{i}", t[i], radii[i], centers[i], normals[i], ncoef=spec.ncoef, nint=spec.nint,
+                   basis=(nb[i], xb[i], yb[i]))
           for i in range(spec.n_fractures)]
     dfn.add_fracture(fr)
-    dfn.get_fracture_intersections(spec.ncoef, spec.nint, rim_clearance=spec.rim_clearance)
+    dfn.get_fracture_intersections(spec.ncoef, spec.nint, rim_clearance=spec.rim_clearance, device=device)
     L = spec.cube
     reach = np.sqrt(2.0) * (0.5 * L + spec.rmax) + spec.rmax
     dfn.set_constant_head_boundary(np.array([0.0, 0.5 * L, 0.5 * L]), np.array([1.0, 0.0, 0.0]), reach,
                                    spec.head0, label="x0", ncoef=spec.ncoef, nint=spec.nint,
-                                   rim_clearance=spec.rim_clearance)
+                                   rim_clearance=spec.rim_clearance, device=device)
     dfn.set_constant_head_boundary(np.array([L, 0.5 * L, 0.5 * L]), np.array([1.0, 0.0, 0.0]), reach,
                                    spec.head1, label="xL", ncoef=spec.ncoef, nint=spec.nint,
-                                   rim_clearance=spec.rim_clearance)
+                                   rim_clearance=spec.rim_clearance, device=device)
     if spec.min_separation > 0:
         drop_near_coincident(dfn, spec.min_separation, spec.min_angle_deg)
-    dfn.keep_largest_connected_component()
+    dfn.keep_largest_connected_component(device=device)
     if spec.n_wells:
         _place_wells(dfn, spec)
     dfn.get_elements()

@@ -332,6 +332,47 @@ int andfn_mi_state_discharges(andfn_mi_handle* handle, double* q, double* consta
 int andfn_mi_device_count(void);
 
 /* Message of the last failure on the calling thread. */
+/* ---- DFN construction ----------------------------------------------------------------------
+ * The input side of the solve: the intersections AnDFN's DFN.get_fracture_intersections creates
+ * (andfn/dfn.py:1393-1458: every pair i < k in (i, k) order, the chord of
+ * andfn/geometry_functions.py:302-356 with the line/circle roots of :359-402, shortened by
+ * shorten_line :974-993), the constant-head lines of set_constant_head_boundary
+ * (andfn/dfn.py:1504-1553, geometry_functions.py:878-926), and the connectivity of the fracture
+ * graph (andfn/dfn.py:1674-1695, geometry_functions.py:1121-1294). */
+
+/* Disks as structure of arrays: centre, unit normal and in-plane basis vectors as [n][3], radius
+ * [n] (the Fracture fields of andfn/fracture.py:55-75). */
+typedef struct andfn_mi_disks {
+  int64_t n;
+  const double* center;
+  const double* normal;
+  const double* x_vector;
+  const double* y_vector;
+  const double* radius;
+} andfn_mi_disks;
+
+/* Candidate pairs i < k whose bounding spheres meet, |c_i - c_k| <= (r_i + tol) + (r_k + tol), in
+ * (i, k) order: every pair AnDFN's pair loop (andfn/dfn.py:1437-1455) can find intersecting.  The
+ * pairs go to pair_i / pair_k (capacity cap) and their number to *count; more than cap pairs
+ * fail with ANDFN_MI_ECAPACITY after *count is set. */
+int andfn_mi_candidate_pairs(int device, const andfn_mi_disks* disks, double tol, int64_t cap, int64_t* pair_i,
+                             int64_t* pair_k, int64_t* count);
+
+/* For p < n, the chord shared by disk ia[p] of a and disk ib[p] of b (andfn/geometry_functions.py
+ * :302-356): its endpoints in a's plane (e_a[p], two complex numbers as four doubles) and in b's
+ * plane (e_b[p]); ok[p] = 1 where the disks cut each other in a chord.  shorten: bit 0 shrinks e_a,
+ * bit 1 e_b about the midpoint by se_factor (geometry_functions.py:974-993).  clip (not in AnDFN;
+ * 0 = off): 1 keeps the part of the chord inside rho r of both disks (the same fraction in both
+ * planes), 2 the part inside rho r of disk b; ok is cleared where nothing remains. */
+int andfn_mi_disk_chords(int device, int64_t n, const int64_t* ia, const int64_t* ib, const andfn_mi_disks* a,
+                         const andfn_mi_disks* b, int32_t shorten, double se_factor, int32_t clip, double rho,
+                         double* e_a, double* e_b, int8_t* ok);
+
+/* Connected components of the graph of n nodes and m edges (a[j], b[j]) -- the fracture graph,
+ * whose edges are the intersections: label[v] is the component of node v, the components
+ * numbered in the order of their smallest node.  Runs on the host. */
+int andfn_mi_components(int64_t n, int64_t m, const int64_t* a, const int64_t* b, int64_t* label);
+
 const char* andfn_mi_last_error(void);
 
 #ifdef __cplusplus
