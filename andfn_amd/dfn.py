@@ -458,6 +458,20 @@ class DFN:
             e.q = float(rec["q"])
             e.error = float(rec["error"])
 
+    # -- checkpoint ------------------------------------------------------------------------
+    def save_dfn(self, filename, overwrite=False):
+        """Save the network and its solution (``andfn/dfn.py:404-458``); see ``checkpoint.py``."""
+        from .checkpoint import save_dfn
+
+        return save_dfn(self, filename, overwrite)
+
+    def load_dfn(self, filename):
+        """Load a saved network with its solution (``andfn/dfn.py:460-609``); the next solve
+        starts from the saved coefficients and discharges."""
+        from .checkpoint import load_dfn
+
+        return load_dfn(self, filename)
+
     # -- solve -----------------------------------------------------------------------------
     def solve(self, unconsolidate=False, **options):
         """Solve the DFN on the GPU (drop-in for ``andfn/dfn.py:1807-1849``)."""
