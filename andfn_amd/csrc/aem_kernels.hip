@@ -436,7 +436,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 template <bool F32>
 __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ tiles, int ntiles,
                                                   const int* __restrict__ sides, const double* __restrict__ val,
-                                                  const int* __restrict__ pt_off, int M, double* __restrict__ dft) {
+                                                  const int* __restrict__ pt_off, int M, double* __restrict__ dft,
+                                                  const double2* __restrict__ trig, const float2* __restrict__ trig32) {
   const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (tile >= ntiles) return;  // the whole wavefront
   const DftTile tl = tiles[tile];
@@ -444,6 +445,7 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
   const int n = tl.n, m = tl.m, ncol = 2 * m;
   const int side_a = r < tl.ns ? sides[tl.s0 + r] : -1;
   const double* va = side_a >= 0 ? val + pt_off[side_a] : nullptr;
+  const int64_t tbase = 2 * (int64_t)n * (n - 1);  // this n's rows of the (cos, sin) tables
   for (int c0 = 0; c0 < ncol; c0 += 16) {
     const int col = c0 + r;
     const bool colok = col < ncol;
@@ -455,22 +457,21 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
       const int i = k0 + kq;
       const bool iok = i < n;
       const double a = (va != nullptr && iok) ? va[i] : 0.0;
+      // angle j theta_i = kk pi / (2 n): kk = 4 (i j mod n) on the full circle (theta_i = 2 pi i / n),
+      // kk = j (2 i + 1) mod 4 n at the midpoint angles (theta_i = (2 i + 1) pi / (2 n))
+      const int kk = tl.family == 1 ? 4 * (int)(((int64_t)i * j) % n) : (int)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n));
       if (F32) {
         float w = 0.0f;
         if (colok && iok) {
-          float sn, cs;
-          if (tl.family == 1) sincospif(2.0f * (float)((int64_t)i * j % n) / (float)n, &sn, &cs);
-          else sincospif((float)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n)) / (float)(2 * n), &sn, &cs);
-          w = sinpart ? sn : cs;
+          const float2 t = trig32[tbase + kk];
+          w = sinpart ? t.y : t.x;
         }
         accf = __builtin_amdgcn_mfma_f32_16x16x4f32((float)a, w, accf, 0, 0, 0);
       } else {
         double w = 0.0;
         if (colok && iok) {
-          double sn, cs;
-          if (tl.family == 1) sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &sn, &cs);
-          else sincospi((double)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n)) / (double)(2 * n), &sn, &cs);
-          w = sinpart ? sn : cs;
+          const double2 t = trig[tbase + kk];
+          w = sinpart ? t.y : t.x;
         }
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, w, acc, 0, 0, 0);
       }
@@ -486,6 +487,27 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
       }
     }
   }
+}
+
+// (cos, sin) of k pi / (2 n) for k < 4 n, block row y = n - 1 (the same sincospi arguments the
+// DFT used to form per lane, so the weights are unchanged to the bit)
+__global__ void trig_table_kernel(int nmax, double2* __restrict__ trig, float2* __restrict__ trig32) {
+  const int n = blockIdx.y + 1;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n > nmax || k >= 4 * n) return;
+  const int64_t e = 2 * (int64_t)n * (n - 1) + k;
+  double s, c;
+  sincospi((double)k / (double)(2 * n), &s, &c);
+  trig[e] = make_double2(c, s);
+  float sf, cf;
+  sincospif((float)k / (float)(2 * n), &sf, &cf);
+  trig32[e] = make_float2(cf, sf);
+}
+
+void launch_trig_tables(int nmax, double2* trig, float2* trig32, hipStream_t st) {
+  if (nmax <= 0) return;
+  hipLaunchKernelGGL(trig_table_kernel, dim3((unsigned)((4 * nmax + 255) / 256), (unsigned)nmax), dim3(256), 0, st,
+                     nmax, trig, trig32);
 }
 
 __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
@@ -1257,10 +1279,10 @@ void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int
   if (ntiles > 0) {
     if (f32)
       hipLaunchKernelGGL(dft_kernel<true>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
-                         pt_off, net.M, dft);
+                         pt_off, net.M, dft, fv.trig, fv.trig32);
     else
       hipLaunchKernelGGL(dft_kernel<false>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
-                         pt_off, net.M, dft);
+                         pt_off, net.M, dft, fv.trig, fv.trig32);
   }
   hipLaunchKernelGGL(fit_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, dft, damping, blend);
 }

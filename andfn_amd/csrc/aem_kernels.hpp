@@ -75,7 +75,14 @@ struct FitView {
   int* cross_sign;
   int* status;             // bit 0: a crossing list overflowed
   const int* stop;         // as NetView::stop
+  // (cos, sin) of k pi / (2 n) for k < 4 n, n = 1 .. nmax, at entry 2 n (n - 1) + k: the DFT
+  // weights of every collocation family (launch_trig_tables)
+  const double2* trig;
+  const float2* trig32;
 };
+// entries of the tables for n = 1 .. nmax
+inline int64_t trig_entries(int nmax) { return 2 * (int64_t)(nmax + 1) * nmax; }
+void launch_trig_tables(int nmax, double2* trig, float2* trig32, hipStream_t st);
 
 // a kernel of the run-ahead iteration returns at once when its stop flag is set (the head
 // residual met the tolerance earlier in the stream), so iterations enqueued past the stop leave

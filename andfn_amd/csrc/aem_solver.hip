@@ -388,6 +388,8 @@ class Solver {
   void calu_reserve(int64_t n, int nblk);
   void calu_factor(double* A, int64_t n, int nblk, int* perm, int64_t* info);
   DBuf<Slot> d_slots_;
+  DBuf<double2> d_trig_;  // DFT weights (cos, sin) of k pi / (2 n), n <= 2 M (launch_trig_tables)
+  DBuf<float2> d_trig32_;
   DBuf<FitItem> d_items_, d_items_g_;
   DBuf<FitItem> d_citems_, d_citems_g_;  // the circle-type items (branch-cut unwrap)
   int ncitems_ = 0, ncitems_g_ = 0;
@@ -463,6 +465,8 @@ class Solver {
     v.cross_sign = d_cross_sign_.p;
     v.status = d_status_.p;
     v.stop = nullptr;
+    v.trig = d_trig_.p;
+    v.trig32 = d_trig32_.p;
     return v;
   }
 
@@ -830,6 +834,13 @@ void Solver::upload() {
   d_grow_.zero(E_, st_);
   d_scal_.alloc(1);
   d_red_.zero(aem::reduce_scratch_bytes(), st_);
+  {
+    const int nmax = (int)(2 * L_.M);  // nint <= 2 M (parse)
+    d_trig_.alloc((size_t)aem::trig_entries(nmax));
+    d_trig32_.alloc((size_t)aem::trig_entries(nmax));
+    aem::launch_trig_tables(nmax, d_trig_.p, d_trig32_.p, st_);
+    HIP_CHECK(hipGetLastError());
+  }
   for (auto& e : ev_) HIP_CHECK(hipEventCreate(&e));
   for (auto& e : rhs_ev_) HIP_CHECK(hipEventCreate(&e));
   for (auto& p : sol_ev_)
