@@ -155,7 +155,13 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
     }
     wi_any |= want_imag[k];
   }
-  const bool any_imag = __any(wi_any);
+  // the stream function (atan2 of every log term) only where a point of the same slot k on some
+  // lane needs it: a fracture's bounding circle is its first side, so its points fill the first
+  // lanes of slot 0 of the fracture's first tile and slot 1 skips the atan2
+  bool any_imag[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) any_imag[k] = __any(want_imag[k]);
+  (void)wi_any;
 
   double ore[P], oim[P];
   float fre[P], fim[P];  // FP32 accumulators of the far-field terms (mixed precision)
@@ -208,7 +214,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
           float gre = sg * hf[k].x, gim = sg * hf[k].y;
           if (qq != 0.0f) {
             gre = fmaf(qq, 0.5f * logf(r2f[k]), gre);
-            if (any_imag) gim = fmaf(qq, atan2f(chif[k].y, chif[k].x), gim);
+            if (any_imag[k]) gim = fmaf(qq, atan2f(chif[k].y, chif[k].x), gim);
           }
           if (src != excl[k]) {
             fre[k] += gre;
@@ -229,7 +235,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         at[k] = 0.0;
         if (qq != 0.0) {
           lg[k] = 0.5 * log_pos(r2);
-          if (any_imag) at[k] = atan2(chi.y, chi.x);
+          if (any_imag[k]) at[k] = atan2(chi.y, chi.x);
         }
         const double inv = rcp_pos(r2);
         w[k] = c2(chi.x * inv, -chi.y * inv);
@@ -242,7 +248,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         tim[k] = sl.sign * h[k].y;
         if (qq != 0.0) {
           tre[k] = fma(qq, lg[k], tre[k]);
-          if (any_imag) tim[k] = fma(qq, at[k], tim[k]);
+          if (any_imag[k]) tim[k] = fma(qq, at[k], tim[k]);
         }
       }
     } else if (sl.kind == KIND_BOUND) {
@@ -263,7 +269,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
         if (qq != 0.0) {
           tre[k] = qq * (0.5 * log_pos(r2));
-          if (any_imag) tim[k] = qq * atan2(chi.y, chi.x);
+          if (any_imag[k]) tim[k] = qq * atan2(chi.y, chi.x);
         }
         if (sqrt(r2) < 1.0 - 1e-12) {
           tre[k] = __builtin_nan("");
