@@ -521,10 +521,27 @@ def gpu_part(args):
         del src, dst
     spec = workload(args.workload, constants={"MAX_ITERATIONS": args.max_iterations}) if args.max_iterations \
         else workload(args.workload)
+    # the network built twice: pair search, chords and connectivity on the GPU / in the library
+    # (the network solved below), and by the host builder (numpy, SciPy) for the time beside it
     t0 = time.perf_counter()
-    dfn = build_network(spec)
+    dfn = build_network(spec, device=True)
     dfn.consolidate_dfn()
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dh = build_network(spec, device=False)
+    dh.consolidate_dfn()
+    t_build_host = time.perf_counter() - t0
+    same_structure = bool(np.array_equal(dh.elements_struc_array_hpc["_type"], dfn.elements_struc_array_hpc["_type"])
+                          and np.array_equal(dh.fractures_struc_array_hpc["elements"],
+                                             dfn.fractures_struc_array_hpc["elements"]))
+    endpoint_diff = float(np.max(np.abs(dh.elements_struc_array_hpc["endpoints0"]
+                                        - dfn.elements_struc_array_hpc["endpoints0"]))) if same_structure else None
+    del dh
+    build = {"device_s": t_build, "host_s": t_build_host, "same_records_structure": same_structure,
+             "max_endpoint_diff_m": endpoint_diff,
+             "note": "build_network(device=True): candidate pairs and chords on the GPU, connectivity in the "
+                     "library's C++; host: numpy KD-tree search and chords, SciPy components; both include the "
+                     "Python objects and consolidation"}
     fa, ea = dfn.fractures_struc_array_hpc, dfn.elements_struc_array_hpc
     fa0, ea0 = fa.copy(), ea.copy()  # the unsolved records (the handle's downloads overwrite fa, ea)
     net = DeviceNetwork(fa, ea, dfn.discharge_int, dfn.constants)
@@ -563,6 +580,30 @@ def gpu_part(args):
                                            "note": "both forms stopped at head residual <= 1e-6 m; heads at 100 "
                                                    "observation points"}
         forms["mixed_over_fp64"] = forms["mixed"]["solves_per_s"] / forms["fp64"]["solves_per_s"]
+    fixed_point = None
+    if not args.no_compare and not args.max_iterations:
+        # how far the states at head residual 1e-6 are from the fixed point: an FP64 solve of the
+        # same network to head residual 1e-9 (MAX_ITERATIONS raised for it)
+        cref = dfn.constants.copy()
+        cref["MAX_ITERATIONS"] = 5000
+        fr_, er_ = fa0.copy(), ea0.copy()
+        nref = DeviceNetwork(fr_, er_, dfn.discharge_int, cref)
+        iref = nref.run("fp64", args.lu_mode, 1e-9)
+        qr, cr = nref.state_discharges()
+        nref.download()
+        ref = (er_["coef"].copy(), qr, cr, nref.state_heads(obs_frac, obs_z))
+        nref.close()
+
+        def rel_(x, y):
+            return float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+
+        for prec, st in states.items():
+            forms[prec]["error_vs_fixed_point"] = {
+                "coef_rel": rel_(st[0], ref[0]), "q_rel": rel_(st[1], ref[1]), "constant_rel": rel_(st[2], ref[2]),
+                "heads_abs_m": float(np.max(np.abs(st[3] - ref[3])))}
+        fixed_point = {"head_residual": iref["head_residual"], "nit": int(iref["nit"]),
+                       "note": "FP64 solve to head residual 1e-9 m; forms.*.error_vs_fixed_point compare the timed "
+                               "forms' states (head residual <= 1e-6 m) with it"}
     adjoint = None
     if not args.no_adjoint:
         # gradient of Q(x=0) + sum of 100 observation heads w.r.t. every log10 T and the two boundary heads
@@ -632,6 +673,8 @@ def gpu_part(args):
         "t_iterate_s": last["t_iterate"],
         "nnz_factors": int(last["nnz_factors"]),
         "forms": forms,
+        "fixed_point": fixed_point,
+        "build": build,
         "drop_in": drop_in,
         "adjoint": adjoint,
         "flow": flow,
