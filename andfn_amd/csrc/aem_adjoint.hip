@@ -43,6 +43,40 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
 }
 
+// One step of the reduce-scatter below: the lanes with bit O clear keep the first ceil(C/2) of their
+// values, the others the rest (a zero stands in for the missing one when C is odd), each adding
+// what its partner at distance O sends.  [base, base + real) is the lane's range of real indices.
+template <int C, int O>
+__device__ __forceinline__ void rs_step(double* a, int lane, int& base, int& real) {
+  if constexpr (O > 0) {
+    constexpr int H = (C + 1) / 2;
+    const bool up = (lane & O) != 0;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const double lo = a[k];
+      const double hi = (k + H < C) ? a[k + H] : 0.0;
+      a[k] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, O, 64);
+    }
+    if (up) {
+      base += H;
+      real -= H;
+    } else {
+      real = min(real, H);
+    }
+    rs_step<H, O / 2>(a, lane, base, real);
+  }
+}
+
+// Wavefront sums of a[0..C) by recursive halving: afterwards a[0] of lane l holds the total of
+// index reduce_scatter(...) when that is >= 0 (C lanes hold one total each, the rest return -1).
+// ceil(C/2) + ceil(C/4) + ... exchanges instead of the 6 C of separate butterfly sums.
+template <int C>
+__device__ __forceinline__ int reduce_scatter(double* a, int lane) {
+  int base = 0, real = C;
+  rs_step<C, 32>(a, lane, base, real);
+  return real >= 1 ? base : -1;
+}
+
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* p, double v) {
   // non-negative doubles order like their bit patterns
   atomicMax(p, (unsigned long long)__double_as_longlong(v));
@@ -57,7 +91,8 @@ constexpr int kAdjChunk = 20;  // coefficient adjoints accumulated per pass over
 // sums sfx[e][k] = sum_{i > k} w_i that the branch-cut correction kappa_k receives.
 __global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem* __restrict__ items, int nitems,
                                                       const double2* __restrict__ lam, const int* __restrict__ pt_off,
-                                                      double2* __restrict__ w, double* __restrict__ sfx) {
+                                                      const double2* __restrict__ trig, double2* __restrict__ w,
+                                                      double* __restrict__ sfx) {
   __shared__ double sw[kMaxPts];
   const int b = blockIdx.x;
   if (b >= nitems) return;
@@ -69,6 +104,7 @@ __global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem
   const int n = net.nint[e];
   const double2* l = lam + (size_t)e * net.M;
   const double two_n = 2.0 / (double)n;
+  const double2* tr = trig + 2 * (int64_t)n * (n - 1);  // (cos, sin)(k pi / (2 n)), k < 4 n
   const bool circle = (ty == 1 || ty == 4 || ty == 5);
   double f0 = 1.0, f1 = 0.0;
   if (ty == 0) {
@@ -86,17 +122,16 @@ __global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem
     if (ty == 1) {
       // c_j = -(2/n) h_j (Y_j + i X_j), h_0 = 1/2, angles 2 pi i j / n
       for (int j = 0; j < m; ++j) {
-        double s, c;
-        sincospi(2.0 * (double)((int64_t)i * j % n) / (double)n, &s, &c);
+        const double2 cs = tr[4 * (int)((int64_t)i * j % n)];  // angle 2 pi i j / n
         const double h = (j == 0) ? 0.5 : 1.0;
-        acc = fma(h, fma(l[j].x, s, l[j].y * c), acc);
+        acc = fma(h, fma(l[j].x, cs.y, l[j].y * cs.x), acc);
       }
       acc *= -two_n;
     } else {
       const int64_t period = 4 * (int64_t)n;
       for (int j = 1; j < m; ++j) {
-        double s, c;
-        sincospi((double)(((int64_t)j * (2 * i + 1)) % period) / (double)(2 * n), &s, &c);
+        const double2 cs = tr[((int64_t)j * (2 * i + 1)) % period];  // angle j (2 i + 1) pi / (2 n)
+        const double s = cs.y, c = cs.x;
         if (ty == 0 || ty == 3) acc = fma(l[j].x, c, acc);
         else if (ty == 4) acc += l[j].x * s - l[j].y * c;  // c_j = ((2/n) Y_j, -(2/n) X_j)
         else acc -= l[j].y * c;                              // c_j = (0, -(2/n) X_j)
@@ -128,9 +163,18 @@ __global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem
 // EVAL^T for one source element per wavefront (items sorted heaviest first).
 //   eslot[eslot_ptr[e] .. eslot_ptr[e+1]) = (slot, fracture) of the element's source sides
 //   target points of fracture f: [fp_beg[f], fp_end[f]) with coordinates pz and owner pelem
-//   w: (Re, Im) dual weight of every target point
+//   w: (Re, Im) dual weight W of every target point
 //   excl: skip the source's own points (collocation); use_q: include the log terms
 //   accumulate: add into gc / gq instead of overwriting
+// A target's weighted value W.(Re, Im)(c_k b^k) (b = 1/chi, z/R) has the derivative
+// (Re Q_k, -Im Q_k) with respect to (Re c_k, Im c_k), Q_k = conj(W) b^k.  Intersections and
+// constant-head lines have real coefficients (their fit keeps Re only), impermeable lines
+// imaginary ones: for them (BOTH = false) one real sequence u_k (Re Q_k or -Im Q_k) is summed,
+// generated by u_{k+1} = 2 Re(b) u_k - |b|^2 u_{k-1} (b and conj b are the roots of
+// x^2 - 2 Re(b) x + |b|^2, so every real combination of b^k and conj(b)^k follows it): two
+// operations per power, with the other component of gc left at zero.  Bounding and impermeable
+// circles (BOTH = true) sum the complex Q_k.
+template <bool BOTH>
 __global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* __restrict__ items, int nitems,
                                                         const int* __restrict__ eslot_ptr,
                                                         const int2* __restrict__ eslot, const int* __restrict__ fp_beg,
@@ -138,31 +182,34 @@ __global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* 
                                                         const int* __restrict__ pelem, const double2* __restrict__ w,
                                                         int excl, int use_q, int accumulate, double2* __restrict__ gc,
                                                         double* __restrict__ gq) {
+  constexpr int NA = BOTH ? kAdjChunk : 1;
   const int b = blockIdx.x;
   if (b >= nitems) return;
   const int src = items[b];
   const int lane = threadIdx.x;
   const int ty = net.etype[src];
   const int m = (ty == 2) ? 0 : net.ncoef[src];
-  const bool has_q = use_q && (ty == 0 || ty == 2 || ty == 3);
+  const bool imc = (ty == 5);  // impermeable line: imaginary coefficients
+  // (the complex-coefficient sources are circles: no log term, no line map)
+  const bool has_q = !BOTH && use_q && (ty == 0 || ty == 2 || ty == 3);
   const int sb = eslot_ptr[src], se = eslot_ptr[src + 1];
   double aq = 0.0;
   int k0 = 0;
   do {
     const int kc = min(kAdjChunk, m - k0);
     const bool do_q = has_q && k0 == 0;
-    double ar[kAdjChunk], ai[kAdjChunk];
+    double acc[kAdjChunk + NA];  // Re Q_k (or -Im Q_k), then -Im Q_k when BOTH
+    double* ar = acc;
+    double* ai = acc + kAdjChunk;
 #pragma unroll
-    for (int k = 0; k < kAdjChunk; ++k) {
-      ar[k] = 0.0;
-      ai[k] = 0.0;
-    }
+    for (int k = 0; k < kAdjChunk + NA; ++k) acc[k] = 0.0;
     for (int s = sb; s < se; ++s) {
       const int2 sf = eslot[s];
       const Slot sl = net.slots[sf.x];
       const int pb = fp_beg[sf.y], pe = fp_end[sf.y];
-      const bool line = (sl.kind == KIND_LINE || sl.kind == KIND_IMPL);
+      const bool line = !BOTH && (sl.kind == KIND_LINE || sl.kind == KIND_IMPL);
       const double sg = line ? sl.sign : 1.0;
+      const bool slot_q = do_q && (sl.kind == KIND_LINE || sl.kind == KIND_WELL);
       for (int p = pb + lane; p < pe; p += 64) {
         if (excl && pelem[p] == src) continue;
         const double2 wd = w[p];
@@ -176,51 +223,89 @@ __global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* 
           const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
           const double inv = rcp_pos(r2);
           base = make_double2(chi.x * inv, -chi.y * inv);  // 1/chi
-          if (do_q && sl.kind == KIND_LINE)
-            aq = fma(sl.sign * kInv2Pi, fma(wd.x, 0.5 * log_pos(r2), wd.y * atan2(chi.y, chi.x)), aq);
+          if (slot_q) aq = fma(sl.sign * kInv2Pi * wd.x, 0.5 * log_pos(r2), aq);
         } else if (sl.kind == KIND_BOUND) {
           base = make_double2(sl.alpha.x * z.x, sl.alpha.x * z.y);  // z / R
         } else {
           const double2 chi = make_double2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
           const double r2 = fma(chi.x, chi.x, chi.y * chi.y);
           if (sl.kind == KIND_WELL) {
-            if (do_q) aq = fma(kInv2Pi, fma(wd.x, 0.5 * log_pos(r2), wd.y * atan2(chi.y, chi.x)), aq);
+            if (slot_q) aq = fma(kInv2Pi * wd.x, 0.5 * log_pos(r2), aq);
           } else {
             const double inv = rcp_pos(r2);
             base = make_double2(chi.x * inv, -chi.y * inv);
           }
         }
         if (kc > 0) {
-          const double wr = sg * wd.x, wi = sg * wd.y;
-          double2 P = make_double2(1.0, 0.0);
-          for (int k = 0; k < k0; ++k) P = cmul(P, base);
+          // Q_{k0} = conj(W) b^{k0}
+          double2 Q = make_double2(sg * wd.x, -sg * wd.y);
+          for (int k = 0; k < k0; ++k) Q = cmul(Q, base);
+          // every power up to the chunk's end is accumulated (|b| <= 1 at every target, so the
+          // sums past ncoef stay finite); only the first kc are written
+          if (BOTH) {
 #pragma unroll
-          for (int k = 0; k < kAdjChunk; ++k) {
-            if (k < kc) {
-              // value = wr Re(c_k P_k) + wi Im(c_k P_k)
-              ar[k] = fma(wr, P.x, fma(wi, P.y, ar[k]));
-              ai[k] = fma(-wr, P.y, fma(wi, P.x, ai[k]));
-              P = cmul(P, base);
+            for (int k = 0; k < kAdjChunk; ++k) {
+              ar[k] += Q.x;
+              ai[k] -= Q.y;
+              Q = cmul(Q, base);
+            }
+          } else {
+            const double2 Q1 = cmul(Q, base);
+            double u0 = imc ? -Q.y : Q.x;
+            double u1 = imc ? -Q1.y : Q1.x;
+            const double tr = 2.0 * base.x;
+            const double nb = -fma(base.x, base.x, base.y * base.y);
+#pragma unroll
+            for (int k = 0; k < kAdjChunk; ++k) {
+              ar[k] += u0;
+              const double u2 = fma(tr, u1, nb * u0);
+              u0 = u1;
+              u1 = u2;
             }
           }
         }
       }
     }
     double2* o = gc + (size_t)src * net.M + k0;
-#pragma unroll
-    for (int k = 0; k < kAdjChunk; ++k) {
-      if (k < kc) {
-        const double a = wsum(ar[k]);
-        const double c = wsum(ai[k]);
-        if (lane == 0) {
-          if (accumulate) o[k] = make_double2(o[k].x + a, o[k].y + c);
-          else o[k] = make_double2(a, c);
-        }
+    const int idx = reduce_scatter<kAdjChunk + NA>(acc, lane);
+    if (BOTH) {
+      // lane of (power, component): one double each
+      const int k = idx % kAdjChunk;
+      if (idx >= 0 && k < kc) {
+        double* d = reinterpret_cast<double*>(o + k) + idx / kAdjChunk;
+        *d = accumulate ? *d + acc[0] : acc[0];
       }
+    } else if (idx >= 0 && idx < kc) {
+      const double2 v = imc ? make_double2(0.0, acc[0]) : make_double2(acc[0], 0.0);
+      o[idx] = accumulate ? make_double2(o[idx].x + v.x, o[idx].y + v.y) : v;
     }
     k0 += kAdjChunk;
   } while (k0 < m);
   if (has_q) {
+    // Im weights times arg(chi) in a pass of their own, after the sums are written: only the
+    // points of circle-type elements carry them, and the atan2 kept out of the series loop leaves
+    // its registers to the sums
+    for (int s = sb; s < se; ++s) {
+      const int2 sf = eslot[s];
+      const Slot sl = net.slots[sf.x];
+      if (sl.kind != KIND_LINE && sl.kind != KIND_WELL) continue;
+      const bool line = (sl.kind == KIND_LINE);
+      const double sg = line ? sl.sign : 1.0;
+      for (int p = fp_beg[sf.y] + lane; p < fp_end[sf.y]; p += 64) {
+        const double wy = w[p].y;
+        if (wy == 0.0 || (excl && pelem[p] == src)) continue;
+        const double2 z = pz[p];
+        double2 chi;
+        if (line) {
+          const double2 Z = make_double2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
+                                         fma(sl.alpha.x, z.y, fma(sl.alpha.y, z.x, sl.beta.y)));
+          chi = joukowsky_exterior(Z);
+        } else {
+          chi = make_double2(fma(sl.alpha.x, z.x, sl.beta.x), fma(sl.alpha.x, z.y, sl.beta.y));
+        }
+        aq = fma(sg * kInv2Pi * wy, atan2(chi.y, chi.x), aq);
+      }
+    }
     aq = wsum(aq);
     if (lane == 0) gq[src] = accumulate ? gq[src] + aq : aq;
   }
@@ -364,29 +449,14 @@ __global__ void adj_refine_in_kernel(const double* __restrict__ z, const double*
   if (i < n) x[i] = (T)(z[i] - aty[i]);
 }
 
-// mu[perm[i]] += rs[perm[i]] * v[i]  (A^-T = R P^T L^-T U^-T); stats[2] = max |correction|,
-// stats[3] = max |mu|
+// mu[perm[i]] += rs[perm[i]] * v[i]  (A^-T = R P^T L^-T U^-T)
 template <class T>
 __global__ void adj_refine_out_kernel(const T* __restrict__ v, const int* __restrict__ perm,
-                                      const double* __restrict__ rs, int n, double* __restrict__ mu,
-                                      unsigned long long* __restrict__ stats) {
+                                      const double* __restrict__ rs, int n, double* __restrict__ mu) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double d = 0.0, a = 0.0;
   if (i < n) {
     const int p = perm[i];
-    d = rs[p] * (double)v[i];
-    const double u = mu[p] + d;
-    mu[p] = u;
-    d = fabs(d);
-    a = fabs(u);
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    d = fmax(d, __shfl_xor(d, o, 64));
-    a = fmax(a, __shfl_xor(a, o, 64));
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomic_max_pos(&stats[2], d);
-    atomic_max_pos(&stats[3], a);
+    mu[p] += rs[p] * (double)v[i];
   }
 }
 
@@ -414,17 +484,22 @@ __global__ void adj_residual_op_kernel(int64_t np, const int64_t* __restrict__ p
 
 // ---- launchers
 void launch_adj_fitT(const NetView& net, const FitItem* items, int nitems, const double2* lam, const int* pt_off,
-                     double2* w, double* sfx, hipStream_t st) {
+                     const double2* trig, double2* w, double* sfx, hipStream_t st) {
   if (nitems <= 0) return;
-  hipLaunchKernelGGL(adj_fitT_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, lam, pt_off, w, sfx);
+  hipLaunchKernelGGL(adj_fitT_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, lam, pt_off, trig, w, sfx);
 }
 
-void launch_adj_gather(const NetView& net, const int* items, int nitems, const int* eslot_ptr, const int2* eslot,
-                       const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem, const double2* w,
-                       int excl, int use_q, int accumulate, double2* gc, double* gq, hipStream_t st) {
-  if (nitems <= 0) return;
-  hipLaunchKernelGGL(adj_gather_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, eslot_ptr, eslot, fp_beg,
-                     fp_end, pz, pelem, w, excl, use_q, accumulate, gc, gq);
+void launch_adj_gather(const NetView& net, const int* items, int nitems, int nsingle, const int* eslot_ptr,
+                       const int2* eslot, const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem,
+                       const double2* w, int excl, int use_q, int accumulate, double2* gc, double* gq,
+                       hipStream_t st) {
+  if (nsingle > 0)
+    hipLaunchKernelGGL(adj_gather_kernel<false>, dim3(nsingle), dim3(64), 0, st, net, items, nsingle, eslot_ptr, eslot,
+                       fp_beg, fp_end, pz, pelem, w, excl, use_q, accumulate, gc, gq);
+  if (nitems > nsingle)
+    hipLaunchKernelGGL(adj_gather_kernel<true>, dim3(nitems - nsingle), dim3(64), 0, st, net, items + nsingle,
+                       nitems - nsingle, eslot_ptr, eslot, fp_beg, fp_end, pz, pelem, w, excl, use_q, accumulate, gc,
+                       gq);
 }
 
 void launch_adj_cross(int nsrc, const int* srcs, const int* cptr, const int4* cent, const double* sfx, int M,
@@ -502,14 +577,11 @@ template void launch_adj_refine_in<double>(const double*, const double*, int, do
 template void launch_adj_refine_in<float>(const double*, const double*, int, float*, hipStream_t);
 
 template <class T>
-void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, unsigned long long* stats,
-                           hipStream_t st) {
+void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(adj_refine_out_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, v, perm, rs, n, mu, stats);
+  hipLaunchKernelGGL(adj_refine_out_kernel<T>, dim3((n + 255) / 256), dim3(256), 0, st, v, perm, rs, n, mu);
 }
-template void launch_adj_refine_out<double>(const double*, const int*, const double*, int, double*,
-                                            unsigned long long*, hipStream_t);
-template void launch_adj_refine_out<float>(const float*, const int*, const double*, int, double*, unsigned long long*,
-                                           hipStream_t);
+template void launch_adj_refine_out<double>(const double*, const int*, const double*, int, double*, hipStream_t);
+template void launch_adj_refine_out<float>(const float*, const int*, const double*, int, double*, hipStream_t);
 
 }  // namespace aem

@@ -394,11 +394,15 @@ void launch_bc_residual(const NetView& net, const FitItem* items, int nitems, co
 // ---- implicit adjoint (aem_adjoint.hip) and its point tables
 void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, int colloc, double2* pz, int* pelem,
                    hipStream_t st);
+// trig: the (cos, sin) tables of launch_trig_tables
 void launch_adj_fitT(const NetView& net, const FitItem* items, int nitems, const double2* lam, const int* pt_off,
-                     double2* w, double* sfx, hipStream_t st);
-void launch_adj_gather(const NetView& net, const int* items, int nitems, const int* eslot_ptr, const int2* eslot,
-                       const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem, const double2* w,
-                       int excl, int use_q, int accumulate, double2* gc, double* gq, hipStream_t st);
+                     const double2* trig, double2* w, double* sfx, hipStream_t st);
+// items[0, nsingle): sources with real or imaginary coefficients (and wells); items[nsingle,
+// nitems): sources with complex coefficients (bounding and impermeable circles)
+void launch_adj_gather(const NetView& net, const int* items, int nitems, int nsingle, const int* eslot_ptr,
+                       const int2* eslot, const int* fp_beg, const int* fp_end, const double2* pz, const int* pelem,
+                       const double2* w, int excl, int use_q, int accumulate, double2* gc, double* gq,
+                       hipStream_t st);
 void launch_adj_cross(int nsrc, const int* srcs, const int* cptr, const int4* cent, const double* sfx, int M,
                       double* gq, hipStream_t st);
 void launch_adj_csum(int nf, const int* fp_beg, const int* fp_end, const double2* w, int accumulate, double* gC,
@@ -421,7 +425,6 @@ void launch_transpose(const T* A, int64_t n, T* B, hipStream_t st, int nblk = 1)
 template <class T>
 void launch_adj_refine_in(const double* z, const double* aty, int n, T* x, hipStream_t st);
 template <class T>
-void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, unsigned long long* stats,
-                           hipStream_t st);
+void launch_adj_refine_out(const T* v, const int* perm, const double* rs, int n, double* mu, hipStream_t st);
 
 }  // namespace aem

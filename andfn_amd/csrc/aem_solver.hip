@@ -3094,7 +3094,7 @@ void Solver::adj_mu_step_sparse() {
   aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
   aem::launch_adj_refine_in<double>(d_z_.p, d_aty_.p, n, d_x_.p, st_);
   mf_.solve_t(d_x_.p, nullptr, d_v_.p, st_);
-  aem::launch_adj_refine_out<double>(d_v_.p, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
+  aem::launch_adj_refine_out<double>(d_v_.p, d_perm_.p, d_req_.p, n, d_mu_.p, st_);
 }
 
 // mu += A^-T (z - A^T mu) with the transposed factors in precision T
@@ -3104,7 +3104,7 @@ void Solver::adj_mu_step(const T* AT, const T* linvT, const T* uinvT, T* x, T* w
   aem::launch_spmv(n, d_csrT_ptr_.p, d_csrT_col_.p, d_csrT_val_.p, d_mu_.p, d_aty_.p, st_);
   aem::launch_adj_refine_in<T>(d_z_.p, d_aty_.p, n, x, st_);
   blocked_solve_all<T>(AT, linvT, uinvT, x, w, v);
-  aem::launch_adj_refine_out<T>(v, d_perm_.p, d_req_.p, n, d_mu_.p, d_stats_.p, st_);
+  aem::launch_adj_refine_out<T>(v, d_perm_.p, d_req_.p, n, d_mu_.p, st_);
 }
 
 int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_options* opt, double* grad_t,
@@ -3155,9 +3155,15 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       const int f = esl[k].y;
       work[e] += full_.pt_off[full_.fsb[f + 1]] - full_.pt_off[full_.fsb[f]];
     }
+  // (sources with one coefficient component first, then bounding and impermeable circles)
   std::vector<int> gitems(E_);
   for (int e = 0; e < E_; ++e) gitems[e] = e;
-  std::stable_sort(gitems.begin(), gitems.end(), [&](int a, int b) { return work[a] > work[b]; });
+  auto both = [&](int e) { return etype_[e] == 1 || etype_[e] == 4; };
+  std::stable_sort(gitems.begin(), gitems.end(), [&](int a, int b) {
+    if (both(a) != both(b)) return !both(a);
+    return work[a] > work[b];
+  });
+  const int gsingle = (int)std::count_if(gitems.begin(), gitems.end(), [&](int e) { return !both(e); });
   d_gitems_.upload(gitems, st_);
   // branch-cut crossings regrouped by crossing source: (circle element, position, sign)
   int nxsrc = 0;
@@ -3239,7 +3245,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     tab_obs_.fpb.upload(fb, st_);
     tab_obs_.fpe.upload(fe, st_);
     d_wobs_.upload(wv, st_);
-    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_obs_.fpb.p, tab_obs_.fpe.p,
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, gsingle, d_eslot_ptr_.p, d_eslot_.p, tab_obs_.fpb.p, tab_obs_.fpe.p,
                            tab_obs_.pz.p, tab_obs_.pelem.p, d_wobs_.p, 0, 1, 0, d_gc_.p, d_gq_.p, st_);
     aem::launch_adj_csum(F_, tab_obs_.fpb.p, tab_obs_.fpe.p, d_wobs_.p, 0, d_gC_.p, st_);
   }
@@ -3265,10 +3271,12 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   // d_gc_ = B lam (+ RHS_c^T A^-T J_y when with_j) and d_mu_ = A^-T (S_y^T lam (+ J_y)), where B is
   // the fixed-point map's linear part on the coefficients: FIT^T, EVAL^T at the collocation
   // points (coefficients, discharges, constants, branch cuts), A^-T, RHS^T
+  // (the collocation gather overwrites every active entry of d_gc_; the entries past an element's
+  // ncoef keep the zeros written above)
   auto apply = [&](bool with_j) {
-    HIP_CHECK(hipMemsetAsync(d_gc_.p, 0, sizeof(double2) * ncf, st_));
-    aem::launch_adj_fitT(nv, d_items_.p, (int)items_.size(), d_lam_.p, full_.d_pt_off.p, d_wfull_.p, d_sfx_.p, st_);
-    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_full_.fpb.p, tab_full_.fpe.p,
+    aem::launch_adj_fitT(nv, d_items_.p, (int)items_.size(), d_lam_.p, full_.d_pt_off.p, d_trig_.p, d_wfull_.p,
+                         d_sfx_.p, st_);
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, gsingle, d_eslot_ptr_.p, d_eslot_.p, tab_full_.fpb.p, tab_full_.fpe.p,
                            tab_full_.pz.p, tab_full_.pelem.p, d_wfull_.p, 1, 1, 0, d_gc_.p, d_gq_.p, st_);
     aem::launch_adj_cross(nxsrc, d_xsrc_.p, d_xptr_.p, d_xent_.p, d_sfx_.p, (int)L_.M, d_gq_.p, st_);
     aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
@@ -3281,7 +3289,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       adj_mu_step<double>(d_AT_.p, d_linvT_.p, d_uinvT_.p, d_x_.p, d_w_.p, d_v_.p);
     }
     aem::launch_adj_rhsT(nv, d_rhs_items_.p, nde, d_de_pos_.p, d_mu_.p, rhs_.d_pt_off.p, K_, d_wrhs_.p, st_);
-    aem::launch_adj_gather(nv, d_gitems_.p, E_, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
+    aem::launch_adj_gather(nv, d_gitems_.p, E_, gsingle, d_eslot_ptr_.p, d_eslot_.p, tab_rhs_.fpb.p, tab_rhs_.fpe.p,
                            tab_rhs_.pz.p, tab_rhs_.pelem.p, d_wrhs_.p, 0, 0, 1, d_gc_.p, d_gq_.p, st_);
   };
   auto read_stats = [&](double* v) {
@@ -3336,8 +3344,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     aem::launch_adj_update((int64_t)ncf, d_jc_.p, d_gc_.p, d_lam_.p, 1.0, d_stats_.p, st_);  // lam = J_c + gc
     aem::launch_adj_pack(np, d_pidx_.p, d_lam_.p, d_gv_.p, st_);
     ++nit;
-    auto op = [&](const double2* v, double2* out) {  // out = (I - B) v
-      HIP_CHECK(hipMemsetAsync(d_lam_.p, 0, sizeof(double2) * ncf, st_));
+    auto op = [&](const double2* v, double2* out) {  // out = (I - B) v (reads only the active entries of lam)
       aem::launch_adj_unpack(np, d_pidx_.p, v, d_lam_.p, st_);
       apply(false);
       aem::launch_adj_residual_op(np, d_pidx_.p, v, d_gc_.p, out, st_);
@@ -3379,16 +3386,22 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       int jd = 0;
       for (int j = 0; j < restart && nit < maxit; ++j) {
         op((const double2*)(V + (size_t)j * N), d_wv_.p);
-        // classical Gram-Schmidt, twice
+        // classical Gram-Schmidt, repeated when the first pass cancels more than 1 - 1/sqrt(2) of
+        // the vector's norm (the criterion of Daniel, Gragg, Kaufman and Stewart, 1976): each
+        // pass reads the j + 1 basis vectors twice, the bulk of the iteration's memory traffic
+        double w0 = 0.0, hn = 0.0;
+        RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &w0));
+        std::fill(h2.begin(), h2.end(), 0.0);
         for (int pass = 0; pass < 2; ++pass) {
           RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_transpose, N, j + 1, &one, V, N, w, 1, &zero, d_hv_.p, 1));
           RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, N, j + 1, &mone, V, N, d_hv_.p, 1, &one, w, 1));
           HIP_CHECK(hipMemcpyAsync(pass ? h2.data() : h.data(), d_hv_.p, sizeof(double) * (j + 1),
                                    hipMemcpyDeviceToHost, st_));
+          RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &hn));
+          HIP_CHECK(hipStreamSynchronize(st_));
+          if (hn >= 0.70710678118654752 * w0) break;
+          w0 = hn;
         }
-        double hn = 0.0;
-        RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &hn));
-        HIP_CHECK(hipStreamSynchronize(st_));
         for (int i = 0; i <= j; ++i) H[(size_t)i * restart + j] = h[i] + h2[i];
         H[(size_t)(j + 1) * restart + j] = hn;
         if (hn > 0.0) {
