@@ -251,6 +251,15 @@ class Solver {
  private:
   int dev_ = 0;
   Layout L_;
+  // coefficients per element held on the device (the stride of every [E][M] array): at least the
+  // largest count any element can reach in this network (record counts, 2 M >= nint and the
+  // discharge points, MAX_NCOEF + COEF_INCREASE - 1 for growth), rounded up to 32 or 64 or a
+  // multiple of 32.  Records are read and written up to Mrec_ = min(M_, L_.M) coefficients; their
+  // entries beyond it are left as they are.  (The stride matters to the collocation sweep's
+  // coefficient loads: on the 20,000-fracture network, 20 coefficients per element, the sweep
+  // takes 926 us per launch at a stride of 20, 924 at 24, 923 at 40, 893 at 32, 892 at 64 and
+  // 897 at 200.)
+  int64_t M_ = 0, Mrec_ = 0;
   char* fr_;
   char* el_;
   int F_, E_, K_;
@@ -372,6 +381,7 @@ class Solver {
   std::vector<int> mf_delay_;            // levels each unknown is lifted (pivots found unstable)
   int mf_rounds_ = 0;                    // factorisations of the last run (1 + analyses redone)
   void mf_analyse_now();
+  int verbose_ = 0;  // the current run's verbose option (factorisation rounds on stderr)
   std::vector<double> fcenter_;          // [F][3] fracture centres (fracture record offset 24)
   std::vector<double> fradius_;          // [F] fracture radii (fracture record offset 16)
   std::vector<double> fframe_;           // [F][12] centre, x vector, y vector, normal (offsets 24-120)
@@ -424,7 +434,7 @@ class Solver {
 
   aem::NetView net() const {
     aem::NetView v;
-    v.M = (int)L_.M;
+    v.M = (int)M_;
     v.etype = d_etype_.p;
     v.efrac0 = d_efrac0_.p;
     v.efrac1 = d_efrac1_.p;
@@ -614,8 +624,22 @@ void Solver::parse() {
   error_old_.resize(E_);
   error_old2_.resize(E_);
   error_coef_.resize(E_);
-  coef_.assign((size_t)E_ * L_.M, make_double2(0, 0));
-  dpsi_.assign((size_t)E_ * 2 * L_.M, 0.0);
+  // the device's coefficient stride M_ (see its declaration)
+  {
+    int64_t mx = 1;
+    for (int i = 0; i < E_; ++i) {
+      const char* p = el_ + (size_t)i * L_.esz;
+      mx = std::max(mx, rd<int64_t>(p, 136));
+      mx = std::max(mx, (rd<int64_t>(p, 144) + 1) / 2);
+    }
+    mx = std::max<int64_t>(mx, (int64_t)C_.MAX_NCOEF + C_.COEF_INCREASE - 1);
+    mx = std::max<int64_t>(mx, (K_ + 1) / 2);
+    mx = std::min<int64_t>(std::max<int64_t>(mx, 1), L_.M);
+    M_ = mx <= 32 ? 32 : mx <= 64 ? 64 : (mx + 31) / 32 * 32;
+    Mrec_ = std::min<int64_t>(M_, L_.M);
+  }
+  coef_.assign((size_t)E_ * M_, make_double2(0, 0));
+  dpsi_.assign((size_t)E_ * 2 * M_, 0.0);
   for (int i = 0; i < E_; ++i) {
     const char* p = el_ + (size_t)i * L_.esz;
     int64_t id = rd<int64_t>(p, 0), ty = rd<int64_t>(p, 8), f0 = rd<int64_t>(p, 16), f1 = rd<int64_t>(p, 24);
@@ -636,8 +660,8 @@ void Solver::parse() {
     center_[i] = d2(rd<cd>(p, 104));
     phi_[i] = rd<double>(p, 128);
     q_[i] = rd<double>(p, 152);
-    for (int64_t j = 0; j < L_.M; ++j) coef_[(size_t)i * L_.M + j] = d2(rd<cd>(p, L_.e_coef + 16 * j));
-    for (int64_t j = 0; j < 2 * L_.M; ++j) dpsi_[(size_t)i * 2 * L_.M + j] = rd<double>(p, L_.e_dpsi + 8 * j);
+    for (int64_t j = 0; j < Mrec_; ++j) coef_[(size_t)i * M_ + j] = d2(rd<cd>(p, L_.e_coef + 16 * j));
+    for (int64_t j = 0; j < 2 * Mrec_; ++j) dpsi_[(size_t)i * 2 * M_ + j] = rd<double>(p, L_.e_dpsi + 8 * j);
     error_[i] = rd<double>(p, L_.e_err);
     error_old_[i] = rd<double>(p, L_.e_err + 8);
     error_old2_[i] = rd<double>(p, L_.e_err + 16);
@@ -648,8 +672,8 @@ void Solver::parse() {
   real_lines_ = std::getenv("ANDFN_MI_COMPLEX_LINES") == nullptr;
   for (int i = 0; i < E_ && real_lines_; ++i)
     if (etype_[i] == 0 || etype_[i] == 3)
-      for (int64_t j = 0; j < L_.M; ++j)
-        if (coef_[(size_t)i * L_.M + j].y != 0.0) {
+      for (int64_t j = 0; j < Mrec_; ++j)
+        if (coef_[(size_t)i * M_ + j].y != 0.0) {
           real_lines_ = false;
           break;
         }
@@ -835,7 +859,7 @@ void Solver::upload() {
   d_scal_.alloc(1);
   d_red_.zero(aem::reduce_scratch_bytes(), st_);
   {
-    const int nmax = (int)(2 * L_.M);  // nint <= 2 M (parse)
+    const int nmax = (int)(2 * M_);  // nint <= 2 M_ (parse)
     d_trig_.alloc((size_t)aem::trig_entries(nmax));
     d_trig32_.alloc((size_t)aem::trig_entries(nmax));
     aem::launch_trig_tables(nmax, d_trig_.p, d_trig32_.p, st_);
@@ -1260,7 +1284,12 @@ void Solver::assemble_and_factor() {
     d_rowscale_.upload(rs, st_);
   }
   sparse_ = lu_mode_ == ANDFN_MI_LU_TRSV && factor_ == ANDFN_MI_FACTOR_SPARSE;
-  if (sparse_ && !mf_.ready() && !mf_failed_) mf_analyse_now();
+  if (sparse_ && !mf_.ready() && !mf_failed_) {
+    const auto ta0 = std::chrono::steady_clock::now();
+    mf_analyse_now();
+    if (verbose_)
+      fprintf(stderr, "analysis: %.1f ms\n", 1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - ta0).count());
+  }
   if (sparse_ && mf_failed_) sparse_ = false;  // the dense tournament LU below instead
   if (sparse_) {
     // A assembled straight into its CSR values, row-equilibrated, factored by the multifrontal LU
@@ -1273,22 +1302,34 @@ void Solver::assemble_and_factor() {
     HIP_CHECK(hipGetLastError());
     d_ay_.alloc(N);
     // Pivots are searched within a front's own rows only.  A pivot whose L21 column grows beyond
-    // kGrowthLimit (threshold pivoting at 1e-5 of its column; the solves refine once) or that is zero is delayed: its
-    // column and row unknowns are lifted one level, the tree is analysed again and A refactored.
-    // The lifts persist on the handle.  After kMaxRounds the dense tournament LU takes over.
+    // kGrowthLimit (threshold pivoting at 1e-5 of its column; the solves refine once) or that is
+    // zero is delayed: its column and row unknowns are lifted to the root front of their part of
+    // the network, where every row is fully summed and partial pivoting is complete, the tree is
+    // analysed again and A refactored.  (Lifting one level at a time took three factorisations and
+    // two analyses on the 20,000-fracture network, where 5 unknowns were delayed and 1 of them
+    // again; a handful of unknowns at the root costs nothing measurable.)  The lifts persist on
+    // the handle.  After kMaxRounds the dense tournament LU takes over.
     constexpr double kGrowthLimit = 1e5;
     constexpr int kMaxRounds = 8;
+    constexpr int kLiftToRoot = 1 << 20;  // levels: more than any tree has
     bool root_zero = false;
     mf_rounds_ = 0;
     for (;;) {
       int bad = 0;
       std::vector<int> lift;
+      const auto tr0 = std::chrono::steady_clock::now();
       try {
         bad = mf_.factor(d_csr_val_.p, d_csr_ptr_.p, d_req_.p, st_);
         ++mf_rounds_;
         mf_.unstable_pivots(kGrowthLimit, lift, root_zero, st_);
       } catch (const std::exception& e) {
         fail(ANDFN_MI_EHIP, "multifrontal factorisation: %s", e.what());
+      }
+      if (verbose_) {
+        size_t again = 0;
+        for (int u : lift) again += !mf_delay_.empty() && mf_delay_[u] > 0;
+        fprintf(stderr, "factorisation round %d: %.1f ms, %zu unknowns to delay (%zu delayed before)\n", mf_rounds_,
+                1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - tr0).count(), lift.size(), again);
       }
       if (lift.empty()) {
         if (bad || root_zero) fail(ANDFN_MI_ESINGULAR, "discharge matrix is exactly singular (zero pivot in front %d)", bad - 1);
@@ -1301,8 +1342,12 @@ void Solver::assemble_and_factor() {
         break;
       }
       if (mf_delay_.empty()) mf_delay_.assign(N, 0);
-      for (int u : lift) ++mf_delay_[u];
+      for (int u : lift) mf_delay_[u] = kLiftToRoot;
+      const auto ta0 = std::chrono::steady_clock::now();
       mf_analyse_now();
+      if (verbose_)
+        fprintf(stderr, "analysis with the delays: %.1f ms\n",
+                1e3 * std::chrono::duration<double>(std::chrono::steady_clock::now() - ta0).count());
       if (mf_failed_) {
         sparse_ = false;
         break;
@@ -1888,10 +1933,10 @@ bool Solver::ensure_rhs_basis(int max_coef, int coef_increase) {
   std::vector<int> mcap(E_, 0);
   for (int e = 0; e < E_; ++e)
     if (etype_[e] != 2)
-      mcap[e] = ncoef_[e] < max_coef ? (int)std::min<int64_t>(L_.M, max_coef + coef_increase - 1) : ncoef_[e];
+      mcap[e] = ncoef_[e] < max_coef ? (int)std::min<int64_t>(M_, max_coef + coef_increase - 1) : ncoef_[e];
   if (rb_ok_ && mcap == rb_mcap_) return true;
   // a column packs (source, 2 j + part) into 32 bits
-  if ((int64_t)E_ > ((int64_t)1 << (32 - aem::kRhsColShift)) - 1 || 2 * L_.M > ((int64_t)1 << aem::kRhsColShift))
+  if ((int64_t)E_ > ((int64_t)1 << (32 - aem::kRhsColShift)) - 1 || 2 * M_ > ((int64_t)1 << aem::kRhsColShift))
     return false;
   const int ns = (int)rhs_.side_elem.size();
   std::vector<int64_t> ptr(ns + 1, 0), coff(std::max(ns, 1), 0);
@@ -2060,6 +2105,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   const int coef_increase = C_.COEF_INCREASE;
   const double damping = C_.DAMPING;
   const int verbose = opt ? opt->verbose : 0;
+  verbose_ = verbose;
   lu_mode_ = opt ? opt->lu_mode : ANDFN_MI_LU_TRSV;
   factor_ = opt ? opt->factor : ANDFN_MI_FACTOR_SPARSE;
   if (factor_ != ANDFN_MI_FACTOR_SPARSE && factor_ != ANDFN_MI_FACTOR_TOURNAMENT && factor_ != ANDFN_MI_FACTOR_ROCSOLVER)
@@ -2133,7 +2179,7 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   d_val_.alloc(std::max<int64_t>(full_.npts(), 1));
   d_valr_.alloc(std::max<int64_t>(rhs_.npts(), 1));
   // DFT sums of every collocation side (a grown subset indexes its own sides from 0)
-  d_dft_.alloc(std::max<size_t>(full_.side_elem.size(), 1) * 2 * (size_t)L_.M);
+  d_dft_.alloc(std::max<size_t>(full_.side_elem.size(), 1) * 2 * (size_t)M_);
   aem::launch_init_errors(fv, E_, st_);
   aem::launch_cross(nv, fv, d_circles_.p, (int)circles_.size(), st_);
   HIP_CHECK(hipGetLastError());
@@ -2662,8 +2708,8 @@ void Solver::write_back() {
     wr<int64_t>(p, 136, ncoef_[i]);
     wr<int64_t>(p, 144, nint_[i]);
     wr<double>(p, 152, q_[i]);
-    for (int64_t j = 0; j < L_.M; ++j) {
-      double2 c = coef_[(size_t)i * L_.M + j];
+    for (int64_t j = 0; j < Mrec_; ++j) {
+      double2 c = coef_[(size_t)i * M_ + j];
       wr<cd>(p, L_.e_coef + 16 * j, cd(c.x, c.y));
     }
     // thetas[:nint] as calc_thetas writes them (hpc_math_functions.py:673-699)
@@ -2675,7 +2721,7 @@ void Solver::write_back() {
     }
     for (int k = 0; k < n; ++k) wr<double>(p, L_.e_thetas + 8 * k, start + k * del);
     if (is_circle_like(etype_[i]))
-      for (int k = 0; k < n - 1; ++k) wr<double>(p, L_.e_dpsi + 8 * k, dpsi_[(size_t)i * 2 * L_.M + k]);
+      for (int k = 0; k < n - 1; ++k) wr<double>(p, L_.e_dpsi + 8 * k, dpsi_[(size_t)i * 2 * M_ + k]);
     wr<double>(p, L_.e_err, error_[i]);
     wr<double>(p, L_.e_err + 8, error_old_[i]);
     wr<double>(p, L_.e_err + 16, error_old2_[i]);
@@ -3130,7 +3176,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     if (obj->obs_frac[k] < 0 || obj->obs_frac[k] >= F_)
       fail(ANDFN_MI_EINVAL, "observation %lld: fracture %lld out of range", (long long)k, (long long)obj->obs_frac[k]);
   const int nde = (int)de_.size(), n = (int)N_;
-  const size_t ncf = (size_t)E_ * L_.M;
+  const size_t ncf = (size_t)E_ * M_;
   aem::NetView nv = net();
 
   // point tables of the collocation and discharge point sets at the converged counts
@@ -3266,7 +3312,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
   d_jzero_.zero(n, st_);
   d_wfull_.zero(std::max<int64_t>(full_.npts(), 1), st_);
   d_wrhs_.zero(std::max<int64_t>(rhs_.npts(), 1), st_);
-  d_sfx_.zero((size_t)E_ * 2 * L_.M, st_);
+  d_sfx_.zero((size_t)E_ * 2 * M_, st_);
   d_stats_.zero(4, st_);
   // d_gc_ = B lam (+ RHS_c^T A^-T J_y when with_j) and d_mu_ = A^-T (S_y^T lam (+ J_y)), where B is
   // the fixed-point map's linear part on the coefficients: FIT^T, EVAL^T at the collocation
@@ -3278,7 +3324,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
                          d_sfx_.p, st_);
     aem::launch_adj_gather(nv, d_gitems_.p, E_, gsingle, d_eslot_ptr_.p, d_eslot_.p, tab_full_.fpb.p, tab_full_.fpe.p,
                            tab_full_.pz.p, tab_full_.pelem.p, d_wfull_.p, 1, 1, 0, d_gc_.p, d_gq_.p, st_);
-    aem::launch_adj_cross(nxsrc, d_xsrc_.p, d_xptr_.p, d_xent_.p, d_sfx_.p, (int)L_.M, d_gq_.p, st_);
+    aem::launch_adj_cross(nxsrc, d_xsrc_.p, d_xptr_.p, d_xent_.p, d_sfx_.p, (int)M_, d_gq_.p, st_);
     aem::launch_adj_csum(F_, tab_full_.fpb.p, tab_full_.fpe.p, d_wfull_.p, 0, d_gC_.p, st_);
     aem::launch_adj_build_z(nde, F_, d_de_.p, sys(), n, with_j ? d_jy_.p : d_jzero_.p, d_gq_.p, d_gC_.p, d_z_.p, st_);
     HIP_CHECK(hipMemsetAsync(d_mu_.p, 0, sizeof(double) * n, st_));
@@ -3327,7 +3373,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     std::vector<int64_t> pidx;
     for (int e = 0; e < E_; ++e)
       if (etype_[e] != 2)
-        for (int k = 0; k < ncoef_[e]; ++k) pidx.push_back((int64_t)e * L_.M + k);
+        for (int k = 0; k < ncoef_[e]; ++k) pidx.push_back((int64_t)e * M_ + k);
     const int64_t np = (int64_t)pidx.size();
     const rocblas_int N = (rocblas_int)(2 * np);
     const int restart = 40;
@@ -3641,10 +3687,24 @@ int andfn_mi_solve(void* fractures, int64_t n_fractures, int64_t max_elements, v
   (void)ntype_elements;  // unused by the reference as well (hpc_solve.py:55-79)
   return guarded([&]() {
     if (!fractures || !elements || !constants) fail(ANDFN_MI_EINVAL, "null record pointer");
-    Solver s(fractures, n_fractures, max_elements, elements, n_elements, max_ncoef, discharge_int, constants,
-             opt ? opt->device : 0);
-    int rc = s.solve(opt, info);
-    s.write_back();
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    int rc;
+    clk::time_point t1, t2, t3;
+    {
+      Solver s(fractures, n_fractures, max_elements, elements, n_elements, max_ncoef, discharge_int, constants,
+               opt ? opt->device : 0);
+      t1 = clk::now();
+      rc = s.solve(opt, info);
+      t2 = clk::now();
+      s.write_back();
+      t3 = clk::now();
+    }
+    if (opt && opt->verbose) {
+      auto ms = [](clk::time_point a, clk::time_point b) { return 1e3 * std::chrono::duration<double>(b - a).count(); };
+      fprintf(stderr, "drop-in call: records in %.1f ms, solve %.1f ms, records out %.1f ms, release %.1f ms\n",
+              ms(t0, t1), ms(t1, t2), ms(t2, t3), ms(t3, clk::now()));
+    }
     return rc;
   });
 }

@@ -140,7 +140,8 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   // the shallowest node holding one of its fracture's discharge elements, so that its continuity
   // row is pivoted only once every discharge it sums is (a row missing some of them can cancel
   // exactly against another fracture's).  An unknown the numerical factorisation found unstable
-  // is moved delay[u] levels further up (each level once per factorisation that rejected it).
+  // is moved delay[u] levels further up, at most to its root (the solver passes a delay larger
+  // than any depth: the root front, where every row is fully summed).
   auto lift = [&](int nd, int64_t g) {
     for (int d = in.delay ? in.delay[g] : 0; d > 0 && nodes[nd].parent >= 0; --d) nd = nodes[nd].parent;
     return nd;
@@ -211,7 +212,7 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   std::vector<std::vector<int>> Bn(nn);
   std::vector<int> bloc(N, -1), stamp(N, -1);
   int stampc = 0;
-  std::vector<std::vector<int>> adj;
+  std::vector<int> aptr, acol;  // the block's pattern, row by row (block-local columns)
   std::vector<int> matchc, matchr, vis, stk_r, stk_e, la;
   for (int sidx = 0; sidx < (int)post.size(); ++sidx) {
     const int s = post[sidx];
@@ -221,16 +222,18 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
       const int k = (int)U.size();
       if (k == 0) break;
       for (int i = 0; i < k; ++i) bloc[U[i]] = i;
-      adj.assign(k, {});
+      aptr.assign(k + 1, 0);
+      acol.clear();
       for (int i = 0; i < k; ++i) {
         ++stampc;
         for (int q = rp[U[i]]; q < rp[U[i] + 1]; ++q) {
           const int l = bloc[cl[q]];
           if (l >= 0 && stamp[cl[q]] != stampc) {
             stamp[cl[q]] = stampc;
-            adj[i].push_back(l);
+            acol.push_back(l);
           }
         }
+        aptr[i + 1] = (int)acol.size();
       }
       // maximum matching of rows to columns: depth-first augmenting paths with look-ahead (each
       // row first takes a free column of its own if it has one; its look-ahead pointer only moves
@@ -240,8 +243,8 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
       la.assign(k, 0);
       vis.assign(k, -1);
       auto free_col = [&](int r) {
-        for (int& e = la[r]; e < (int)adj[r].size(); ++e)
-          if (matchc[adj[r][e]] < 0) return adj[r][e];
+        for (int& e = la[r]; e < aptr[r + 1] - aptr[r]; ++e)
+          if (matchc[acol[aptr[r] + e]] < 0) return acol[aptr[r] + e];
         return -1;
       };
       // augment along the stack: each row on it takes the column it went through, the top row col
@@ -267,12 +270,12 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
         while (!stk_r.empty()) {
           const int r = stk_r.back();
           int& e = stk_e.back();
-          if (e >= (int)adj[r].size()) {
+          if (e >= aptr[r + 1] - aptr[r]) {
             stk_r.pop_back();
             stk_e.pop_back();
             continue;
           }
-          const int r2 = matchc[adj[r][e++]];  // every column of a row on the stack is matched
+          const int r2 = matchc[acol[aptr[r] + e++]];  // every column of a row on the stack is matched
           if (vis[r2] == i0) continue;
           vis[r2] = i0;
           stk_r.push_back(r2);

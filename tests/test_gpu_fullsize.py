@@ -132,7 +132,19 @@ def test_mixed_form_against_fp64(cfg2, fp64):
     assert head.max() <= HEAD_TOL * (1 + 1e-6)
 
 
-def test_fixed_count_values_match_the_oracle(cfg2):
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _errs(fx, ex, fy, ey):
+    return dict(coef=_rel(ex["coef"], ey["coef"]), q=_rel(ex["q"], ey["q"]),
+                constant=_rel(fx["constant"], fy["constant"]))
+
+
+@pytest.fixture(scope="module")
+def fixed160(cfg2):
+    """The GPU's and the refined oracle's states after 160 iterations at fixed counts (each oracle
+    run takes about a minute and a half: the two comparisons are separate tests)."""
     consts = np.array(cfg2.constants, copy=True)
     consts["MAX_ERROR"] = 0.0
     consts["MAX_ITERATIONS"] = 160
@@ -142,28 +154,32 @@ def test_fixed_count_values_match_the_oracle(cfg2):
     print("cfg2: GPU, 160 iterations", flush=True)
     solve(fa, ea, cfg2.discharge_int, consts)
     info = last_solve_info()
-    fp, ep = fo.copy(), eo.copy()
     print("cfg2: oracle, 160 iterations, refined SuperLU solves", flush=True)
     oinfo = {}
     oracle.solve(fo, eo, cfg2.discharge_int, consts, info=oinfo, refine=1)
+    return dict(consts=consts, gpu=(fa, ea, info), ref=(fo, eo, oinfo))
+
+
+def test_fixed_count_values_match_the_oracle(fixed160):
+    fa, ea, info = fixed160["gpu"]
+    fo, eo, oinfo = fixed160["ref"]
     assert info["nit"] == oinfo["nit"] == 160
     assert np.array_equal(ea["ncoef"], eo["ncoef"]) and np.array_equal(ea["nint"], eo["nint"])
-
-    def rel(a, b):
-        return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
-
-    def errs(fx, ex, fy, ey):
-        return dict(coef=rel(ex["coef"], ey["coef"]), q=rel(ex["q"], ey["q"]),
-                    constant=rel(fx["constant"], fy["constant"]))
-
-    e_ref = errs(fa, ea, fo, eo)
+    e_ref = _errs(fa, ea, fo, eo)
     print(f"cfg2 160 iterations, GPU vs oracle (refined solves): {e_ref}; head residual GPU "
           f"{info['head_residual']:.3e}, oracle {oinfo['head_residual']:.3e}", flush=True)
     assert e_ref["coef"] < 1e-9 and e_ref["q"] < 1e-9 and e_ref["constant"] < 1e-9, e_ref
+
+
+def test_fixed_count_values_against_the_unrefined_oracle(cfg2, fixed160):
+    fa, ea, _ = fixed160["gpu"]
+    fo, eo, _ = fixed160["ref"]
+    fp = cfg2.fractures_struc_array_hpc.copy()
+    ep = cfg2.elements_struc_array_hpc.copy()
     print("cfg2: oracle, 160 iterations, SuperLU solves as in AnDFN", flush=True)
-    oracle.solve(fp, ep, cfg2.discharge_int, consts)
-    e_gpu = errs(fa, ea, fp, ep)
-    e_slu = errs(fo, eo, fp, ep)
+    oracle.solve(fp, ep, cfg2.discharge_int, fixed160["consts"])
+    e_gpu = _errs(fa, ea, fp, ep)
+    e_slu = _errs(fo, eo, fp, ep)
     print(f"cfg2: GPU vs unrefined oracle {e_gpu}; refined vs unrefined oracle {e_slu}")
     for k in e_gpu:
         assert e_gpu[k] <= e_slu[k] + 1e-10, (k, e_gpu, e_slu)
