@@ -396,6 +396,9 @@ void launch_points(const NetView& net, const TargetView& tg, int ntiles, int K, 
 __global__ __launch_bounds__(64) void unwrap_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
                                                     double* __restrict__ val, const int* __restrict__ pt_off) {
   __shared__ double sdpsi[kMaxPts];
+  __shared__ double sd[kMaxPts];
+  __shared__ double scv[64];
+  __shared__ int scp[64];
   if (stopped(net)) return;
   const FitItem it = items[blockIdx.x];
   const int e = it.elem;
@@ -404,29 +407,41 @@ __global__ __launch_bounds__(64) void unwrap_kernel(NetView net, FitView fv, con
   const int lane = threadIdx.x;
   const int n = net.nint[e];
   // kappa_i = sum q * sign over the cached crossings, then unwrap (get_dpsi_corr
-  // hpc_math_functions.py:406-447; cauchy_integral_domega 583-595)
+  // hpc_math_functions.py:406-447; cauchy_integral_domega 583-595).  The lanes fetch the
+  // crossings and the values; the two running sums stay sequential in lane 0, in the reference's
+  // order, over values already in LDS.
   for (int i = lane; i < n; i += 64) sdpsi[i] = 0.0;
+  const int cb = fv.cross_ptr[e];
+  const int cn = fv.cross_cnt[e];
+  for (int c0 = 0; c0 < cn; c0 += 64) {
+    __syncthreads();
+    if (c0 + lane < cn) {
+      const int2 rec = fv.cross[cb + c0 + lane];  // (position, element); sign in cross_sign
+      scp[lane] = rec.x;
+      scv[lane] = net.q[rec.y] * (double)fv.cross_sign[cb + c0 + lane];
+    }
+    __syncthreads();
+    if (lane == 0)
+      for (int c = 0; c < min(64, cn - c0); ++c) sdpsi[scp[c]] += scv[c];
+  }
+  __syncthreads();
+  double* v = val + pt_off[it.s0];
+  double* gd = fv.dpsi_corr + (size_t)e * 2 * net.M;
+  // d_i = (v_i - v_{i-1}) - kappa_{i-1}, then v_i = v_0 + d_1 + ... + d_i summed in order
+  for (int i = lane; i < n - 1; i += 64) {
+    gd[i] = sdpsi[i];
+    sd[i + 1] = (v[i + 1] - v[i]) - sdpsi[i];
+  }
   __syncthreads();
   if (lane == 0) {
-    const int cb = fv.cross_ptr[e];
-    const int cn = fv.cross_cnt[e];
-    for (int c = 0; c < cn; ++c) {
-      const int2 rec = fv.cross[cb + c];  // (position, element); sign in cross_sign
-      sdpsi[rec.x] += net.q[rec.y] * (double)fv.cross_sign[cb + c];
-    }
-    double* gd = fv.dpsi_corr + (size_t)e * 2 * net.M;
-    for (int i = 0; i < n - 1; ++i) gd[i] = sdpsi[i];
-    double* v = val + pt_off[it.s0];
     double psi0 = v[0];
-    double prev = v[0];
     for (int i = 1; i < n; ++i) {
-      const double cur = v[i];
-      const double d = (cur - prev) - sdpsi[i - 1];
-      prev = cur;
-      psi0 = psi0 + d;
-      v[i] = psi0;
+      psi0 = psi0 + sd[i];
+      sd[i] = psi0;
     }
   }
+  __syncthreads();
+  for (int i = lane + 1; i < n; i += 64) v[i] = sd[i];
 }
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
@@ -516,26 +531,41 @@ void launch_trig_tables(int nmax, double2* trig, float2* trig32, hipStream_t st)
                      nmax, trig, trig32);
 }
 
-__global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
-                                                 const double* __restrict__ dft, double damping, int blend) {
+// sum over the G lanes of a group (G a power of two <= 64, groups aligned to G lanes): the last
+// log2 G steps of wave_sum, so for a group whose other lanes hold zeros the result is the same
+template <int G>
+__device__ __forceinline__ double group_sum(double v) {
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Cauchy/DFT fit of one element per group of G lanes (lanes over its coefficients), 256 / G
+// elements per workgroup.  blend: 0 = the fit only; 1 = also the first damping step; 2 = both
+// damping steps of the iteration (hpc_solve.py:561-575: the one blend_kernel would apply next)
+template <int G>
+__global__ __launch_bounds__(256) void fit_kernel(NetView net, FitView fv, const FitItem* __restrict__ items,
+                                                  int nitems, const double* __restrict__ dft, double damping,
+                                                  int blend) {
   if (stopped(net)) return;
-  const FitItem it = items[blockIdx.x];
+  const int b = blockIdx.x * (256 / G) + (int)threadIdx.x / G;
+  if (b >= nitems) return;  // the whole group
+  const FitItem it = items[b];
   const int e = it.elem;
-  const int lane = threadIdx.x;
+  const int lane = (int)threadIdx.x % G;
   const int ty = net.etype[e];
   const int m = net.ncoef[e];
   const int n = net.nint[e];
   const double* d0 = dft + (size_t)it.s0 * 2 * net.M;
   const double* d1 = dft + (size_t)(ty == 0 ? it.s1 : it.s0) * 2 * net.M;
 
-  // per-lane coefficients j = lane, lane + 64, ...
+  // per-lane coefficients j = lane, lane + G, ...
   double2* wc = net.wcoef + (size_t)e * net.M;
   double2* cf = net.coef + (size_t)e * net.M;
   double s_new_re = 0.0, s_new_im = 0.0, s_tail_re = 0.0, s_tail_im = 0.0, s_old5 = 0.0, s_diff = 0.0;
   const int jtail = (int)(m * 0.9);
   const double t0 = (ty == 0) ? net.t[net.efrac0[e]] : 0.0;
   const double t1 = (ty == 0) ? net.t[net.efrac1[e]] : 0.0;
-  for (int j = lane; j < m; j += 64) {
+  for (int j = lane; j < m; j += G) {
     const double x0 = d0[j], y0 = d0[net.M + j], x1 = ty == 0 ? d1[j] : 0.0;
     const double two_n = 2.0 / (double)n;
     double2 cnew;
@@ -566,14 +596,18 @@ __global__ __launch_bounds__(64) void fit_kernel(NetView net, FitView fv, const 
     }
     if (j < 5) s_old5 += hypot(old.x, old.y);
     s_diff += hypot(cnew.x - old.x, cnew.y - old.y);
-    if (blend) cf[j] = c2(damping * cnew.x + (1.0 - damping) * old.x, damping * cnew.y + (1.0 - damping) * old.y);
+    if (blend) {
+      double2 c = c2(damping * cnew.x + (1.0 - damping) * old.x, damping * cnew.y + (1.0 - damping) * old.y);
+      if (blend == 2) c = c2(damping * cnew.x + (1.0 - damping) * c.x, damping * cnew.y + (1.0 - damping) * c.y);
+      cf[j] = c;
+    }
   }
-  s_new_re = wave_sum(s_new_re);
-  s_new_im = wave_sum(s_new_im);
-  s_tail_re = wave_sum(s_tail_re);
-  s_tail_im = wave_sum(s_tail_im);
-  s_old5 = wave_sum(s_old5);
-  s_diff = wave_sum(s_diff);
+  s_new_re = group_sum<G>(s_new_re);
+  s_new_im = group_sum<G>(s_new_im);
+  s_tail_re = group_sum<G>(s_tail_re);
+  s_tail_im = group_sum<G>(s_tail_im);
+  s_old5 = group_sum<G>(s_old5);
+  s_diff = group_sum<G>(s_diff);
   if (lane == 0) {
     double err = 0.0, errc = 0.0;
     if (!(s_old5 < 1e-15)) {
@@ -602,8 +636,10 @@ __global__ void blend_kernel(NetView net, int ne, double damping) {
   }
 }
 
-// wells: error = 0 and the first damping step (their work coefficients stay zero)
-__global__ void well_kernel(NetView net, FitView fv, const int* __restrict__ wells, int nw, double damping) {
+// wells: error = 0 and the first damping step (their work coefficients stay zero); steps = 2 also
+// applies the second (the one blend_kernel would apply next)
+__global__ void well_kernel(NetView net, FitView fv, const int* __restrict__ wells, int nw, double damping,
+                            int steps) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nw || stopped(net)) return;
   const int e = wells[k];
@@ -613,7 +649,9 @@ __global__ void well_kernel(NetView net, FitView fv, const int* __restrict__ wel
   const double2* wc = net.wcoef + (size_t)e * net.M;
   for (int j = 0; j < m; ++j) {
     double2 a = wc[j], b = cf[j];
-    cf[j] = c2(damping * a.x + (1.0 - damping) * b.x, damping * a.y + (1.0 - damping) * b.y);
+    for (int r = 0; r < steps; ++r)
+      b = c2(damping * a.x + (1.0 - damping) * b.x, damping * a.y + (1.0 - damping) * b.y);
+    cf[j] = b;
   }
 }
 
@@ -1088,9 +1126,11 @@ __device__ __forceinline__ void nanmax_merge(double& a, int& ia, double b, int i
 // one block of 1024 threads: max error (with argmax), max error_coef, max |y - y_old|, max |y_old|
 // Two-stage reductions: each workgroup reduces a contiguous chunk and writes its partial; the
 // last workgroup to finish (atomic counter, reset by that workgroup for the next launch) merges
-// the partials in increasing chunk order, so ties resolve to the first index as in np.argmax.
+// the partials, one per thread, by the same tree.  Every merge keeps the larger value, a NaN
+// over any number, and the lower index between equal values or two NaNs, so the result does
+// not depend on the merge order: ties resolve to the first index as in np.argmax.
 constexpr int kRedThreads = 256;
-constexpr int kRedMaxGroups = 128;
+constexpr int kRedMaxGroups = 256;  // <= kRedThreads: the last workgroup merges one partial per thread
 struct RedPart {
   double e, c, d, o;
   int ie, pad;
@@ -1136,49 +1176,53 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(FitView fv, int ne,
       absmax_merge(om, fabs(yold[k]));
     }
   }
+  auto tree = [&]() {
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        double a = se[tid];
+        int ia = si[tid];
+        nanmax_merge(a, ia, se[tid + s], si[tid + s]);
+        se[tid] = a;
+        si[tid] = ia;
+        int d0 = 0;
+        double c = sc[tid];
+        nanmax_merge(c, d0, sc[tid + s], 0);
+        sc[tid] = c;
+        double d = sd[tid];
+        absmax_merge(d, sd[tid + s]);
+        sd[tid] = d;
+        double o = so[tid];
+        absmax_merge(o, so[tid + s]);
+        so[tid] = o;
+      }
+      __syncthreads();
+    }
+  };
   se[tid] = em;
   si[tid] = ie;
   sc[tid] = cm;
   sd[tid] = dm;
   so[tid] = om;
-  __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      double a = se[tid];
-      int ia = si[tid];
-      nanmax_merge(a, ia, se[tid + s], si[tid + s]);
-      se[tid] = a;
-      si[tid] = ia;
-      int d0 = 0;
-      double c = sc[tid];
-      nanmax_merge(c, d0, sc[tid + s], 0);
-      sc[tid] = c;
-      double d = sd[tid];
-      absmax_merge(d, sd[tid + s]);
-      sd[tid] = d;
-      double o = so[tid];
-      absmax_merge(o, so[tid + s]);
-      so[tid] = o;
-    }
-    __syncthreads();
-  }
+  tree();
   if (tid == 0) rs->parts[g] = RedPart{se[0], sc[0], sd[0], so[0], si[0], 0};
   if (!last_group(&rs->count_reduce)) return;
+  {
+    RedPart q{-__builtin_inf(), -__builtin_inf(), 0.0, 0.0, 0x7fffffff, 0};
+    if (tid < G) q = rs->parts[tid];
+    se[tid] = q.e;
+    si[tid] = q.ie;
+    sc[tid] = q.c;
+    sd[tid] = q.d;
+    so[tid] = q.o;
+  }
+  tree();
   if (tid == 0) {
-    RedPart r = rs->parts[0];
-    for (int k = 1; k < G; ++k) {
-      const RedPart q = rs->parts[k];
-      nanmax_merge(r.e, r.ie, q.e, q.ie);
-      int d0 = 0;
-      nanmax_merge(r.c, d0, q.c, 0);
-      absmax_merge(r.d, q.d);
-      absmax_merge(r.o, q.o);
-    }
-    out->error = r.e;
-    out->argmax = r.ie;
-    out->error_coef = r.c;
-    out->dq_max = r.d;
-    out->yold_max = r.o;
+    out->error = se[0];
+    out->argmax = si[0];
+    out->error_coef = sc[0];
+    out->dq_max = sd[0];
+    out->yold_max = so[0];
     rs->count_reduce = 0;
   }
 }
@@ -1290,7 +1334,14 @@ void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int
       hipLaunchKernelGGL(dft_kernel<false>, dim3((ntiles + 3) / 4), dim3(256), 0, st, tiles, ntiles, dft_sides, val,
                          pt_off, net.M, dft, fv.trig, fv.trig32);
   }
-  hipLaunchKernelGGL(fit_kernel, dim3(nitems), dim3(64), 0, st, net, fv, items, dft, damping, blend);
+  // a group of 32 lanes per element when every count fits (two coefficient passes at most for
+  // 64), one wavefront per element otherwise
+  if (net.M <= 32)
+    hipLaunchKernelGGL(fit_kernel<32>, dim3((nitems + 7) / 8), dim3(256), 0, st, net, fv, items, nitems, dft, damping,
+                       blend);
+  else
+    hipLaunchKernelGGL(fit_kernel<64>, dim3((nitems + 3) / 4), dim3(256), 0, st, net, fv, items, nitems, dft, damping,
+                       blend);
 }
 
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st) {
@@ -1298,9 +1349,10 @@ void launch_blend(const NetView& net, int ne, double damping, hipStream_t st) {
   hipLaunchKernelGGL(blend_kernel, dim3(ne), dim3(64), 0, st, net, ne, damping);
 }
 
-void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st) {
+void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st,
+                  int steps) {
   if (nw <= 0) return;
-  hipLaunchKernelGGL(well_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, net, fv, wells, nw, damping);
+  hipLaunchKernelGGL(well_kernel, dim3((nw + 255) / 256), dim3(256), 0, st, net, fv, wells, nw, damping, steps);
 }
 
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
@@ -1755,34 +1807,30 @@ __global__ __launch_bounds__(kRedThreads) void residual_kernel(const double* __r
       ir = j;
     }
   }
+  auto tree = [&]() {
+    __syncthreads();
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
+      if (tid < s) {
+        double a = sr[tid];
+        int ia = si[tid];
+        nanmax_merge(a, ia, sr[tid + s], si[tid + s]);
+        sr[tid] = a;
+        si[tid] = ia;
+      }
+      __syncthreads();
+    }
+  };
   sr[tid] = r;
   si[tid] = ir;
-  __syncthreads();
-  for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-    if (tid < s) {
-      const double a = sr[tid], c = sr[tid + s];
-      if (a == a && (c != c || c > a || (c == a && si[tid + s] < si[tid]))) {
-        sr[tid] = c;
-        si[tid] = si[tid + s];
-      }
-    }
-    __syncthreads();
-  }
+  tree();
   if (tid == 0) rs->parts[g] = RedPart{sr[0], 0.0, 0.0, 0.0, si[0], 0};
   if (!last_group(&rs->count_residual)) return;
+  sr[tid] = tid < G ? rs->parts[tid].e : 0.0;
+  si[tid] = tid < G ? rs->parts[tid].ie : 0x7fffffff;
+  tree();
   if (tid == 0) {
-    double m = rs->parts[0].e;
-    int im = rs->parts[0].ie;
-    for (int k = 1; k < G; ++k) {
-      const double c = rs->parts[k].e;
-      const int ic = rs->parts[k].ie;
-      if (m == m && (c != c || c > m || (c == m && ic < im))) {
-        m = c;
-        im = ic;
-      }
-    }
-    out->residual = m;
-    out->arg_residual = im;
+    out->residual = sr[0];
+    out->arg_residual = si[0];
     rs->count_residual = 0;
   }
 }

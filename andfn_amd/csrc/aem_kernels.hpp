@@ -278,13 +278,17 @@ void launch_trace_pack(const TracePack& tp, hipStream_t st);
 void launch_coef_to_f32(const NetView& net, int ne, float2* out, hipStream_t st);
 // unwrap (circle-type sides, in place in val), DFT sums of every side into dft (side s: X_j at
 // s 2M + j, Y_j at s 2M + M + j), then the element solves
-// citems: the circle-type items among items (types 1, 4, 5), whose sides are unwrapped
+// citems: the circle-type items among items (types 1, 4, 5), whose sides are unwrapped.
+// blend: 0 the fit only, 1 with the first damping step, 2 with both damping steps of the
+// iteration (the second is otherwise launch_blend's)
 void launch_fit(const NetView& net, const FitView& fv, const FitItem* items, int nitems, const FitItem* citems,
                 int ncitems, double* val,
                 const int* pt_off, const DftTile* tiles, int ntiles, const int* dft_sides, double* dft, double damping,
                 int blend, hipStream_t st, bool f32 = false);
 void launch_blend(const NetView& net, int ne, double damping, hipStream_t st);
-void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st);
+// steps = 2: also the second damping step of the iteration (the one launch_blend applies)
+void launch_wells(const NetView& net, const FitView& fv, const int* wells, int nw, double damping, hipStream_t st,
+                  int steps = 1);
 void launch_grow(const NetView& net, const FitView& fv, int ne, GrowParams gp, const int* sched, int* grow,
                  hipStream_t st);
 void launch_csr_gather(const double* A, const SysLayout& sys, int64_t nrows, const int* row_ptr, const int* col,

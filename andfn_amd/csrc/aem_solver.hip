@@ -2037,12 +2037,13 @@ void Solver::enqueue_iteration(const aem::NetView& nv, const aem::FitView& fv, b
   aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 2, 0, st_);
   launch_sweep_shape(aem::MODE_COLLOC, nv, full_, d_val_.p, phase32, ppl_[0] ? ppl_[0] : 1, persist_[0]);
   aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 3, 0, st_);
+  // both damping steps of the iteration inside the fit and well kernels (the fit items and the
+  // wells are every element: no blend launch)
   aem::launch_fit(nv, fv, d_items_.p, (int)items_.size(), d_citems_.p, ncitems_, d_val_.p, full_.d_pt_off.p,
-                  full_.d_dft_tiles.p, (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, 1, st_,
+                  full_.d_dft_tiles.p, (int)full_.dft_tiles.size(), full_.d_dft_sides.p, d_dft_.p, damping, 2, st_,
                   phase32);
-  aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_);
+  aem::launch_wells(nv, fv, d_wells_.p, (int)wells_.size(), damping, st_, 2);
   aem::launch_iter_stamp(d_ctl_.p, stamps, it_cap_, 4, 0, st_);
-  aem::launch_blend(nv, E_, damping, st_);
   aem::launch_reduce(fv, E_, d_y_.p, d_yold_.p, n, d_red_.p, d_scal_.p, st_);
 }
 
