@@ -112,18 +112,22 @@ __device__ void lds_upper_solve(const double* L, const double* rd, double* X, in
 }
 
 // Partial factorisation of a front with k <= KM pivots, one workgroup: LU of the pivot block with
-// partial pivoting over the front's own k rows, then in tiles of KM columns (rows) staged in LDS:
+// partial pivoting over the front's own k rows, then in tiles of XC columns (rows) staged in LDS:
 // U12 = L11^-1 P F12, L21 = F21 U11^-1, and L11^-1, U11^-1.  The Schur update F22 -= L21 U12
-// follows in mf_schur_kernel.
-template <int KM>
+// follows in mf_schur_kernel.  (KM = 128 holds the pivot block in 129 KiB of LDS and stages
+// 24-column tiles in the rest of the 160 KiB.)
+template <int KM, int XC>
 __global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __restrict__ fr,
                                                               const int* __restrict__ list, double* __restrict__ F,
                                                               double* __restrict__ INV, int* __restrict__ piv,
                                                               int* __restrict__ status,
                                                               unsigned long long* __restrict__ growth) {
-  constexpr int LD = KM + 1;
+  static_assert(KM <= 128 && XC <= KM, "pivot search covers two rows per lane of one wavefront");
+  constexpr int LD = KM + 1;   // leading dimension of L and of X's column tiles (k rows)
+  constexpr int LDR = XC + 1;  // leading dimension of X's transposed row tiles (XC rows)
+  constexpr int XN = XC * LD > KM * LDR ? XC * LD : KM * LDR;
   __shared__ double L[KM * LD];  // L[c * LD + r] = (r, c) of the pivot block
-  __shared__ double X[KM * LD];  // a tile of U12, L21 (transposed) or an inverse
+  __shared__ double X[XN];       // a tile of U12, L21 (transposed) or an inverse
   __shared__ double rdiag[KM];
   __shared__ unsigned long long gmax[KM];  // bits of max |L21(:, j)| (inf: zero pivot)
   __shared__ int prm[KM];
@@ -144,8 +148,13 @@ __global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __r
   __syncthreads();
   for (int j = 0; j < k; ++j) {
     if (t < 64) {
+      // rows t and t + 64 (the key's index field holds the row)
       unsigned long long key = 0ull;
       if (t >= j && t < k) key = lu_pivot_key(L[j * LD + t], t);
+      if (KM > 64 && t + 64 >= j && t + 64 < k) {
+        const unsigned long long k2 = lu_pivot_key(L[j * LD + t + 64], t + 64);
+        key = k2 > key ? k2 : key;
+      }
       key = wave_max_u64(key);
       if (t == 0) spiv = lu_pivot_thread(key);
     }
@@ -186,9 +195,9 @@ __global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __r
   }
   if (t < k) piv[f.piv + t] = prm[t];
   __syncthreads();
-  // U12 = L11^-1 P F12, KM columns at a time
-  for (int c0 = 0; c0 < b; c0 += KM) {
-    const int nc = min(KM, b - c0);
+  // U12 = L11^-1 P F12, XC columns at a time
+  for (int c0 = 0; c0 < b; c0 += XC) {
+    const int nc = min(XC, b - c0);
     for (int e = t; e < k * nc; e += 256) {
       const int r = e % k, c = e / k;
       X[c * LD + r] = A[prm[r] + (int64_t)(k + c0 + c) * m];
@@ -201,28 +210,28 @@ __global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __r
     }
     __syncthreads();
   }
-  // L21 = F21 U11^-1, KM rows at a time: X holds the rows' transposes, so U11^T X = F21^T is the
+  // L21 = F21 U11^-1, XC rows at a time: X holds the rows' transposes, so U11^T X = F21^T is the
   // same upper solve from the left once transposed: solved here as forward substitution over columns
-  for (int r0 = 0; r0 < b; r0 += KM) {
-    const int nr = min(KM, b - r0);
+  for (int r0 = 0; r0 < b; r0 += XC) {
+    const int nr = min(XC, b - r0);
     for (int e = t; e < k * nr; e += 256) {
       const int r = e % nr, c = e / nr;
-      X[c * LD + r] = A[(k + r0 + r) + (int64_t)c * m];
+      X[c * LDR + r] = A[(k + r0 + r) + (int64_t)c * m];
     }
     __syncthreads();
     for (int j = 0; j < k; ++j) {
-      for (int r = t; r < nr; r += 256) X[j * LD + r] *= rdiag[j];
+      for (int r = t; r < nr; r += 256) X[j * LDR + r] *= rdiag[j];
       __syncthreads();
       const int nc2 = k - j - 1;
       for (int e = t; e < nr * nc2; e += 256) {
         const int r = e % nr, c = j + 1 + e / nr;
-        X[c * LD + r] = fma(-X[j * LD + r], L[c * LD + j], X[c * LD + r]);
+        X[c * LDR + r] = fma(-X[j * LDR + r], L[c * LD + j], X[c * LDR + r]);
       }
       __syncthreads();
     }
     for (int e = t; e < k * nr; e += 256) {
       const int r = e % nr, c = e / nr;
-      A[(k + r0 + r) + (int64_t)c * m] = X[c * LD + r];
+      A[(k + r0 + r) + (int64_t)c * m] = X[c * LDR + r];
     }
     __syncthreads();
   }
@@ -233,63 +242,72 @@ __global__ __launch_bounds__(256) void mf_small_factor_kernel(const MfFront* __r
   }
   __syncthreads();
   if (t < k) growth[f.piv + t] = gmax[t];
-  // L11^-1 and U11^-1 against the identity
+  // L11^-1 and U11^-1 against the identity, XC columns at a time
   double* Li = INV + f.inv;
   double* Ui = Li + (int64_t)k * k;
   for (int pass = 0; pass < 2; ++pass) {
-    for (int e = t; e < k * k; e += 256) {
-      const int r = e % k, c = e / k;
-      X[c * LD + r] = r == c ? 1.0 : 0.0;
+    for (int c0 = 0; c0 < k; c0 += XC) {
+      const int nc = min(XC, k - c0);
+      for (int e = t; e < k * nc; e += 256) {
+        const int r = e % k, c = e / k;
+        X[c * LD + r] = r == c0 + c ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      if (pass == 0) lds_lower_unit_solve<LD>(L, X, k, nc);
+      else lds_upper_solve<LD>(L, rdiag, X, k, nc);
+      double* out = pass == 0 ? Li : Ui;
+      for (int e = t; e < k * nc; e += 256) {
+        const int r = e % k, c = e / k;
+        out[r + (int64_t)(c0 + c) * k] = X[c * LD + r];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    if (pass == 0) lds_lower_unit_solve<LD>(L, X, k, k);
-    else lds_upper_solve<LD>(L, rdiag, X, k, k);
-    double* out = pass == 0 ? Li : Ui;
-    for (int e = t; e < k * k; e += 256) {
-      const int r = e % k, c = e / k;
-      out[r + (int64_t)c * k] = X[c * LD + r];
-    }
-    __syncthreads();
   }
 }
 
 // Schur update of small fronts: tile (front, first boundary row, first boundary column) of 64 x 64
-// entries of F22 -= L21 U12 (k <= 64), L21 and U12 panels staged in LDS, 4 x 4 outputs per thread
+// entries of F22 -= L21 U12 (k <= kMfSmallMax), L21 and U12 panels staged in LDS 64 pivot columns
+// at a time, 4 x 4 outputs per thread
 __global__ __launch_bounds__(256) void mf_schur_kernel(const MfFront* __restrict__ fr, const int4* __restrict__ tiles,
                                                        double* __restrict__ F) {
-  __shared__ double Ls[kMfSmallMax * 65];
-  __shared__ double Us[kMfSmallMax * 65];
+  constexpr int KC = 64;
+  __shared__ double Ls[KC * 65];
+  __shared__ double Us[KC * 65];
   const int4 tl = tiles[blockIdx.x];
   const MfFront f = fr[tl.x];
   const int m = f.m, k = f.k, b = m - k, i0 = tl.y, j0 = tl.z;
   double* A = F + f.off;
   const int t = threadIdx.x;
-  for (int e = t; e < k * 64; e += 256) {
-    const int i = e & 63, c = e >> 6;
-    Ls[c * 65 + i] = (i0 + i < b) ? A[(k + i0 + i) + (int64_t)c * m] : 0.0;
-  }
-  for (int e = t; e < k * 64; e += 256) {
-    const int c = e % k, j = e / k;
-    Us[c * 65 + j] = (j0 + j < b) ? A[c + (int64_t)(k + j0 + j) * m] : 0.0;
-  }
-  __syncthreads();
   const int ti = t & 15, tj = t >> 4;
   double acc[4][4];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[q][r] = 0.0;
-  for (int c = 0; c < k; ++c) {
-    double a[4], u[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      a[q] = Ls[c * 65 + ti + 16 * q];
-      u[q] = Us[c * 65 + tj + 16 * q];
+  for (int c0 = 0; c0 < k; c0 += KC) {
+    const int kc = min(KC, k - c0);
+    if (c0 > 0) __syncthreads();
+    for (int e = t; e < kc * 64; e += 256) {
+      const int i = e & 63, c = e >> 6;
+      Ls[c * 65 + i] = (i0 + i < b) ? A[(k + i0 + i) + (int64_t)(c0 + c) * m] : 0.0;
     }
+    for (int e = t; e < kc * 64; e += 256) {
+      const int c = e % kc, j = e / kc;
+      Us[c * 65 + j] = (j0 + j < b) ? A[(c0 + c) + (int64_t)(k + j0 + j) * m] : 0.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < kc; ++c) {
+      double a[4], u[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
+      for (int q = 0; q < 4; ++q) {
+        a[q] = Ls[c * 65 + ti + 16 * q];
+        u[q] = Us[c * 65 + tj + 16 * q];
+      }
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[q][r] = fma(a[q], u[r], acc[q][r]);
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[q][r] = fma(a[q], u[r], acc[q][r]);
+    }
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -747,23 +765,12 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
       if (n > 0)
         hipLaunchKernelGGL(mf_extend_kernel, dim3(n), dim3(256), 0, st, d_fr_, d_ext_ + L.ext_beg[s], d_rel_, d_F_);
     }
-    const int n0 = L.small_end[0] - L.small_beg[0], n1 = L.small_end[1] - L.small_beg[1],
-              n2 = L.small_end[2] - L.small_beg[2];
-    if (n0 > 0)
-      hipLaunchKernelGGL(mf_small_factor_kernel<16>, dim3(n0), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[0],
-                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
-    if (n1 > 0)
-      hipLaunchKernelGGL(mf_small_factor_kernel<32>, dim3(n1), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[1],
-                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
-    if (n2 > 0)
-      hipLaunchKernelGGL(mf_small_factor_kernel<64>, dim3(n2), dim3(256), 0, st, d_fr_, d_small_ + L.small_beg[2],
-                         d_F_, d_inv_, d_piv_, d_status_, d_growth_);
-    if (L.tile_end > L.tile_beg)
-      hipLaunchKernelGGL(mf_schur_kernel, dim3(L.tile_end - L.tile_beg), dim3(256), 0, st, d_fr_,
-                         d_tiles_ + L.tile_beg, d_F_);
+    // the level's large fronts on the lanes and its small fronts on st run concurrently (they
+    // write disjoint fronts); the lanes join st before the next level's extend-add
+    bool used[kMfLanes] = {false};
     if (!L.big.empty()) {
       // the level's large fronts are independent: spread over the lanes (largest first, each to
-      // the least loaded lane), forked from and joined back into st
+      // the least loaded lane), forked from st after the extend-add
       MF_HIP(hipEventRecord(fork_, st));
       std::vector<int> order(L.big);
       auto work = [&](int i) {
@@ -772,7 +779,6 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
       };
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
       double load[kMfLanes] = {0.0};
-      bool used[kMfLanes] = {false};
       for (int fi : order) {
         const int li = (int)(std::min_element(load, load + kMfLanes) - load);
         if (!used[li]) {
@@ -782,12 +788,29 @@ int MultifrontalLU::factor(const double* csr_val, const int* csr_ptr, const doub
         factor_big(fi, lanes_[li]);
         load[li] += work(fi) + 1e7;  // (a launch-latency floor per front)
       }
-      for (int li = 0; li < kMfLanes; ++li)
-        if (used[li]) {
-          MF_HIP(hipEventRecord(lanes_[li].done, lanes_[li].s));
-          MF_HIP(hipStreamWaitEvent(st, lanes_[li].done, 0));
-        }
     }
+    const int n0 = L.small_end[0] - L.small_beg[0], n1 = L.small_end[1] - L.small_beg[1],
+              n2 = L.small_end[2] - L.small_beg[2], n3 = L.small_end[3] - L.small_beg[3];
+    if (n0 > 0)
+      hipLaunchKernelGGL((mf_small_factor_kernel<16, 16>), dim3(n0), dim3(256), 0, st, d_fr_,
+                         d_small_ + L.small_beg[0], d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (n1 > 0)
+      hipLaunchKernelGGL((mf_small_factor_kernel<32, 32>), dim3(n1), dim3(256), 0, st, d_fr_,
+                         d_small_ + L.small_beg[1], d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (n2 > 0)
+      hipLaunchKernelGGL((mf_small_factor_kernel<64, 64>), dim3(n2), dim3(256), 0, st, d_fr_,
+                         d_small_ + L.small_beg[2], d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (n3 > 0)
+      hipLaunchKernelGGL((mf_small_factor_kernel<128, 24>), dim3(n3), dim3(256), 0, st, d_fr_,
+                         d_small_ + L.small_beg[3], d_F_, d_inv_, d_piv_, d_status_, d_growth_);
+    if (L.tile_end > L.tile_beg)
+      hipLaunchKernelGGL(mf_schur_kernel, dim3(L.tile_end - L.tile_beg), dim3(256), 0, st, d_fr_,
+                         d_tiles_ + L.tile_beg, d_F_);
+    for (int li = 0; li < kMfLanes; ++li)
+      if (used[li]) {
+        MF_HIP(hipEventRecord(lanes_[li].done, lanes_[li].s));
+        MF_HIP(hipStreamWaitEvent(st, lanes_[li].done, 0));
+      }
   }
   pack_tiles(false, st);
   tiles_t_ready_ = false;

@@ -466,18 +466,18 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   std::vector<MfI2>& ext = out.ext;
   for (int h = 0; h < H; ++h) {
     MfLevel& L = lev[h];
-    for (int cls = 0; cls < 3; ++cls) {
+    for (int cls = 0; cls < 4; ++cls) {
       const int kmax = 16 << cls, kmin = cls ? (8 << cls) : 0;
       L.small_beg[cls] = (int)small_list.size();
       for (int i : byh[h])
-        if (fr[i].k > kmin && fr[i].k <= kmax) small_list.push_back(i);
+        if (fr[i].k > kmin && fr[i].k <= kmax && mf_is_small(fr[i].k, fr[i].m - fr[i].k)) small_list.push_back(i);
       L.small_end[cls] = (int)small_list.size();
     }
     L.tile_beg = (int)tiles.size();
     for (int i : byh[h]) {
       const MfFront& f = fr[i];
       const int b = f.m - f.k;
-      if (f.k > kMfSmallMax) {
+      if (!mf_is_small(f.k, b)) {
         L.big.push_back(i);
         continue;
       }
@@ -547,7 +547,7 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
     for (int l = 0; l < fr[i].k; ++l) out.piv0[fr[i].piv + l] = l;
   out.big_k = 0;
   for (int i = 0; i < nf; ++i)
-    if (fr[i].k > kMfSmallMax) out.big_k = std::max(out.big_k, fr[i].k);
+    if (!mf_is_small(fr[i].k, fr[i].m - fr[i].k)) out.big_k = std::max(out.big_k, fr[i].k);
 }
 
 }  // namespace aem

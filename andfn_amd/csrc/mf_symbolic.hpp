@@ -15,7 +15,11 @@
 
 namespace aem {
 
-constexpr int kMfSmallMax = 64;     // fronts with at most this many pivots: one workgroup in LDS
+constexpr int kMfSmallMax = 128;    // fronts with at most this many pivots: one workgroup in LDS ...
+constexpr int kMfSmallMaxB = 512;   // ... if they have more than 64, only with at most this many boundary rows
+// (the one-workgroup kernel stages the boundary of a 65-128 pivot front 24 rows or columns at a
+// time, one barrier per pivot and tile: with a long boundary the large-front kernels are faster)
+inline bool mf_is_small(int k, int b) { return k <= 64 || (k <= kMfSmallMax && b <= kMfSmallMaxB); }
 constexpr int kMfLeafUnknowns = 48;  // a fracture set is split while its interior has more unknowns
 
 struct MfFront {
@@ -55,8 +59,8 @@ struct MfInput {
 };
 
 struct MfLevel {
-  int small_beg[3], small_end[3];     // small-front lists per pivot class (<= 16, 32, 64)
-  std::vector<int> big;               // fronts with more than kMfSmallMax pivots
+  int small_beg[4], small_end[4];     // small-front lists per pivot class (<= 16, 32, 64, 128)
+  std::vector<int> big;               // fronts that are not small (mf_is_small)
   int tile_beg, tile_end;             // Schur tiles of its small fronts
   std::vector<int> ext_beg, ext_end;  // extend-add items per child slot
   int gv_beg[4], gv_end[4];           // solve panels, per kind (mf_panel_shape / mf_panel_columns)

@@ -78,16 +78,16 @@ void mf_test_fronts(long long* out) {
   }
 }
 
-// levels as 26 ints each: small_beg[3], small_end[3], tile_beg, tile_end, number of big fronts,
+// levels as 28 ints each: small_beg[4], small_end[4], tile_beg, tile_end, number of big fronts,
 // number of extend slots, gv_beg[4], gv_end[4], prow[4], pwide[4]
 void mf_test_levels(int* out) {
   for (size_t h = 0; h < g_sym.levels.size(); ++h) {
     const aem::MfLevel& L = g_sym.levels[h];
-    int* o = out + 26 * h;
-    for (int c = 0; c < 3; ++c) { o[c] = L.small_beg[c]; o[3 + c] = L.small_end[c]; }
-    o[6] = L.tile_beg; o[7] = L.tile_end; o[8] = (int)L.big.size(); o[9] = (int)L.ext_beg.size();
+    int* o = out + 28 * h;
+    for (int c = 0; c < 4; ++c) { o[c] = L.small_beg[c]; o[4 + c] = L.small_end[c]; }
+    o[8] = L.tile_beg; o[9] = L.tile_end; o[10] = (int)L.big.size(); o[11] = (int)L.ext_beg.size();
     for (int c = 0; c < 4; ++c) {
-      o[10 + c] = L.gv_beg[c]; o[14 + c] = L.gv_end[c]; o[18 + c] = L.prow[c]; o[22 + c] = L.pwide[c];
+      o[12 + c] = L.gv_beg[c]; o[16 + c] = L.gv_end[c]; o[20 + c] = L.prow[c]; o[24 + c] = L.pwide[c];
     }
   }
 }

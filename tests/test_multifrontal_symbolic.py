@@ -97,7 +97,7 @@ def _analyse(L, net, delay=None, panel_rows=0):
                                                       "gv_total"])}
     fr = np.zeros((sz["fronts"], 11), np.int64)
     L.mf_test_fronts(_p(fr))
-    lv = np.zeros((sz["levels"], 26), np.int32)
+    lv = np.zeros((sz["levels"], 28), np.int32)
     L.mf_test_levels(_p(lv))
     t = dict(sz=sz, fr=fr, lv=lv)
     for w, k, shape in ((2, "idx", None), (3, "rel", None), (4, "piv0", None), (5, "gptr", None), (6, "gsrc", None),
@@ -112,10 +112,10 @@ def _analyse(L, net, delay=None, panel_rows=0):
     L.mf_test_asmap(_p(t["asmap"]))
     t["big"], t["ext_slots"] = [], []
     for h in range(sz["levels"]):
-        b = np.zeros(max(lv[h, 8], 1), np.int32)
+        b = np.zeros(max(lv[h, 10], 1), np.int32)
         L.mf_test_level_big(h, _p(b))
-        t["big"].append(b[: lv[h, 8]])
-        ns = lv[h, 9]
+        t["big"].append(b[: lv[h, 10]])
+        ns = lv[h, 11]
         eb, ee = np.zeros(max(ns, 1), np.int32), np.zeros(max(ns, 1), np.int32)
         L.mf_test_level_ext(h, _p(eb), _p(ee))
         t["ext_slots"].append(list(zip(eb[:ns], ee[:ns])))
@@ -151,23 +151,25 @@ def _check_bounds(t, net):
     a = t["asmap"]
     assert len(a) == len(net["col"]) and np.all((a >= 0) & (a < sz["front_total"])) and len(np.unique(a)) == len(a)
     ti = t["tiles"]
-    assert np.all(k[ti[:, 0]] <= 64) and np.all(ti[:, 1] < b[ti[:, 0]]) and np.all(ti[:, 2] < b[ti[:, 0]])
+    small = (k <= 64) | ((k <= 128) & (b <= 512))
+    assert np.all(small[ti[:, 0]]) and np.all(ti[:, 1] < b[ti[:, 0]]) and np.all(ti[:, 2] < b[ti[:, 0]])
     ex = t["ext"]
     assert np.all(par[ex[:, 0]] >= 0) and np.all(ex[:, 1] < b[ex[:, 0]])
     level = np.full(nf, -1)
     for h, row in enumerate(t["lv"]):
-        fs = np.concatenate([t["small"][lo:hi] for lo, hi in zip(row[0:3], row[3:6])] + [t["big"][h]])
+        fs = np.concatenate([t["small"][lo:hi] for lo, hi in zip(row[0:4], row[4:8])] + [t["big"][h]])
         assert np.all(level[fs] == -1)
         level[fs] = h
-        for c, (lo, hi) in enumerate(zip(row[0:3], row[3:6])):
+        for c, (lo, hi) in enumerate(zip(row[0:4], row[4:8])):
             kk = k[t["small"][lo:hi]]
             assert np.all(kk <= 16 << c) and np.all(kk > (8 << c if c else 0))
-        assert np.all(k[t["big"][h]] > 64)
+            assert np.all(small[t["small"][lo:hi]])
+        assert not np.any(small[t["big"][h]])
         for lo, hi in t["ext_slots"][h]:
             assert np.all(level[par[ex[lo:hi, 0]]] == h) if hi > lo else True
     assert np.all(level >= 0)
     assert np.all((par < 0) | (level[np.maximum(par, 0)] > level))
-    nsmall = sum(hi - lo for row in t["lv"] for lo, hi in zip(row[0:3], row[3:6]))
+    nsmall = sum(hi - lo for row in t["lv"] for lo, hi in zip(row[0:4], row[4:8]))
     assert nsmall + sum(len(x) for x in t["big"]) == nf
     _check_solve_items(t, level)
     return level
@@ -192,8 +194,8 @@ def _check_solve_items(t, level):
     pos = 0
     for h, row in enumerate(lv):
         for kind in range(4):
-            lo, hi = row[10 + kind], row[14 + kind]
-            R, wide = row[18 + kind], row[22 + kind]
+            lo, hi = row[12 + kind], row[16 + kind]
+            R, wide = row[20 + kind], row[24 + kind]
             assert R in (4, 8, 16) and wide in (0, 1) and (R == 16 or not wide)
             C = BLOCK // R
             assert lo == pos and hi >= lo
@@ -280,8 +282,8 @@ def _replay(t, net, level):
         """One launch of the panel products: each panel's rows in full over its column blocks,
         then the kind's epilogue (the tables drive everything, as on the device)."""
         gv, row = t["gv"], t["lv"][h]
-        R = row[18 + kind]
-        for f, r0, c_lo, nblk in gv[row[10 + kind]: row[14 + kind]]:
+        R = row[20 + kind]
+        for f, r0, c_lo, nblk in gv[row[12 + kind]: row[16 + kind]]:
             kk, bb = k[f], m[f] - k[f]
             rows, cols = _kind_shape(kind, kk, bb)
             cs = np.arange(c_lo, min(cols, c_lo + BLOCK // R * nblk))
@@ -325,8 +327,8 @@ def _replay(t, net, level):
     def products_t(kind, h, V, ys, cvec):
         gv, row = t["gv"], t["lv"][h]
         s0 = kind - 4
-        R = row[18 + s0]
-        for f, r0, c_lo, nblk in gv[row[10 + s0]: row[14 + s0]]:
+        R = row[20 + s0]
+        for f, r0, c_lo, nblk in gv[row[12 + s0]: row[16 + s0]]:
             kk, bb = k[f], m[f] - k[f]
             rows, cols = _kind_shape(s0, kk, bb)
             cs = np.arange(c_lo, min(cols, c_lo + BLOCK // R * nblk))
@@ -413,7 +415,7 @@ def test_short_solve_panels(lib, panel_rows):
     net = _network("dfn200")
     t = _analyse(lib, net, panel_rows=panel_rows)
     level = _check_bounds(t, net)
-    assert set(t["lv"][:, 19:22].ravel()) == {panel_rows} and set(t["lv"][:, 18]) == {16}
+    assert set(t["lv"][:, 21:24].ravel()) == {panel_rows} and set(t["lv"][:, 20]) == {16}
     solve, solve_t = _replay(t, net, level)
     rng = np.random.default_rng(5)
     A = net["A"].tocsc()
