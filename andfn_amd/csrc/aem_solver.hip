@@ -175,6 +175,10 @@ struct TargetSet {
   // launch grids: the device copies of the tile lists are laid out for the XCDs (xcd_order) and
   // padded with empty tiles, so they are at least as long as the host lists
   int ndev[2] = {0, 0};
+  // the counts the tables were built for (a whole set only): a run that starts from the same
+  // counts keeps them
+  bool keyed = false;
+  std::vector<int> key_nint, key_ncoef;
   int64_t npts() const { return pt_off.empty() ? 0 : pt_off.back(); }
   int ntiles(int ppl) const { return ppl == 2 ? ndev[1] : ndev[0]; }
   aem::TargetView view(int ppl = 1) const {
@@ -484,7 +488,7 @@ class Solver {
   void upload();
   void snapshot();
   void restore();
-  void build_targets(TargetSet& ts, const std::vector<char>* only, bool rhs);
+  bool build_targets(TargetSet& ts, const std::vector<char>* only, bool rhs);
   void upload_targets(TargetSet& ts);
   std::vector<Tile> xcd_order(const std::vector<Tile>& tiles, const TargetSet& ts) const;
   void build_items(const TargetSet& ts, std::vector<FitItem>& items, const std::vector<char>* only);
@@ -920,7 +924,11 @@ void Solver::restore() {
 // Pack the target points of every fracture contiguously and cut them into 64-point tiles.
 // Collocation set: every non-well element side (an intersection has one side per fracture);
 // discharge set (rhs): every discharge element side with K points each.
-void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rhs) {
+// Returns false when the set's tables (host and device) were kept: a whole set (only == nullptr)
+// already built for the current counts (the discharge-point set depends on the structure alone).
+bool Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rhs) {
+  if (!only && ts.keyed && (rhs || (ts.key_nint == nint_ && ts.key_ncoef == ncoef_))) return false;
+  ts.keyed = false;
   ts.side_elem.clear();
   ts.side_which.clear();
   ts.pt_off.assign(1, 0);
@@ -986,6 +994,12 @@ void Solver::build_targets(TargetSet& ts, const std::vector<char>* only, bool rh
     }
   }
   upload_targets(ts);
+  if (!only) {
+    ts.keyed = true;
+    ts.key_nint = nint_;
+    ts.key_ncoef = ncoef_;
+  }
+  return true;
 }
 
 // Device order of a tile list.  Workgroups go to the 8 XCDs of the MI355X round-robin
@@ -2158,12 +2172,13 @@ int Solver::solve(const andfn_mi_options* opt, andfn_mi_info* info) {
   aem::FitView fv = fit();
 
   // setup: tables, initial discharges, errors, branch cuts, matrix
-  build_targets(full_, nullptr, false);
-  build_items(full_, items_, nullptr);
-  d_items_.upload(items_, st_);
-  ncitems_ = upload_circle_items(items_, d_citems_);
-  build_targets(rhs_, nullptr, true);
-  {
+  // (a later run from the same counts keeps the tables, the items and their device copies)
+  if (build_targets(full_, nullptr, false)) {
+    build_items(full_, items_, nullptr);
+    d_items_.upload(items_, st_);
+    ncitems_ = upload_circle_items(items_, d_citems_);
+  }
+  if (build_targets(rhs_, nullptr, true)) {
     std::vector<int> s0(E_, -1), s1(E_, -1);
     for (size_t s = 0; s < rhs_.side_elem.size(); ++s) (rhs_.side_which[s] ? s1 : s0)[rhs_.side_elem[s]] = (int)s;
     rhs_items_.clear();
