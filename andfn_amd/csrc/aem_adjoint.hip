@@ -460,26 +460,23 @@ __global__ void adj_refine_out_kernel(const T* __restrict__ v, const int* __rest
   }
 }
 
-// packed coefficient vectors (only k < ncoef of elements with coefficients): pidx maps a packed
-// entry to its [E][M] position
-__global__ void adj_pack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ full,
-                                double2* __restrict__ packed) {
+// packed coefficient vectors (the real components that the fit reads, k < ncoef): pidx maps a
+// packed entry to its position in the [E][M] complex array taken as doubles (2 (e M + k) + part)
+__global__ void adj_pack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double* __restrict__ full,
+                                double* __restrict__ packed) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) packed[i] = full[pidx[i]];
 }
-__global__ void adj_unpack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ packed,
-                                  double2* __restrict__ full) {
+__global__ void adj_unpack_kernel(int64_t np, const int64_t* __restrict__ pidx, const double* __restrict__ packed,
+                                  double* __restrict__ full) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < np) full[pidx[i]] = packed[i];
 }
 // out = v - packed(gc)
-__global__ void adj_residual_op_kernel(int64_t np, const int64_t* __restrict__ pidx, const double2* __restrict__ v,
-                                       const double2* __restrict__ gc, double2* __restrict__ out) {
+__global__ void adj_residual_op_kernel(int64_t np, const int64_t* __restrict__ pidx, const double* __restrict__ v,
+                                       const double* __restrict__ gc, double* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < np) {
-    const double2 a = v[i], b = gc[pidx[i]];
-    out[i] = make_double2(a.x - b.x, a.y - b.y);
-  }
+  if (i < np) out[i] = v[i] - gc[pidx[i]];
 }
 
 // ---- launchers
@@ -544,15 +541,15 @@ void launch_adj_tgrad(const NetView& net, const FitItem* items, int nitems, cons
   hipLaunchKernelGGL(adj_tgrad_kernel, dim3(nitems), dim3(64), 0, st, net, items, nitems, lam, val, pt_off, s_out);
 }
 
-void launch_adj_pack(int64_t np, const int64_t* pidx, const double2* full, double2* packed, hipStream_t st) {
+void launch_adj_pack(int64_t np, const int64_t* pidx, const double* full, double* packed, hipStream_t st) {
   if (np <= 0) return;
   hipLaunchKernelGGL(adj_pack_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, full, packed);
 }
-void launch_adj_unpack(int64_t np, const int64_t* pidx, const double2* packed, double2* full, hipStream_t st) {
+void launch_adj_unpack(int64_t np, const int64_t* pidx, const double* packed, double* full, hipStream_t st) {
   if (np <= 0) return;
   hipLaunchKernelGGL(adj_unpack_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, packed, full);
 }
-void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, const double2* gc, double2* out,
+void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double* v, const double* gc, double* out,
                             hipStream_t st) {
   if (np <= 0) return;
   hipLaunchKernelGGL(adj_residual_op_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, np, pidx, v, gc,

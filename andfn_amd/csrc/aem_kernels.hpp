@@ -419,9 +419,9 @@ void launch_adj_update(int64_t cnt, const double2* jc, const double2* gc, double
                        unsigned long long* stats, hipStream_t st);
 void launch_adj_tgrad(const NetView& net, const FitItem* items, int nitems, const double2* lam, const double* val,
                       const int* pt_off, double* s_out, hipStream_t st);
-void launch_adj_pack(int64_t np, const int64_t* pidx, const double2* full, double2* packed, hipStream_t st);
-void launch_adj_unpack(int64_t np, const int64_t* pidx, const double2* packed, double2* full, hipStream_t st);
-void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double2* v, const double2* gc, double2* out,
+void launch_adj_pack(int64_t np, const int64_t* pidx, const double* full, double* packed, hipStream_t st);
+void launch_adj_unpack(int64_t np, const int64_t* pidx, const double* packed, double* full, hipStream_t st);
+void launch_adj_residual_op(int64_t np, const int64_t* pidx, const double* v, const double* gc, double* out,
                             hipStream_t st);
 // B_b = A_b^T for the nblk column-major n x n slabs
 template <class T>

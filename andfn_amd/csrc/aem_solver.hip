@@ -594,7 +594,7 @@ class Solver {
   DBuf<unsigned long long> d_stats_;
   DBuf<double> d_jzero_, d_hv_;
   DBuf<int64_t> d_pidx_;
-  DBuf<double2> d_gv_, d_xv_, d_wv_;
+  DBuf<double> d_gv_, d_xv_, d_wv_;
   DBuf<double> d_V_;
   DBuf<double> d_AT_, d_linvT_, d_uinvT_, d_csrT_val_;
   DBuf<int> d_csrT_ptr_, d_csrT_col_;
@@ -3385,13 +3385,23 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     }
   } else {
     // restarted GMRES on (I - B) lam = g, g = J_c + RHS_c^T A^-T J_y, in the packed space of the
-    // active coefficients (k < ncoef of every element with coefficients)
+    // active coefficients (k < ncoef of every element with coefficients), real components only
+    // where the fit reads only one: Re for intersections and constant-head lines (real
+    // coefficients), Im for impermeable lines.  The other component of such an element is read
+    // by nothing and receives nothing (the gather writes it zero), an identity block of I - B
+    // that GMRES need not carry.
     std::vector<int64_t> pidx;
-    for (int e = 0; e < E_; ++e)
-      if (etype_[e] != 2)
-        for (int k = 0; k < ncoef_[e]; ++k) pidx.push_back((int64_t)e * M_ + k);
+    for (int e = 0; e < E_; ++e) {
+      const int ty = etype_[e];
+      if (ty == 2) continue;
+      for (int k = 0; k < ncoef_[e]; ++k) {
+        const int64_t q = 2 * ((int64_t)e * M_ + k);
+        if (ty != 5) pidx.push_back(q);
+        if (ty != 0 && ty != 3) pidx.push_back(q + 1);
+      }
+    }
     const int64_t np = (int64_t)pidx.size();
-    const rocblas_int N = (rocblas_int)(2 * np);
+    const rocblas_int N = (rocblas_int)np;
     const int restart = 40;
     d_pidx_.upload(pidx, st_);
     d_gv_.alloc(np);
@@ -3404,20 +3414,20 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
     apply(true);
     HIP_CHECK(hipMemsetAsync(d_stats_.p, 0, sizeof(unsigned long long) * 4, st_));
     aem::launch_adj_update((int64_t)ncf, d_jc_.p, d_gc_.p, d_lam_.p, 1.0, d_stats_.p, st_);  // lam = J_c + gc
-    aem::launch_adj_pack(np, d_pidx_.p, d_lam_.p, d_gv_.p, st_);
+    aem::launch_adj_pack(np, d_pidx_.p, (const double*)d_lam_.p, d_gv_.p, st_);
     ++nit;
-    auto op = [&](const double2* v, double2* out) {  // out = (I - B) v (reads only the active entries of lam)
-      aem::launch_adj_unpack(np, d_pidx_.p, v, d_lam_.p, st_);
+    auto op = [&](const double* v, double* out) {  // out = (I - B) v (reads only the packed entries of lam)
+      aem::launch_adj_unpack(np, d_pidx_.p, v, (double*)d_lam_.p, st_);
       apply(false);
-      aem::launch_adj_residual_op(np, d_pidx_.p, v, d_gc_.p, out, st_);
+      aem::launch_adj_residual_op(np, d_pidx_.p, v, (const double*)d_gc_.p, out, st_);
       ++nit;
     };
     double gnorm = 0.0;
-    RB_CHECK(rocblas_dnrm2(rb_, N, (const double*)d_gv_.p, 1, &gnorm));
+    RB_CHECK(rocblas_dnrm2(rb_, N, d_gv_.p, 1, &gnorm));
     double* V = d_V_.p;
     std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), sv(restart + 1), h(restart + 1),
         h2(restart + 1);
-    double* w = (double*)d_wv_.p;
+    double* w = d_wv_.p;
     if (gnorm == 0.0) {
       conv = true;
       delta = 0.0;
@@ -3431,7 +3441,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       } else {
         op(d_xv_.p, d_wv_.p);
         RB_CHECK(rocblas_dscal(rb_, N, &mone, w, 1));
-        RB_CHECK(rocblas_daxpy(rb_, N, &one, (const double*)d_gv_.p, 1, w, 1));
+        RB_CHECK(rocblas_daxpy(rb_, N, &one, d_gv_.p, 1, w, 1));
       }
       double beta = 0.0;
       RB_CHECK(rocblas_dnrm2(rb_, N, w, 1, &beta));
@@ -3447,7 +3457,7 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
       sv[0] = beta;
       int jd = 0;
       for (int j = 0; j < restart && nit < maxit; ++j) {
-        op((const double2*)(V + (size_t)j * N), d_wv_.p);
+        op(V + (size_t)j * N, d_wv_.p);
         // classical Gram-Schmidt, repeated when the first pass cancels more than 1 - 1/sqrt(2) of
         // the vector's norm (the criterion of Daniel, Gragg, Kaufman and Stewart, 1976): each
         // pass reads the j + 1 basis vectors twice, the bulk of the iteration's memory traffic
@@ -3500,12 +3510,12 @@ int Solver::adjoint(const andfn_mi_objective* obj, const andfn_mi_adjoint_option
         y[i] = acc / H[(size_t)i * restart + i];
       }
       HIP_CHECK(hipMemcpyAsync(d_hv_.p, y.data(), sizeof(double) * jd, hipMemcpyHostToDevice, st_));
-      RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, N, jd, &one, V, N, d_hv_.p, 1, &one, (double*)d_xv_.p, 1));
+      RB_CHECK(rocblas_dgemv(rb_, rocblas_operation_none, N, jd, &one, V, N, d_hv_.p, 1, &one, d_xv_.p, 1));
       HIP_CHECK(hipStreamSynchronize(st_));
       if (delta <= tol) conv = true;
     }
     HIP_CHECK(hipMemsetAsync(d_lam_.p, 0, sizeof(double2) * ncf, st_));
-    aem::launch_adj_unpack(np, d_pidx_.p, d_xv_.p, d_lam_.p, st_);
+    aem::launch_adj_unpack(np, d_pidx_.p, d_xv_.p, (double*)d_lam_.p, st_);
   }
   // mu of the final lam
   apply(true);
