@@ -15,9 +15,10 @@
 //   U12 = L11^-1 P F12, F22 -= L21 U12; the inverses of L11 and U11 are kept for the solves;
 //   solve (A x = b and A^T y = c), four launches per level, each covering every front of the level.
 //
-// Fronts with at most 64 pivots are factored by one workgroup each in LDS (all fronts of a level in
-// one launch, their Schur updates in one tiled launch); larger fronts by the tournament-pivoting
-// kernels of aem_lu.hip with the pivot search limited to the front's rows, and rocBLAS.
+// Fronts with at most 64 pivots, and with 65 to 128 and a boundary of at most 512 rows, are
+// factored by one workgroup each in LDS (all fronts of a level and class in one launch, their Schur
+// updates in one tiled launch); larger fronts by the tournament-pivoting kernels of aem_lu.hip with
+// the pivot search limited to the front's rows, and rocBLAS, on several streams beside them.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -112,8 +113,11 @@ class MultifrontalLU {
   double* d_vec_ = nullptr;  // per-front solve vectors
   double* d_ys_ = nullptr;   // transposed solve: unscaled result in the solver's layout
   // The large fronts of a level are factored concurrently on kMfLanes streams of default priority,
-  // each with its own rocBLAS handle (its own workspace) and tournament work buffers.
-  static constexpr int kMfLanes = 4;
+  // each with its own rocBLAS handle (its own workspace) and tournament work buffers.  HIP shares
+  // the process's hardware queues (4 by default) among its streams; 6 lanes beside the handle's
+  // stream measured fastest (factorisation of the 64-realisation batch: 163, 94, 120, 96, 73 and
+  // 79 ms with 1, 2, 3, 4, 6 and 8 lanes; cfg3: 185, 124, 138, 122, 108 and 111 ms).
+  static constexpr int kMfLanes = 6;
   struct Lane {
     hipStream_t s = nullptr;
     rocblas_handle rb = nullptr;
