@@ -210,11 +210,24 @@ __global__ __launch_bounds__(64) void adj_gather_kernel(NetView net, const int* 
       const bool line = !BOTH && (sl.kind == KIND_LINE || sl.kind == KIND_IMPL);
       const double sg = line ? sl.sign : 1.0;
       const bool slot_q = do_q && (sl.kind == KIND_LINE || sl.kind == KIND_WELL);
+      // the next point's owner, weight and position are requested before this point's work
+      int pel_n = -1;
+      double2 wd_n = make_double2(0.0, 0.0), z_n = make_double2(0.0, 0.0);
+      if (pb + lane < pe) {
+        pel_n = excl ? pelem[pb + lane] : -1;
+        wd_n = w[pb + lane];
+        z_n = pz[pb + lane];
+      }
       for (int p = pb + lane; p < pe; p += 64) {
-        if (excl && pelem[p] == src) continue;
-        const double2 wd = w[p];
+        const int pel = pel_n;
+        const double2 wd = wd_n, z = z_n;
+        if (p + 64 < pe) {
+          pel_n = excl ? pelem[p + 64] : -1;
+          wd_n = w[p + 64];
+          z_n = pz[p + 64];
+        }
+        if (pel == src) continue;
         if (wd.x == 0.0 && wd.y == 0.0) continue;
-        const double2 z = pz[p];
         double2 base = make_double2(0.0, 0.0);
         if (line) {
           const double2 Z = make_double2(fma(sl.alpha.x, z.x, fma(-sl.alpha.y, z.y, sl.beta.x)),
