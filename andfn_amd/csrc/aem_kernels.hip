@@ -174,10 +174,21 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
   }
   const int kbeg = net.mem_ptr[f];
   const int kend = net.mem_ptr[f + 1];
+  // the next source's slot, coefficient count and discharge are requested before this source's
+  // work (a dependent chain of scalar loads otherwise paid at the head of every source)
+  Slot sl_n = net.slots[kbeg < kend ? kbeg : 0];
+  int m_n = kbeg < kend ? net.ncoef[sl_n.elem] : 0;
+  double q_n = kbeg < kend ? net.q[sl_n.elem] : 0.0;
   for (int ks = kbeg; ks < kend; ++ks) {
-    const Slot sl = net.slots[ks];
+    const Slot sl = sl_n;
     const int src = sl.elem;
-    const int m = net.ncoef[src];
+    const int m = m_n;
+    const double q_src = q_n;
+    if (ks + 1 < kend) {
+      sl_n = net.slots[ks + 1];
+      m_n = net.ncoef[sl_n.elem];
+      q_n = net.q[sl_n.elem];
+    }
     const double2* __restrict__ cf = net.coef + (size_t)src * net.M;
     double tre[P], tim[P];
 #pragma unroll
@@ -208,7 +219,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         if (sl.kind == KIND_LINE && net.real_lines) horner_real_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
         else horner_f_p<P>(net.coef32 + (size_t)src * net.M, m, wf, hf);
         const float sg = (float)sl.sign;
-        const float qq = (sl.kind == KIND_LINE && use_q) ? (float)(sl.sign * net.q[src] * kInv2Pi) : 0.0f;
+        const float qq = (sl.kind == KIND_LINE && use_q) ? (float)(sl.sign * q_src * kInv2Pi) : 0.0f;
 #pragma unroll
         for (int k = 0; k < P; ++k) {
           float gre = sg * hf[k].x, gim = sg * hf[k].y;
@@ -224,7 +235,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         continue;
       }
       // the log term first, so chi and |chi|^2 are dead during the series
-      const double qq = (sl.kind == KIND_LINE && use_q) ? sl.sign * net.q[src] * kInv2Pi : 0.0;
+      const double qq = (sl.kind == KIND_LINE && use_q) ? sl.sign * q_src * kInv2Pi : 0.0;
       double2 w[P], h[P];
       double lg[P], at[P];
 #pragma unroll
@@ -262,7 +273,7 @@ __device__ __forceinline__ void sweep_tile(const NetView net, const TargetView t
         tim[k] = h[k].y;
       }
     } else if (sl.kind == KIND_WELL) {
-      const double qq = use_q ? net.q[src] * kInv2Pi : 0.0;
+      const double qq = use_q ? q_src * kInv2Pi : 0.0;
 #pragma unroll
       for (int k = 0; k < P; ++k) {
         const double2 chi = c2(fma(sl.alpha.x, z[k].x, sl.beta.x), fma(sl.alpha.x, z[k].y, sl.beta.y));
