@@ -9,8 +9,11 @@
 #include "mf_symbolic.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <stdexcept>
+#include <string>
+#include <thread>
 #include <utility>
 
 namespace aem {
@@ -412,28 +415,66 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   std::vector<std::vector<int>> fkids(nf);
   for (int i = 0; i < nf; ++i)
     for (int c : kids[post[i]]) fkids[i].push_back(front_of_node[c]);
+  // the assembly map, front by front on a few host threads (every matrix entry belongs to one
+  // front, so the fronts write disjoint entries); the rest in front order below
+  {
+    const unsigned hc = std::thread::hardware_concurrency();
+    const int nth = (int)std::max(1u, std::min(8u, hc ? hc : 1u));
+    std::atomic<int> next{0};
+    std::atomic<bool> failed{false};
+    std::string why;
+    auto work = [&](std::vector<int>& lc) {
+      for (;;) {
+        const int i0 = next.fetch_add(16);
+        if (i0 >= nf || failed.load()) return;
+        for (int i = i0; i < std::min(nf, i0 + 16); ++i) {
+          const MfFront& f = fr[i];
+          const int* ix = idx.data() + f.idx;
+          for (int l = 0; l < f.m; ++l) lc[ix[l]] = l;
+          const int first = elim[ix[0]];
+          const char* err = nullptr;
+          for (int lu = 0; lu < f.k && !err; ++lu) {
+            const int u = ix[lu];
+            for (int q = rp[u]; q < rp[u + 1]; ++q) {
+              const int v = cl[q];
+              if (elim[v] < first) continue;  // v eliminated in a descendant, which placed this entry
+              const int lv = lc[v];
+              if (lv < 0) {
+                err = "multifrontal analysis: entry outside its front";
+                break;
+              }
+              asmap[q] = f.off + lu + (int64_t)lv * f.m;
+              if (lv >= f.k) {  // (row v, column u) with v in the boundary
+                const int* b0 = cl + rp[v];
+                const int* b1 = cl + rp[v + 1];
+                const int* hit = std::lower_bound(b0, b1, u);
+                if (hit == b1 || *hit != u) {
+                  err = "multifrontal analysis: pattern is not symmetric";
+                  break;
+                }
+                asmap[hit - cl] = f.off + lv + (int64_t)lu * f.m;
+              }
+            }
+          }
+          for (int l = 0; l < f.m; ++l) lc[ix[l]] = -1;
+          if (err) {
+            if (!failed.exchange(true)) why = err;
+            return;
+          }
+        }
+      }
+    };
+    std::vector<std::vector<int>> locs(nth - 1, std::vector<int>(N, -1));
+    std::vector<std::thread> pool;
+    for (int t = 0; t + 1 < nth; ++t) pool.emplace_back(work, std::ref(locs[t]));
+    work(loc);
+    for (auto& th : pool) th.join();
+    if (failed.load()) throw std::runtime_error(why);
+  }
   for (int i = 0; i < nf; ++i) {
     const MfFront& f = fr[i];
     const int* ix = idx.data() + f.idx;
     for (int l = 0; l < f.m; ++l) loc[ix[l]] = l;
-    const int first = elim[ix[0]];
-    for (int lu = 0; lu < f.k; ++lu) {
-      const int u = ix[lu];
-      for (int q = rp[u]; q < rp[u + 1]; ++q) {
-        const int v = cl[q];
-        if (elim[v] < first) continue;  // v eliminated in a descendant, which placed this entry
-        const int lv = loc[v];
-        if (lv < 0) throw std::runtime_error("multifrontal analysis: entry outside its front");
-        asmap[q] = f.off + lu + (int64_t)lv * f.m;
-        if (lv >= f.k) {  // (row v, column u) with v in the boundary
-          const int* b0 = cl + rp[v];
-          const int* b1 = cl + rp[v + 1];
-          const int* hit = std::lower_bound(b0, b1, u);
-          if (hit == b1 || *hit != u) throw std::runtime_error("multifrontal analysis: pattern is not symmetric");
-          asmap[hit - cl] = f.off + lv + (int64_t)lu * f.m;
-        }
-      }
-    }
     // children: their boundaries inside this front
     gl.assign(f.m, {});
     for (int c : fkids[i]) {
