@@ -120,17 +120,25 @@ __global__ __launch_bounds__(64) void adj_fitT_kernel(NetView net, const FitItem
   for (int i = lane; i < n; i += 64) {
     double acc = 0.0;
     if (ty == 1) {
-      // c_j = -(2/n) h_j (Y_j + i X_j), h_0 = 1/2, angles 2 pi i j / n
+      // c_j = -(2/n) h_j (Y_j + i X_j), h_0 = 1/2, angles 2 pi i j / n: table entry 4 (i j mod n),
+      // i j mod n stepped by i (i < n, one subtraction)
+      int r = 0;
       for (int j = 0; j < m; ++j) {
-        const double2 cs = tr[4 * (int)((int64_t)i * j % n)];  // angle 2 pi i j / n
+        const double2 cs = tr[4 * r];  // angle 2 pi i j / n
         const double h = (j == 0) ? 0.5 : 1.0;
         acc = fma(h, fma(l[j].x, cs.y, l[j].y * cs.x), acc);
+        r += i;
+        if (r >= n) r -= n;
       }
       acc *= -two_n;
     } else {
-      const int64_t period = 4 * (int64_t)n;
+      // angle j (2 i + 1) pi / (2 n): table entry j (2 i + 1) mod 4 n, stepped by 2 i + 1 < 4 n
+      const int period = 4 * n, step = 2 * i + 1;
+      int r = step;
       for (int j = 1; j < m; ++j) {
-        const double2 cs = tr[((int64_t)j * (2 * i + 1)) % period];  // angle j (2 i + 1) pi / (2 n)
+        const double2 cs = tr[r];
+        r += step;
+        if (r >= period) r -= period;
         const double s = cs.y, c = cs.x;
         if (ty == 0 || ty == 3) acc = fma(l[j].x, c, acc);
         else if (ty == 4) acc += l[j].x * s - l[j].y * c;  // c_j = ((2/n) Y_j, -(2/n) X_j)

@@ -485,13 +485,16 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
     const int j = sinpart ? col - m : col;
     f64x4 acc = {0.0, 0.0, 0.0, 0.0};
     f32x4 accf = {0.0f, 0.0f, 0.0f, 0.0f};
+    // angle j theta_i = kk pi / (2 n): kk = 4 (i j mod n) on the full circle (theta_i = 2 pi i / n),
+    // kk = j (2 i + 1) mod 4 n at the midpoint angles (theta_i = (2 i + 1) pi / (2 n)); i steps by
+    // 4 per pass, so kk steps by 4 (4 j mod n) or 8 j mod 4 n (below the period: one subtraction)
+    const int period = 4 * n;
+    const int d = tl.family == 1 ? 4 * ((4 * j) % n) : (8 * j) % period;
+    int kk = tl.family == 1 ? 4 * ((kq * j) % n) : (j * (2 * kq + 1)) % period;
     for (int k0 = 0; k0 < n; k0 += 4) {
       const int i = k0 + kq;
       const bool iok = i < n;
       const double a = (va != nullptr && iok) ? va[i] : 0.0;
-      // angle j theta_i = kk pi / (2 n): kk = 4 (i j mod n) on the full circle (theta_i = 2 pi i / n),
-      // kk = j (2 i + 1) mod 4 n at the midpoint angles (theta_i = (2 i + 1) pi / (2 n))
-      const int kk = tl.family == 1 ? 4 * (int)(((int64_t)i * j) % n) : (int)(((int64_t)j * (2 * i + 1)) % (4 * (int64_t)n));
       if (F32) {
         float w = 0.0f;
         if (colok && iok) {
@@ -507,6 +510,8 @@ __global__ __launch_bounds__(256) void dft_kernel(const DftTile* __restrict__ ti
         }
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, w, acc, 0, 0, 0);
       }
+      kk += d;
+      if (kk >= period) kk -= period;
     }
     if (colok) {
 #pragma unroll
