@@ -505,20 +505,24 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
   std::vector<int>& small_list = out.small_list;
   std::vector<MfI4>& tiles = out.tiles;
   std::vector<MfI2>& ext = out.ext;
+  std::vector<char> small(nf, 0);
   for (int h = 0; h < H; ++h) {
     MfLevel& L = lev[h];
+    int n128 = 0;  // fronts of the 65-128 class on this level
+    for (int i : byh[h]) n128 += fr[i].k > 64 && mf_is_small(fr[i].k, fr[i].m - fr[i].k);
+    for (int i : byh[h]) small[i] = fr[i].k <= 64 || (n128 >= kMfSmall128Min && mf_is_small(fr[i].k, fr[i].m - fr[i].k));
     for (int cls = 0; cls < 4; ++cls) {
       const int kmax = 16 << cls, kmin = cls ? (8 << cls) : 0;
       L.small_beg[cls] = (int)small_list.size();
       for (int i : byh[h])
-        if (fr[i].k > kmin && fr[i].k <= kmax && mf_is_small(fr[i].k, fr[i].m - fr[i].k)) small_list.push_back(i);
+        if (fr[i].k > kmin && fr[i].k <= kmax && small[i]) small_list.push_back(i);
       L.small_end[cls] = (int)small_list.size();
     }
     L.tile_beg = (int)tiles.size();
     for (int i : byh[h]) {
       const MfFront& f = fr[i];
       const int b = f.m - f.k;
-      if (!mf_is_small(f.k, b)) {
+      if (!small[i]) {
         L.big.push_back(i);
         continue;
       }
@@ -588,7 +592,7 @@ void mf_analyse(const MfInput& in, MfSymbolic& out) {
     for (int l = 0; l < fr[i].k; ++l) out.piv0[fr[i].piv + l] = l;
   out.big_k = 0;
   for (int i = 0; i < nf; ++i)
-    if (!mf_is_small(fr[i].k, fr[i].m - fr[i].k)) out.big_k = std::max(out.big_k, fr[i].k);
+    if (!small[i]) out.big_k = std::max(out.big_k, fr[i].k);
 }
 
 }  // namespace aem

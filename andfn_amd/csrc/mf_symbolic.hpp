@@ -17,8 +17,12 @@ namespace aem {
 
 constexpr int kMfSmallMax = 128;    // fronts with at most this many pivots: one workgroup in LDS ...
 constexpr int kMfSmallMaxB = 512;   // ... if they have more than 64, only with at most this many boundary rows
+constexpr int kMfSmall128Min = 16;  // ... and only on a level with at least this many such fronts
 // (the one-workgroup kernel stages the boundary of a 65-128 pivot front 24 rows or columns at a
-// time, one barrier per pivot and tile: with a long boundary the large-front kernels are faster)
+// time, one barrier per pivot and tile, and pays off when a level holds many such fronts, as the
+// 64-realisation batch does with 29, 130, 156 and 57 on four levels: 104 against 73 ms per
+// factorisation; cfg3's levels hold at most 7, for which the large-front streams are faster:
+// 104 against 108 ms)
 inline bool mf_is_small(int k, int b) { return k <= 64 || (k <= kMfSmallMax && b <= kMfSmallMaxB); }
 constexpr int kMfLeafUnknowns = 48;  // a fracture set is split while its interior has more unknowns
 
@@ -60,7 +64,7 @@ struct MfInput {
 
 struct MfLevel {
   int small_beg[4], small_end[4];     // small-front lists per pivot class (<= 16, 32, 64, 128)
-  std::vector<int> big;               // fronts that are not small (mf_is_small)
+  std::vector<int> big;               // fronts that are not small (mf_is_small, kMfSmall128Min)
   int tile_beg, tile_end;             // Schur tiles of its small fronts
   std::vector<int> ext_beg, ext_end;  // extend-add items per child slot
   int gv_beg[4], gv_end[4];           // solve panels, per kind (mf_panel_shape / mf_panel_columns)

@@ -151,6 +151,8 @@ def _check_bounds(t, net):
     a = t["asmap"]
     assert len(a) == len(net["col"]) and np.all((a >= 0) & (a < sz["front_total"])) and len(np.unique(a)) == len(a)
     ti = t["tiles"]
+    # small fronts: k <= 64, or 65-128 pivots with at most 512 boundary rows on a level holding at
+    # least 16 such fronts (a level with fewer leaves them to the large-front kernels)
     small = (k <= 64) | ((k <= 128) & (b <= 512))
     assert np.all(small[ti[:, 0]]) and np.all(ti[:, 1] < b[ti[:, 0]]) and np.all(ti[:, 2] < b[ti[:, 0]])
     ex = t["ext"]
@@ -164,7 +166,13 @@ def _check_bounds(t, net):
             kk = k[t["small"][lo:hi]]
             assert np.all(kk <= 16 << c) and np.all(kk > (8 << c if c else 0))
             assert np.all(small[t["small"][lo:hi]])
-        assert not np.any(small[t["big"][h]])
+        bigs = t["big"][h]
+        assert np.all(k[bigs] > 64)
+        n128 = int(np.sum(small[fs] & (k[fs] > 64)))
+        if n128 >= 16:
+            assert not np.any(small[bigs])
+        else:
+            assert not np.any((k[fs] > 64) & np.isin(fs, np.concatenate([t["small"][lo:hi] for lo, hi in zip(row[0:4], row[4:8])])))
         for lo, hi in t["ext_slots"][h]:
             assert np.all(level[par[ex[lo:hi, 0]]] == h) if hi > lo else True
     assert np.all(level >= 0)
